@@ -1,0 +1,289 @@
+"""se_e2_a model parameters and the compressed embedding table.
+
+A model here is the data the se_e2_a energy path consumes:
+
+* descriptor: ``rcut``, ``rcut_smth``, ``sel``, embedding widths
+  ``neuron`` (1 -> n0 -> n1 -> n2 = M), ``axis_neuron`` (A), normalisation
+  ``davg``/``dstd`` of shape [ntypes, nnei, 4], and one embedding net per
+  (centre type, neighbour type) pair, or one per neighbour type when
+  ``type_one_side`` (deepmd/pt/model/descriptor/se_a.py:706-740, 785-839);
+* fitting: one net per centre type, D = M*A -> fit[0] -> fit[1] -> ... -> 1
+  with tanh, residual on equal widths, optional ``idt`` (resnet_dt), and a
+  per-type ``bias_atom_e`` (deepmd/pt/model/task/fitting.py:1079-1104,
+  deepmd/pt/model/network/mlp.py:188-219, deepmd/dpmodel/utils/network.py:850-1030);
+* the compressed embedding table: per net a ``[nspline, 6*M]`` array of
+  quintic Hermite coefficients and ``table_info = [lower, upper,
+  upper*extrapolate, stride0, stride1, check_frequency]``
+  (deepmd/utils/tabulate.py:67-370, 468-496).
+
+Embedding-net index convention: net ``e = tj * ntypes + ti`` serves centre
+type ti and neighbour type tj (se_a.py:715-719); with ``type_one_side`` net
+``tj`` serves every centre type.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+
+@dataclasses.dataclass
+class Layer:
+    w: np.ndarray  # [n_in, n_out]
+    b: np.ndarray  # [n_out]
+    idt: np.ndarray | None = None  # [n_out] (resnet_dt timestep)
+    act: bool = True  # tanh or linear
+    resnet: bool = True
+
+
+def layer_forward(layer: Layer, x: np.ndarray) -> np.ndarray:
+    """mlp.py:188-219: linear -> tanh -> *idt -> residual (x or [x, x])."""
+    y = x @ layer.w + layer.b
+    if layer.act:
+        y = np.tanh(y)
+    if layer.idt is not None:
+        y = y * layer.idt
+    if layer.resnet:
+        if x.shape[-1] == y.shape[-1]:
+            y = y + x
+        elif 2 * x.shape[-1] == y.shape[-1]:
+            y = y + np.concatenate([x, x], axis=-1)
+    return y
+
+
+@dataclasses.dataclass
+class SeAModel:
+    ntypes: int
+    sel: list
+    rcut: float
+    rcut_smth: float
+    neuron: list
+    axis_neuron: int
+    fit_neuron: list
+    type_one_side: bool
+    davg: np.ndarray  # [ntypes, nnei, 4]
+    dstd: np.ndarray  # [ntypes, nnei, 4]
+    embed: list  # list (per net) of list[Layer]
+    fitting: list  # list (per centre type) of list[Layer]
+    bias_atom_e: np.ndarray  # [ntypes]
+    tables: list | None = None  # per net [nspline, 6*M] float64
+    table_info: np.ndarray | None = None  # [nnets, 6] float64
+
+    @property
+    def nnei(self) -> int:
+        return int(sum(self.sel))
+
+    @property
+    def sec(self) -> list:
+        out = [0]
+        for s in self.sel:
+            out.append(out[-1] + int(s))
+        return out
+
+    @property
+    def M(self) -> int:
+        return int(self.neuron[-1])
+
+    @property
+    def dim_descrpt(self) -> int:
+        return self.M * self.axis_neuron
+
+    @property
+    def nnets(self) -> int:
+        return self.ntypes if self.type_one_side else self.ntypes * self.ntypes
+
+    def net_index(self, ti: int, tj: int) -> int:
+        return tj if self.type_one_side else tj * self.ntypes + ti
+
+    # ------------------------------------------------------------ builders
+    @classmethod
+    def random(
+        cls,
+        ntypes: int,
+        sel,
+        rcut: float,
+        rcut_smth: float,
+        neuron,
+        axis_neuron: int,
+        fit_neuron,
+        seed: int = 1,
+        type_one_side: bool = False,
+        resnet_dt: bool = False,
+    ) -> "SeAModel":
+        """Seeded random weights N(0, 1/fan_in), zero biases, davg 0, dstd 1."""
+        rng = np.random.default_rng(seed)
+        nnei = int(sum(sel))
+        nnets = ntypes if type_one_side else ntypes * ntypes
+
+        def dense(n_in, n_out, act=True, resnet=True, dt=False):
+            w = rng.normal(0.0, 1.0 / math.sqrt(n_in), size=(n_in, n_out))
+            b = np.zeros(n_out)
+            idt = np.full(n_out, 0.1) if dt else None
+            return Layer(w, b, idt, act, resnet)
+
+        embed = []
+        for _ in range(nnets):
+            layers, n_in = [], 1
+            for n in neuron:
+                layers.append(dense(n_in, n))
+                n_in = n
+            embed.append(layers)
+        D = int(neuron[-1]) * axis_neuron
+        fitting = []
+        for _ in range(ntypes):
+            layers, n_in = [], D
+            for n in fit_neuron:
+                layers.append(dense(n_in, n, dt=resnet_dt))
+                n_in = n
+            layers.append(dense(n_in, 1, act=False, resnet=False))
+            fitting.append(layers)
+        return cls(
+            ntypes=ntypes,
+            sel=list(sel),
+            rcut=float(rcut),
+            rcut_smth=float(rcut_smth),
+            neuron=list(neuron),
+            axis_neuron=int(axis_neuron),
+            fit_neuron=list(fit_neuron),
+            type_one_side=type_one_side,
+            davg=np.zeros((ntypes, nnei, 4)),
+            dstd=np.ones((ntypes, nnei, 4)),
+            embed=embed,
+            fitting=fitting,
+            bias_atom_e=np.zeros(ntypes),
+        )
+
+    # ------------------------------------------------------ embedding net
+    def embed_eval(self, net: int, x: np.ndarray):
+        """Embedding net output and its first and second derivative w.r.t.
+        the scalar input, all [len(x), M] (tanh' = 1 - y^2, tanh'' =
+        -2 y (1 - y^2); residual concat carried through the derivatives).
+        Same quantities as deepmd/utils/tabulate_math.py:353-470."""
+        h = x.reshape(-1, 1).astype(np.float64)
+        dh = np.ones_like(h)
+        d2h = np.zeros_like(h)
+        for layer in self.embed[net]:
+            z = h @ layer.w + layer.b
+            dz = dh @ layer.w
+            d2z = d2h @ layer.w
+            y = np.tanh(z)
+            g1 = 1.0 - y * y
+            g2 = -2.0 * y * g1
+            yo = y
+            dyo = g1 * dz
+            d2yo = g2 * dz * dz + g1 * d2z
+            if layer.idt is not None:
+                yo, dyo, d2yo = yo * layer.idt, dyo * layer.idt, d2yo * layer.idt
+            if h.shape[1] == yo.shape[1]:
+                yo, dyo, d2yo = yo + h, dyo + dh, d2yo + d2h
+            elif 2 * h.shape[1] == yo.shape[1]:
+                cat = lambda a: np.concatenate([a, a], axis=1)  # noqa: E731
+                yo, dyo, d2yo = yo + cat(h), dyo + cat(dh), d2yo + cat(d2h)
+            h, dh, d2h = yo, dyo, d2yo
+        return h, dh, d2h
+
+    # -------------------------------------------------- compressed table
+    def build_tables(
+        self,
+        min_nbor_dist: float,
+        extrapolate: float = 5.0,
+        stride0: float = 0.01,
+        stride1: float = 0.1,
+        check_frequency: float = -1.0,
+    ) -> None:
+        """Quintic Hermite table per embedding net (tabulate.py:67-370).
+
+        Knots: arange(lower, upper, stride0) ++ arange(upper, ext*upper,
+        stride1) ++ [ext*upper], with lower = floor(-davg/dstd) and upper =
+        ceil((s(d_min)/d_min - davg)/dstd) per centre type (tabulate.py:468-486).
+        On each interval of width t the coefficients a0..a5 are the unique
+        quintic matching value, slope and curvature at both knots.
+        """
+        sw = _switch(min_nbor_dist, self.rcut_smth, self.rcut)
+        davg0 = self.davg[:, 0, 0]
+        dstd0 = self.dstd[:, 0, 0]
+        lower_t = np.floor(-davg0 / dstd0)
+        upper_t = np.ceil(((1.0 / min_nbor_dist) * sw - davg0) / dstd0)
+        tables, infos = [], []
+        for net in range(self.nnets):
+            if self.type_one_side:
+                uu = float(np.max(upper_t))
+                ll = float(np.min(lower_t))
+            else:
+                ti = net % self.ntypes
+                uu = float(upper_t[ti])
+                ll = float(lower_t[ti])
+            xx = np.arange(ll, uu, stride0, dtype=np.float64)
+            xx = np.append(xx, np.arange(uu, extrapolate * uu, stride1, dtype=np.float64))
+            xx = np.append(xx, np.array([extrapolate * uu], dtype=np.float64))
+            nspline = int((uu - ll) / stride0 + (extrapolate * uu - uu) / stride1)
+            vv, dd, d2 = self.embed_eval(net, xx)
+            M = self.M
+            tt = np.full((nspline, M), stride1)
+            tt[: int((uu - ll) / stride0), :] = stride0
+            y0, y1 = vv[:nspline], vv[1 : nspline + 1]
+            g0, g1 = dd[:nspline], dd[1 : nspline + 1]
+            c0, c1 = d2[:nspline], d2[1 : nspline + 1]
+            hh = y1 - y0
+            coef = np.empty((nspline, M, 6))
+            coef[:, :, 0] = y0
+            coef[:, :, 1] = g0
+            coef[:, :, 2] = 0.5 * c0
+            coef[:, :, 3] = (20 * hh - (8 * g1 + 12 * g0) * tt - (3 * c0 - c1) * tt * tt) / (2 * tt**3)
+            coef[:, :, 4] = (-30 * hh + (14 * g1 + 16 * g0) * tt + (3 * c0 - 2 * c1) * tt * tt) / (2 * tt**4)
+            coef[:, :, 5] = (12 * hh - 6 * (g1 + g0) * tt + (c1 - c0) * tt * tt) / (2 * tt**5)
+            tables.append(np.ascontiguousarray(coef.reshape(nspline, 6 * M)))
+            infos.append([ll, uu, uu * extrapolate, stride0, stride1, check_frequency])
+        self.tables = tables
+        self.table_info = np.asarray(infos, dtype=np.float64)
+
+    def table_stack(self) -> tuple[np.ndarray, np.ndarray]:
+        """All tables padded to a common row count -> [nnets, nspline, 6M]."""
+        assert self.tables is not None, "build_tables() first"
+        ns = max(t.shape[0] for t in self.tables)
+        out = np.zeros((len(self.tables), ns, self.tables[0].shape[1]))
+        for k, t in enumerate(self.tables):
+            out[k, : t.shape[0]] = t
+        return out, self.table_info.copy()
+
+    # ---------------------------------------------------------- fitting
+    def fitting_arrays(self):
+        """Per-type fitting weights as dense arrays for the device path:
+        list over layers of (W [ntypes, n_in, n_out], b [ntypes, n_out],
+        idt [ntypes, n_out] or None, act, resnet)."""
+        out = []
+        for li in range(len(self.fitting[0])):
+            ls = [self.fitting[t][li] for t in range(self.ntypes)]
+            W = np.stack([l.w for l in ls])
+            b = np.stack([l.b for l in ls])
+            idt = None if ls[0].idt is None else np.stack([l.idt for l in ls])
+            out.append((W, b, idt, ls[0].act, ls[0].resnet))
+        return out
+
+
+def _switch(x: float, rmin: float, rmax: float) -> float:
+    if x < rmin:
+        return 1.0
+    if x < rmax:
+        u = (x - rmin) / (rmax - rmin)
+        return u * u * u * (-6 * u * u + 15 * u - 10) + 1
+    return 0.0
+
+
+def water_model(seed: int = 1, min_nbor_dist: float = 0.5, type_one_side: bool = False) -> SeAModel:
+    """The water se_e2_a model of the benchmark: rcut 6.0, rcut_smth 0.5,
+    sel [46, 92], embedding [25, 50, 100], axis 16, fitting [240]*3, tanh,
+    N(0, 1/fan_in) weights, zero biases, compressed at strides 0.01/0.1."""
+    m = SeAModel.random(2, [46, 92], 6.0, 0.5, [25, 50, 100], 16, [240, 240, 240],
+                        seed=seed, type_one_side=type_one_side)
+    m.build_tables(min_nbor_dist)
+    return m
+
+
+def copper_model(seed: int = 1, min_nbor_dist: float = 1.0) -> SeAModel:
+    """Single-type fcc Cu model: rcut 8.0, rcut_smth 2.0, sel [200],
+    embedding [32, 64, 128], axis 16, fitting [240]*3."""
+    m = SeAModel.random(1, [200], 8.0, 2.0, [32, 64, 128], 16, [240, 240, 240], seed=seed)
+    m.build_tables(min_nbor_dist)
+    return m
