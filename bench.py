@@ -1,0 +1,179 @@
+"""Benchmark of the se_e2_a energy/force/virial path on one MI355X.
+
+Workload (the 12,288-atom water configuration): 4096 H2O in a 49.68 A cubic
+box, se_e2_a rcut 6.0 / rcut_smth 0.5, sel [46, 92], embedding [25, 50, 100]
+compressed at strides 0.01/0.1, axis 16, fitting [240, 240, 240] tanh, seeded
+random weights; synthetic seeded coordinates.  One step = one full E+F+V
+evaluation from raw coordinates (wrap, ghosts, cell-list neighbour list,
+environment matrix, table embedding + contraction, fitting forward and
+backward, force/virial, ghost fold, per-frame sums) with the coordinates
+already resident in HBM.
+
+Prints one JSON line (rank 0).  Also reports per-stage device times, the
+roofline of the dominant kernel (live HIP-event timing over the timed
+region) and the reference CPU path timed on a bounded sample of the same
+workload on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "deepmd-kit-lr-pt_amd"))
+
+METRIC = "atom-steps/s (E+F+V), 12k-atom water, mixed vs FP64 GPU; force RMSE vs FP64"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def stage_model(m, nloc, nall, precision):
+    """Algorithmic bytes / flops per launch of each engine stage (per local
+    atom figures times nloc).  N = sum(sel), M = embedding width, A = axis,
+    D = M*A, s = bytes per float."""
+    N, M, A = m.nnei, m.M, m.axis_neuron
+    D = M * A
+    s = 8
+    rho = nall / nloc
+    fit = [D] + list(m.fit_neuron) + [1]
+    fit_flops = 2 * sum(fit[i] * fit[i + 1] for i in range(len(fit) - 1))
+    per_atom = {
+        # reads nlist (4N) and coordinates, writes em/em_deriv/rij (19 N s)
+        "env_mat": ("hbm", 4 * N + 19 * N * s + (3 * s + 4) * rho),
+        # reads em (4 N s), writes xyz (4 M s)
+        "tabulate": ("flop", 18 * N * M),
+        "tabulate_grad": ("flop", 36 * N * M),
+        # reads net_deriv, em_deriv, rij, nlist; scatters forces
+        "force_virial": ("hbm", 4 * N * s + 12 * N * s + 3 * N * s + 4 * N + 3 * s * rho),
+        # candidate gather + sort: reads the cell lists and ghost coordinates
+        "nlist": ("hbm", 4 * N),
+        "fitting_fwd": ("flop", fit_flops),
+        "fitting_bwd": ("flop", fit_flops),
+        "descriptor": ("hbm", 4 * M * s + D * s),
+        "descriptor_bwd": ("hbm", 4 * M * s + D * s + 4 * M * s),
+    }
+    return {k: (b, amt * nloc) for k, (b, amt) in per_atom.items()}
+
+
+def cpu_baseline(m, coord, atype, box, max_seconds=25.0):
+    """Reference CPU path (the compiled reference lib where present, else the
+    oracle restatement) plus numpy glue for the fitting, on the host cores."""
+    from oracle import oracle_lib, ref_available, ref_lib
+    from oracle.sea import evaluate
+
+    lib = ref_lib() if ref_available() else oracle_lib()
+    kind = "reference" if ref_available() else "port"
+    t0 = time.perf_counter()
+    r = evaluate(m, coord, atype, box, lib)  # warm-up, also the CPU result
+    first = time.perf_counter() - t0
+    n = max(1, min(3, int(max_seconds / max(first, 1e-3))))
+    t0 = time.perf_counter()
+    for _ in range(n):
+        evaluate(m, coord, atype, box, lib)
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return r, {"value": len(atype) * n / dt, "unit": "atom-steps/s", "cores": cores, "kind": kind,
+               "sample": f"{n} E+F+V evaluations of the same 12,288-atom water frame "
+                         f"(reference lib + numpy glue, OMP threads {cores})"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--nmol", type=int, default=4096)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from dpmi import model as dmodel
+    from dpmi import systems
+    from dpmi.engine import DeviceModel, Engine
+
+    m = dmodel.water_model(seed=1)
+    # the 12,288-atom configuration uses the 12.42 A 64-molecule box scaled by 4
+    coord, atype, box = systems.water(args.nmol, seed=2024, box=49.68 if args.nmol == 4096 else None)
+    nloc = len(atype)
+    dm = DeviceModel(m)
+    eng = Engine(dm, [nloc], atype, precision=args.precision)
+    x = torch.as_tensor(coord, device="cuda")
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        eng.compute(x, box)
+    eng.status()
+    eng.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.compute(x, box)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    eng.status()
+    stages = eng.timing()
+    eng.set_timing(False)
+    e, f, v = eng.compute(x, box)
+    torch.cuda.synchronize()
+    nall = int(eng.fetch("nall")[0])
+    ms = dt / args.steps * 1e3
+    value = nloc * args.steps / dt
+
+    model = stage_model(m, nloc, nall, args.precision)
+    dom = max((k for k in stages if k in model), key=lambda k: stages[k])
+    kind, amount = model[dom]
+    t_ms = stages[dom]
+    if kind == "hbm":
+        roof = {"kernel": dom, "bound": "hbm", "achieved": amount / (t_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s"}
+    else:
+        roof = {"kernel": dom, "bound": "mfma", "achieved": amount / (t_ms * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+    roof["avg_ms"] = t_ms
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"water {args.nmol} H2O ({nloc} atoms, box {box[0]:.2f} A), se_e2_a compressed, "
+                               f"{args.precision} engine, one frame", "nloc": nloc, "nall": nall,
+                   "precision": args.precision, "parallelism": "single-gpu"},
+        "roofline": roof,
+        "stages_ms": {k: round(vv, 5) for k, vv in stages.items()},
+    }
+    if not args.no_cpu_baseline:
+        ref, cb = cpu_baseline(m, coord, atype, box)
+        out["cpu_baseline"] = cb
+        fe = f.cpu().numpy()
+        out["accuracy_vs_cpu_reference"] = {
+            "energy_rel": abs(e.item() - ref.energy) / abs(ref.energy),
+            "force_max_rel": float(np.max(np.abs(fe - ref.force)) / np.max(np.abs(ref.force))),
+            "virial_max_rel": float(np.max(np.abs(v[0].cpu().numpy() - ref.virial)) / np.max(np.abs(ref.virial))),
+        }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

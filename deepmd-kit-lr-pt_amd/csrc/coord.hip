@@ -1,0 +1,290 @@
+// Coordinate wrapping, cell binning and ghost-image emission for packed
+// batches of periodic frames (gfx950).
+//
+// Behaviour follows the reference CPU path normalize_coord_cpu
+// (coord.cc:14-28) -> copy_coord_cpu (coord.cc:31-66 -> neighbor_list.cc
+// :43-148, 747-847): extended atoms are the locals followed, for every
+// ghost cell of the extended grid in (x, y, z) lexicographic order, by the
+// atoms of its periodic source cell in ascending index.  The layout is
+// produced with counting sorts and prefix sums instead of the reference's
+// per-cell vectors:
+//   1. k_wrap_bin      one thread per local atom: wrap, bin, count per cell
+//   2. k_scan_cells    one workgroup per frame: exclusive scan of the counts
+//   3. k_scatter_cells one thread per local atom: place into its cell
+//   4. k_sort_cells    one thread per cell: ascending index inside the cell
+//   5. k_scan_xcells   one workgroup per frame: prefix over extended cells
+//                      (all atoms, and ghosts only) -> nall
+//   6. k_emit_ghosts   one wave per extended cell: write ghost images and
+//                      the extended cell -> atom index lists
+// Floating-point expressions that decide cell membership and image
+// coordinates are evaluated without contraction, in the reference's order.
+#include "dpmi_common.h"
+#include "geometry.h"
+#include "kernels.h"
+
+namespace dpmi {
+
+namespace {
+
+__device__ __forceinline__ int image_shift(int idx, int n) {
+  // neighbor_list.cc:730-745
+  int s = 0;
+  if (idx < 0) {
+    s = 1;
+    while (idx + s * n < 0) ++s;
+  } else if (idx >= n) {
+    s = -1;
+    while (idx + s * n >= n) --s;
+  }
+  return s;
+}
+
+__global__ void k_wrap_bin(const double* __restrict__ coord_in, const int* __restrict__ type_in,
+                           const int* __restrict__ frame_of, const FrameGeom* __restrict__ geom, int ntot,
+                           double* __restrict__ coord_ext, int* __restrict__ type_ext, int* __restrict__ map_ext,
+                           int* __restrict__ cell_of, int* __restrict__ cell_cnt, int do_wrap) {
+#pragma clang fp contract(off)
+  const int gidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gidx >= ntot) return;
+  const FrameGeom& g = geom[frame_of[gidx]];
+  const int i = gidx - g.loc_off;
+  double p[3] = {coord_in[3 * gidx + 0], coord_in[3 * gidx + 1], coord_in[3 * gidx + 2]};
+  if (do_wrap) {
+    double f[3];
+    for (int d = 0; d < 3; ++d)
+      f[d] = p[0] * g.rec[d * 3 + 0] + p[1] * g.rec[d * 3 + 1] + p[2] * g.rec[d * 3 + 2];
+    for (int d = 0; d < 3; ++d) {
+      f[d] = fmod(f[d], 1.0);
+      if (f[d] < 0.) f[d] += 1.;
+    }
+    for (int d = 0; d < 3; ++d) p[d] = f[0] * g.t[0 * 3 + d] + f[1] * g.t[1 * 3 + d] + f[2] * g.t[2 * 3 + d];
+  }
+  const size_t e = (size_t)g.ext_off + i;
+  coord_ext[3 * e + 0] = p[0];
+  coord_ext[3 * e + 1] = p[1];
+  coord_ext[3 * e + 2] = p[2];
+  type_ext[e] = type_in[gidx];
+  map_ext[e] = i;
+  // bin on the wrapped coordinate (neighbor_list.cc:88-113)
+  int id[3];
+  for (int d = 0; d < 3; ++d) {
+    const double fr = p[0] * g.rec[d * 3 + 0] + p[1] * g.rec[d * 3 + 1] + p[2] * g.rec[d * 3 + 2];
+    const double cs = 1. / g.nc[d];
+    int k = (int)((fr - 0.) / cs);
+    if (fr - 0. < 0.) k--;
+    if (k < 0) k = 0;
+    if (k >= g.nc[d]) k = g.nc[d] - 1;
+    id[d] = k;
+  }
+  const int c = (id[0] * g.nc[1] + id[1]) * g.nc[2] + id[2];
+  cell_of[gidx] = c;
+  atomicAdd(&cell_cnt[g.cell_off + c], 1);
+}
+
+// Block-wide exclusive scan of n ints (one workgroup, blockDim = 1024).
+__device__ int block_exclusive_scan(const int* __restrict__ in, int* __restrict__ out, int n, int* sh) {
+  const int t = threadIdx.x;
+  const int nt = blockDim.x;
+  const int chunk = (n + nt - 1) / nt;
+  const int b = t * chunk;
+  const int e = min(n, b + chunk);
+  int s = 0;
+  for (int k = b; k < e; ++k) s += in[k];
+  sh[t] = s;
+  __syncthreads();
+  // Hillis-Steele over thread totals
+  for (int off = 1; off < nt; off <<= 1) {
+    int v = (t >= off) ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += v;
+    __syncthreads();
+  }
+  int run = (t == 0) ? 0 : sh[t - 1];
+  const int total = sh[nt - 1];
+  for (int k = b; k < e; ++k) {
+    const int v = in[k];
+    out[k] = run;
+    run += v;
+  }
+  __syncthreads();
+  return total;
+}
+
+__global__ void __launch_bounds__(1024) k_scan_cells(const FrameGeom* __restrict__ geom, const int* __restrict__ cell_cnt,
+                                                     int* __restrict__ cell_start, int* __restrict__ cell_fill) {
+  __shared__ int sh[1024];
+  const FrameGeom& g = geom[blockIdx.x];
+  const int n = g.nc[0] * g.nc[1] * g.nc[2];
+  block_exclusive_scan(cell_cnt + g.cell_off, cell_start + g.cell_off, n, sh);
+  for (int k = threadIdx.x; k < n; k += blockDim.x) cell_fill[g.cell_off + k] = 0;
+}
+
+__global__ void k_scatter_cells(const int* __restrict__ frame_of, const FrameGeom* __restrict__ geom, int ntot,
+                                const int* __restrict__ cell_of, const int* __restrict__ cell_start,
+                                int* __restrict__ cell_fill, int* __restrict__ sorted_loc) {
+  const int gidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gidx >= ntot) return;
+  const FrameGeom& g = geom[frame_of[gidx]];
+  const int c = g.cell_off + cell_of[gidx];
+  const int slot = atomicAdd(&cell_fill[c], 1);
+  sorted_loc[g.loc_off + cell_start[c] + slot] = gidx - g.loc_off;
+}
+
+__global__ void k_sort_cells(const int* __restrict__ cell_frame, const FrameGeom* __restrict__ geom, int ncells_tot,
+                             const int* __restrict__ cell_cnt, const int* __restrict__ cell_start,
+                             int* __restrict__ sorted_loc) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells_tot) return;
+  const FrameGeom& g = geom[cell_frame[c]];
+  int* a = sorted_loc + g.loc_off + cell_start[c];
+  const int n = cell_cnt[c];
+  for (int k = 1; k < n; ++k) {
+    const int v = a[k];
+    int m = k - 1;
+    while (m >= 0 && a[m] > v) {
+      a[m + 1] = a[m];
+      --m;
+    }
+    a[m + 1] = v;
+  }
+}
+
+__device__ __forceinline__ int src_cell(const FrameGeom& g, int e, int* cx, bool* ghost) {
+  int c[3];
+  c[2] = e % g.nx[2];
+  const int r = e / g.nx[2];
+  c[1] = r % g.nx[1];
+  c[0] = r / g.nx[1];
+  bool gh = false;
+  int s[3];
+  for (int d = 0; d < 3; ++d) {
+    c[d] -= g.ng[d];
+    gh = gh || c[d] < 0 || c[d] >= g.nc[d];
+    s[d] = c[d] + image_shift(c[d], g.nc[d]) * g.nc[d];
+    if (cx) cx[d] = c[d];
+  }
+  *ghost = gh;
+  return (s[0] * g.nc[1] + s[1]) * g.nc[2] + s[2];
+}
+
+__global__ void __launch_bounds__(1024) k_scan_xcells(const FrameGeom* __restrict__ geom, const int* __restrict__ cell_cnt,
+                                                      int* __restrict__ xcnt, int* __restrict__ xstart,
+                                                      int* __restrict__ xgcnt, int* __restrict__ xgstart,
+                                                      int* __restrict__ nall_out, int* __restrict__ flag) {
+  __shared__ int sh[1024];
+  const int f = blockIdx.x;
+  const FrameGeom& g = geom[f];
+  const int n = g.nx[0] * g.nx[1] * g.nx[2];
+  const int off = g.xcell_off;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    bool gh;
+    const int s = src_cell(g, e, nullptr, &gh);
+    const int c = cell_cnt[g.cell_off + s];
+    xcnt[off + e] = c;
+    xgcnt[off + e] = gh ? c : 0;
+  }
+  __syncthreads();
+  const int tot = block_exclusive_scan(xcnt + off, xstart + off, n, sh);
+  const int gtot = block_exclusive_scan(xgcnt + off, xgstart + off, n, sh);
+  if (threadIdx.x == 0) {
+    xstart[off + n] = tot;
+    const int nall = g.nloc + gtot;
+    nall_out[f] = nall;
+    if (nall > g.ext_cap) atomicOr(flag, kFlagGhostCapacity);
+  }
+}
+
+__global__ void k_emit_ghosts(const int* __restrict__ xcell_frame, const FrameGeom* __restrict__ geom, int nx_tot,
+                              const int* __restrict__ cell_start, const int* __restrict__ sorted_loc,
+                              const int* __restrict__ xcnt, const int* __restrict__ xstart,
+                              const int* __restrict__ xgstart, double* __restrict__ coord_ext,
+                              int* __restrict__ type_ext, int* __restrict__ map_ext, int* __restrict__ cell_atoms) {
+#pragma clang fp contract(off)
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (wave >= nx_tot) return;
+  const FrameGeom& g = geom[xcell_frame[wave]];
+  const int e = wave - g.xcell_off;
+  if (e >= g.nx[0] * g.nx[1] * g.nx[2]) return;  // the frame's scan sentinel
+  int c[3];
+  bool gh;
+  const int s = src_cell(g, e, c, &gh);
+  const int cnt = xcnt[wave];
+  const int* src = sorted_loc + g.loc_off + cell_start[g.cell_off + s];
+  double sv[3] = {0., 0., 0.};
+  if (gh) {
+    double sh[3];
+    for (int d = 0; d < 3; ++d) sh[d] = (double)image_shift(c[d], g.nc[d]);
+    for (int d = 0; d < 3; ++d) sv[d] = sh[0] * g.t[0 * 3 + d] + sh[1] * g.t[1 * 3 + d] + sh[2] * g.t[2 * 3 + d];
+  }
+  for (int k = lane; k < cnt; k += kWave) {
+    const int p = src[k];
+    const int slot = xstart[wave] + k;
+    if (gh) {
+      const int gi = g.nloc + xgstart[wave] + k;
+      if (gi < g.ext_cap) {
+        const size_t a = (size_t)g.ext_off + p, b = (size_t)g.ext_off + gi;
+        coord_ext[3 * b + 0] = coord_ext[3 * a + 0] - sv[0];
+        coord_ext[3 * b + 1] = coord_ext[3 * a + 1] - sv[1];
+        coord_ext[3 * b + 2] = coord_ext[3 * a + 2] - sv[2];
+        type_ext[b] = type_ext[a];
+        map_ext[b] = p;
+      }
+      if (slot < g.ext_cap) cell_atoms[g.ext_off + slot] = gi;
+    } else {
+      if (slot < g.ext_cap) cell_atoms[g.ext_off + slot] = p;
+    }
+  }
+}
+
+}  // namespace
+
+void launch_build_extended(const CellWork& w, hipStream_t st) {
+  DPMI_CHECK(hipMemsetAsync(w.cell_cnt, 0, sizeof(int) * w.ncells_tot, st));
+  const int tpb = 256;
+  hipLaunchKernelGGL(k_wrap_bin, dim3(ceil_div(w.ntot, tpb)), dim3(tpb), 0, st, w.coord_in, w.type_in, w.frame_of,
+                     w.geom, w.ntot, w.coord_ext, w.type_ext, w.map_ext, w.cell_of, w.cell_cnt, w.do_wrap);
+  DPMI_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_scan_cells, dim3(w.nframes), dim3(1024), 0, st, w.geom, w.cell_cnt, w.cell_start, w.cell_fill);
+  DPMI_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_scatter_cells, dim3(ceil_div(w.ntot, tpb)), dim3(tpb), 0, st, w.frame_of, w.geom, w.ntot,
+                     w.cell_of, w.cell_start, w.cell_fill, w.sorted_loc);
+  DPMI_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_sort_cells, dim3(ceil_div(w.ncells_tot, tpb)), dim3(tpb), 0, st, w.cell_frame, w.geom,
+                     w.ncells_tot, w.cell_cnt, w.cell_start, w.sorted_loc);
+  DPMI_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_scan_xcells, dim3(w.nframes), dim3(1024), 0, st, w.geom, w.cell_cnt, w.xcnt, w.xstart, w.xgcnt,
+                     w.xgstart, w.nall, w.flag);
+  DPMI_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_emit_ghosts, dim3(ceil_div((long long)w.nx_tot * kWave, tpb)), dim3(tpb), 0, st,
+                     w.xcell_frame, w.geom, w.nx_tot, w.cell_start, w.sorted_loc, w.xcnt, w.xstart, w.xgstart,
+                     w.coord_ext, w.type_ext, w.map_ext, w.cell_atoms);
+  DPMI_LAUNCH_CHECK();
+}
+
+}  // namespace dpmi
+
+namespace dpmi {
+namespace {
+__global__ void k_normalize(double* __restrict__ coord, int natom, FrameGeom g) {
+#pragma clang fp contract(off)
+  // coord.cc:14-28
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= natom) return;
+  double p[3] = {coord[3 * i], coord[3 * i + 1], coord[3 * i + 2]};
+  double f[3];
+  for (int d = 0; d < 3; ++d) f[d] = p[0] * g.rec[d * 3 + 0] + p[1] * g.rec[d * 3 + 1] + p[2] * g.rec[d * 3 + 2];
+  for (int d = 0; d < 3; ++d) {
+    f[d] = fmod(f[d], 1.0);
+    if (f[d] < 0.) f[d] += 1.;
+  }
+  for (int d = 0; d < 3; ++d) coord[3 * i + d] = f[0] * g.t[0 * 3 + d] + f[1] * g.t[1 * 3 + d] + f[2] * g.t[2 * 3 + d];
+}
+}  // namespace
+
+void launch_normalize(double* coord, int natom, const FrameGeom& g, hipStream_t st) {
+  if (natom <= 0) return;
+  hipLaunchKernelGGL(k_normalize, dim3(ceil_div(natom, 256)), dim3(256), 0, st, coord, natom, g);
+  DPMI_LAUNCH_CHECK();
+}
+}  // namespace dpmi

@@ -1,0 +1,579 @@
+// Whole-path se_e2_a engine: packed frames -> E, F, V on one GPU.
+//
+// Host orchestration of the HIP kernels for the path of the reference TF
+// descriptor (deepmd/tf/descriptor/se_a.py:578-779) and PT model
+// (deepmd/pt/model/descriptor/se_a.py:742-859, task/fitting.py:1079-1104,
+// model/transform_output.py:70-211): cell list -> neighbour list ->
+// environment matrix -> table embedding + contraction -> descriptor ->
+// fitting (per-type row blocks) -> backward to net_deriv -> force/virial ->
+// ghost fold and per-frame reductions.  Everything after the per-step box
+// upload runs asynchronously on the caller's stream.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "dpmi_common.h"
+#include "engine.h"
+#include "kernels.h"
+
+namespace dpmi {
+
+namespace {
+
+template <typename T>
+T* dalloc(size_t n) {
+  if (n == 0) n = 1;
+  void* p = nullptr;
+  DPMI_CHECK(hipMalloc(&p, n * sizeof(T)));
+  return static_cast<T*>(p);
+}
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+template <typename T>
+T* dupload(const T* h, size_t n) {
+  T* d = dalloc<T>(n);
+  if (n) DPMI_CHECK(hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice));
+  return d;
+}
+
+void rb_check(rocblas_status s, const char* what) {
+  if (s != rocblas_status_success) throw HipError(std::string("rocBLAS error in ") + what + ": " + std::to_string(s));
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------- model --
+SeaModel::SeaModel(const dpmi_sea_model_desc& d) {
+  ntypes = d.ntypes;
+  if (ntypes <= 0 || ntypes > 127) throw std::invalid_argument("ntypes must be in [1, 127]");
+  sel.assign(d.sel, d.sel + ntypes);
+  sec.assign(ntypes + 1, 0);
+  for (int t = 0; t < ntypes; ++t) sec[t + 1] = sec[t] + sel[t];
+  nnei = sec[ntypes];
+  rcut = d.rcut;
+  rcut_smth = d.rcut_smth;
+  M = d.last_layer_size;
+  A = d.axis_neuron;
+  if (A > M) throw std::invalid_argument("axis_neuron must not exceed the last embedding width");
+  type_one_side = d.type_one_side != 0;
+  nnets = d.nnets;
+  if (nnets != (type_one_side ? ntypes : ntypes * ntypes)) throw std::invalid_argument("nnets mismatch");
+  davg.assign(d.davg, d.davg + (size_t)ntypes * nnei * 4);
+  dstd.assign(d.dstd, d.dstd + (size_t)ntypes * nnei * 4);
+  tab_off.assign(nnets, 0);
+  long long tot = 0;
+  for (int n = 0; n < nnets; ++n) {
+    tab_off[n] = tot;
+    tot += (long long)d.nspline[n] * 6 * M;
+  }
+  table_host.resize(tot);
+  for (int n = 0; n < nnets; ++n)
+    std::memcpy(table_host.data() + tab_off[n], d.tables[n], sizeof(double) * (size_t)d.nspline[n] * 6 * M);
+  info_host.assign(d.table_info, d.table_info + nnets * 6);
+  net_of.assign(ntypes * ntypes, 0);
+  for (int ti = 0; ti < ntypes; ++ti)
+    for (int tj = 0; tj < ntypes; ++tj) net_of[ti * ntypes + tj] = type_one_side ? tj : tj * ntypes + ti;
+  nlayers = d.fit_nlayers;
+  width.assign(d.fit_width, d.fit_width + nlayers + 1);
+  if (width[0] != M * A) throw std::invalid_argument("fitting input width must be M*A");
+  if (width[nlayers] != 1) throw std::invalid_argument("fitting output width must be 1");
+  act.assign(d.fit_act, d.fit_act + nlayers);
+  res_mode.assign(nlayers, 0);
+  for (int l = 0; l < nlayers; ++l) {
+    if (d.fit_resnet[l]) {
+      if (width[l] == width[l + 1])
+        res_mode[l] = 1;
+      else if (2 * width[l] == width[l + 1])
+        res_mode[l] = 2;
+    }
+  }
+  bias_atom_e.assign(d.bias_atom_e, d.bias_atom_e + ntypes);
+  has_idt.assign(nlayers, 0);
+  for (int l = 0; l < nlayers; ++l) has_idt[l] = d.fit_idt[l] != nullptr;
+
+  // device copies
+  d_table = dupload(table_host.data(), table_host.size());
+  d_tab_off = dupload(tab_off.data(), tab_off.size());
+  void* infos = make_table_infos(info_host.data(), nnets, false);
+  d_infos = dupload(static_cast<const char*>(infos), table_info_bytes(false) * nnets);
+  free_table_infos(infos, false);
+  d_net_of = dupload(net_of.data(), net_of.size());
+  d_sec = dupload(sec.data(), sec.size());
+  d_davg = dupload(davg.data(), davg.size());
+  d_dstd = dupload(dstd.data(), dstd.size());
+  fit_w.resize((size_t)ntypes * nlayers);
+  fit_b.resize((size_t)ntypes * nlayers);
+  fit_idt.resize((size_t)ntypes * nlayers, nullptr);
+  out_w_host.resize(ntypes);
+  out_b_host.resize(ntypes);
+  for (int t = 0; t < ntypes; ++t)
+    for (int l = 0; l < nlayers; ++l) {
+      const size_t k = (size_t)t * nlayers + l;
+      const int ni = width[l], no = width[l + 1];
+      fit_w[k] = dupload(d.fit_w[k], (size_t)ni * no);
+      fit_b[k] = dupload(d.fit_b[k], (size_t)no);
+      if (d.fit_idt[k]) fit_idt[k] = dupload(d.fit_idt[k], (size_t)no);
+      if (l == nlayers - 1) {
+        out_w_host[t].assign(d.fit_w[k], d.fit_w[k] + ni);
+        out_b_host[t] = d.fit_b[k][0];
+      }
+    }
+}
+
+SeaModel::~SeaModel() {
+  dfree(d_table);
+  dfree(d_tab_off);
+  dfree(d_infos);
+  dfree(d_net_of);
+  dfree(d_sec);
+  dfree(d_davg);
+  dfree(d_dstd);
+  for (auto& p : fit_w) dfree(p);
+  for (auto& p : fit_b) dfree(p);
+  for (auto& p : fit_idt) dfree(p);
+}
+
+// ---------------------------------------------------------------- engine --
+SeaEngine::SeaEngine(const SeaModel* m, int nframes_, const int* natoms_h, const int* atype_h, int precision_,
+                     double nall_per_atom_)
+    : model(m), nframes(nframes_), precision(precision_), nall_per_atom(nall_per_atom_) {
+  if (precision != DPMI_FP64) throw std::invalid_argument("only DPMI_FP64 is available in this build");
+  natoms.assign(natoms_h, natoms_h + nframes);
+  loc_off.assign(nframes + 1, 0);
+  for (int f = 0; f < nframes; ++f) loc_off[f + 1] = loc_off[f] + natoms[f];
+  ntot = loc_off[nframes];
+  atype.assign(atype_h, atype_h + ntot);
+  // per-row frame, centre, type permutation
+  std::vector<int> frame_of(ntot), center(ntot);
+  for (int f = 0; f < nframes; ++f)
+    for (int i = 0; i < natoms[f]; ++i) {
+      frame_of[loc_off[f] + i] = f;
+      center[loc_off[f] + i] = i;
+    }
+  const int nt = m->ntypes;
+  tstart.assign(nt + 2, 0);
+  std::vector<int> cnt(nt + 1, 0);
+  for (int g = 0; g < ntot; ++g) {
+    const int t = atype[g];
+    if (t >= nt) throw std::invalid_argument("atom type >= ntypes");
+    cnt[t < 0 ? nt : t]++;
+  }
+  for (int t = 0; t <= nt; ++t) tstart[t + 1] = tstart[t] + cnt[t];
+  std::vector<int> fill(tstart.begin(), tstart.end() - 1), pos(ntot), perm(ntot);
+  for (int g = 0; g < ntot; ++g) {
+    const int t = atype[g] < 0 ? nt : atype[g];
+    pos[g] = fill[t]++;
+    perm[pos[g]] = g;
+  }
+  d_frame_of = dupload(frame_of.data(), ntot);
+  d_center = dupload(center.data(), ntot);
+  d_type = dupload(atype.data(), ntot);
+  d_pos = dupload(pos.data(), ntot);
+  d_perm = dupload(perm.data(), ntot);
+  geom_h.resize(nframes);
+  DPMI_CHECK(hipHostMalloc((void**)&geom_pinned, sizeof(FrameGeom) * nframes, hipHostMallocDefault));
+  d_geom = dalloc<FrameGeom>(nframes);
+  d_flag = dalloc<int>(1);
+  d_nall = dalloc<int>(nframes);
+  d_frame_ext_off = dalloc<int>(nframes);
+  const size_t nn = (size_t)ntot * m->nnei;
+  d_nlist = dalloc<int>(nn);
+  d_ncand = dalloc<int>(ntot);
+  d_cell_of = dalloc<int>(ntot);
+  d_sorted_loc = dalloc<int>(ntot);
+  d_em = dalloc<double>(nn * 4);
+  d_em_deriv = dalloc<double>(nn * 12);
+  d_rij = dalloc<double>(nn * 3);
+  d_net_deriv = dalloc<double>(nn * 4);
+  d_xyz = dalloc<double>((size_t)ntot * 4 * m->M);
+  d_gxyz = dalloc<double>((size_t)ntot * 4 * m->M);
+  const int D = m->M * m->A;
+  d_D = dalloc<double>((size_t)ntot * D);
+  d_gD = dalloc<double>((size_t)ntot * D);
+  DPMI_CHECK(hipMemset(d_gD, 0, sizeof(double) * (size_t)ntot * D));
+  d_atom_e = dalloc<double>(ntot);
+  d_vpart = dalloc<double>((size_t)ntot * 9);
+  for (int l = 0; l < m->nlayers; ++l) {
+    d_fz.push_back(dalloc<double>((size_t)ntot * m->width[l + 1]));
+    d_fy.push_back(dalloc<double>((size_t)ntot * m->width[l + 1]));
+    d_fx.push_back(dalloc<double>((size_t)ntot * m->width[l + 1]));
+  }
+  int wmax = 0;
+  for (int w : m->width) wmax = std::max(wmax, w);
+  d_g0 = dalloc<double>((size_t)ntot * wmax);
+  d_g1 = dalloc<double>((size_t)ntot * wmax);
+  d_outw.resize(nt, nullptr);
+  for (int t = 0; t < nt; ++t) d_outw[t] = dupload(m->out_w_host[t].data(), m->out_w_host[t].size());
+  rocblas_handle h;
+  rb_check(rocblas_create_handle(&h), "create_handle");
+  blas = h;
+  nlist_cap = nlist_row_capacity(std::max(128, 2 * m->nnei));
+}
+
+SeaEngine::~SeaEngine() {
+  if (blas) rocblas_destroy_handle((rocblas_handle)blas);
+  if (geom_pinned) (void)hipHostFree(geom_pinned);
+  for (auto* p : {&d_frame_of, &d_center, &d_type, &d_pos, &d_perm, &d_flag, &d_nall, &d_frame_ext_off, &d_nlist,
+                  &d_ncand, &d_cell_of, &d_sorted_loc, &d_cell_frame, &d_xcell_frame, &d_cell_cnt, &d_cell_start,
+                  &d_cell_fill, &d_xcnt, &d_xstart, &d_xgcnt, &d_xgstart, &d_type_ext, &d_map_ext, &d_cell_atoms,
+                  &d_xframe_of_ext})
+    dfree(*p);
+  for (auto* p : {&d_em, &d_em_deriv, &d_rij, &d_net_deriv, &d_xyz, &d_gxyz, &d_D, &d_gD, &d_atom_e, &d_vpart,
+                  &d_coord_ext, &d_force_ext, &d_g0, &d_g1})
+    dfree(*p);
+  dfree(d_geom);
+  for (auto& p : d_fz) dfree(p);
+  for (auto& p : d_fy) dfree(p);
+  for (auto& p : d_fx) dfree(p);
+  for (auto& p : d_outw) dfree(p);
+  for (auto& e : events) (void)hipEventDestroy(e);
+}
+
+// Box-dependent layout: cell grids and extended capacities per frame.
+void SeaEngine::prepare_geometry(const double* box_h) {
+  int ncells = 0, nxcells = 0;
+  long long next = 0;
+  bool grid_changed = (int)cell_frame_h.size() == 0;
+  for (int f = 0; f < nframes; ++f) {
+    FrameGeom& g = geom_h[f];
+    const FrameGeom old = g;
+    geom_from_box(g, box_h + 9 * f, model->rcut);
+    for (int d = 0; d < 3; ++d) grid_changed = grid_changed || g.nc[d] != old.nc[d] || g.ng[d] != old.ng[d];
+    g.nloc = natoms[f];
+    g.loc_off = loc_off[f];
+    g.cell_off = ncells;
+    g.xcell_off = nxcells;
+    ncells += geom_local_cells(g);
+    nxcells += geom_ext_cells(g) + 1;
+    const int cap =
+        nall_per_atom > 0 ? (int)std::ceil(natoms[f] * nall_per_atom) + 64 : geom_nall_estimate(g, natoms[f], 1.25);
+    g.ext_cap = std::max(cap, natoms[f]);
+    if (!grid_changed && f < (int)ext_cap_h.size()) g.ext_cap = std::max(g.ext_cap, ext_cap_h[f]);
+    g.ext_off = (int)next;
+    next += g.ext_cap;
+  }
+  if (grid_changed || next > next_tot || ncells > ncells_tot || nxcells > nx_tot) {
+    DPMI_CHECK(hipDeviceSynchronize());
+    std::vector<int> cf(ncells), xf(nxcells), ef(next), eoff(nframes);
+    ext_cap_h.resize(nframes);
+    for (int f = 0; f < nframes; ++f) {
+      const FrameGeom& g = geom_h[f];
+      std::fill(cf.begin() + g.cell_off, cf.begin() + g.cell_off + geom_local_cells(g), f);
+      std::fill(xf.begin() + g.xcell_off, xf.begin() + g.xcell_off + geom_ext_cells(g) + 1, f);
+      std::fill(ef.begin() + g.ext_off, ef.begin() + g.ext_off + g.ext_cap, f);
+      eoff[f] = g.ext_off;
+      ext_cap_h[f] = g.ext_cap;
+    }
+    for (auto* p : {&d_cell_frame, &d_xcell_frame, &d_cell_cnt, &d_cell_start, &d_cell_fill, &d_xcnt, &d_xstart,
+                    &d_xgcnt, &d_xgstart, &d_type_ext, &d_map_ext, &d_cell_atoms, &d_xframe_of_ext})
+      dfree(*p);
+    dfree(d_coord_ext);
+    dfree(d_force_ext);
+    d_cell_frame = dupload(cf.data(), cf.size());
+    d_xcell_frame = dupload(xf.data(), xf.size());
+    d_xframe_of_ext = dupload(ef.data(), ef.size());
+    d_cell_cnt = dalloc<int>(ncells);
+    d_cell_start = dalloc<int>(ncells);
+    d_cell_fill = dalloc<int>(ncells);
+    d_xcnt = dalloc<int>(nxcells);
+    d_xstart = dalloc<int>(nxcells);
+    d_xgcnt = dalloc<int>(nxcells);
+    d_xgstart = dalloc<int>(nxcells);
+    d_type_ext = dalloc<int>(next);
+    d_map_ext = dalloc<int>(next);
+    d_cell_atoms = dalloc<int>(next);
+    d_coord_ext = dalloc<double>((size_t)next * 3);
+    d_force_ext = dalloc<double>((size_t)next * 3);
+    DPMI_CHECK(hipMemcpy(d_frame_ext_off, eoff.data(), sizeof(int) * nframes, hipMemcpyHostToDevice));
+    cell_frame_h = cf;
+    ncells_tot = ncells;
+    nx_tot = nxcells;
+    next_tot = next;
+  }
+}
+
+// Stage events are kept per call (up to kMaxTimedCalls) so that a timed loop
+// of many asynchronous calls can be read back once at the end.
+constexpr int kMaxTimedCalls = 4096;
+constexpr int kMaxStages = 16;
+
+void SeaEngine::mark(hipStream_t st, int stage) {
+  if (!timing || timed_calls >= kMaxTimedCalls) return;
+  const size_t k = (size_t)timed_calls * kMaxStages + stage;
+  while (events.size() <= k) {
+    hipEvent_t e;
+    DPMI_CHECK(hipEventCreate(&e));
+    events.push_back(e);
+  }
+  DPMI_CHECK(hipEventRecord(events[k], st));
+}
+
+static const char* kStageNames = "geometry;nlist;env_mat;tabulate;descriptor;fitting_fwd;fitting_bwd;descriptor_bwd;"
+                                 "tabulate_grad;force_virial;reduce";
+
+void SeaEngine::compute(const double* coord, const double* box_h, double* energy, double* force, double* virial,
+                        double* atom_energy, hipStream_t st) {
+  const SeaModel& m = *model;
+  prepare_geometry(box_h);
+  last_stream = st;
+  int stage = 0;
+  mark(st, stage++);
+  // upload the per-frame layout only when it changed; the pinned staging
+  // buffer is rewritten only after the previous upload has completed
+  if (!geom_uploaded || std::memcmp(geom_pinned, geom_h.data(), sizeof(FrameGeom) * nframes) != 0) {
+    DPMI_CHECK(hipStreamSynchronize(st));
+    std::memcpy(geom_pinned, geom_h.data(), sizeof(FrameGeom) * nframes);
+    DPMI_CHECK(hipMemcpyAsync(d_geom, geom_pinned, sizeof(FrameGeom) * nframes, hipMemcpyHostToDevice, st));
+    geom_uploaded = true;
+  }
+  DPMI_CHECK(hipMemsetAsync(d_flag, 0, sizeof(int), st));
+
+  CellWork w;
+  w.nframes = nframes;
+  w.ntot = ntot;
+  w.ncells_tot = ncells_tot;
+  w.nx_tot = nx_tot;
+  w.do_wrap = 1;
+  w.coord_in = coord;
+  w.type_in = d_type;
+  w.frame_of = d_frame_of;
+  w.cell_frame = d_cell_frame;
+  w.xcell_frame = d_xcell_frame;
+  w.geom = d_geom;
+  w.coord_ext = d_coord_ext;
+  w.type_ext = d_type_ext;
+  w.map_ext = d_map_ext;
+  w.cell_atoms = d_cell_atoms;
+  w.cell_of = d_cell_of;
+  w.cell_cnt = d_cell_cnt;
+  w.cell_start = d_cell_start;
+  w.cell_fill = d_cell_fill;
+  w.sorted_loc = d_sorted_loc;
+  w.xcnt = d_xcnt;
+  w.xstart = d_xstart;
+  w.xgcnt = d_xgcnt;
+  w.xgstart = d_xgstart;
+  w.nall = d_nall;
+  w.flag = d_flag;
+  launch_build_extended(w, st);
+  mark(st, stage++);
+
+  NlistArgs na;
+  na.cap = nlist_cap;
+  na.sec = m.d_sec;
+  na.ntypes = m.ntypes;
+  na.nnei = m.nnei;
+  na.rc2 = (double)(m.rcut * m.rcut);
+  na.rc2f = m.rcut * m.rcut;
+  na.nlist = d_nlist;
+  na.ncand = d_ncand;
+  launch_cell_nlist(w, na, st);
+  mark(st, stage++);
+
+  EnvArgs ea;
+  ea.coord = d_coord_ext;
+  ea.type = d_type_ext;
+  ea.nlist = d_nlist;
+  ea.row_frame = d_frame_of;
+  ea.row_ext_off = d_frame_ext_off;
+  ea.row_center = d_center;
+  ea.nrows = ntot;
+  ea.nnei = m.nnei;
+  ea.avg = m.d_davg;
+  ea.std_ = m.d_dstd;
+  ea.rmin = (double)m.rcut_smth;
+  ea.rmax = (double)m.rcut;
+  ea.em = d_em;
+  ea.em_deriv = d_em_deriv;
+  ea.rij = d_rij;
+  launch_env_mat_a(ea, st);
+  mark(st, stage++);
+
+  TabRowsArgs ta;
+  ta.xyz = d_xyz;
+  ta.net_deriv = d_net_deriv;
+  ta.gxyz = d_gxyz;
+  ta.tables = m.d_table;
+  ta.tab_off = m.d_tab_off;
+  ta.infos = m.d_infos;
+  ta.net_of = m.d_net_of;
+  ta.em = d_em;
+  ta.row_type = d_type;
+  ta.sec = m.d_sec;
+  ta.ntypes = m.ntypes;
+  ta.nrows = ntot;
+  ta.nnei = m.nnei;
+  ta.M = m.M;
+  launch_tab_fwd_rows(ta, st);
+  mark(st, stage++);
+
+  const double inv_nnei = 1.0 / m.nnei;
+  launch_descriptor(d_xyz, d_pos, ntot, m.M, m.A, inv_nnei, d_D, st);
+  mark(st, stage++);
+
+  // fitting forward, one row block per type
+  rocblas_handle h = (rocblas_handle)blas;
+  rb_check(rocblas_set_stream(h, st), "set_stream");
+  const double one = 1.0, zero = 0.0;
+  launch_fill_double(d_atom_e, 0.0, ntot, st);
+  const int L = m.nlayers;
+  for (int t = 0; t < m.ntypes; ++t) {
+    const long long p0 = tstart[t], n = tstart[t + 1] - tstart[t];
+    if (n == 0) continue;
+    const double* x = d_D + (size_t)p0 * m.width[0];
+    for (int l = 0; l < L; ++l) {
+      const int ni = m.width[l], no = m.width[l + 1];
+      const size_t k = (size_t)t * L + l;
+      double* z = d_fz[l] + (size_t)p0 * no;
+      rb_check(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, no, (int)n, ni, &one, m.fit_w[k], no,
+                             x, ni, &zero, z, no),
+               "dgemm fwd");
+      if (l < L - 1) {
+        launch_fit_act(z, m.fit_b[k], m.fit_idt[k], x, m.act[l], m.res_mode[l], n, ni, no, d_fy[l] + (size_t)p0 * no,
+                       d_fx[l] + (size_t)p0 * no, st);
+        x = d_fx[l] + (size_t)p0 * no;
+      } else {
+        launch_fit_out(z, m.out_b_host[t], m.bias_atom_e[t], d_perm, p0, n, d_atom_e, st);
+      }
+    }
+  }
+  mark(st, stage++);
+
+  // fitting backward: dE/dD
+  for (int t = 0; t < m.ntypes; ++t) {
+    const long long p0 = tstart[t], n = tstart[t + 1] - tstart[t];
+    if (n == 0) continue;
+    // g = dE/d(x_{L-1}) = w_out broadcast over rows
+    const int wl = m.width[L - 1];
+    double* g = d_g0;
+    double* gn = d_g1;
+    launch_fill_rows(g, d_outw[t], n, wl, st);
+    for (int l = L - 2; l >= 0; --l) {
+      const int ni = m.width[l], no = m.width[l + 1];
+      const size_t k = (size_t)t * L + l;
+      double* gz = d_fz[l] + (size_t)p0 * no;  // reuse z storage for g_z
+      launch_fit_act_bwd(g, d_fy[l] + (size_t)p0 * no, m.fit_idt[k], m.act[l], n, no, gz, st);
+      double* gin = (l == 0) ? d_gD + (size_t)p0 * ni : gn;
+      rb_check(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, ni, (int)n, no, &one,
+                             m.fit_w[k], no, gz, no, &zero, gin, ni),
+               "dgemm bwd");
+      launch_fit_res_bwd(g, m.res_mode[l], n, ni, no, gin, st);
+      if (l > 0) std::swap(g, gn);
+    }
+  }
+  mark(st, stage++);
+
+  launch_descriptor_bwd(d_xyz, d_gD, d_pos, ntot, m.M, m.A, inv_nnei, d_gxyz, st);
+  mark(st, stage++);
+  launch_tab_grad_rows(ta, st);
+  mark(st, stage++);
+
+  DPMI_CHECK(hipMemsetAsync(d_force_ext, 0, sizeof(double) * (size_t)next_tot * 3, st));
+  ForceRowsArgs fa;
+  fa.net_deriv = d_net_deriv;
+  fa.em_deriv = d_em_deriv;
+  fa.rij = d_rij;
+  fa.nlist = d_nlist;
+  fa.row_frame = d_frame_of;
+  fa.row_center = d_center;
+  fa.frame_ext_off = d_frame_ext_off;
+  fa.nrows = ntot;
+  fa.nnei = m.nnei;
+  fa.force_ext = d_force_ext;
+  fa.vpart = d_vpart;
+  launch_force_virial_rows(fa, st);
+  DPMI_CHECK(hipMemsetAsync(force, 0, sizeof(double) * (size_t)ntot * 3, st));
+  launch_fold_ghost_forces(d_force_ext, d_map_ext, d_xframe_of_ext, d_geom, d_nall, next_tot, force, st);
+  mark(st, stage++);
+  launch_frame_reduce(d_atom_e, d_vpart, d_geom, nframes, energy, virial, st);
+  if (atom_energy)
+    DPMI_CHECK(hipMemcpyAsync(atom_energy, d_atom_e, sizeof(double) * ntot, hipMemcpyDeviceToDevice, st));
+  mark(st, stage++);
+  nstages = stage - 1;
+  if (timing && timed_calls < kMaxTimedCalls) ++timed_calls;
+}
+
+int SeaEngine::status() {
+  DPMI_CHECK(hipStreamSynchronize(last_stream));
+  int flag = 0;
+  DPMI_CHECK(hipMemcpy(&flag, d_flag, sizeof(int), hipMemcpyDeviceToHost));
+  if (flag & kFlagNlistCapacity) return DPMI_ERR_NLIST_CAPACITY;
+  if (flag & kFlagGhostCapacity) return DPMI_ERR_MEMORY;
+  if (flag & (kFlagIndexRange | kFlagTypeRange)) return DPMI_ERR_RANGE;
+  return DPMI_OK;
+}
+
+long long SeaEngine::fetch(int frame, int what, void* dst) {
+  const SeaModel& m = *model;
+  DPMI_CHECK(hipStreamSynchronize(last_stream));
+  if (frame < 0 || frame >= nframes) throw std::invalid_argument("frame out of range");
+  int nall = 0;
+  DPMI_CHECK(hipMemcpy(&nall, d_nall + frame, sizeof(int), hipMemcpyDeviceToHost));
+  const FrameGeom& g = geom_h[frame];
+  const size_t r0 = loc_off[frame];
+  const int nloc = natoms[frame];
+  const void* src = nullptr;
+  long long n = 0;
+  size_t es = 0;
+  switch (what) {
+    case 0:
+      if (dst) std::memcpy(dst, &nall, sizeof(int));
+      return 1;
+    case 1: src = d_coord_ext + (size_t)g.ext_off * 3; n = 3LL * nall; es = 8; break;
+    case 2: src = d_type_ext + g.ext_off; n = nall; es = 4; break;
+    case 3: src = d_map_ext + g.ext_off; n = nall; es = 4; break;
+    case 4: src = d_nlist + r0 * m.nnei; n = (long long)nloc * m.nnei; es = 4; break;
+    case 5: src = d_em + r0 * m.nnei * 4; n = (long long)nloc * m.nnei * 4; es = 8; break;
+    case 6: {
+      // descriptor rows are stored type-permuted
+      const int D = m.M * m.A;
+      n = (long long)nloc * D;
+      if (dst) {
+        std::vector<int> pos(ntot);
+        DPMI_CHECK(hipMemcpy(pos.data(), d_pos, sizeof(int) * ntot, hipMemcpyDeviceToHost));
+        for (int i = 0; i < nloc; ++i)
+          DPMI_CHECK(hipMemcpy(static_cast<double*>(dst) + (size_t)i * D, d_D + (size_t)pos[r0 + i] * D,
+                               sizeof(double) * D, hipMemcpyDeviceToHost));
+      }
+      return n;
+    }
+    case 7: src = d_net_deriv + r0 * m.nnei * 4; n = (long long)nloc * m.nnei * 4; es = 8; break;
+    case 8: src = d_atom_e + r0; n = nloc; es = 8; break;
+    default: throw std::invalid_argument("unknown fetch target");
+  }
+  if (dst && n) DPMI_CHECK(hipMemcpy(dst, src, es * n, hipMemcpyDeviceToHost));
+  return n;
+}
+
+// Average per-stage device time over the calls recorded since timing was
+// (re)enabled.
+int SeaEngine::timing_report(double* ms, int max_stages, char* names, int names_len) {
+  if (!timing || nstages <= 0 || timed_calls == 0) return 0;
+  DPMI_CHECK(hipStreamSynchronize(last_stream));
+  const int n = std::min(nstages, max_stages);
+  for (int s = 0; s < n; ++s) {
+    double acc = 0.;
+    for (int c = 0; c < timed_calls; ++c) {
+      float t = 0.f;
+      DPMI_CHECK(hipEventElapsedTime(&t, events[(size_t)c * kMaxStages + s], events[(size_t)c * kMaxStages + s + 1]));
+      acc += t;
+    }
+    ms[s] = acc / timed_calls;
+  }
+  if (names && names_len > 0) {
+    std::strncpy(names, kStageNames, names_len - 1);
+    names[names_len - 1] = 0;
+  }
+  return n;
+}
+
+}  // namespace dpmi

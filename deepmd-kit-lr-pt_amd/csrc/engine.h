@@ -1,0 +1,96 @@
+// Host-side model and engine objects behind dpmi_sea_* (include/dpmi.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../include/dpmi.h"
+#include "geometry.h"
+
+namespace dpmi {
+
+struct SeaModel {
+  explicit SeaModel(const dpmi_sea_model_desc& d);
+  ~SeaModel();
+  int ntypes = 0, nnei = 0, M = 0, A = 0, nnets = 0, nlayers = 0;
+  float rcut = 0, rcut_smth = 0;
+  bool type_one_side = false;
+  std::vector<int> sel, sec, net_of, width, act, res_mode, has_idt;
+  std::vector<double> davg, dstd, table_host, info_host, bias_atom_e, out_b_host;
+  std::vector<std::vector<double>> out_w_host;
+  std::vector<long long> tab_off;
+  // device
+  double* d_table = nullptr;
+  long long* d_tab_off = nullptr;
+  char* d_infos = nullptr;
+  int* d_net_of = nullptr;
+  int* d_sec = nullptr;
+  double* d_davg = nullptr;
+  double* d_dstd = nullptr;
+  std::vector<double*> fit_w, fit_b, fit_idt;  // [ntypes * nlayers]
+};
+
+struct SeaEngine {
+  SeaEngine(const SeaModel* m, int nframes, const int* natoms, const int* atype, int precision,
+            double nall_per_atom);
+  ~SeaEngine();
+  void compute(const double* coord, const double* box_h, double* energy, double* force, double* virial,
+               double* atom_energy, hipStream_t st);
+  int status();
+  long long fetch(int frame, int what, void* dst);
+  int timing_report(double* ms, int max_stages, char* names, int names_len);
+
+  const SeaModel* model;
+  int nframes, precision;
+  double nall_per_atom;
+  int ntot = 0;
+  std::vector<int> natoms, loc_off, atype, tstart;
+  bool timing = false;
+
+ private:
+  void prepare_geometry(const double* box_h);
+  void mark(hipStream_t st, int stage);
+
+  std::vector<FrameGeom> geom_h;
+  FrameGeom* geom_pinned = nullptr;
+  std::vector<int> cell_frame_h, ext_cap_h;
+  int ncells_tot = 0, nx_tot = 0;
+  long long next_tot = 0;
+  int nlist_cap = 256;
+  void* blas = nullptr;
+  hipStream_t last_stream = nullptr;
+  std::vector<hipEvent_t> events;
+  int nstages = 0;
+  int timed_calls = 0;
+  bool geom_uploaded = false;
+
+ public:
+  void reset_timing() { timed_calls = 0; }
+
+ private:
+
+  FrameGeom* d_geom = nullptr;
+  int *d_frame_of = nullptr, *d_center = nullptr, *d_type = nullptr, *d_pos = nullptr, *d_perm = nullptr;
+  int *d_flag = nullptr, *d_nall = nullptr, *d_frame_ext_off = nullptr;
+  int *d_nlist = nullptr, *d_ncand = nullptr, *d_cell_of = nullptr, *d_sorted_loc = nullptr;
+  int *d_cell_frame = nullptr, *d_xcell_frame = nullptr, *d_cell_cnt = nullptr, *d_cell_start = nullptr,
+      *d_cell_fill = nullptr;
+  int *d_xcnt = nullptr, *d_xstart = nullptr, *d_xgcnt = nullptr, *d_xgstart = nullptr;
+  int *d_type_ext = nullptr, *d_map_ext = nullptr, *d_cell_atoms = nullptr, *d_xframe_of_ext = nullptr;
+  double *d_coord_ext = nullptr, *d_force_ext = nullptr;
+  double *d_em = nullptr, *d_em_deriv = nullptr, *d_rij = nullptr, *d_net_deriv = nullptr;
+  double *d_xyz = nullptr, *d_gxyz = nullptr, *d_D = nullptr, *d_gD = nullptr, *d_atom_e = nullptr,
+         *d_vpart = nullptr;
+  std::vector<double*> d_fz, d_fy, d_fx, d_outw;
+  double *d_g0 = nullptr, *d_g1 = nullptr;
+};
+
+}  // namespace dpmi
+
+struct dpmi_sea_model {
+  dpmi::SeaModel* impl;
+};
+struct dpmi_sea_engine {
+  dpmi::SeaEngine* impl;
+};

@@ -1,0 +1,121 @@
+// Environment matrix of se_e2_a (gfx950): one thread per (centre, slot).
+//
+// For slot s of centre i with neighbour j = nlist[i, s]:
+//   r = x_j - x_i,  value = s(|r|) * [1/|r|, r_x/|r|^2, r_y/|r|^2, r_z/|r|^2]
+//   deriv[a][d] = d(value_a)/d(x_i)_d            (reference env_mat.cc:124-222)
+//   em = (value - avg[t_i]) / std[t_i], em_deriv = deriv / std[t_i]
+//   padding slots: em = -avg/std, deriv = 0, rij = 0; centre type < 0: zeros
+//   (prod_env_mat.cc:107-123).
+// Consecutive threads own consecutive slots, so the 4-, 12- and 3-wide rows
+// are written as contiguous, coalesced spans.
+#include "dpmi_common.h"
+#include "kernels.h"
+
+namespace dpmi {
+
+namespace {
+
+template <typename T>
+__device__ __forceinline__ void smooth_switch(T& v, T& dv, T r, T rmin, T rmax) {
+  // switcher.h:61-80
+  if (r < rmin) {
+    v = 1;
+    dv = 0;
+  } else if (r < rmax) {
+    const T u = (r - rmin) / (rmax - rmin);
+    const T du = (T)1 / (rmax - rmin);
+    v = u * u * u * ((T)-6 * u * u + (T)15 * u - (T)10) + (T)1;
+    dv = ((T)3 * u * u * ((T)-6 * u * u + (T)15 * u - (T)10) + u * u * u * ((T)-12 * u + (T)15)) * du;
+  } else {
+    v = 0;
+    dv = 0;
+  }
+}
+
+__global__ void k_env_mat_a(const double* __restrict__ coord, const int* __restrict__ type,
+                            const int* __restrict__ nlist, const int* __restrict__ row_frame,
+                            const int* __restrict__ row_ext_off, const int* __restrict__ row_center,
+                            int strided_nloc, long long nrows, int nnei, int nall_stride, const double* __restrict__ avg,
+                            const double* __restrict__ stdv, double rmin, double rmax, double* __restrict__ em,
+                            double* __restrict__ em_deriv, double* __restrict__ rij) {
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= nrows * nnei) return;
+  const long long g = tid / nnei;
+  const int s = (int)(tid - g * nnei);
+  // row -> (frame extended offset, centre index in the frame)
+  size_t eo;
+  int i;
+  if (row_ext_off) {
+    eo = (size_t)row_ext_off[row_frame[g]];
+    i = row_center[g];
+  } else {
+    eo = (size_t)(g / strided_nloc) * nall_stride;
+    i = (int)(g % strided_nloc);
+  }
+  const int ti = type[eo + i];
+  const int j = nlist[g * nnei + s];
+  double val[4] = {0., 0., 0., 0.};
+  double der[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) der[k] = 0.;
+  double rr[3] = {0., 0., 0.};
+  if (ti >= 0 && j >= 0) {
+    for (int d = 0; d < 3; ++d) rr[d] = coord[3 * (eo + j) + d] - coord[3 * (eo + i) + d];
+    const double nr2 = rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2];
+    const double inr = 1. / sqrt(nr2);
+    const double nr = nr2 * inr;
+    const double inr2 = inr * inr;
+    const double inr4 = inr2 * inr2;
+    const double inr3 = inr4 * nr;
+    double sw, dsw;
+    smooth_switch(sw, dsw, nr, rmin, rmax);
+    val[0] = 1. / nr;
+    val[1] = rr[0] / nr2;
+    val[2] = rr[1] / nr2;
+    val[3] = rr[2] / nr2;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) der[d] = rr[d] * inr3 * sw - val[0] * dsw * rr[d] * inr;
+#pragma unroll
+    for (int a = 1; a < 4; ++a) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double t = 2. * rr[a - 1] * rr[d] * inr4;
+        if (d == a - 1) t -= inr2;
+        der[a * 3 + d] = t * sw - val[a] * dsw * rr[d] * inr;
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) val[a] *= sw;
+  }
+  const size_t base = (size_t)g * nnei + s;
+  if (ti >= 0) {
+    const double* av = avg + (size_t)ti * nnei * 4 + s * 4;
+    const double* sd = stdv + (size_t)ti * nnei * 4 + s * 4;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      em[base * 4 + a] = (val[a] - av[a]) / sd[a];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) em_deriv[base * 12 + a * 3 + d] = der[a * 3 + d] / sd[a];
+    }
+  } else {
+#pragma unroll
+    for (int a = 0; a < 4; ++a) em[base * 4 + a] = 0.;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) em_deriv[base * 12 + k] = 0.;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) rij[base * 3 + d] = rr[d];
+}
+
+}  // namespace
+
+void launch_env_mat_a(const EnvArgs& a, hipStream_t st) {
+  const long long n = a.nrows * (long long)a.nnei;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_env_mat_a, dim3(ceil_div(n, 256)), dim3(256), 0, st, a.coord, a.type, a.nlist, a.row_frame,
+                     a.row_ext_off, a.row_center, a.strided_nloc, a.nrows, a.nnei, a.nall_stride, a.avg, a.std_, a.rmin, a.rmax,
+                     a.em, a.em_deriv, a.rij);
+  DPMI_LAUNCH_CHECK();
+}
+
+}  // namespace dpmi

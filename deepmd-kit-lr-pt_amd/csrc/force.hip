@@ -1,0 +1,343 @@
+// Forces and virials from net_deriv = dE/d(em) (gfx950), and their
+// adjoints w.r.t. net_deriv.
+//
+// prod_force_a (prod_force.cc:24-87): with f_s = sum_a nd[s,a] ed[s,a,:],
+//   F_i -= sum_s f_s, F_j(s) += f_s.
+// prod_virial_a (prod_virial.cc:23-69): with v_s[p][q] = sum_a nd[s,a]
+//   ed[s,a,p] rij[s,q], virial += sum v_s and atom_virial[j(s)] += v_s.
+// One wavefront per centre row, lanes over neighbour slots (4-, 12- and
+// 3-wide rows read as contiguous spans); the scatter to neighbours uses FP64
+// atomics, per-row sums use a wave butterfly, and totals are reduced per row
+// into a partial buffer that a second kernel sums with compensation.
+#include "dpmi_common.h"
+#include "kernels.h"
+
+namespace dpmi {
+
+namespace {
+
+__device__ __forceinline__ void slot_force(const double* nd, const double* ed, double* f) {
+  f[0] = f[1] = f[2] = 0.;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) f[d] += nd[a] * ed[a * 3 + d];
+}
+
+__device__ __forceinline__ void slot_virial(const double* nd, const double* ed, const double* r, double* v) {
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      double s = 0.;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) s += nd[a] * ed[a * 3 + p];
+      v[p * 3 + q] = s * r[q];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_prod_force(double* __restrict__ force, const double* __restrict__ nd,
+                                                    const double* __restrict__ ed, const int* __restrict__ nlist,
+                                                    int nloc, int nall, int nnei, int nframes) {
+  const int g = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = lane_id();
+  if (g >= nframes * nloc) return;
+  const int f = g / nloc, i = g % nloc;
+  double fi[3] = {0., 0., 0.};
+  for (int s = lane; s < nnei; s += kWave) {
+    const size_t b = (size_t)g * nnei + s;
+    double n4[4], e12[12], fs[3];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) n4[a] = nd[b * 4 + a];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) e12[k] = ed[b * 12 + k];
+    slot_force(n4, e12, fs);
+    fi[0] -= fs[0];
+    fi[1] -= fs[1];
+    fi[2] -= fs[2];
+    const int j = nlist[b];
+    if (j >= 0) {
+      double* fj = force + ((size_t)f * nall + j) * 3;
+      atomicAdd(fj + 0, fs[0]);
+      atomicAdd(fj + 1, fs[1]);
+      atomicAdd(fj + 2, fs[2]);
+    }
+  }
+  for (int d = 0; d < 3; ++d) fi[d] = wave_sum(fi[d]);
+  if (lane == 0) {
+    double* fo = force + ((size_t)f * nall + i) * 3;
+    atomicAdd(fo + 0, fi[0]);
+    atomicAdd(fo + 1, fi[1]);
+    atomicAdd(fo + 2, fi[2]);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_prod_virial(double* __restrict__ vpart, double* __restrict__ atom_virial,
+                                                     const double* __restrict__ nd, const double* __restrict__ ed,
+                                                     const double* __restrict__ rij, const int* __restrict__ nlist,
+                                                     int nloc, int nnei) {
+  const int i = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = lane_id();
+  if (i >= nloc) return;
+  double vs[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) vs[k] = 0.;
+  for (int s = lane; s < nnei; s += kWave) {
+    const size_t b = (size_t)i * nnei + s;
+    const int j = nlist[b];
+    if (j < 0) continue;
+    double n4[4], e12[12], r3[3], v[9];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) n4[a] = nd[b * 4 + a];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) e12[k] = ed[b * 12 + k];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) r3[d] = rij[b * 3 + d];
+    slot_virial(n4, e12, r3, v);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      vs[k] += v[k];
+      if (atom_virial) atomicAdd(atom_virial + (size_t)j * 9 + k, v[k]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) vs[k] = wave_sum(vs[k]);
+  if (lane < 9) {
+    double out = 0.;
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+      if (lane == k) out = vs[k];
+    vpart[(size_t)i * 9 + lane] = out;
+  }
+}
+
+// Sum of rows [nrows, 9] -> out9 (one workgroup, compensated).
+__global__ void __launch_bounds__(256) k_reduce_rows9(double* __restrict__ out, const double* __restrict__ part,
+                                                      int nrows) {
+  __shared__ double sh[256][9];
+  KahanD acc[9];
+  for (int r = threadIdx.x; r < nrows; r += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < 9; ++k) acc[k].add(part[(size_t)r * 9 + k]);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) sh[threadIdx.x][k] = acc[k].value();
+  __syncthreads();
+  if (threadIdx.x < 9) {
+    KahanD t;
+    for (int r = 0; r < (int)blockDim.x; ++r) t.add(sh[r][threadIdx.x]);
+    out[threadIdx.x] = t.value();
+  }
+}
+
+// prod_force_grad.cc:23-75 (ghost index folded with j % nloc)
+__global__ void k_prod_force_grad(double* __restrict__ gn, const double* __restrict__ grad,
+                                  const double* __restrict__ ed, const int* __restrict__ nlist, int nloc, int nnei,
+                                  int nframes) {
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= (long long)nframes * nloc * nnei) return;
+  const long long g = tid / nnei;
+  const int s = (int)(tid - g * nnei);
+  const int f = (int)(g / nloc);
+  int j = nlist[g * nnei + s];
+  if (j >= nloc) j = j % nloc;
+  const double* e = ed + ((size_t)g * nnei + s) * 12;
+  const double gi[3] = {grad[g * 3 + 0], grad[g * 3 + 1], grad[g * 3 + 2]};
+  double gj[3] = {0., 0., 0.};
+  if (j >= 0)
+    for (int d = 0; d < 3; ++d) gj[d] = grad[((size_t)f * nloc + j) * 3 + d];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    double v = 0.;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) v -= gi[d] * e[a * 3 + d];
+    if (j >= 0)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) v += gj[d] * e[a * 3 + d];
+    gn[((size_t)g * nnei + s) * 4 + a] = v;
+  }
+}
+
+// prod_virial_grad.cc:22-66
+__global__ void k_prod_virial_grad(double* __restrict__ gn, const double* __restrict__ grad,
+                                   const double* __restrict__ ed, const double* __restrict__ rij,
+                                   const int* __restrict__ nlist, int nloc, int nnei) {
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= (long long)nloc * nnei) return;
+  const int j = nlist[tid];
+  const double* e = ed + (size_t)tid * 12;
+  const double* r = rij + (size_t)tid * 3;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    double v = 0.;
+    if (j >= 0)
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) v += grad[p * 3 + q] * r[q] * e[a * 3 + p];
+    gn[(size_t)tid * 4 + a] = v;
+  }
+}
+
+// Engine: packed rows, extended force buffer per frame, virial partials.
+__global__ void __launch_bounds__(256) k_force_virial_rows(const double* __restrict__ nd, const double* __restrict__ ed,
+                                                           const double* __restrict__ rij, const int* __restrict__ nlist,
+                                                           const int* __restrict__ row_frame,
+                                                           const int* __restrict__ row_center,
+                                                           const int* __restrict__ frame_ext_off, long long nrows,
+                                                           int nnei, double* __restrict__ force_ext,
+                                                           double* __restrict__ vpart) {
+  const long long g = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = lane_id();
+  if (g >= nrows) return;
+  const size_t eo = (size_t)frame_ext_off[row_frame[g]];
+  const int i = row_center[g];
+  double fi[3] = {0., 0., 0.};
+  double vs[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) vs[k] = 0.;
+  for (int s = lane; s < nnei; s += kWave) {
+    const size_t b = (size_t)g * nnei + s;
+    const int j = nlist[b];
+    if (j < 0) continue;  // padding slots have zero em_deriv
+    double n4[4], e12[12], r3[3], fs[3], v[9];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) n4[a] = nd[b * 4 + a];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) e12[k] = ed[b * 12 + k];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) r3[d] = rij[b * 3 + d];
+    slot_force(n4, e12, fs);
+    slot_virial(n4, e12, r3, v);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      fi[d] -= fs[d];
+      atomicAdd(force_ext + (eo + j) * 3 + d, fs[d]);
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) vs[k] += v[k];
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) fi[d] = wave_sum(fi[d]);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) vs[k] = wave_sum(vs[k]);
+  if (lane < 3) {
+    double o = lane == 0 ? fi[0] : (lane == 1 ? fi[1] : fi[2]);
+    atomicAdd(force_ext + (eo + i) * 3 + lane, o);
+  }
+  if (lane < 9) {
+    double out = 0.;
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+      if (lane == k) out = vs[k];
+    vpart[(size_t)g * 9 + lane] = out;
+  }
+}
+
+__global__ void k_fold_ghosts(const double* __restrict__ force_ext, const int* __restrict__ map_ext,
+                              const int* __restrict__ xframe_of_ext, const FrameGeom* __restrict__ geom,
+                              const int* __restrict__ nall, long long next_tot, double* __restrict__ force) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= next_tot) return;
+  const int f = xframe_of_ext[e];
+  const FrameGeom& g = geom[f];
+  const int k = (int)(e - g.ext_off);
+  if (k >= nall[f] || k >= g.ext_cap) return;
+  const int owner = g.loc_off + map_ext[e];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) atomicAdd(force + (size_t)owner * 3 + d, force_ext[e * 3 + d]);
+}
+
+__global__ void __launch_bounds__(256) k_frame_reduce(const double* __restrict__ atom_e,
+                                                      const double* __restrict__ vpart,
+                                                      const FrameGeom* __restrict__ geom, double* __restrict__ energy,
+                                                      double* __restrict__ virial) {
+  __shared__ double sh[256][10];
+  const FrameGeom& g = geom[blockIdx.x];
+  KahanD acc[10];
+  for (int r = threadIdx.x; r < g.nloc; r += blockDim.x) {
+    const size_t row = (size_t)g.loc_off + r;
+    acc[0].add(atom_e[row]);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) acc[k + 1].add(vpart[row * 9 + k]);
+  }
+#pragma unroll
+  for (int k = 0; k < 10; ++k) sh[threadIdx.x][k] = acc[k].value();
+  __syncthreads();
+  if (threadIdx.x < 10) {
+    KahanD t;
+    for (int r = 0; r < (int)blockDim.x; ++r) t.add(sh[r][threadIdx.x]);
+    if (threadIdx.x == 0)
+      energy[blockIdx.x] = t.value();
+    else
+      virial[blockIdx.x * 9 + threadIdx.x - 1] = t.value();
+  }
+}
+
+}  // namespace
+
+void launch_prod_force_a(double* force, const double* net_deriv, const double* env_deriv, const int* nlist, int nloc,
+                         int nall, int nnei, int nframes, hipStream_t st) {
+  DPMI_CHECK(hipMemsetAsync(force, 0, sizeof(double) * (size_t)nframes * nall * 3, st));
+  const long long rows = (long long)nframes * nloc;
+  if (rows <= 0 || nnei <= 0) return;
+  hipLaunchKernelGGL(k_prod_force, dim3(ceil_div(rows * kWave, 256)), dim3(256), 0, st, force, net_deriv, env_deriv,
+                     nlist, nloc, nall, nnei, nframes);
+  DPMI_LAUNCH_CHECK();
+}
+
+void launch_prod_virial_a(double* vpart, double* atom_virial, const double* net_deriv, const double* env_deriv,
+                          const double* rij, const int* nlist, int nloc, int nall, int nnei, hipStream_t st) {
+  if (atom_virial) DPMI_CHECK(hipMemsetAsync(atom_virial, 0, sizeof(double) * (size_t)nall * 9, st));
+  if (nloc <= 0) return;
+  hipLaunchKernelGGL(k_prod_virial, dim3(ceil_div((long long)nloc * kWave, 256)), dim3(256), 0, st, vpart,
+                     atom_virial, net_deriv, env_deriv, rij, nlist, nloc, nnei);
+  DPMI_LAUNCH_CHECK();
+}
+
+void launch_reduce_rows9(double* out9, const double* part, int nrows, hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce_rows9, dim3(1), dim3(256), 0, st, out9, part, nrows);
+  DPMI_LAUNCH_CHECK();
+}
+
+void launch_prod_force_grad_a(double* grad_net, const double* grad, const double* env_deriv, const int* nlist,
+                              int nloc, int nnei, int nframes, hipStream_t st) {
+  const long long n = (long long)nframes * nloc * nnei;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_prod_force_grad, dim3(ceil_div(n, 256)), dim3(256), 0, st, grad_net, grad, env_deriv, nlist,
+                     nloc, nnei, nframes);
+  DPMI_LAUNCH_CHECK();
+}
+
+void launch_prod_virial_grad_a(double* grad_net, const double* grad, const double* env_deriv, const double* rij,
+                               const int* nlist, int nloc, int nnei, hipStream_t st) {
+  const long long n = (long long)nloc * nnei;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_prod_virial_grad, dim3(ceil_div(n, 256)), dim3(256), 0, st, grad_net, grad, env_deriv, rij,
+                     nlist, nloc, nnei);
+  DPMI_LAUNCH_CHECK();
+}
+
+void launch_force_virial_rows(const ForceRowsArgs& a, hipStream_t st) {
+  if (a.nrows <= 0) return;
+  hipLaunchKernelGGL(k_force_virial_rows, dim3(ceil_div(a.nrows * kWave, 256)), dim3(256), 0, st, a.net_deriv,
+                     a.em_deriv, a.rij, a.nlist, a.row_frame, a.row_center, a.frame_ext_off, a.nrows, a.nnei,
+                     a.force_ext, a.vpart);
+  DPMI_LAUNCH_CHECK();
+}
+
+void launch_fold_ghost_forces(const double* force_ext, const int* map_ext, const int* xframe_of_ext,
+                              const FrameGeom* geom, const int* nall, long long next_tot, double* force,
+                              hipStream_t st) {
+  if (next_tot <= 0) return;
+  hipLaunchKernelGGL(k_fold_ghosts, dim3(ceil_div(next_tot, 256)), dim3(256), 0, st, force_ext, map_ext,
+                     xframe_of_ext, geom, nall, next_tot, force);
+  DPMI_LAUNCH_CHECK();
+}
+
+void launch_frame_reduce(const double* atom_e, const double* vpart, const FrameGeom* geom, int nframes,
+                         double* energy, double* virial, hipStream_t st) {
+  hipLaunchKernelGGL(k_frame_reduce, dim3(nframes), dim3(256), 0, st, atom_e, vpart, geom, energy, virial);
+  DPMI_LAUNCH_CHECK();
+}
+
+}  // namespace dpmi

@@ -1,0 +1,81 @@
+// Periodic box, cell grid and per-frame layout shared by host and device.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+namespace dpmi {
+
+// One frame of a packed batch.  Local atoms of frame f are the packed rows
+// [loc_off, loc_off + nloc); its ghost-extended atoms occupy
+// [ext_off, ext_off + ext_cap) of the extended arrays (frame-relative
+// extended index 0..nall-1, locals first).  The cell grid has nc local cells
+// per dimension, ng ghost layers on each side and nx = nc + 2 ng extended
+// cells per dimension.
+struct FrameGeom {
+  double t[9];    // box, rows are cell vectors
+  double rec[9];  // reciprocal box (fractional = rec . phys)
+  int nc[3], ng[3], nx[3];
+  int nloc, loc_off;
+  int ext_off, ext_cap;
+  int cell_off;   // offset of the frame's local cells in packed cell arrays
+  int xcell_off;  // offset of the frame's extended cells (+ frame index for the scan sentinel)
+  int pad[2];
+};
+
+// Box inverse and cell grid with the same operation order as the reference
+// (region.cc:40-85, SimulationRegion_Impl.h:364-372, coord.cc:69-108,
+// neighbor_list.cc:758-766) so that fractional coordinates, cell indices and
+// image shifts agree bit for bit.
+inline void geom_from_box(FrameGeom& g, const double* boxt, float rcut) {
+#pragma clang fp contract(off)
+  std::memcpy(g.t, boxt, sizeof(double) * 9);
+  const double* m = g.t;
+  double vol = m[0] * (m[4] * m[8] - m[7] * m[5]) - m[1] * (m[3] * m[8] - m[6] * m[5]) +
+               m[2] * (m[3] * m[7] - m[6] * m[4]);
+  vol = std::fabs(vol);
+  const double vi = 1.0 / vol;
+  double* r = g.rec;
+  r[0] = (m[4] * m[8] - m[7] * m[5]) * vi;
+  r[4] = (m[0] * m[8] - m[6] * m[2]) * vi;
+  r[8] = (m[0] * m[4] - m[3] * m[1]) * vi;
+  r[1] = (-m[3] * m[8] + m[6] * m[5]) * vi;
+  r[2] = (m[3] * m[7] - m[6] * m[4]) * vi;
+  r[3] = (-m[1] * m[8] + m[7] * m[2]) * vi;
+  r[5] = (-m[0] * m[7] + m[6] * m[1]) * vi;
+  r[6] = (m[1] * m[5] - m[4] * m[2]) * vi;
+  r[7] = (-m[0] * m[5] + m[3] * m[2]) * vi;
+  const double* a[3] = {m, m + 3, m + 6};
+  double tf[3];
+  for (int d = 0; d < 3; ++d) {
+    const double* u = a[(d + 1) % 3];
+    const double* v = a[(d + 2) % 3];
+    const double c0 = u[1] * v[2] - u[2] * v[1];
+    const double c1 = u[2] * v[0] - u[0] * v[2];
+    const double c2 = u[0] * v[1] - u[1] * v[0];
+    tf[d] = vol * (1. / std::sqrt(c0 * c0 + c1 * c1 + c2 * c2));
+  }
+  const double rc = rcut;
+  for (int d = 0; d < 3; ++d) {
+    int n = (int)(tf[d] / rc);
+    if (n == 0) n = 1;
+    const double cs = tf[d] / n;
+    g.nc[d] = n;
+    g.ng[d] = int(rc / cs) + 1;
+    g.nx[d] = n + 2 * g.ng[d];
+  }
+}
+
+inline int geom_local_cells(const FrameGeom& g) { return g.nc[0] * g.nc[1] * g.nc[2]; }
+inline int geom_ext_cells(const FrameGeom& g) { return g.nx[0] * g.nx[1] * g.nx[2]; }
+
+// Upper bound of nall for a frame: every extended cell holds at most the
+// densest local cell; callers size with the average density and a margin.
+inline int geom_nall_estimate(const FrameGeom& g, int nloc, double margin) {
+  const double ratio = (double)geom_ext_cells(g) / (double)geom_local_cells(g);
+  return (int)std::ceil(nloc * ratio * margin) + 64;
+}
+
+}  // namespace dpmi

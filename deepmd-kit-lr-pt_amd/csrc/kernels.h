@@ -1,0 +1,181 @@
+// Internal launch interface between the host code and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "geometry.h"
+
+namespace dpmi {
+
+// ------------------------------------------------------------- coord.hip --
+struct CellWork {
+  int nframes = 0;
+  int ntot = 0;          // packed local atoms of all frames
+  int ncells_tot = 0;    // packed local cells of all frames
+  int nx_tot = 0;        // packed extended cells (+1 sentinel per frame)
+  int do_wrap = 1;
+  const double* coord_in = nullptr;  // [ntot, 3]
+  const int* type_in = nullptr;      // [ntot]
+  const int* frame_of = nullptr;     // [ntot]
+  const int* cell_frame = nullptr;   // [ncells_tot]
+  const int* xcell_frame = nullptr;  // [nx_tot]
+  const FrameGeom* geom = nullptr;   // [nframes]
+  double* coord_ext = nullptr;       // [sum ext_cap, 3]
+  int* type_ext = nullptr;
+  int* map_ext = nullptr;
+  int* cell_atoms = nullptr;         // [sum ext_cap]
+  int* cell_of = nullptr;            // [ntot]
+  int* cell_cnt = nullptr;           // [ncells_tot]
+  int* cell_start = nullptr;
+  int* cell_fill = nullptr;
+  int* sorted_loc = nullptr;         // [ntot]
+  int* xcnt = nullptr;               // [nx_tot]
+  int* xstart = nullptr;
+  int* xgcnt = nullptr;
+  int* xgstart = nullptr;
+  int* nall = nullptr;               // [nframes]
+  int* flag = nullptr;               // device status word
+};
+void launch_build_extended(const CellWork& w, hipStream_t st);
+// one frame, caller buffers; returns a dpmi_status (capi.cpp)
+int copy_coord_run(FrameGeom g, double* out_c, int* out_t, int* mapping, int* nall_host, const double* in_c,
+                   const int* in_t, int nloc, int mem_nall, void* stream);
+void launch_normalize(double* coord, int natom, const FrameGeom& g, hipStream_t st);
+
+// ------------------------------------------------------------- nlist.hip --
+struct NlistArgs {
+  int cap = 256;        // LDS row capacity (power of two, 128..4096)
+  const int* sec = nullptr;  // [ntypes + 1] device
+  int ntypes = 0;
+  int nnei = 0;
+  double rc2 = 0;
+  float rc2f = 0;
+  int* nlist = nullptr;
+  int* ncand = nullptr;
+};
+int nlist_row_capacity(int want);
+void launch_cell_nlist(const CellWork& w, const NlistArgs& a, hipStream_t st);
+// rows: jlist + r * jstride, or firstneigh[r] (device pointer array) if given
+void launch_row_format(int inum, const int* ilist, const int* numneigh, const int* jlist, int jstride,
+                       const double* coord, const int* type, int nloc, int nall, const NlistArgs& a, int* flag,
+                       hipStream_t st, const int* const* firstneigh = nullptr);
+void launch_brute_nlist(const double* c, int nloc, int nall, int mem_size, double rc2, int nframes, const int* type,
+                        int* ilist, int* numneigh, int* jlist, hipStream_t st, int* const* firstneigh = nullptr);
+
+// --------------------------------------------------------------- env.hip --
+struct EnvArgs {
+  const double* coord = nullptr;
+  const int* type = nullptr;
+  const int* nlist = nullptr;
+  // packed rows: row g -> frame row_frame[g], extended offset
+  // row_ext_off[frame], centre row_center[g]; when row_ext_off is null the
+  // rows are strided frames of strided_nloc rows over nall_stride atoms.
+  const int* row_frame = nullptr;
+  const int* row_ext_off = nullptr;
+  const int* row_center = nullptr;
+  int strided_nloc = 0;
+  long long nrows = 0;
+  int nnei = 0;
+  int nall_stride = 0;
+  const double* avg = nullptr;
+  const double* std_ = nullptr;
+  double rmin = 0, rmax = 0;
+  double* em = nullptr;
+  double* em_deriv = nullptr;
+  double* rij = nullptr;
+};
+void launch_env_mat_a(const EnvArgs& a, hipStream_t st);
+
+// ---------------------------------------------------------- tabulate.hip --
+template <typename T>
+int launch_tab_fwd(T* out, const T* table, const T* info_host, const T* em_x, const T* em, int nloc, int nnei,
+                   int M, int sorted, int nd, hipStream_t st);
+template <typename T>
+int launch_tab_grad(T* dy_dem_x, T* dy_dem, const T* table, const T* info_host, const T* em_x, const T* em,
+                    const T* dy, int nloc, int nnei, int M, int sorted, int nd, hipStream_t st);
+template <typename T>
+int launch_tab_grad_grad(T* dz_dy, const T* table, const T* info_host, const T* em_x, const T* em, const T* dz_dx,
+                         const T* dz_dem, int nloc, int nnei, int M, int sorted, int nd, hipStream_t st);
+
+struct TabRowsArgs {
+  void* xyz = nullptr;        // fwd output [nrows, 4, M]
+  void* net_deriv = nullptr;  // grad output [nrows, nnei, 4]
+  const void* gxyz = nullptr; // grad input [nrows, 4, M]
+  const void* tables = nullptr;
+  const long long* tab_off = nullptr;  // [nnets] element offsets
+  const void* infos = nullptr;         // device Info<T>[nnets]
+  const int* net_of = nullptr;         // [ntypes * ntypes]: net of (ti, tj)
+  const void* em = nullptr;            // [nrows, nnei, 4]; em_x is column 0
+  const int* row_type = nullptr;       // [nrows]
+  const int* sec = nullptr;            // [ntypes + 1]
+  int ntypes = 0;
+  long long nrows = 0;
+  int nnei = 0;
+  int M = 0;
+};
+void* make_table_infos(const double* infos_host, int nnets, bool fp32);
+size_t table_info_bytes(bool fp32);
+void free_table_infos(void* p, bool fp32);
+void launch_tab_fwd_rows(const TabRowsArgs& a, hipStream_t st);
+void launch_tab_grad_rows(const TabRowsArgs& a, hipStream_t st);
+
+// ------------------------------------------------------------- force.hip --
+void launch_prod_force_a(double* force, const double* net_deriv, const double* env_deriv, const int* nlist,
+                         int nloc, int nall, int nnei, int nframes, hipStream_t st);
+// per-row virial partials [nloc, 9] (caller reduces) and atom_virial
+void launch_prod_virial_a(double* vpart, double* atom_virial, const double* net_deriv, const double* env_deriv,
+                          const double* rij, const int* nlist, int nloc, int nall, int nnei, hipStream_t st);
+void launch_reduce_rows9(double* out9, const double* part, int nrows, hipStream_t st);
+void launch_prod_force_grad_a(double* grad_net, const double* grad, const double* env_deriv, const int* nlist,
+                              int nloc, int nnei, int nframes, hipStream_t st);
+void launch_prod_virial_grad_a(double* grad_net, const double* grad, const double* env_deriv, const double* rij,
+                               const int* nlist, int nloc, int nnei, hipStream_t st);
+
+// engine: force/virial over packed rows into extended force buffers
+struct ForceRowsArgs {
+  const double* net_deriv = nullptr;  // [nrows, nnei*4]
+  const double* em_deriv = nullptr;   // [nrows, nnei*12]
+  const double* rij = nullptr;        // [nrows, nnei*3]
+  const int* nlist = nullptr;
+  const int* row_frame = nullptr;
+  const int* row_center = nullptr;
+  const int* frame_ext_off = nullptr;
+  long long nrows = 0;
+  int nnei = 0;
+  double* force_ext = nullptr;        // [sum ext_cap, 3] (zeroed by caller)
+  double* vpart = nullptr;            // [nrows, 9]
+};
+void launch_force_virial_rows(const ForceRowsArgs& a, hipStream_t st);
+// force[loc] += force_ext[e] for every extended atom e < nall of each frame
+void launch_fold_ghost_forces(const double* force_ext, const int* map_ext, const int* xframe_of_ext,
+                              const FrameGeom* geom, const int* nall, long long next_tot, double* force,
+                              hipStream_t st);
+// per-frame energy and virial (Neumaier-compensated): one workgroup per frame
+void launch_frame_reduce(const double* atom_e, const double* vpart, const FrameGeom* geom, int nframes,
+                         double* energy, double* virial, hipStream_t st);
+
+// ----------------------------------------------------------- fitting.hip --
+// D[pos[g]] = T^T T[:, :A] with T = xyz[g] / nnei, row length M*A
+void launch_descriptor(const double* xyz, const int* pos, long long nrows, int M, int A, double inv_nnei, double* D,
+                       hipStream_t st);
+// gxyz[g] = dT / nnei from G = dE/dD[pos[g]]
+void launch_descriptor_bwd(const double* xyz, const double* gD, const int* pos, long long nrows, int M, int A,
+                           double inv_nnei, double* gxyz, hipStream_t st);
+// y = act(z + b); x_next = y * idt + residual(x); stores y (for backward)
+void launch_fit_act(const double* z, const double* b, const double* idt, const double* x, int act, int res_mode,
+                    long long n, int n_in, int n_out, double* y, double* x_next, hipStream_t st);
+// g_z = g_out * idt * (1 - y^2)  (act) ; residual contribution added later
+void launch_fit_act_bwd(const double* g_out, const double* y, const double* idt, int act, long long n, int n_out,
+                        double* g_z, hipStream_t st);
+// g_in += residual(g_out)
+void launch_fit_res_bwd(const double* g_out, int res_mode, long long n, int n_in, int n_out, double* g_in,
+                        hipStream_t st);
+// atom_e[perm[p]] = z[p] + b + bias ; fills g_out rows with w_out
+void launch_fit_out(const double* z, double b_out, double bias, const int* perm, long long p0, long long n,
+                    double* atom_e, hipStream_t st);
+void launch_fill_rows(double* g, const double* w, long long n, int width, hipStream_t st);
+
+void launch_fill_int(int* p, int v, long long n, hipStream_t st);
+void launch_fill_double(double* p, double v, long long n, hipStream_t st);
+
+}  // namespace dpmi

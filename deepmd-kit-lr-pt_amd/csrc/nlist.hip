@@ -1,0 +1,328 @@
+// Neighbour search and formatting (gfx950): one wavefront per centre atom.
+//
+// The formatted row of centre i holds, per neighbour type t, the sel[t]
+// nearest candidates in the order of the reference CPU path
+// (fmt_nlist.cc:99-144): candidates are the j != i with FP64
+// |x_i - x_j|^2 < rcut^2 (neighbor_list.cc:905-913) and FP32
+// |(float)x_j - (float)x_i|^2 <= rcut^2 computed from rounded coordinates
+// (fmt_nlist.cc:114-128), ordered by (type, FP32 r^2, index).  Both
+// distance expressions are evaluated without contraction in the reference's
+// order, so rows are bit-identical to the reference's.
+//
+// Per wave: candidates are gathered (from the 27 -- or (2 ng + 1)^3 --
+// surrounding cells of the extended cell grid, or from a caller-given
+// candidate row) with ballot compaction into an LDS key array
+//   key = type << 56 | float_bits(r^2) << 24 | index      (index < 2^24)
+// which is bitonic-sorted in LDS; per-type first/end positions then map
+// slot s of type t to key[first[t] + s - sec[t]].
+#include "dpmi_common.h"
+#include "geometry.h"
+#include "kernels.h"
+
+namespace dpmi {
+
+namespace {
+
+constexpr int kMaxTypes = 128;
+
+__device__ __forceinline__ unsigned long long make_key(int t, float r2, int j) {
+  return ((unsigned long long)(unsigned)t << 56) | ((unsigned long long)__float_as_uint(r2) << 24) |
+         (unsigned long long)(unsigned)j;
+}
+
+// Appends the keys of the lanes with keep set; returns the new count.
+template <int CAP>
+__device__ __forceinline__ int wave_append(unsigned long long* keys, int n, bool keep, unsigned long long key,
+                                           int* flag) {
+  const unsigned long long mask = __ballot(keep);
+  const int pos = n + __popcll(mask & ((1ull << lane_id()) - 1ull));
+  if (keep) {
+    if (pos < CAP)
+      keys[pos] = key;
+    else
+      atomicOr(flag, kFlagNlistCapacity);
+  }
+  return n + __popcll(mask);
+}
+
+// FP64 acceptance then FP32 formatting test for candidate j of centre i.
+__device__ __forceinline__ bool accept(const double* ci, const double* cj, double rc2, float rc2f, float* fr2) {
+#pragma clang fp contract(off)
+  const double d0 = ci[0] - cj[0], d1 = ci[1] - cj[1], d2 = ci[2] - cj[2];
+  const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+  if (!(r2 < rc2)) return false;
+  const float f0 = (float)cj[0] - (float)ci[0];
+  const float f1 = (float)cj[1] - (float)ci[1];
+  const float f2 = (float)cj[2] - (float)ci[2];
+  const float q = f0 * f0 + f1 * f1 + f2 * f2;
+  *fr2 = q;
+  return q <= rc2f;
+}
+
+// Sorts keys[0, n) in LDS and writes the formatted row.
+template <int CAP>
+__device__ void sort_and_slot(unsigned long long* keys, int n, int* tfirst, int* tend, const int* sec, int ntypes,
+                              int nnei, int* __restrict__ row) {
+  const int lane = lane_id();
+  if (n > CAP) n = CAP;
+  int P = 64;
+  while (P < n) P <<= 1;
+  for (int t = n + lane; t < P; t += kWave) keys[t] = ~0ull;
+  for (int t = lane; t < ntypes; t += kWave) {
+    tfirst[t] = 0;
+    tend[t] = 0;
+  }
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = lane; t < P; t += kWave) {
+        const int u = t ^ j;
+        if (u > t) {
+          const unsigned long long a = keys[t], b = keys[u];
+          const bool up = (t & k) == 0;
+          if ((a > b) == up) {
+            keys[t] = b;
+            keys[u] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int t = lane; t < n; t += kWave) {
+    const int ty = (int)(keys[t] >> 56);
+    if (t == 0 || (int)(keys[t - 1] >> 56) != ty) tfirst[ty] = t;
+    if (t == n - 1 || (int)(keys[t + 1] >> 56) != ty) tend[ty] = t + 1;
+  }
+  __syncthreads();
+  for (int s = lane; s < nnei; s += kWave) {
+    int ty = 0;
+    while (ty + 1 < ntypes && s >= sec[ty + 1]) ++ty;
+    const int r = s - sec[ty];
+    const int cnt = tend[ty] - tfirst[ty];
+    row[s] = r < cnt ? (int)(keys[tfirst[ty] + r] & 0xFFFFFFull) : -1;
+  }
+}
+
+// Engine path: candidates from the extended cell grid.
+template <int CAP>
+__global__ void __launch_bounds__(64) k_cell_nlist(const int* __restrict__ frame_of, const FrameGeom* __restrict__ geom,
+                                                   int ntot, const double* __restrict__ coord_ext,
+                                                   const int* __restrict__ type_ext, const int* __restrict__ cell_of,
+                                                   const int* __restrict__ xstart, const int* __restrict__ cell_atoms,
+                                                   const int* __restrict__ sec_g, int ntypes, int nnei, double rc2,
+                                                   float rc2f, int* __restrict__ nlist, int* __restrict__ ncand,
+                                                   int* __restrict__ flag) {
+  __shared__ unsigned long long keys[CAP];
+  __shared__ int tfirst[kMaxTypes], tend[kMaxTypes], sec[kMaxTypes + 1];
+  __shared__ int cstart[kWave], cpre[kWave + 1];
+  const int gi = blockIdx.x;
+  const int lane = lane_id();
+  if (gi >= ntot) return;
+  for (int t = lane; t <= ntypes; t += kWave) sec[t] = sec_g[t];
+  const FrameGeom& g = geom[frame_of[gi]];
+  const int i = gi - g.loc_off;
+  const size_t eo = (size_t)g.ext_off;
+  const double ci[3] = {coord_ext[3 * (eo + i) + 0], coord_ext[3 * (eo + i) + 1], coord_ext[3 * (eo + i) + 2]};
+  const int ti = type_ext[eo + i];
+  const int c = cell_of[gi];
+  const int cz = c % g.nc[2], cy = (c / g.nc[2]) % g.nc[1], cx = c / (g.nc[2] * g.nc[1]);
+  const int r0 = g.ng[0], r1 = g.ng[1], r2 = g.ng[2];
+  const int w1 = 2 * r1 + 1, w2 = 2 * r2 + 1;
+  const int nsearch = (2 * r0 + 1) * w1 * w2;
+  int n = 0;
+  if (ti >= 0) {
+    for (int cb = 0; cb < nsearch; cb += kWave) {
+      const int k = cb + lane;
+      int cnt = 0, st = 0;
+      if (k < nsearch) {
+        const int dz = k % w2 - r2, dy = (k / w2) % w1 - r1, dx = k / (w2 * w1) - r0;
+        const int e = ((cx + g.ng[0] + dx) * g.nx[1] + (cy + g.ng[1] + dy)) * g.nx[2] + (cz + g.ng[2] + dz);
+        st = xstart[g.xcell_off + e];
+        cnt = xstart[g.xcell_off + e + 1] - st;
+      }
+      // inclusive scan of cnt over the wave
+      int inc = cnt;
+#pragma unroll
+      for (int off = 1; off < kWave; off <<= 1) {
+        const int v = __shfl_up(inc, off, kWave);
+        if (lane >= off) inc += v;
+      }
+      cstart[lane] = st;
+      cpre[lane + 1] = inc;
+      if (lane == 0) cpre[0] = 0;
+      const int total = __shfl(inc, kWave - 1, kWave);
+      __syncthreads();
+      for (int p0 = 0; p0 < total; p0 += kWave) {
+        const int p = p0 + lane;
+        bool keep = false;
+        unsigned long long key = 0;
+        if (p < total) {
+          int lo = 0, hi = kWave;  // find cell q with cpre[q] <= p < cpre[q+1]
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (cpre[mid] <= p)
+              lo = mid;
+            else
+              hi = mid;
+          }
+          // slots and indices beyond the frame's capacity exist only after a
+          // ghost-capacity overflow (already flagged): skip them
+          const int slot = cstart[lo] + (p - cpre[lo]);
+          const int j = slot < g.ext_cap ? cell_atoms[eo + slot] : -1;
+          const int tj = (j >= 0 && j < g.ext_cap) ? type_ext[eo + j] : -1;
+          if (j != i && tj >= 0) {
+            const double cj[3] = {coord_ext[3 * (eo + j) + 0], coord_ext[3 * (eo + j) + 1],
+                                  coord_ext[3 * (eo + j) + 2]};
+            float fr2;
+            if (accept(ci, cj, rc2, rc2f, &fr2)) {
+              keep = true;
+              if (j >= (1 << 24)) atomicOr(flag, kFlagIndexRange);
+              if (tj >= ntypes) atomicOr(flag, kFlagTypeRange);
+              key = make_key(tj, fr2, j);
+            }
+          }
+        }
+        n = wave_append<CAP>(keys, n, keep, key, flag);
+      }
+      __syncthreads();
+    }
+  }
+  if (ncand && lane == 0) ncand[gi] = n;
+  sort_and_slot<CAP>(keys, n, tfirst, tend, sec, ntypes, nnei, nlist + (size_t)gi * nnei);
+}
+
+// Drop-in path: candidates from caller rows (InputNlist), one frame stride.
+// Rows are jlist + r * jstride, or firstneigh[r] when firstneigh is given
+// (InputNlist with device row pointers).
+template <int CAP>
+__global__ void __launch_bounds__(64) k_row_format(int inum, const int* __restrict__ ilist,
+                                                   const int* __restrict__ numneigh, const int* __restrict__ jlist,
+                                                   const int* const* __restrict__ firstneigh, int jstride,
+                                                   const double* __restrict__ coord, const int* __restrict__ type,
+                                                   int nloc, int nall, const int* __restrict__ sec_g, int ntypes,
+                                                   int nnei, float rc2f, int* __restrict__ nlist,
+                                                   int* __restrict__ flag) {
+  __shared__ unsigned long long keys[CAP];
+  __shared__ int tfirst[kMaxTypes], tend[kMaxTypes], sec[kMaxTypes + 1];
+  const int r = blockIdx.x;
+  const int lane = lane_id();
+  if (r >= inum) return;
+  for (int t = lane; t <= ntypes; t += kWave) sec[t] = sec_g[t];
+  const int f = r / nloc;
+  const int i = ilist[r];
+  const double* cf = coord + (size_t)f * nall * 3;
+  const int* tf = type + (size_t)f * nall;
+  const double ci[3] = {cf[3 * i], cf[3 * i + 1], cf[3 * i + 2]};
+  const int cnt = numneigh[r];
+  const int* cand = firstneigh ? firstneigh[r] : jlist + (size_t)r * jstride;
+  __syncthreads();
+  int n = 0;
+  for (int p0 = 0; p0 < cnt; p0 += kWave) {
+    const int p = p0 + lane;
+    bool keep = false;
+    unsigned long long key = 0;
+    if (p < cnt) {
+      const int j = cand[p];
+      const int tj = tf[j];
+      if (tj >= 0) {
+#pragma clang fp contract(off)
+        const float f0 = (float)cf[3 * j + 0] - (float)ci[0];
+        const float f1 = (float)cf[3 * j + 1] - (float)ci[1];
+        const float f2 = (float)cf[3 * j + 2] - (float)ci[2];
+        const float q = f0 * f0 + f1 * f1 + f2 * f2;
+        if (q <= rc2f) {
+          keep = true;
+          if (j >= (1 << 24)) atomicOr(flag, kFlagIndexRange);
+          if (tj >= ntypes) atomicOr(flag, kFlagTypeRange);
+          key = make_key(tj, q, j);
+        }
+      }
+    }
+    n = wave_append<CAP>(keys, n, keep, key, flag);
+  }
+  __syncthreads();
+  sort_and_slot<CAP>(keys, n, tfirst, tend, sec, ntypes, nnei, nlist + ((size_t)f * nloc + i) * nnei);
+}
+
+// Brute-force candidate rows in ascending j (the build_nlist contract):
+// one wave per row, lanes over j, ballot compaction keeps the order.
+__global__ void __launch_bounds__(256) k_brute_nlist(const double* __restrict__ c, int nloc, int nall, int mem_size,
+                                                     double rc2, int nframes, const int* __restrict__ type,
+                                                     int* __restrict__ ilist, int* __restrict__ numneigh,
+                                                     int* __restrict__ jlist, int* const* __restrict__ firstneigh) {
+#pragma clang fp contract(off)
+  const int r = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = lane_id();
+  if (r >= nframes * nloc) return;
+  const int i = r % nloc, f = r / nloc;
+  const double* cf = c + (size_t)f * nall * 3;
+  const int* tf = type ? type + (size_t)f * nall : nullptr;
+  const double ci[3] = {cf[3 * i], cf[3 * i + 1], cf[3 * i + 2]};
+  const bool ci_ok = !tf || tf[i] >= 0;
+  int n = 0;
+  int* out = firstneigh ? firstneigh[r] : jlist + (size_t)r * mem_size;
+  for (int j0 = 0; j0 < nall; j0 += kWave) {
+    const int j = j0 + lane;
+    bool keep = false;
+    if (j < nall && j != i && ci_ok && (!tf || tf[j] >= 0)) {
+      const double d0 = ci[0] - cf[3 * j], d1 = ci[1] - cf[3 * j + 1], d2 = ci[2] - cf[3 * j + 2];
+      keep = d0 * d0 + d1 * d1 + d2 * d2 < rc2;
+    }
+    const unsigned long long m = __ballot(keep);
+    const int pos = n + __popcll(m & ((1ull << lane) - 1ull));
+    if (keep && pos < mem_size) out[pos] = j;
+    n += __popcll(m);
+  }
+  if (lane == 0) {
+    ilist[r] = i;
+    numneigh[r] = n;  // may exceed mem_size: the host reports the capacity error
+  }
+}
+
+}  // namespace
+
+#define DPMI_CAP_DISPATCH(CAPV, ...)                 \
+  switch (CAPV) {                                    \
+    case 128: { constexpr int CAP = 128; __VA_ARGS__; } break; \
+    case 256: { constexpr int CAP = 256; __VA_ARGS__; } break; \
+    case 512: { constexpr int CAP = 512; __VA_ARGS__; } break; \
+    case 1024: { constexpr int CAP = 1024; __VA_ARGS__; } break; \
+    case 2048: { constexpr int CAP = 2048; __VA_ARGS__; } break; \
+    default: { constexpr int CAP = 4096; __VA_ARGS__; } break; \
+  }
+
+int nlist_row_capacity(int want) {
+  int c = 128;
+  while (c < want && c < 4096) c <<= 1;
+  return c;
+}
+
+void launch_cell_nlist(const CellWork& w, const NlistArgs& a, hipStream_t st) {
+  DPMI_CAP_DISPATCH(a.cap, hipLaunchKernelGGL(k_cell_nlist<CAP>, dim3(w.ntot), dim3(kWave), 0, st, w.frame_of,
+                                              w.geom, w.ntot, w.coord_ext, w.type_ext, w.cell_of, w.xstart,
+                                              w.cell_atoms, a.sec, a.ntypes, a.nnei, a.rc2, a.rc2f, a.nlist,
+                                              a.ncand, w.flag));
+  DPMI_LAUNCH_CHECK();
+}
+
+void launch_row_format(int inum, const int* ilist, const int* numneigh, const int* jlist, int jstride,
+                       const double* coord, const int* type, int nloc, int nall, const NlistArgs& a, int* flag,
+                       hipStream_t st, const int* const* firstneigh) {
+  if (inum <= 0) return;
+  DPMI_CAP_DISPATCH(a.cap, hipLaunchKernelGGL(k_row_format<CAP>, dim3(inum), dim3(kWave), 0, st, inum, ilist,
+                                              numneigh, jlist, firstneigh, jstride, coord, type, nloc, nall, a.sec,
+                                              a.ntypes, a.nnei, a.rc2f, a.nlist, flag));
+  DPMI_LAUNCH_CHECK();
+}
+
+void launch_brute_nlist(const double* c, int nloc, int nall, int mem_size, double rc2, int nframes, const int* type,
+                        int* ilist, int* numneigh, int* jlist, hipStream_t st, int* const* firstneigh) {
+  const long long rows = (long long)nframes * nloc;
+  if (rows <= 0) return;
+  hipLaunchKernelGGL(k_brute_nlist, dim3(ceil_div(rows * kWave, 256)), dim3(256), 0, st, c, nloc, nall, mem_size,
+                     rc2, nframes, type, ilist, numneigh, jlist, firstneigh);
+  DPMI_LAUNCH_CHECK();
+}
+
+}  // namespace dpmi

@@ -1,0 +1,146 @@
+"""Python handle of the whole-path engine (dpmi_sea_* in include/dpmi.h)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from .lib import DPMI_FP64, DPMI_MIXED, ModelDesc, check, load
+
+FETCH = {"nall": (0, np.int32), "coord_ext": (1, np.float64), "atype_ext": (2, np.int32),
+         "mapping": (3, np.int32), "nlist": (4, np.int32), "em": (5, np.float64), "descriptor": (6, np.float64),
+         "net_deriv": (7, np.float64), "atom_energy": (8, np.float64)}
+
+
+def _arr(a, dtype):
+    return np.ascontiguousarray(np.asarray(a, dtype=dtype))
+
+
+class DeviceModel:
+    """An SeAModel (dpmi.model) uploaded to the GPU."""
+
+    def __init__(self, model):
+        if model.tables is None:
+            raise ValueError("the model has no compressed table: call build_tables() first")
+        keep = []  # numpy buffers referenced by the descriptor until creation returns
+
+        def ptr(a, ct):
+            a = _arr(a, np.float64 if ct is C.c_double else np.int32)
+            keep.append(a)
+            return a.ctypes.data_as(C.POINTER(ct))
+
+        def ptr_array(arrs):
+            ps = (C.POINTER(C.c_double) * len(arrs))()
+            for k, a in enumerate(arrs):
+                ps[k] = C.POINTER(C.c_double)() if a is None else ptr(a, C.c_double)
+            keep.append(ps)
+            return C.cast(ps, C.POINTER(C.POINTER(C.c_double)))
+
+        d = ModelDesc()
+        nt = model.ntypes
+        d.ntypes = nt
+        d.sel = ptr(model.sel, C.c_int)
+        d.rcut = model.rcut
+        d.rcut_smth = model.rcut_smth
+        d.last_layer_size = model.M
+        d.axis_neuron = model.axis_neuron
+        d.type_one_side = int(model.type_one_side)
+        d.davg = ptr(model.davg.reshape(nt, -1), C.c_double)
+        d.dstd = ptr(model.dstd.reshape(nt, -1), C.c_double)
+        d.nnets = model.nnets
+        d.nspline = ptr([t.shape[0] for t in model.tables], C.c_int)
+        d.tables = ptr_array(model.tables)
+        d.table_info = ptr(model.table_info, C.c_double)
+        fa = model.fitting_arrays()
+        L = len(fa)
+        d.fit_nlayers = L
+        d.fit_width = ptr([model.dim_descrpt] + [W.shape[2] for (W, *_r) in fa], C.c_int)
+        ws, bs, idts = [], [], []
+        for t in range(nt):
+            for (W, b, idt, act, res) in fa:
+                ws.append(W[t])
+                bs.append(b[t])
+                idts.append(None if idt is None else idt[t])
+        d.fit_w = ptr_array(ws)
+        d.fit_b = ptr_array(bs)
+        d.fit_idt = ptr_array(idts)
+        d.fit_act = ptr([int(act) for (_W, _b, _i, act, _r) in fa], C.c_int)
+        d.fit_resnet = ptr([int(res) for (_W, _b, _i, _a, res) in fa], C.c_int)
+        d.bias_atom_e = ptr(model.bias_atom_e, C.c_double)
+        h = C.c_void_p()
+        check(load().dpmi_sea_model_create(C.byref(h), C.byref(d)))
+        self.handle = h
+        self.model = model
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            load().dpmi_sea_model_destroy(h)
+            self.handle = None
+
+
+class Engine:
+    """E+F+V of a fixed set of frames (types fixed, coordinates and boxes per
+    call).  ``natoms`` lists the atom count of each frame; ``atype`` holds the
+    types of all frames back to back."""
+
+    def __init__(self, dmodel: DeviceModel, natoms, atype, precision: str = "fp64",
+                 nall_per_atom: float = 0.0):
+        self.dmodel = dmodel
+        self.natoms = _arr(natoms, np.int32).reshape(-1)
+        self.atype = _arr(atype, np.int32).reshape(-1)
+        assert self.atype.size == int(self.natoms.sum())
+        self.nframes = len(self.natoms)
+        prec = {"fp64": DPMI_FP64, "mixed": DPMI_MIXED}[precision]
+        h = C.c_void_p()
+        check(load().dpmi_sea_engine_create(C.byref(h), dmodel.handle, self.nframes,
+                                            self.natoms.ctypes.data_as(C.c_void_p),
+                                            self.atype.ctypes.data_as(C.c_void_p), prec, float(nall_per_atom)))
+        self.handle = h
+        self.ntot = int(self.natoms.sum())
+        dev = torch.device("cuda")
+        self.energy = torch.empty(self.nframes, dtype=torch.float64, device=dev)
+        self.force = torch.empty((self.ntot, 3), dtype=torch.float64, device=dev)
+        self.virial = torch.empty((self.nframes, 9), dtype=torch.float64, device=dev)
+        self.atom_energy = torch.empty(self.ntot, dtype=torch.float64, device=dev)
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            load().dpmi_sea_engine_destroy(h)
+            self.handle = None
+
+    def compute(self, coord: torch.Tensor, box, atom_energy: bool = False):
+        """coord: [ntot, 3] float64 device tensor; box: [nframes, 9] host
+        array.  Returns (energy [nf], force [ntot, 3], virial [nf, 9]) views
+        of engine-owned buffers (overwritten by the next call)."""
+        assert coord.dtype == torch.float64 and coord.is_cuda and coord.is_contiguous()
+        b = _arr(box, np.float64).reshape(self.nframes, 9)
+        st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        check(load().dpmi_sea_engine_compute(self.handle, C.c_void_p(coord.data_ptr()), b.ctypes.data_as(C.c_void_p),
+                                             C.c_void_p(self.energy.data_ptr()), C.c_void_p(self.force.data_ptr()),
+                                             C.c_void_p(self.virial.data_ptr()),
+                                             C.c_void_p(self.atom_energy.data_ptr()) if atom_energy else None, st))
+        return self.energy, self.force, self.virial
+
+    def status(self) -> None:
+        check(load().dpmi_sea_engine_status(self.handle))
+
+    def fetch(self, what: str, frame: int = 0) -> np.ndarray:
+        code, dt = FETCH[what]
+        n = C.c_longlong(0)
+        check(load().dpmi_sea_engine_fetch(self.handle, frame, code, None, C.byref(n)))
+        out = np.zeros(max(n.value, 1), dtype=dt)
+        check(load().dpmi_sea_engine_fetch(self.handle, frame, code, out.ctypes.data_as(C.c_void_p), C.byref(n)))
+        return out[: n.value]
+
+    def set_timing(self, on: bool = True) -> None:
+        load().dpmi_sea_engine_set_timing(self.handle, int(on))
+
+    def timing(self) -> dict:
+        ms = np.zeros(32)
+        names = C.create_string_buffer(1024)
+        k = load().dpmi_sea_engine_timing(self.handle, ms.ctypes.data_as(C.c_void_p), 32, names, 1024)
+        labels = names.value.decode().split(";")
+        return {labels[i] if i < len(labels) else f"stage{i}": float(ms[i]) for i in range(max(k, 0))}

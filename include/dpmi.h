@@ -1,0 +1,208 @@
+/*
+ * dpmi -- MI355X (gfx950) se_e2_a energy/force/virial path, C ABI.
+ *
+ * Plain pointers and sizes only.  Every array argument is DEVICE memory
+ * unless its name ends in _host.  All entry points are asynchronous on the
+ * given HIP stream (NULL = the default stream) except where a host output
+ * (suffix _host) forces a synchronisation, and return a dpmi_status; the
+ * message of the last failure on the calling thread is dpmi_last_error().
+ *
+ * Two groups:
+ *   1. kernel-level entry points, one per reference lib function on the path
+ *      (reference: source/lib/include, the deepmd::..._gpu templates).
+ *      Layouts, units and conventions are exactly the reference's; the
+ *      deepmd:: C++ templates of include/deepmd_lib.h forward to these.
+ *   2. the whole-path engine (dpmi_sea_*): cell list -> neighbour list ->
+ *      environment matrix -> compressed embedding + G^T R contraction ->
+ *      fitting net -> net_deriv -> force/virial, for one or many frames,
+ *      in FP64 or in mixed precision.  This is what the PyTorch op layer,
+ *      the C++ DeepPot-style caller and bench.py drive.
+ */
+#ifndef DPMI_H
+#define DPMI_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum dpmi_status {
+  DPMI_OK = 0,
+  DPMI_ERR_ARGUMENT = 1,         /* invalid argument (std::invalid_argument in the reference) */
+  DPMI_ERR_HIP = 2,              /* HIP runtime error */
+  DPMI_ERR_NLIST_CAPACITY = 3,   /* a neighbour row exceeds its capacity (deepmd_exception_nlist_capacity) */
+  DPMI_ERR_MEMORY = 4,           /* nall > mem_nall or allocation failure (reference returns 1 / deepmd_exception_oom) */
+  DPMI_ERR_RANGE = 5,            /* index/type outside the sort-key range (reference traps) */
+  DPMI_ERR_UNSUPPORTED = 6       /* ndescrpt not in {4,9,16,25}, last_layer_size outside (0,1024] */
+} dpmi_status;
+
+/* Message of the last failure on this thread ("" if none). */
+const char* dpmi_last_error(void);
+/* Library build string (arch, precision flags). */
+const char* dpmi_version(void);
+
+/* ===================================================================== */
+/* 1. kernel-level entry points (reference source/lib/include)          */
+/* ===================================================================== */
+
+/* coord.h:50-54 normalize_coord_gpu<double>: wrap coord[natom*3] into the
+ * primary cell of the box given by boxt_host[9] (rows = cell vectors). */
+int dpmi_normalize_coord_f64(double* coord, int natom, const double* boxt_host, void* stream);
+
+/* coord.h:56-82 copy_coord_gpu<double>: ghost images of nloc normalised
+ * atoms inside an rcut shell.  Output order equals the reference CPU
+ * copy_coord (locals, then ghost cells in lexicographic order, each holding
+ * its source cell's atoms in ascending index).  *nall_host receives nall;
+ * returns DPMI_ERR_MEMORY (reference: return 1) when nall > mem_nall. */
+int dpmi_copy_coord_f64(double* out_c, int* out_t, int* mapping, int* nall_host, const double* in_c,
+                        const int* in_t, int nloc, int mem_nall, float rcut, const double* boxt_host,
+                        void* stream);
+
+/* neighbor_list.h:230-247 build_nlist_gpu<double>: for every row
+ * r = frame*nloc + i, all j != i of that frame with |x_i - x_j|^2 < rcut^2
+ * (and type >= 0 when type != NULL), ascending j, written to
+ * jlist[r*mem_size ...]; ilist[r] = i, numneigh[r] = count.
+ * *max_list_size_host receives the longest row; returns
+ * DPMI_ERR_NLIST_CAPACITY when a row exceeds mem_size (reference: 1). */
+int dpmi_build_nlist_f64(int* ilist, int* numneigh, int* jlist, int* max_list_size_host,
+                         const double* c_cpy, int nloc, int nall, int mem_size, float rcut, int nframes,
+                         const int* type, void* stream);
+
+/* fmt_nlist.h:23-35 format_nbor_list_gpu + prod_env_mat.h:82-117
+ * prod_env_mat_a_gpu<double>: formats the candidate rows (ilist/numneigh,
+ * row r at jlist + r*jstride) into nlist[nf*nloc, nnei] sorted by
+ * (type, float r^2, index) and truncated at sel, then writes
+ * em[nf*nloc, nnei*4], em_deriv[nf*nloc, nnei*12], rij[nf*nloc, nnei*3]
+ * normalised by avg/std[ntypes, nnei*4] of the centre type.
+ * sel_host[ntypes]. */
+int dpmi_prod_env_mat_a_f64(double* em, double* em_deriv, double* rij, int* nlist, const double* coord,
+                            const int* type, int inum, const int* ilist, const int* numneigh,
+                            const int* jlist, int jstride, const double* avg, const double* std_,
+                            int nloc, int nall, int nframes, float rcut, float rcut_smth,
+                            const int* sel_host, int ntypes, void* stream);
+
+/* tabulate.h:175-218 tabulate_fusion_se_a{,_grad,_grad_grad}_gpu<T>
+ * (two_embed == NULL: the se_a path).  table_info_host[6] is a host array
+ * (the reference reads it on the host).  ndescrpt in {4,9,16,25}. */
+int dpmi_tabulate_fusion_se_a_f64(double* out, const double* table, const double* table_info_host,
+                                  const double* em_x, const double* em, int nloc, int nnei,
+                                  int last_layer_size, int is_sorted, int ndescrpt, void* stream);
+int dpmi_tabulate_fusion_se_a_f32(float* out, const float* table, const float* table_info_host,
+                                  const float* em_x, const float* em, int nloc, int nnei,
+                                  int last_layer_size, int is_sorted, int ndescrpt, void* stream);
+int dpmi_tabulate_fusion_se_a_grad_f64(double* dy_dem_x, double* dy_dem, const double* table,
+                                       const double* table_info_host, const double* em_x, const double* em,
+                                       const double* dy, int nloc, int nnei, int last_layer_size,
+                                       int is_sorted, int ndescrpt, void* stream);
+int dpmi_tabulate_fusion_se_a_grad_f32(float* dy_dem_x, float* dy_dem, const float* table,
+                                       const float* table_info_host, const float* em_x, const float* em,
+                                       const float* dy, int nloc, int nnei, int last_layer_size,
+                                       int is_sorted, int ndescrpt, void* stream);
+int dpmi_tabulate_fusion_se_a_grad_grad_f64(double* dz_dy, const double* table, const double* table_info_host,
+                                            const double* em_x, const double* em, const double* dz_dy_dem_x,
+                                            const double* dz_dy_dem, int nloc, int nnei,
+                                            int last_layer_size, int is_sorted, int ndescrpt, void* stream);
+int dpmi_tabulate_fusion_se_a_grad_grad_f32(float* dz_dy, const float* table, const float* table_info_host,
+                                            const float* em_x, const float* em, const float* dz_dy_dem_x,
+                                            const float* dz_dy_dem, int nloc, int nnei,
+                                            int last_layer_size, int is_sorted, int ndescrpt, void* stream);
+
+/* prod_force.h:65-73 prod_force_a_gpu<double>: force[nf, nall, 3]. */
+int dpmi_prod_force_a_f64(double* force, const double* net_deriv, const double* in_deriv, const int* nlist,
+                          int nloc, int nall, int nnei, int nframes, void* stream);
+/* prod_virial.h:30-39 prod_virial_a_gpu<double> (one frame):
+ * virial[9], atom_virial[nall, 9]. */
+int dpmi_prod_virial_a_f64(double* virial, double* atom_virial, const double* net_deriv,
+                           const double* env_deriv, const double* rij, const int* nlist, int nloc, int nall,
+                           int nnei, void* stream);
+/* prod_force_grad.h:27-34 prod_force_grad_a_gpu<double>. */
+int dpmi_prod_force_grad_a_f64(double* grad_net, const double* grad, const double* env_deriv,
+                               const int* nlist, int nloc, int nnei, int nframes, void* stream);
+/* prod_virial_grad.h:27-34 prod_virial_grad_a_gpu<double> (one frame). */
+int dpmi_prod_virial_grad_a_f64(double* grad_net, const double* grad, const double* env_deriv,
+                                const double* rij, const int* nlist, int nloc, int nnei, void* stream);
+
+/* ===================================================================== */
+/* 2. whole-path engine                                                  */
+/* ===================================================================== */
+
+typedef struct dpmi_sea_model dpmi_sea_model;
+typedef struct dpmi_sea_engine dpmi_sea_engine;
+
+/* Model description (all host pointers, copied at creation).
+ * Embedding net e serves centre type ti / neighbour type tj with
+ * e = tj*ntypes + ti, or e = tj when type_one_side (reference
+ * deepmd/pt/model/descriptor/se_a.py:715-719). */
+typedef struct dpmi_sea_model_desc {
+  int ntypes;
+  const int* sel;            /* [ntypes] */
+  float rcut, rcut_smth;
+  int last_layer_size;       /* M */
+  int axis_neuron;           /* A */
+  int type_one_side;
+  const double* davg;        /* [ntypes, nnei*4] */
+  const double* dstd;        /* [ntypes, nnei*4] */
+  int nnets;                 /* ntypes or ntypes^2 */
+  const int* nspline;        /* [nnets] table rows */
+  const double* const* tables;     /* [nnets] -> [nspline, 6*M] */
+  const double* table_info;  /* [nnets, 6] */
+  int fit_nlayers;           /* hidden layers + output layer */
+  const int* fit_width;      /* [fit_nlayers + 1]: M*A, h0, ..., 1 */
+  const double* const* fit_w;   /* [ntypes*fit_nlayers] -> [n_in, n_out] row-major */
+  const double* const* fit_b;   /* [ntypes*fit_nlayers] -> [n_out] */
+  const double* const* fit_idt; /* [ntypes*fit_nlayers] -> [n_out] or NULL */
+  const int* fit_act;        /* [fit_nlayers] 1 = tanh, 0 = linear */
+  const int* fit_resnet;     /* [fit_nlayers] residual allowed (x or [x, x]) */
+  const double* bias_atom_e; /* [ntypes] */
+} dpmi_sea_model_desc;
+
+int dpmi_sea_model_create(dpmi_sea_model** model, const dpmi_sea_model_desc* desc);
+void dpmi_sea_model_destroy(dpmi_sea_model* model);
+
+/* precision */
+#define DPMI_FP64 0   /* every stage FP64 (the plain path) */
+#define DPMI_MIXED 1  /* FP32 table + FP16/BF16 MFMA contraction and fitting; FP64 image shift, s(r), E/V sums */
+
+/* Engine for frames with fixed atom types.  nframes frames, frame f holding
+ * natoms_host[f] atoms whose types are atype_host[offset_f ...] (frames
+ * packed back to back).  Buffers are sized here for mem_nall_per_atom
+ * ghost-extended atoms per local atom (0 = automatic). */
+int dpmi_sea_engine_create(dpmi_sea_engine** engine, const dpmi_sea_model* model, int nframes,
+                           const int* natoms_host, const int* atype_host, int precision,
+                           double mem_nall_per_atom);
+void dpmi_sea_engine_destroy(dpmi_sea_engine* engine);
+
+/* One E+F+V evaluation of every frame.  coord [sum natoms, 3] device,
+ * box_host [nframes, 9] host.  Outputs (device): energy [nframes],
+ * force [sum natoms, 3], virial [nframes, 9]; atom_energy
+ * [sum natoms] may be NULL.  Asynchronous: capacity errors detected on the
+ * device are reported by dpmi_sea_engine_status(). */
+int dpmi_sea_engine_compute(dpmi_sea_engine* engine, const double* coord, const double* box_host,
+                            double* energy, double* force, double* virial, double* atom_energy, void* stream);
+
+/* Synchronises the stream of the last compute and returns its device-side
+ * status (DPMI_OK, DPMI_ERR_NLIST_CAPACITY, DPMI_ERR_MEMORY, DPMI_ERR_RANGE). */
+int dpmi_sea_engine_status(dpmi_sea_engine* engine);
+
+/* Copies an intermediate of the last compute of frame `frame` to host memory
+ * (tests and diagnostics).  what: 0 nall (int, 1), 1 extended coord
+ * (double, nall*3), 2 extended type (int, nall), 3 mapping (int, nall),
+ * 4 nlist (int, nloc*nnei), 5 em (double, nloc*nnei*4), 6 descriptor
+ * (double, nloc*M*A), 7 net_deriv (double, nloc*nnei*4), 8 atom energy
+ * (double, nloc).  *count_host receives the element count; dst_host may be
+ * NULL to query it. */
+int dpmi_sea_engine_fetch(dpmi_sea_engine* engine, int frame, int what, void* dst_host, long long* count_host);
+
+/* Device-time breakdown of the last compute in milliseconds per stage,
+ * recorded when timing is enabled: stage names are written to names_host as
+ * a ';'-separated list.  Returns the number of stages. */
+int dpmi_sea_engine_set_timing(dpmi_sea_engine* engine, int enable);
+int dpmi_sea_engine_timing(dpmi_sea_engine* engine, double* ms_host, int max_stages, char* names_host,
+                           int names_len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DPMI_H */
