@@ -130,22 +130,38 @@ __global__ void k_scatter_cells(const int* __restrict__ frame_of, const FrameGeo
   sorted_loc[g.loc_off + cell_start[c] + slot] = gidx - g.loc_off;
 }
 
+// One wave per cell: ascending atom index inside the cell (bitonic network
+// over the 64 lanes; cells holding more than 64 atoms fall back to an
+// insertion sort by lane 0).
 __global__ void k_sort_cells(const int* __restrict__ cell_frame, const FrameGeom* __restrict__ geom, int ncells_tot,
                              const int* __restrict__ cell_cnt, const int* __restrict__ cell_start,
                              int* __restrict__ sorted_loc) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
   if (c >= ncells_tot) return;
   const FrameGeom& g = geom[cell_frame[c]];
   int* a = sorted_loc + g.loc_off + cell_start[c];
   const int n = cell_cnt[c];
-  for (int k = 1; k < n; ++k) {
-    const int v = a[k];
-    int m = k - 1;
-    while (m >= 0 && a[m] > v) {
-      a[m + 1] = a[m];
-      --m;
+  if (n <= kWave) {
+    int v = lane < n ? a[lane] : 0x7fffffff;
+    for (int k = 2; k <= kWave; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        const int p = __shfl_xor(v, j, kWave);
+        const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+        v = keep_min ? min(v, p) : max(v, p);
+      }
     }
-    a[m + 1] = v;
+    if (lane < n) a[lane] = v;
+  } else if (lane == 0) {
+    for (int k = 1; k < n; ++k) {
+      const int v = a[k];
+      int m = k - 1;
+      while (m >= 0 && a[m] > v) {
+        a[m + 1] = a[m];
+        --m;
+      }
+      a[m + 1] = v;
+    }
   }
 }
 
@@ -250,7 +266,8 @@ void launch_build_extended(const CellWork& w, hipStream_t st) {
   hipLaunchKernelGGL(k_scatter_cells, dim3(ceil_div(w.ntot, tpb)), dim3(tpb), 0, st, w.frame_of, w.geom, w.ntot,
                      w.cell_of, w.cell_start, w.cell_fill, w.sorted_loc);
   DPMI_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_sort_cells, dim3(ceil_div(w.ncells_tot, tpb)), dim3(tpb), 0, st, w.cell_frame, w.geom,
+  hipLaunchKernelGGL(k_sort_cells, dim3(ceil_div((long long)w.ncells_tot * kWave, tpb)), dim3(tpb), 0, st,
+                     w.cell_frame, w.geom,
                      w.ncells_tot, w.cell_cnt, w.cell_start, w.sorted_loc);
   DPMI_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_scan_xcells, dim3(w.nframes), dim3(1024), 0, st, w.geom, w.cell_cnt, w.xcnt, w.xstart, w.xgcnt,

@@ -64,6 +64,8 @@ SeaModel::SeaModel(const dpmi_sea_model_desc& d) {
   M = d.last_layer_size;
   A = d.axis_neuron;
   if (A > M) throw std::invalid_argument("axis_neuron must not exceed the last embedding width");
+  if (A > 16) throw std::invalid_argument("axis_neuron above 16 is not supported by the fused descriptor kernels");
+  if (M > 1024) throw std::invalid_argument("last_layer_size must be at most 1024");
   type_one_side = d.type_one_side != 0;
   nnets = d.nnets;
   if (nnets != (type_one_side ? ntypes : ntypes * ntypes)) throw std::invalid_argument("nnets mismatch");
@@ -202,6 +204,14 @@ SeaEngine::SeaEngine(const SeaModel* m, int nframes_, const int* natoms_h, const
   DPMI_CHECK(hipMemset(d_gD, 0, sizeof(double) * (size_t)ntot * D));
   d_atom_e = dalloc<double>(ntot);
   d_vpart = dalloc<double>((size_t)ntot * 9);
+  {
+    std::vector<int> cxy, cst;
+    frame_chunks(natoms.data(), nframes, cxy, cst);
+    nchunks = (int)cxy.size() / 2;
+    d_chunks = dupload(cxy.data(), cxy.size());
+    d_chunk_start = dupload(cst.data(), cst.size());
+    d_part = dalloc<double>((size_t)std::max(nchunks, 1) * 10);
+  }
   for (int l = 0; l < m->nlayers; ++l) {
     d_fz.push_back(dalloc<double>((size_t)ntot * m->width[l + 1]));
     d_fy.push_back(dalloc<double>((size_t)ntot * m->width[l + 1]));
@@ -225,10 +235,10 @@ SeaEngine::~SeaEngine() {
   for (auto* p : {&d_frame_of, &d_center, &d_type, &d_pos, &d_perm, &d_flag, &d_nall, &d_frame_ext_off, &d_nlist,
                   &d_ncand, &d_cell_of, &d_sorted_loc, &d_cell_frame, &d_xcell_frame, &d_cell_cnt, &d_cell_start,
                   &d_cell_fill, &d_xcnt, &d_xstart, &d_xgcnt, &d_xgstart, &d_type_ext, &d_map_ext, &d_cell_atoms,
-                  &d_xframe_of_ext})
+                  &d_xframe_of_ext, &d_chunks, &d_chunk_start})
     dfree(*p);
   for (auto* p : {&d_em, &d_em_deriv, &d_rij, &d_net_deriv, &d_xyz, &d_gxyz, &d_D, &d_gD, &d_atom_e, &d_vpart,
-                  &d_coord_ext, &d_force_ext, &d_g0, &d_g1})
+                  &d_coord_ext, &d_force_ext, &d_g0, &d_g1, &d_part})
     dfree(*p);
   dfree(d_geom);
   for (auto& p : d_fz) dfree(p);
@@ -317,8 +327,7 @@ void SeaEngine::mark(hipStream_t st, int stage) {
   DPMI_CHECK(hipEventRecord(events[k], st));
 }
 
-static const char* kStageNames = "geometry;nlist;env_mat;tabulate;descriptor;fitting_fwd;fitting_bwd;descriptor_bwd;"
-                                 "tabulate_grad;force_virial;reduce";
+static const char* kStageNames = "geometry;nlist;env_mat;embed_fwd;fitting_fwd;fitting_bwd;embed_bwd;force_virial;reduce";
 
 void SeaEngine::compute(const double* coord, const double* box_h, double* energy, double* force, double* virial,
                         double* atom_energy, hipStream_t st) {
@@ -398,10 +407,12 @@ void SeaEngine::compute(const double* coord, const double* box_h, double* energy
   launch_env_mat_a(ea, st);
   mark(st, stage++);
 
-  TabRowsArgs ta;
+  EmbedArgs ta;
   ta.xyz = d_xyz;
+  ta.D = d_D;
+  ta.gD = d_gD;
   ta.net_deriv = d_net_deriv;
-  ta.gxyz = d_gxyz;
+  ta.pos = d_pos;
   ta.tables = m.d_table;
   ta.tab_off = m.d_tab_off;
   ta.infos = m.d_infos;
@@ -413,11 +424,9 @@ void SeaEngine::compute(const double* coord, const double* box_h, double* energy
   ta.nrows = ntot;
   ta.nnei = m.nnei;
   ta.M = m.M;
-  launch_tab_fwd_rows(ta, st);
-  mark(st, stage++);
-
-  const double inv_nnei = 1.0 / m.nnei;
-  launch_descriptor(d_xyz, d_pos, ntot, m.M, m.A, inv_nnei, d_D, st);
+  ta.A = m.A;
+  ta.inv_nnei = 1.0 / m.nnei;
+  launch_embed_fwd<double, double>(ta, st);
   mark(st, stage++);
 
   // fitting forward, one row block per type
@@ -472,9 +481,7 @@ void SeaEngine::compute(const double* coord, const double* box_h, double* energy
   }
   mark(st, stage++);
 
-  launch_descriptor_bwd(d_xyz, d_gD, d_pos, ntot, m.M, m.A, inv_nnei, d_gxyz, st);
-  mark(st, stage++);
-  launch_tab_grad_rows(ta, st);
+  launch_embed_bwd<double, double>(ta, st);
   mark(st, stage++);
 
   DPMI_CHECK(hipMemsetAsync(d_force_ext, 0, sizeof(double) * (size_t)next_tot * 3, st));
@@ -494,7 +501,7 @@ void SeaEngine::compute(const double* coord, const double* box_h, double* energy
   DPMI_CHECK(hipMemsetAsync(force, 0, sizeof(double) * (size_t)ntot * 3, st));
   launch_fold_ghost_forces(d_force_ext, d_map_ext, d_xframe_of_ext, d_geom, d_nall, next_tot, force, st);
   mark(st, stage++);
-  launch_frame_reduce(d_atom_e, d_vpart, d_geom, nframes, energy, virial, st);
+  launch_frame_reduce(d_atom_e, d_vpart, d_chunks, nchunks, d_chunk_start, nframes, d_part, energy, virial, st);
   if (atom_energy)
     DPMI_CHECK(hipMemcpyAsync(atom_energy, d_atom_e, sizeof(double) * ntot, hipMemcpyDeviceToDevice, st));
   mark(st, stage++);

@@ -84,6 +84,9 @@ struct SeaEngine {
          *d_vpart = nullptr;
   std::vector<double*> d_fz, d_fy, d_fx, d_outw;
   double *d_g0 = nullptr, *d_g1 = nullptr;
+  int nchunks = 0;
+  int *d_chunks = nullptr, *d_chunk_start = nullptr;
+  double* d_part = nullptr;
 };
 
 }  // namespace dpmi

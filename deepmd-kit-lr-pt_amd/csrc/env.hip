@@ -32,12 +32,16 @@ __device__ __forceinline__ void smooth_switch(T& v, T& dv, T r, T rmin, T rmax) 
   }
 }
 
+// TO: storage type of em / em_deriv / rij (double for the FP64 engine,
+// float for the mixed engine: the image displacement, the switch and every
+// intermediate are evaluated in FP64 and only the stored values are rounded).
+template <typename TO>
 __global__ void k_env_mat_a(const double* __restrict__ coord, const int* __restrict__ type,
                             const int* __restrict__ nlist, const int* __restrict__ row_frame,
                             const int* __restrict__ row_ext_off, const int* __restrict__ row_center,
                             int strided_nloc, long long nrows, int nnei, int nall_stride, const double* __restrict__ avg,
-                            const double* __restrict__ stdv, double rmin, double rmax, double* __restrict__ em,
-                            double* __restrict__ em_deriv, double* __restrict__ rij) {
+                            const double* __restrict__ stdv, double rmin, double rmax, TO* __restrict__ em,
+                            TO* __restrict__ em_deriv, TO* __restrict__ rij) {
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= nrows * nnei) return;
   const long long g = tid / nnei;
@@ -93,18 +97,18 @@ __global__ void k_env_mat_a(const double* __restrict__ coord, const int* __restr
     const double* sd = stdv + (size_t)ti * nnei * 4 + s * 4;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
-      em[base * 4 + a] = (val[a] - av[a]) / sd[a];
+      em[base * 4 + a] = (TO)((val[a] - av[a]) / sd[a]);
 #pragma unroll
-      for (int d = 0; d < 3; ++d) em_deriv[base * 12 + a * 3 + d] = der[a * 3 + d] / sd[a];
+      for (int d = 0; d < 3; ++d) em_deriv[base * 12 + a * 3 + d] = (TO)(der[a * 3 + d] / sd[a]);
     }
   } else {
 #pragma unroll
-    for (int a = 0; a < 4; ++a) em[base * 4 + a] = 0.;
+    for (int a = 0; a < 4; ++a) em[base * 4 + a] = (TO)0;
 #pragma unroll
-    for (int k = 0; k < 12; ++k) em_deriv[base * 12 + k] = 0.;
+    for (int k = 0; k < 12; ++k) em_deriv[base * 12 + k] = (TO)0;
   }
 #pragma unroll
-  for (int d = 0; d < 3; ++d) rij[base * 3 + d] = rr[d];
+  for (int d = 0; d < 3; ++d) rij[base * 3 + d] = (TO)rr[d];
 }
 
 }  // namespace
@@ -112,9 +116,18 @@ __global__ void k_env_mat_a(const double* __restrict__ coord, const int* __restr
 void launch_env_mat_a(const EnvArgs& a, hipStream_t st) {
   const long long n = a.nrows * (long long)a.nnei;
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_env_mat_a, dim3(ceil_div(n, 256)), dim3(256), 0, st, a.coord, a.type, a.nlist, a.row_frame,
+  hipLaunchKernelGGL(k_env_mat_a<double>, dim3(ceil_div(n, 256)), dim3(256), 0, st, a.coord, a.type, a.nlist, a.row_frame,
                      a.row_ext_off, a.row_center, a.strided_nloc, a.nrows, a.nnei, a.nall_stride, a.avg, a.std_, a.rmin, a.rmax,
                      a.em, a.em_deriv, a.rij);
+  DPMI_LAUNCH_CHECK();
+}
+
+void launch_env_mat_a_f32(const EnvArgs& a, float* em, float* em_deriv, float* rij, hipStream_t st) {
+  const long long n = a.nrows * (long long)a.nnei;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_env_mat_a<float>, dim3(ceil_div(n, 256)), dim3(256), 0, st, a.coord, a.type, a.nlist,
+                     a.row_frame, a.row_ext_off, a.row_center, a.strided_nloc, a.nrows, a.nnei, a.nall_stride, a.avg,
+                     a.std_, a.rmin, a.rmax, em, em_deriv, rij);
   DPMI_LAUNCH_CHECK();
 }
 

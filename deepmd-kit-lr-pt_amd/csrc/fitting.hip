@@ -1,9 +1,5 @@
-// Descriptor contraction and fitting-net element-wise stages (gfx950), FP64.
+// Fitting-net element-wise stages of the FP64 engine (gfx950).
 //
-// Descriptor (deepmd/pt/model/descriptor/se_a.py:844-851):
-//   T = xyz / nnei  [4, M];  D[m, a] = sum_c T[c, m] T[c, a]  (a < A)
-// written row-permuted so that atoms of one type form a contiguous block
-// for the per-type fitting GEMMs.
 // Fitting layer (deepmd/pt/model/network/mlp.py:188-219): z = x W (GEMM),
 //   y = tanh(z + b), out = y * idt + (x | [x, x] | 0).
 #include "dpmi_common.h"
@@ -12,51 +8,6 @@
 namespace dpmi {
 
 namespace {
-
-__global__ void k_descriptor(const double* __restrict__ xyz, const int* __restrict__ pos, long long nrows, int M,
-                             int A, double inv_nnei, double* __restrict__ D) {
-  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= nrows * M) return;
-  const long long g = tid / M;
-  const int m = (int)(tid - g * M);
-  const double* x = xyz + (size_t)g * 4 * M;
-  const double t0 = x[m] * inv_nnei, t1 = x[M + m] * inv_nnei, t2 = x[2 * M + m] * inv_nnei,
-               t3 = x[3 * M + m] * inv_nnei;
-  double* out = D + ((size_t)pos[g] * M + m) * A;
-  for (int a = 0; a < A; ++a) {
-    const double v = t0 * (x[a] * inv_nnei) + t1 * (x[M + a] * inv_nnei) + t2 * (x[2 * M + a] * inv_nnei) +
-                     t3 * (x[3 * M + a] * inv_nnei);
-    out[a] = v;
-  }
-}
-
-// gT[c, m] = sum_a G[m, a] T[c, a] + [m < A] sum_m' G[m', m] T[c, m']
-__global__ void k_descriptor_bwd(const double* __restrict__ xyz, const double* __restrict__ gD,
-                                 const int* __restrict__ pos, long long nrows, int M, int A, double inv_nnei,
-                                 double* __restrict__ gxyz) {
-  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= nrows * M) return;
-  const long long g = tid / M;
-  const int m = (int)(tid - g * M);
-  const double* x = xyz + (size_t)g * 4 * M;
-  const double* G = gD + (size_t)pos[g] * M * A;
-  double acc[4] = {0., 0., 0., 0.};
-  for (int a = 0; a < A; ++a) {
-    const double w = G[(size_t)m * A + a];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) acc[c] += w * x[c * M + a];
-  }
-  if (m < A) {
-    for (int mp = 0; mp < M; ++mp) {
-      const double w = G[(size_t)mp * A + m];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) acc[c] += w * x[c * M + mp];
-    }
-  }
-  // d/dxyz = inv_nnei * d/dT ; T itself carries one inv_nnei factor
-#pragma unroll
-  for (int c = 0; c < 4; ++c) gxyz[((size_t)g * 4 + c) * M + m] = acc[c] * inv_nnei * inv_nnei;
-}
 
 __global__ void k_fit_act(const double* __restrict__ z, const double* __restrict__ b, const double* __restrict__ idt,
                           const double* __restrict__ x, int act, int res_mode, long long n, int n_in, int n_out,
@@ -128,23 +79,6 @@ __global__ void k_fill_double(double* __restrict__ p, double v, long long n) {
 }
 
 }  // namespace
-
-void launch_descriptor(const double* xyz, const int* pos, long long nrows, int M, int A, double inv_nnei, double* D,
-                       hipStream_t st) {
-  const long long n = nrows * M;
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_descriptor, dim3(ceil_div(n, 256)), dim3(256), 0, st, xyz, pos, nrows, M, A, inv_nnei, D);
-  DPMI_LAUNCH_CHECK();
-}
-
-void launch_descriptor_bwd(const double* xyz, const double* gD, const int* pos, long long nrows, int M, int A,
-                           double inv_nnei, double* gxyz, hipStream_t st) {
-  const long long n = nrows * M;
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_descriptor_bwd, dim3(ceil_div(n, 256)), dim3(256), 0, st, xyz, gD, pos, nrows, M, A, inv_nnei,
-                     gxyz);
-  DPMI_LAUNCH_CHECK();
-}
 
 void launch_fit_act(const double* z, const double* b, const double* idt, const double* x, int act, int res_mode,
                     long long n, int n_in, int n_out, double* y, double* x_next, hipStream_t st) {

@@ -16,20 +16,22 @@ namespace dpmi {
 
 namespace {
 
-__device__ __forceinline__ void slot_force(const double* nd, const double* ed, double* f) {
-  f[0] = f[1] = f[2] = 0.;
+template <typename T>
+__device__ __forceinline__ void slot_force(const T* nd, const T* ed, T* f) {
+  f[0] = f[1] = f[2] = 0;
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int d = 0; d < 3; ++d) f[d] += nd[a] * ed[a * 3 + d];
 }
 
-__device__ __forceinline__ void slot_virial(const double* nd, const double* ed, const double* r, double* v) {
+template <typename T>
+__device__ __forceinline__ void slot_virial(const T* nd, const T* ed, const T* r, T* v) {
 #pragma unroll
   for (int p = 0; p < 3; ++p)
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      double s = 0.;
+      T s = 0;
 #pragma unroll
       for (int a = 0; a < 4; ++a) s += nd[a] * ed[a * 3 + p];
       v[p * 3 + q] = s * r[q];
@@ -179,8 +181,11 @@ __global__ void k_prod_virial_grad(double* __restrict__ gn, const double* __rest
 }
 
 // Engine: packed rows, extended force buffer per frame, virial partials.
-__global__ void __launch_bounds__(256) k_force_virial_rows(const double* __restrict__ nd, const double* __restrict__ ed,
-                                                           const double* __restrict__ rij, const int* __restrict__ nlist,
+// TI: storage and arithmetic type of the per-slot terms (double, or float in
+// the mixed engine); forces and virials are accumulated in FP64.
+template <typename TI>
+__global__ void __launch_bounds__(256) k_force_virial_rows(const TI* __restrict__ nd, const TI* __restrict__ ed,
+                                                           const TI* __restrict__ rij, const int* __restrict__ nlist,
                                                            const int* __restrict__ row_frame,
                                                            const int* __restrict__ row_center,
                                                            const int* __restrict__ frame_ext_off, long long nrows,
@@ -199,7 +204,7 @@ __global__ void __launch_bounds__(256) k_force_virial_rows(const double* __restr
     const size_t b = (size_t)g * nnei + s;
     const int j = nlist[b];
     if (j < 0) continue;  // padding slots have zero em_deriv
-    double n4[4], e12[12], r3[3], fs[3], v[9];
+    TI n4[4], e12[12], r3[3], fs[3], v[9];
 #pragma unroll
     for (int a = 0; a < 4; ++a) n4[a] = nd[b * 4 + a];
 #pragma unroll
@@ -247,30 +252,47 @@ __global__ void k_fold_ghosts(const double* __restrict__ force_ext, const int* _
   for (int d = 0; d < 3; ++d) atomicAdd(force + (size_t)owner * 3 + d, force_ext[e * 3 + d]);
 }
 
-__global__ void __launch_bounds__(256) k_frame_reduce(const double* __restrict__ atom_e,
+// Two-level compensated per-frame sums of the atomic energies and of the
+// per-row virial partials.  Level 1: one workgroup per chunk of at most
+// kChunkRows rows of one frame (chunk list built on the host): each thread
+// sums its rows with compensation, the block combines the thread sums with
+// compensation into 10 values.  Level 2: one wave per frame combines its
+// chunks.
+constexpr int kChunkRows = 1024;
+
+__global__ void __launch_bounds__(256) k_chunk_reduce(const double* __restrict__ atom_e,
                                                       const double* __restrict__ vpart,
-                                                      const FrameGeom* __restrict__ geom, double* __restrict__ energy,
-                                                      double* __restrict__ virial) {
-  __shared__ double sh[256][10];
-  const FrameGeom& g = geom[blockIdx.x];
+                                                      const int2* __restrict__ chunks, double* __restrict__ part) {
+  __shared__ double sh[10][256];
+  const int2 ch = chunks[blockIdx.x];  // (first row, row count)
   KahanD acc[10];
-  for (int r = threadIdx.x; r < g.nloc; r += blockDim.x) {
-    const size_t row = (size_t)g.loc_off + r;
+  for (int r = threadIdx.x; r < ch.y; r += blockDim.x) {
+    const size_t row = (size_t)ch.x + r;
     acc[0].add(atom_e[row]);
 #pragma unroll
     for (int k = 0; k < 9; ++k) acc[k + 1].add(vpart[row * 9 + k]);
   }
 #pragma unroll
-  for (int k = 0; k < 10; ++k) sh[threadIdx.x][k] = acc[k].value();
+  for (int k = 0; k < 10; ++k) sh[k][threadIdx.x] = acc[k].value();
   __syncthreads();
   if (threadIdx.x < 10) {
     KahanD t;
-    for (int r = 0; r < (int)blockDim.x; ++r) t.add(sh[r][threadIdx.x]);
-    if (threadIdx.x == 0)
-      energy[blockIdx.x] = t.value();
-    else
-      virial[blockIdx.x * 9 + threadIdx.x - 1] = t.value();
+    for (int r = 0; r < (int)blockDim.x; ++r) t.add(sh[threadIdx.x][r]);
+    part[(size_t)blockIdx.x * 10 + threadIdx.x] = t.value();
   }
+}
+
+__global__ void k_frame_combine(const double* __restrict__ part, const int* __restrict__ chunk_start, int nframes,
+                                double* __restrict__ energy, double* __restrict__ virial) {
+  const int f = blockIdx.x;
+  const int q = threadIdx.x;
+  if (f >= nframes || q >= 10) return;
+  KahanD t;
+  for (int c = chunk_start[f]; c < chunk_start[f + 1]; ++c) t.add(part[(size_t)c * 10 + q]);
+  if (q == 0)
+    energy[f] = t.value();
+  else
+    virial[f * 9 + q - 1] = t.value();
 }
 
 }  // namespace
@@ -319,9 +341,14 @@ void launch_prod_virial_grad_a(double* grad_net, const double* grad, const doubl
 
 void launch_force_virial_rows(const ForceRowsArgs& a, hipStream_t st) {
   if (a.nrows <= 0) return;
-  hipLaunchKernelGGL(k_force_virial_rows, dim3(ceil_div(a.nrows * kWave, 256)), dim3(256), 0, st, a.net_deriv,
-                     a.em_deriv, a.rij, a.nlist, a.row_frame, a.row_center, a.frame_ext_off, a.nrows, a.nnei,
-                     a.force_ext, a.vpart);
+  if (a.fp32)
+    hipLaunchKernelGGL(k_force_virial_rows<float>, dim3(ceil_div(a.nrows * kWave, 256)), dim3(256), 0, st,
+                       (const float*)a.net_deriv, (const float*)a.em_deriv, (const float*)a.rij, a.nlist,
+                       a.row_frame, a.row_center, a.frame_ext_off, a.nrows, a.nnei, a.force_ext, a.vpart);
+  else
+    hipLaunchKernelGGL(k_force_virial_rows<double>, dim3(ceil_div(a.nrows * kWave, 256)), dim3(256), 0, st,
+                       (const double*)a.net_deriv, (const double*)a.em_deriv, (const double*)a.rij, a.nlist,
+                       a.row_frame, a.row_center, a.frame_ext_off, a.nrows, a.nnei, a.force_ext, a.vpart);
   DPMI_LAUNCH_CHECK();
 }
 
@@ -334,9 +361,30 @@ void launch_fold_ghost_forces(const double* force_ext, const int* map_ext, const
   DPMI_LAUNCH_CHECK();
 }
 
-void launch_frame_reduce(const double* atom_e, const double* vpart, const FrameGeom* geom, int nframes,
-                         double* energy, double* virial, hipStream_t st) {
-  hipLaunchKernelGGL(k_frame_reduce, dim3(nframes), dim3(256), 0, st, atom_e, vpart, geom, energy, virial);
+void frame_chunks(const int* natoms, int nframes, std::vector<int>& chunks_xy, std::vector<int>& chunk_start) {
+  chunks_xy.clear();
+  chunk_start.assign(nframes + 1, 0);
+  int off = 0;
+  for (int f = 0; f < nframes; ++f) {
+    chunk_start[f] = (int)chunks_xy.size() / 2;
+    for (int r = 0; r < natoms[f]; r += kChunkRows) {
+      chunks_xy.push_back(off + r);
+      chunks_xy.push_back(std::min(kChunkRows, natoms[f] - r));
+    }
+    off += natoms[f];
+  }
+  chunk_start[nframes] = (int)chunks_xy.size() / 2;
+}
+
+void launch_frame_reduce(const double* atom_e, const double* vpart, const int* chunks_xy, int nchunks,
+                         const int* chunk_start, int nframes, double* part, double* energy, double* virial,
+                         hipStream_t st) {
+  if (nchunks > 0) {
+    hipLaunchKernelGGL(k_chunk_reduce, dim3(nchunks), dim3(256), 0, st, atom_e, vpart,
+                       reinterpret_cast<const int2*>(chunks_xy), part);
+    DPMI_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_frame_combine, dim3(nframes), dim3(kWave), 0, st, part, chunk_start, nframes, energy, virial);
   DPMI_LAUNCH_CHECK();
 }
 

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "geometry.h"
 
 namespace dpmi {
@@ -85,6 +87,8 @@ struct EnvArgs {
   double* rij = nullptr;
 };
 void launch_env_mat_a(const EnvArgs& a, hipStream_t st);
+// same, storing em / em_deriv / rij in FP32 (em, em_deriv, rij of a are ignored)
+void launch_env_mat_a_f32(const EnvArgs& a, float* em, float* em_deriv, float* rij, hipStream_t st);
 
 // ---------------------------------------------------------- tabulate.hip --
 template <typename T>
@@ -97,27 +101,37 @@ template <typename T>
 int launch_tab_grad_grad(T* dz_dy, const T* table, const T* info_host, const T* em_x, const T* em, const T* dz_dx,
                          const T* dz_dem, int nloc, int nnei, int M, int sorted, int nd, hipStream_t st);
 
-struct TabRowsArgs {
-  void* xyz = nullptr;        // fwd output [nrows, 4, M]
-  void* net_deriv = nullptr;  // grad output [nrows, nnei, 4]
-  const void* gxyz = nullptr; // grad input [nrows, 4, M]
+// ------------------------------------------------------------- embed.hip --
+// Engine descriptor: table embedding + contraction + descriptor (fwd) and the
+// fused adjoint from dE/dD to net_deriv (bwd).  T: table/arithmetic type,
+// TD: descriptor storage type (the fitting input), TG: dE/dD storage type.
+struct EmbedArgs {
+  void* xyz = nullptr;        // [nrows, 4, M] (T)
+  void* D = nullptr;          // [nrows, M*A] (TD), rows permuted by pos
+  const void* gD = nullptr;   // [nrows, M*A] (TG), rows permuted by pos
+  void* net_deriv = nullptr;  // [nrows, nnei, 4] (T)
+  const int* pos = nullptr;   // [nrows] row -> permuted descriptor row
   const void* tables = nullptr;
   const long long* tab_off = nullptr;  // [nnets] element offsets
   const void* infos = nullptr;         // device Info<T>[nnets]
   const int* net_of = nullptr;         // [ntypes * ntypes]: net of (ti, tj)
-  const void* em = nullptr;            // [nrows, nnei, 4]; em_x is column 0
+  const void* em = nullptr;            // [nrows, nnei, 4] (T); em_x is column 0
   const int* row_type = nullptr;       // [nrows]
   const int* sec = nullptr;            // [ntypes + 1]
   int ntypes = 0;
   long long nrows = 0;
   int nnei = 0;
   int M = 0;
+  int A = 0;  // axis_neuron, at most 16
+  double inv_nnei = 0;
 };
 void* make_table_infos(const double* infos_host, int nnets, bool fp32);
 size_t table_info_bytes(bool fp32);
 void free_table_infos(void* p, bool fp32);
-void launch_tab_fwd_rows(const TabRowsArgs& a, hipStream_t st);
-void launch_tab_grad_rows(const TabRowsArgs& a, hipStream_t st);
+template <typename T, typename TD>
+void launch_embed_fwd(const EmbedArgs& a, hipStream_t st);
+template <typename T, typename TG>
+void launch_embed_bwd(const EmbedArgs& a, hipStream_t st);
 
 // ------------------------------------------------------------- force.hip --
 void launch_prod_force_a(double* force, const double* net_deriv, const double* env_deriv, const int* nlist,
@@ -133,9 +147,10 @@ void launch_prod_virial_grad_a(double* grad_net, const double* grad, const doubl
 
 // engine: force/virial over packed rows into extended force buffers
 struct ForceRowsArgs {
-  const double* net_deriv = nullptr;  // [nrows, nnei*4]
-  const double* em_deriv = nullptr;   // [nrows, nnei*12]
-  const double* rij = nullptr;        // [nrows, nnei*3]
+  const void* net_deriv = nullptr;  // [nrows, nnei*4]  (double, or float when fp32)
+  const void* em_deriv = nullptr;   // [nrows, nnei*12]
+  const void* rij = nullptr;        // [nrows, nnei*3]
+  bool fp32 = false;
   const int* nlist = nullptr;
   const int* row_frame = nullptr;
   const int* row_center = nullptr;
@@ -150,17 +165,14 @@ void launch_force_virial_rows(const ForceRowsArgs& a, hipStream_t st);
 void launch_fold_ghost_forces(const double* force_ext, const int* map_ext, const int* xframe_of_ext,
                               const FrameGeom* geom, const int* nall, long long next_tot, double* force,
                               hipStream_t st);
-// per-frame energy and virial (Neumaier-compensated): one workgroup per frame
-void launch_frame_reduce(const double* atom_e, const double* vpart, const FrameGeom* geom, int nframes,
-                         double* energy, double* virial, hipStream_t st);
+// per-frame energy and virial (Neumaier-compensated, two levels): chunks of
+// at most 1024 rows of one frame -> partials [nchunks, 10] -> frame totals
+void frame_chunks(const int* natoms, int nframes, std::vector<int>& chunks_xy, std::vector<int>& chunk_start);
+void launch_frame_reduce(const double* atom_e, const double* vpart, const int* chunks_xy, int nchunks,
+                         const int* chunk_start, int nframes, double* part, double* energy, double* virial,
+                         hipStream_t st);
 
 // ----------------------------------------------------------- fitting.hip --
-// D[pos[g]] = T^T T[:, :A] with T = xyz[g] / nnei, row length M*A
-void launch_descriptor(const double* xyz, const int* pos, long long nrows, int M, int A, double inv_nnei, double* D,
-                       hipStream_t st);
-// gxyz[g] = dT / nnei from G = dE/dD[pos[g]]
-void launch_descriptor_bwd(const double* xyz, const double* gD, const int* pos, long long nrows, int M, int A,
-                           double inv_nnei, double* gxyz, hipStream_t st);
 // y = act(z + b); x_next = y * idt + residual(x); stores y (for backward)
 void launch_fit_act(const double* z, const double* b, const double* idt, const double* x, int act, int res_mode,
                     long long n, int n_in, int n_out, double* y, double* x_next, hipStream_t st);
@@ -174,6 +186,27 @@ void launch_fit_res_bwd(const double* g_out, int res_mode, long long n, int n_in
 void launch_fit_out(const double* z, double b_out, double bias, const int* perm, long long p0, long long n,
                     double* atom_e, hipStream_t st);
 void launch_fill_rows(double* g, const double* w, long long n, int width, hipStream_t st);
+
+// ------------------------------------------------------------- fit16.hip --
+// One fitting layer of the mixed engine, widths padded to multiples of 64.
+struct Fit16Layer {
+  const _Float16* wt = nullptr;  // [n_pad][k_pad] forward operand (W^T)
+  const _Float16* w = nullptr;   // [k_pad][n_pad] backward operand (W)
+  const float* b = nullptr;      // [n_pad]
+  const float* idt = nullptr;    // [n_pad] or null
+  const float* out_w = nullptr;  // [n_pad] output-layer weights (last hidden layer) or null
+  int act = 1, res_mode = 0;
+  int k_pad = 0, n_pad = 0;
+};
+// y = act(x W + b) (FP32), xo = y * idt + xres (FP16); e_acc[r] += xo . out_w
+void launch_fit16_fwd(const Fit16Layer& L, const _Float16* x, int ldx, int rows, const _Float16* xres, int ldr,
+                      float* y, int ldy, _Float16* xo, int ldxo, float* e_acc, hipStream_t st);
+// g_in = (g * idt * (1 - y^2)) W^T + (residual ? g : 0); g_bcast: g is one row [n_pad]
+void launch_fit16_bwd(const Fit16Layer& L, const float* g, int ldg, bool g_bcast, const float* y, int ldy, int rows,
+                      bool residual, float* g_in, int ld_in, hipStream_t st);
+// atom_e[g] = e_perm[pos[g]] + shift[type[g]]
+void launch_energy_out(const float* e_perm, const int* pos, const int* row_type, const double* shift, long long n,
+                       double* atom_e, hipStream_t st);
 
 void launch_fill_int(int* p, int v, long long n, hipStream_t st);
 void launch_fill_double(double* p, double v, long long n, hipStream_t st);

@@ -1,0 +1,173 @@
+// Table lookup helpers shared by the tabulate kernels (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "dpmi_common.h"
+
+namespace dpmi {
+namespace tab {
+
+template <typename T>
+struct Info {
+  T lower, upper, mx, s0, s1;
+  int n0, ntail;  // int((upper-lower)/s0) and the high-tail segment index
+};
+
+template <typename T>
+__device__ __forceinline__ void locate(const Info<T>& f, T& x, int& idx, T& delta) {
+#pragma clang fp contract(off)
+  // tabulate.cc:46-73
+  const T x0 = x;
+  delta = (T)0;
+  if (x < f.lower) {
+    idx = 0;
+    x = (T)0;
+    delta = x0 - f.lower;
+  } else if (x < f.upper) {
+    idx = (int)((x - f.lower) / f.s0);
+    x -= (idx * f.s0 + f.lower);
+  } else if (x < f.mx) {
+    // min(): a quotient rounded up to the tail count must not address the
+    // row past the last segment
+    idx = min(f.n0 + (int)((x - f.upper) / f.s1), f.ntail);
+    x -= ((idx - f.n0) * f.s1 + f.upper);
+  } else {
+    idx = f.ntail;
+    x = f.mx - ((idx - f.n0) * f.s1 + f.upper);
+    delta = x0 - f.mx;
+  }
+}
+
+__device__ __forceinline__ double bcast(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float bcast(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ int bcast(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+template <typename T>
+__device__ __forceinline__ void load6(const T* p, T* a) {
+#pragma unroll
+  for (int q = 0; q < 6; ++q) a[q] = p[q];
+}
+
+template <typename T>
+__device__ __forceinline__ void quintic(const T* a, T x, T delta, T& v, T& d) {
+  // tabulate.cc:122-158
+  d = a[1] + ((T)2 * a[2] + ((T)3 * a[3] + ((T)4 * a[4] + (T)5 * a[5] * x) * x) * x) * x;
+  v = a[0] + (a[1] + (a[2] + (a[3] + (a[4] + a[5] * x) * x) * x) * x) * x;
+  v += d * delta;
+}
+
+// Per-lane neighbour record of a 64-neighbour chunk.
+template <typename T, int ND>
+struct Nbr {
+  T x, delta;
+  int idx;
+  T ll[ND];
+};
+
+template <typename T, int ND>
+__device__ __forceinline__ bool load_nbr(Nbr<T, ND>& r, const Info<T>& f, const T* em_x, int xs, const T* em, int j,
+                                         int nnei, T last, bool sorted) {
+  if (j >= nnei) {
+    r.x = 0;
+    r.delta = 0;
+    r.idx = 0;
+#pragma unroll
+    for (int m = 0; m < ND; ++m) r.ll[m] = 0;
+    return false;
+  }
+  T x = em_x[(size_t)j * xs];
+#pragma unroll
+  for (int m = 0; m < ND; ++m) r.ll[m] = em[(size_t)j * ND + m];
+  const bool tail = sorted && last == x && r.ll[1] == (T)0 && r.ll[2] == (T)0 && r.ll[3] == (T)0;
+  locate(f, x, r.idx, r.delta);
+  r.x = x;
+  return tail;
+}
+
+// Accumulates one row block (nnei neighbours) into acc[KQ][ND].
+template <typename T, int ND, int KQ>
+__device__ __forceinline__ void fwd_block(T (&acc)[KQ][ND], const T* __restrict__ table, const Info<T>& f,
+                                          const T* __restrict__ em_x, int xs, const T* __restrict__ em, int nnei,
+                                          int M, bool sorted) {
+  if (nnei <= 0) return;
+  const int lane = lane_id();
+  const T last = em_x[(size_t)(nnei - 1) * xs];
+  for (int c0 = 0; c0 < nnei; c0 += kWave) {
+    Nbr<T, ND> r;
+    const bool tail = load_nbr(r, f, em_x + (size_t)c0 * xs, xs, em + (size_t)c0 * ND, lane, nnei - c0, last, sorted);
+    const unsigned long long tm = __ballot(tail);
+    int jend = min(kWave, nnei - c0);
+    if (tm) jend = __ffsll((long long)tm);  // up to and including the first tail neighbour
+    for (int q = 0; q < jend; ++q) {
+      const int idx = bcast(r.idx, q);
+      const T x = bcast(r.x, q), delta = bcast(r.delta, q);
+      T ll[ND];
+#pragma unroll
+      for (int m = 0; m < ND; ++m) ll[m] = bcast(r.ll[m], q);
+      const T scale = (tm && q == jend - 1) ? (T)(nnei - (c0 + q)) : (T)1;
+      const T* row = table + (size_t)idx * M * 6;
+#pragma unroll
+      for (int kq = 0; kq < KQ; ++kq) {
+        const int k = lane + kq * kWave;
+        if (k < M) {
+          T a[6];
+          load6(row + 6 * k, a);
+          T v, d;
+          quintic(a, x, delta, v, d);
+          const T w = v * scale;
+#pragma unroll
+          for (int m = 0; m < ND; ++m) acc[kq][m] += w * ll[m];
+        }
+      }
+    }
+    if (tm) break;
+  }
+}
+
+
+template <typename T>
+Info<T> make_info(const T* ti) {
+  Info<T> f;
+  f.lower = ti[0];
+  f.upper = ti[1];
+  f.mx = ti[2];
+  f.s0 = ti[3];
+  f.s1 = ti[4];
+  f.n0 = int((f.upper - f.lower) / f.s0);
+  // tabulate.cc:21-30: the last segment below max
+  const T edge = std::nextafter(f.mx, f.lower);
+  f.ntail = f.n0 + int((edge - f.upper) / f.s1);
+  return f;
+}
+
+#define DPMI_KQ_DISPATCH(M, ...)                                      \
+  do {                                                                \
+    const int kq_ = ((M) + kWave - 1) / kWave;                         \
+    if (kq_ <= 1) { constexpr int KQ = 1; __VA_ARGS__; }              \
+    else if (kq_ == 2) { constexpr int KQ = 2; __VA_ARGS__; }         \
+    else if (kq_ <= 4) { constexpr int KQ = 4; __VA_ARGS__; }         \
+    else if (kq_ <= 8) { constexpr int KQ = 8; __VA_ARGS__; }         \
+    else { constexpr int KQ = 16; __VA_ARGS__; }                      \
+  } while (0)
+
+#define DPMI_ND_DISPATCH(nd, ...)                                     \
+  do {                                                                \
+    if ((nd) == 4) { constexpr int ND = 4; __VA_ARGS__; }             \
+    else if ((nd) == 9) { constexpr int ND = 9; __VA_ARGS__; }        \
+    else if ((nd) == 16) { constexpr int ND = 16; __VA_ARGS__; }      \
+    else { constexpr int ND = 25; __VA_ARGS__; }                      \
+  } while (0)
+
+
+}  // namespace tab
+}  // namespace dpmi
