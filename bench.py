@@ -29,34 +29,38 @@ sys.path.insert(0, os.path.join(ROOT, "deepmd-kit-lr-pt_amd"))
 METRIC = "atom-steps/s (E+F+V), 12k-atom water, mixed vs FP64 GPU; force RMSE vs FP64"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP16_PEAK_TFLOPS = 2500.0
 
 
 def stage_model(m, nloc, nall, precision):
-    """Algorithmic bytes / flops per launch of each engine stage (per local
-    atom figures times nloc).  N = sum(sel), M = embedding width, A = axis,
-    D = M*A, s = bytes per float."""
+    """Algorithmic bytes or flops per launch of each engine stage: per local
+    atom figures times nloc.  N = sum(sel), M = embedding width, A = axis,
+    D = M*A, s = bytes per stored float (8 FP64, 4 mixed), h = bytes per
+    descriptor element (8 FP64, 2 mixed), rho = nall / nloc."""
     N, M, A = m.nnei, m.M, m.axis_neuron
     D = M * A
-    s = 8
+    mixed = precision == "mixed"
+    s = 4 if mixed else 8
+    h = 2 if mixed else 8
     rho = nall / nloc
     fit = [D] + list(m.fit_neuron) + [1]
     fit_flops = 2 * sum(fit[i] * fit[i + 1] for i in range(len(fit) - 1))
+    mm_peak = FP16_PEAK_TFLOPS if mixed else FP64_PEAK_TFLOPS
     per_atom = {
-        # reads nlist (4N) and coordinates, writes em/em_deriv/rij (19 N s)
-        "env_mat": ("hbm", 4 * N + 19 * N * s + (3 * s + 4) * rho),
-        # reads em (4 N s), writes xyz (4 M s)
-        "tabulate": ("flop", 18 * N * M),
-        "tabulate_grad": ("flop", 36 * N * M),
-        # reads net_deriv, em_deriv, rij, nlist; scatters forces
-        "force_virial": ("hbm", 4 * N * s + 12 * N * s + 3 * N * s + 4 * N + 3 * s * rho),
-        # candidate gather + sort: reads the cell lists and ghost coordinates
-        "nlist": ("hbm", 4 * N),
-        "fitting_fwd": ("flop", fit_flops),
-        "fitting_bwd": ("flop", fit_flops),
-        "descriptor": ("hbm", 4 * M * s + D * s),
-        "descriptor_bwd": ("hbm", 4 * M * s + D * s + 4 * M * s),
+        # reads nlist (4N) and extended coordinates, writes em/em_deriv/rij (19 N s)
+        "env_mat": ("hbm", 4 * N + 19 * N * s + 24 * rho, None),
+        # reads em (4 N s), writes xyz (4 M s) and the descriptor (D h)
+        "embed_fwd": ("hbm", 4 * N * s + 4 * M * s + D * h, None),
+        # reads em, xyz, dE/dD (D s); writes net_deriv (4 N s)
+        "embed_bwd": ("hbm", 8 * N * s + 4 * M * s + D * (4 if mixed else 8), None),
+        # reads net_deriv, em_deriv, rij, nlist; scatters forces (FP64)
+        "force_virial": ("hbm", 19 * N * s + 4 * N + 24 * rho + 72, None),
+        # reads the cell lists and ghost coordinates, writes the sorted rows
+        "nlist": ("hbm", 8 * N, None),
+        "fitting_fwd": ("flop", fit_flops, mm_peak),
+        "fitting_bwd": ("flop", fit_flops - 2 * fit[-2], mm_peak),
     }
-    return {k: (b, amt * nloc) for k, (b, amt) in per_atom.items()}
+    return {k: (b, amt * nloc, pk) for k, (b, amt, pk) in per_atom.items()}
 
 
 def cpu_baseline(m, coord, atype, box, max_seconds=25.0):
@@ -81,12 +85,51 @@ def cpu_baseline(m, coord, atype, box, max_seconds=25.0):
                          f"(reference lib + numpy glue, OMP threads {cores})"}
 
 
+def time_engine(eng, x, box, steps, warmup):
+    """Warm-up, then `steps` timed evaluations bracketed by device syncs;
+    returns (seconds, per-stage device ms averaged over the timed calls)."""
+    import torch
+
+    for _ in range(warmup):
+        eng.compute(x, box)
+    eng.status()
+    eng.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.compute(x, box)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    eng.status()
+    stages = eng.timing()
+    eng.set_timing(False)
+    return dt, stages
+
+
+def roofline(m, nloc, nall, precision, stages):
+    model = stage_model(m, nloc, nall, precision)
+    dom = max((k for k in stages if k in model), key=lambda k: stages[k])
+    kind, amount, peak = model[dom]
+    t_ms = stages[dom]
+    if kind == "hbm":
+        roof = {"kernel": dom, "bound": "hbm", "achieved": amount / (t_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s"}
+    else:
+        roof = {"kernel": dom, "bound": "mfma", "achieved": amount / (t_ms * 1e-3) / 1e12, "peak": peak,
+                "unit": "TFLOP/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+    roof["avg_ms"] = t_ms
+    return roof
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"],
+                    help="path reported as value (the other one is timed beside it)")
     ap.add_argument("--nmol", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -103,45 +146,34 @@ def main() -> int:
     from dpmi.engine import DeviceModel, Engine
 
     m = dmodel.water_model(seed=1)
-    # the 12,288-atom configuration uses the 12.42 A 64-molecule box scaled by 4
+    # the 12,288-atom configuration: 4096 molecules in a 49.68 A cubic box
     coord, atype, box = systems.water(args.nmol, seed=2024, box=49.68 if args.nmol == 4096 else None)
     nloc = len(atype)
     dm = DeviceModel(m)
-    eng = Engine(dm, [nloc], atype, precision=args.precision)
     x = torch.as_tensor(coord, device="cuda")
-    torch.cuda.synchronize()
-    for _ in range(args.warmup):
-        eng.compute(x, box)
-    eng.status()
-    eng.set_timing(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        eng.compute(x, box)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    eng.status()
-    stages = eng.timing()
-    eng.set_timing(False)
-    e, f, v = eng.compute(x, box)
-    torch.cuda.synchronize()
-    nall = int(eng.fetch("nall")[0])
-    ms = dt / args.steps * 1e3
-    value = nloc * args.steps / dt
-
-    model = stage_model(m, nloc, nall, args.precision)
-    dom = max((k for k in stages if k in model), key=lambda k: stages[k])
-    kind, amount = model[dom]
-    t_ms = stages[dom]
-    if kind == "hbm":
-        roof = {"kernel": dom, "bound": "hbm", "achieved": amount / (t_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s"}
-    else:
-        roof = {"kernel": dom, "bound": "mfma", "achieved": amount / (t_ms * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s"}
-    roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
-    roof["avg_ms"] = t_ms
+    other = "fp64" if args.precision == "mixed" else "mixed"
+    res = {}
+    for prec in (args.precision, other):
+        eng = Engine(dm, [nloc], atype, precision=prec)
+        dt, stages = time_engine(eng, x, box, args.steps, args.warmup)
+        e, f, v = eng.compute(x, box)
+        torch.cuda.synchronize()
+        nall = int(eng.fetch("nall")[0])
+        res[prec] = {"dt": dt, "stages": stages, "nall": nall, "e": e.cpu().numpy().copy(),
+                     "f": f.cpu().numpy().copy(), "v": v.cpu().numpy().copy()}
+        del eng
+    r = res[args.precision]
+    ms = r["dt"] / args.steps * 1e3
+    value = nloc * args.steps / r["dt"]
+    fp64_value = nloc * args.steps / res["fp64"]["dt"]
+    mixed_value = nloc * args.steps / res["mixed"]["dt"]
+    rm, rd = res["mixed"], res["fp64"]
+    mixed_err = {
+        "energy_meV_atom": float(abs(rm["e"][0] - rd["e"][0]) / nloc * 1e3),
+        "force_rmse_meV_A": float(np.sqrt(np.mean((rm["f"] - rd["f"]) ** 2)) * 1e3),
+        "force_max_meV_A": float(np.max(np.abs(rm["f"] - rd["f"])) * 1e3),
+        "virial_meV_atom": float(np.max(np.abs(rm["v"][0] - rd["v"][0])) / nloc * 1e3),
+    }
 
     out = {
         "metric": METRIC,
@@ -154,22 +186,26 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "fp16-mfma/fp32 (fp64 geometry, s(r), sums)" if args.precision == "mixed" else "f64",
         "data": "synthetic",
         "config": {"workload": f"water {args.nmol} H2O ({nloc} atoms, box {box[0]:.2f} A), se_e2_a compressed, "
-                               f"{args.precision} engine, one frame", "nloc": nloc, "nall": nall,
-                   "precision": args.precision, "parallelism": "single-gpu"},
-        "roofline": roof,
-        "stages_ms": {k: round(vv, 5) for k, vv in stages.items()},
+                               f"{args.precision} engine, one frame, E+F+V per step", "nloc": nloc,
+                   "nall": r["nall"], "precision": args.precision, "parallelism": "single-gpu"},
+        "roofline": roofline(m, nloc, r["nall"], args.precision, r["stages"]),
+        "stages_ms": {k: round(vv, 5) for k, vv in r["stages"].items()},
+        "fp64_value": fp64_value,
+        "mixed_value": mixed_value,
+        "mixed_over_fp64": mixed_value / fp64_value,
+        "mixed_error_vs_fp64": mixed_err,
+        "other_stages_ms": {k: round(vv, 5) for k, vv in res[other]["stages"].items()},
     }
     if not args.no_cpu_baseline:
         ref, cb = cpu_baseline(m, coord, atype, box)
         out["cpu_baseline"] = cb
-        fe = f.cpu().numpy()
-        out["accuracy_vs_cpu_reference"] = {
-            "energy_rel": abs(e.item() - ref.energy) / abs(ref.energy),
-            "force_max_rel": float(np.max(np.abs(fe - ref.force)) / np.max(np.abs(ref.force))),
-            "virial_max_rel": float(np.max(np.abs(v[0].cpu().numpy() - ref.virial)) / np.max(np.abs(ref.virial))),
+        out["fp64_accuracy_vs_cpu_reference"] = {
+            "energy_rel": abs(rd["e"][0] - ref.energy) / abs(ref.energy),
+            "force_max_rel": float(np.max(np.abs(rd["f"] - ref.force)) / np.max(np.abs(ref.force))),
+            "virial_max_rel": float(np.max(np.abs(rd["v"][0] - ref.virial)) / np.max(np.abs(ref.virial))),
         }
     print(json.dumps(out), flush=True)
     return 0

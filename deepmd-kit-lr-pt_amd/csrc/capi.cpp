@@ -33,6 +33,8 @@ int guarded(F&& f) {
   try {
     g_err.clear();
     return f();
+  } catch (const dpmi::Unsupported& e) {
+    return fail(DPMI_ERR_UNSUPPORTED, e.what());
   } catch (const dpmi::HipError& e) {
     return fail(DPMI_ERR_HIP, e.what());
   } catch (const std::invalid_argument& e) {
@@ -352,7 +354,6 @@ int dpmi_sea_engine_create(dpmi_sea_engine** engine, const dpmi_sea_model* model
   return guarded([&] {
     if (!engine || !model || nframes <= 0) throw std::invalid_argument("invalid engine arguments");
     if (precision != DPMI_FP64 && precision != DPMI_MIXED) throw std::invalid_argument("unknown precision");
-    if (precision == DPMI_MIXED) return fail(DPMI_ERR_UNSUPPORTED, "mixed precision engine not built");
     auto* e = new dpmi_sea_engine{nullptr};
     try {
       e->impl = new dpmi::SeaEngine(model->impl, nframes, natoms_host, atype_host, precision, mem_nall_per_atom);

@@ -27,6 +27,10 @@ enum DeviceFlag : int {
 struct HipError : std::runtime_error {
   explicit HipError(const std::string& s) : std::runtime_error(s) {}
 };
+// a valid request this build does not implement (DPMI_ERR_UNSUPPORTED)
+struct Unsupported : std::runtime_error {
+  explicit Unsupported(const std::string& s) : std::runtime_error(s) {}
+};
 
 inline void check(hipError_t e, const char* what, const char* file, int line) {
   if (e != hipSuccess) {

@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(256) k_embed_fwd(T* __restrict__ xyz, TD* __re
                                                    const Info<T>* __restrict__ infos, const int* __restrict__ net_of,
                                                    const T* __restrict__ em, const int* __restrict__ row_type,
                                                    const int* __restrict__ sec, int ntypes, long long nrows, int nnei,
-                                                   int M, int A, T inv_nnei) {
+                                                   int M, int A, int ldD, T inv_nnei) {
   const long long g = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   if (g >= nrows) return;
   const int lane = lane_id();
@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(256) k_embed_fwd(T* __restrict__ xyz, TD* __re
       for (int c = 0; c < 4; ++c) xyz[((size_t)g * 4 + c) * M + k] = acc[kq][c];
   }
   // descriptor row block: columns a < A live in lanes 0..A-1 (kq = 0)
-  TD* out = D + (size_t)pos[g] * M * A;
+  TD* out = D + (size_t)pos[g] * ldD;
   for (int a = 0; a < A; ++a) {
     T ta[4];
 #pragma unroll
@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(64) k_embed_bwd(T* __restrict__ net_deriv, con
                                                   const Info<T>* __restrict__ infos, const int* __restrict__ net_of,
                                                   const T* __restrict__ em, const int* __restrict__ row_type,
                                                   const int* __restrict__ sec, int ntypes, long long nrows, int nnei,
-                                                  int M, int A, T inv_nnei) {
+                                                  int M, int A, int ldD, T inv_nnei) {
   __shared__ T dyk[KQ * kWave * 4];
   const long long g = blockIdx.x;
   const int lane = lane_id();
@@ -184,7 +184,7 @@ __global__ void __launch_bounds__(64) k_embed_bwd(T* __restrict__ net_deriv, con
 #pragma unroll
     for (int c = 0; c < 4; ++c) Tk[kq][c] = k < M ? xyz[((size_t)g * 4 + c) * M + k] * inv_nnei : (T)0;
   }
-  const TG* G = gD + (size_t)pos[g] * M * A;
+  const TG* G = gD + (size_t)pos[g] * ldD;
   // sum_a G[k, a] T[c, a]
   T dT[KQ][4];
 #pragma unroll
@@ -266,7 +266,7 @@ void launch_embed_fwd(const EmbedArgs& a, hipStream_t st) {
   DPMI_KQ_DISPATCH(a.M, hipLaunchKernelGGL((k_embed_fwd<T, TD, KQ>), grid, dim3(256), 0, st, (T*)a.xyz, (TD*)a.D,
                                            a.pos, (const T*)a.tables, a.tab_off, (const Info<T>*)a.infos, a.net_of,
                                            (const T*)a.em, a.row_type, a.sec, a.ntypes, a.nrows, a.nnei, a.M, a.A,
-                                           (T)a.inv_nnei));
+                                           a.ldD > 0 ? a.ldD : a.M * a.A, (T)a.inv_nnei));
   DPMI_LAUNCH_CHECK();
 }
 
@@ -277,7 +277,7 @@ void launch_embed_bwd(const EmbedArgs& a, hipStream_t st) {
                                            (T*)a.net_deriv, (const T*)a.xyz, (const TG*)a.gD, a.pos,
                                            (const T*)a.tables, a.tab_off, (const Info<T>*)a.infos, a.net_of,
                                            (const T*)a.em, a.row_type, a.sec, a.ntypes, a.nrows, a.nnei, a.M, a.A,
-                                           (T)a.inv_nnei));
+                                           a.ldD > 0 ? a.ldD : a.M * a.A, (T)a.inv_nnei));
   DPMI_LAUNCH_CHECK();
 }
 

@@ -129,6 +129,63 @@ SeaModel::SeaModel(const dpmi_sea_model_desc& d) {
         out_b_host[t] = d.fit_b[k][0];
       }
     }
+  build_mixed(d);
+}
+
+// FP32 tables and FP16 fitting layers of the mixed engine.  Hidden layer l of
+// type t maps x [rows, kpad[l]] -> [rows, kpad[l+1]]; padded weight rows and
+// columns, biases and timesteps are zero, so padded activations stay zero.
+void SeaModel::build_mixed(const dpmi_sea_model_desc& d) {
+  if (nlayers < 2)
+    mixed_unsupported = "the mixed engine needs at least one hidden fitting layer";
+  for (int l = 0; l < nlayers; ++l)
+    if (res_mode[l] == 2) mixed_unsupported = "the mixed engine does not support width-doubling residual layers";
+  kpad.resize(nlayers + 1);
+  for (int l = 0; l <= nlayers; ++l) kpad[l] = (width[l] + 63) / 64 * 64;
+  std::vector<float> tf(table_host.begin(), table_host.end());
+  d_table_f = dupload(tf.data(), tf.size());
+  void* infos = make_table_infos(info_host.data(), nnets, true);
+  d_infos_f = dupload(static_cast<const char*>(infos), table_info_bytes(true) * nnets);
+  free_table_infos(infos, true);
+  std::vector<double> shift(ntypes);
+  for (int t = 0; t < ntypes; ++t) shift[t] = out_b_host[t] + bias_atom_e[t];
+  d_shift = dupload(shift.data(), shift.size());
+  if (!mixed_unsupported.empty()) return;
+  const int H = nlayers - 1;
+  f16.resize((size_t)ntypes * H);
+  for (int t = 0; t < ntypes; ++t)
+    for (int l = 0; l < H; ++l) {
+      const size_t k = (size_t)t * nlayers + l;
+      const int ni = width[l], no = width[l + 1], kp = kpad[l], np = kpad[l + 1];
+      std::vector<_Float16> wt((size_t)np * kp, (_Float16)0.f), w((size_t)kp * np, (_Float16)0.f);
+      for (int i = 0; i < ni; ++i)
+        for (int o = 0; o < no; ++o) {
+          const _Float16 v = (_Float16)d.fit_w[k][(size_t)i * no + o];
+          wt[(size_t)o * kp + i] = v;
+          w[(size_t)i * np + o] = v;
+        }
+      std::vector<float> b(np, 0.f), idt(np, 0.f), ow(np, 0.f);
+      for (int o = 0; o < no; ++o) {
+        b[o] = (float)d.fit_b[k][o];
+        if (d.fit_idt[k]) idt[o] = (float)d.fit_idt[k][o];
+      }
+      Fit16Layer& L = f16[(size_t)t * H + l];
+      L.wt = dupload(wt.data(), wt.size());
+      L.w = dupload(w.data(), w.size());
+      L.b = dupload(b.data(), b.size());
+      L.idt = d.fit_idt[k] ? dupload(idt.data(), idt.size()) : nullptr;
+      if (l == H - 1) {
+        for (int o = 0; o < no; ++o) ow[o] = (float)out_w_host[t][o];
+        L.out_w = dupload(ow.data(), ow.size());
+      }
+      L.act = act[l];
+      L.res_mode = res_mode[l];
+      L.k_pad = kp;
+      L.n_pad = np;
+      for (const void* p : {(const void*)L.wt, (const void*)L.w, (const void*)L.b, (const void*)L.idt,
+                            (const void*)L.out_w})
+        if (p) f16_mem.push_back(const_cast<void*>(p));
+    }
 }
 
 SeaModel::~SeaModel() {
@@ -142,13 +199,18 @@ SeaModel::~SeaModel() {
   for (auto& p : fit_w) dfree(p);
   for (auto& p : fit_b) dfree(p);
   for (auto& p : fit_idt) dfree(p);
+  dfree(d_table_f);
+  dfree(d_infos_f);
+  dfree(d_shift);
+  for (auto& p : f16_mem) dfree(p);
 }
 
 // ---------------------------------------------------------------- engine --
 SeaEngine::SeaEngine(const SeaModel* m, int nframes_, const int* natoms_h, const int* atype_h, int precision_,
                      double nall_per_atom_)
     : model(m), nframes(nframes_), precision(precision_), nall_per_atom(nall_per_atom_) {
-  if (precision != DPMI_FP64) throw std::invalid_argument("only DPMI_FP64 is available in this build");
+  if (precision != DPMI_FP64 && precision != DPMI_MIXED) throw std::invalid_argument("unknown precision");
+  if (precision == DPMI_MIXED && !m->mixed_unsupported.empty()) throw Unsupported(m->mixed_unsupported);
   natoms.assign(natoms_h, natoms_h + nframes);
   loc_off.assign(nframes + 1, 0);
   for (int f = 0; f < nframes; ++f) loc_off[f + 1] = loc_off[f] + natoms[f];
@@ -192,16 +254,47 @@ SeaEngine::SeaEngine(const SeaModel* m, int nframes_, const int* natoms_h, const
   d_ncand = dalloc<int>(ntot);
   d_cell_of = dalloc<int>(ntot);
   d_sorted_loc = dalloc<int>(ntot);
-  d_em = dalloc<double>(nn * 4);
-  d_em_deriv = dalloc<double>(nn * 12);
-  d_rij = dalloc<double>(nn * 3);
-  d_net_deriv = dalloc<double>(nn * 4);
-  d_xyz = dalloc<double>((size_t)ntot * 4 * m->M);
-  d_gxyz = dalloc<double>((size_t)ntot * 4 * m->M);
   const int D = m->M * m->A;
-  d_D = dalloc<double>((size_t)ntot * D);
-  d_gD = dalloc<double>((size_t)ntot * D);
-  DPMI_CHECK(hipMemset(d_gD, 0, sizeof(double) * (size_t)ntot * D));
+  if (precision == DPMI_FP64) {
+    d_em = dalloc<double>(nn * 4);
+    d_em_deriv = dalloc<double>(nn * 12);
+    d_rij = dalloc<double>(nn * 3);
+    d_net_deriv = dalloc<double>(nn * 4);
+    d_xyz = dalloc<double>((size_t)ntot * 4 * m->M);
+    d_D = dalloc<double>((size_t)ntot * D);
+    d_gD = dalloc<double>((size_t)ntot * D);
+    DPMI_CHECK(hipMemset(d_gD, 0, sizeof(double) * (size_t)ntot * D));
+    for (int l = 0; l < m->nlayers; ++l) {
+      d_fz.push_back(dalloc<double>((size_t)ntot * m->width[l + 1]));
+      d_fy.push_back(dalloc<double>((size_t)ntot * m->width[l + 1]));
+      d_fx.push_back(dalloc<double>((size_t)ntot * m->width[l + 1]));
+    }
+    int wmax = 0;
+    for (int w : m->width) wmax = std::max(wmax, w);
+    d_g0 = dalloc<double>((size_t)ntot * wmax);
+    d_g1 = dalloc<double>((size_t)ntot * wmax);
+  } else {
+    d_em_f = dalloc<float>(nn * 4);
+    d_em_deriv_f = dalloc<float>(nn * 12);
+    d_rij_f = dalloc<float>(nn * 3);
+    d_net_deriv_f = dalloc<float>(nn * 4);
+    d_xyz_f = dalloc<float>((size_t)ntot * 4 * m->M);
+    const int k0 = m->kpad[0];
+    // padded descriptor columns are never written and must stay zero
+    d_Dh = dalloc<_Float16>((size_t)ntot * k0);
+    DPMI_CHECK(hipMemset(d_Dh, 0, sizeof(_Float16) * (size_t)ntot * k0));
+    d_gD_f = dalloc<float>((size_t)ntot * k0);
+    d_eperm = dalloc<float>(ntot);
+    int kmax = 0;
+    for (int l = 0; l + 1 < m->nlayers; ++l) {
+      const int np = m->kpad[l + 1];
+      kmax = std::max(kmax, np);
+      d_y16.push_back(dalloc<float>((size_t)ntot * np));
+      d_x16.push_back(l + 2 < m->nlayers ? dalloc<_Float16>((size_t)ntot * np) : nullptr);
+    }
+    d_ga = dalloc<float>((size_t)ntot * kmax);
+    d_gb = dalloc<float>((size_t)ntot * kmax);
+  }
   d_atom_e = dalloc<double>(ntot);
   d_vpart = dalloc<double>((size_t)ntot * 9);
   {
@@ -212,15 +305,6 @@ SeaEngine::SeaEngine(const SeaModel* m, int nframes_, const int* natoms_h, const
     d_chunk_start = dupload(cst.data(), cst.size());
     d_part = dalloc<double>((size_t)std::max(nchunks, 1) * 10);
   }
-  for (int l = 0; l < m->nlayers; ++l) {
-    d_fz.push_back(dalloc<double>((size_t)ntot * m->width[l + 1]));
-    d_fy.push_back(dalloc<double>((size_t)ntot * m->width[l + 1]));
-    d_fx.push_back(dalloc<double>((size_t)ntot * m->width[l + 1]));
-  }
-  int wmax = 0;
-  for (int w : m->width) wmax = std::max(wmax, w);
-  d_g0 = dalloc<double>((size_t)ntot * wmax);
-  d_g1 = dalloc<double>((size_t)ntot * wmax);
   d_outw.resize(nt, nullptr);
   for (int t = 0; t < nt; ++t) d_outw[t] = dupload(m->out_w_host[t].data(), m->out_w_host[t].size());
   rocblas_handle h;
@@ -237,9 +321,14 @@ SeaEngine::~SeaEngine() {
                   &d_cell_fill, &d_xcnt, &d_xstart, &d_xgcnt, &d_xgstart, &d_type_ext, &d_map_ext, &d_cell_atoms,
                   &d_xframe_of_ext, &d_chunks, &d_chunk_start})
     dfree(*p);
-  for (auto* p : {&d_em, &d_em_deriv, &d_rij, &d_net_deriv, &d_xyz, &d_gxyz, &d_D, &d_gD, &d_atom_e, &d_vpart,
+  for (auto* p : {&d_em, &d_em_deriv, &d_rij, &d_net_deriv, &d_xyz, &d_D, &d_gD, &d_atom_e, &d_vpart,
                   &d_coord_ext, &d_force_ext, &d_g0, &d_g1, &d_part})
     dfree(*p);
+  for (auto* p : {&d_em_f, &d_em_deriv_f, &d_rij_f, &d_net_deriv_f, &d_xyz_f, &d_gD_f, &d_eperm, &d_ga, &d_gb})
+    dfree(*p);
+  dfree(d_Dh);
+  for (auto& p : d_y16) dfree(p);
+  for (auto& p : d_x16) dfree(p);
   dfree(d_geom);
   for (auto& p : d_fz) dfree(p);
   for (auto& p : d_fy) dfree(p);
@@ -309,6 +398,186 @@ void SeaEngine::prepare_geometry(const double* box_h) {
     nx_tot = nxcells;
     next_tot = next;
   }
+}
+
+EnvArgs SeaEngine::env_args() const {
+  const SeaModel& m = *model;
+  EnvArgs ea;
+  ea.coord = d_coord_ext;
+  ea.type = d_type_ext;
+  ea.nlist = d_nlist;
+  ea.row_frame = d_frame_of;
+  ea.row_ext_off = d_frame_ext_off;
+  ea.row_center = d_center;
+  ea.nrows = ntot;
+  ea.nnei = m.nnei;
+  ea.avg = m.d_davg;
+  ea.std_ = m.d_dstd;
+  ea.rmin = (double)m.rcut_smth;
+  ea.rmax = (double)m.rcut;
+  return ea;
+}
+
+EmbedArgs SeaEngine::embed_args() const {
+  const SeaModel& m = *model;
+  EmbedArgs ta;
+  ta.pos = d_pos;
+  ta.tab_off = m.d_tab_off;
+  ta.net_of = m.d_net_of;
+  ta.row_type = d_type;
+  ta.sec = m.d_sec;
+  ta.ntypes = m.ntypes;
+  ta.nrows = ntot;
+  ta.nnei = m.nnei;
+  ta.M = m.M;
+  ta.A = m.A;
+  ta.inv_nnei = 1.0 / m.nnei;
+  return ta;
+}
+
+// FP64 stages env_mat .. embed_bwd: rocBLAS DGEMM fitting per type block.
+void SeaEngine::compute_fp64_descriptor_fitting(hipStream_t st, int& stage) {
+  const SeaModel& m = *model;
+  EnvArgs ea = env_args();
+  ea.em = d_em;
+  ea.em_deriv = d_em_deriv;
+  ea.rij = d_rij;
+  launch_env_mat_a(ea, st);
+  mark(st, stage++);
+
+  EmbedArgs ta = embed_args();
+  ta.xyz = d_xyz;
+  ta.D = d_D;
+  ta.gD = d_gD;
+  ta.net_deriv = d_net_deriv;
+  ta.tables = m.d_table;
+  ta.infos = m.d_infos;
+  ta.em = d_em;
+  launch_embed_fwd<double, double>(ta, st);
+  mark(st, stage++);
+
+  // fitting forward, one row block per type
+  rocblas_handle h = (rocblas_handle)blas;
+  rb_check(rocblas_set_stream(h, st), "set_stream");
+  const double one = 1.0, zero = 0.0;
+  launch_fill_double(d_atom_e, 0.0, ntot, st);
+  const int L = m.nlayers;
+  for (int t = 0; t < m.ntypes; ++t) {
+    const long long p0 = tstart[t], n = tstart[t + 1] - tstart[t];
+    if (n == 0) continue;
+    const double* x = d_D + (size_t)p0 * m.width[0];
+    for (int l = 0; l < L; ++l) {
+      const int ni = m.width[l], no = m.width[l + 1];
+      const size_t k = (size_t)t * L + l;
+      double* z = d_fz[l] + (size_t)p0 * no;
+      rb_check(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, no, (int)n, ni, &one, m.fit_w[k], no,
+                             x, ni, &zero, z, no),
+               "dgemm fwd");
+      if (l < L - 1) {
+        launch_fit_act(z, m.fit_b[k], m.fit_idt[k], x, m.act[l], m.res_mode[l], n, ni, no, d_fy[l] + (size_t)p0 * no,
+                       d_fx[l] + (size_t)p0 * no, st);
+        x = d_fx[l] + (size_t)p0 * no;
+      } else {
+        launch_fit_out(z, m.out_b_host[t], m.bias_atom_e[t], d_perm, p0, n, d_atom_e, st);
+      }
+    }
+  }
+  mark(st, stage++);
+
+  // fitting backward: dE/dD
+  for (int t = 0; t < m.ntypes; ++t) {
+    const long long p0 = tstart[t], n = tstart[t + 1] - tstart[t];
+    if (n == 0) continue;
+    // g = dE/d(x_{L-1}) = w_out broadcast over rows
+    const int wl = m.width[L - 1];
+    double* g = d_g0;
+    double* gn = d_g1;
+    launch_fill_rows(g, d_outw[t], n, wl, st);
+    for (int l = L - 2; l >= 0; --l) {
+      const int ni = m.width[l], no = m.width[l + 1];
+      const size_t k = (size_t)t * L + l;
+      double* gz = d_fz[l] + (size_t)p0 * no;  // reuse z storage for g_z
+      launch_fit_act_bwd(g, d_fy[l] + (size_t)p0 * no, m.fit_idt[k], m.act[l], n, no, gz, st);
+      double* gin = (l == 0) ? d_gD + (size_t)p0 * ni : gn;
+      rb_check(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, ni, (int)n, no, &one,
+                             m.fit_w[k], no, gz, no, &zero, gin, ni),
+               "dgemm bwd");
+      launch_fit_res_bwd(g, m.res_mode[l], n, ni, no, gin, st);
+      if (l > 0) std::swap(g, gn);
+    }
+  }
+  mark(st, stage++);
+
+  launch_embed_bwd<double, double>(ta, st);
+  mark(st, stage++);
+
+}
+
+// Mixed stages env_mat .. embed_bwd: FP32 env matrix (FP64 geometry), FP32
+// table embedding and contraction, FP16 descriptor, FP16 MFMA fitting GEMMs
+// with FP32 accumulation and FP32 activations/gradients.
+void SeaEngine::compute_mixed_descriptor_fitting(hipStream_t st, int& stage) {
+  const SeaModel& m = *model;
+  launch_env_mat_a_f32(env_args(), d_em_f, d_em_deriv_f, d_rij_f, st);
+  mark(st, stage++);
+
+  const int k0 = m.kpad[0];
+  EmbedArgs ta = embed_args();
+  ta.xyz = d_xyz_f;
+  ta.D = d_Dh;
+  ta.gD = d_gD_f;
+  ta.net_deriv = d_net_deriv_f;
+  ta.tables = m.d_table_f;
+  ta.infos = m.d_infos_f;
+  ta.em = d_em_f;
+  ta.ldD = k0;
+  launch_embed_fwd<float, _Float16>(ta, st);
+  mark(st, stage++);
+
+  const int H = m.nlayers - 1;
+  DPMI_CHECK(hipMemsetAsync(d_eperm, 0, sizeof(float) * (size_t)ntot, st));
+  for (int t = 0; t < m.ntypes; ++t) {
+    const long long p0 = tstart[t], n = tstart[t + 1] - tstart[t];
+    if (n == 0) continue;
+    const _Float16* x = d_Dh + (size_t)p0 * k0;
+    int ldx = k0;
+    for (int l = 0; l < H; ++l) {
+      const Fit16Layer& L = m.f16[(size_t)t * H + l];
+      const int ldo = m.kpad[l + 1];
+      const bool last = l == H - 1;
+      _Float16* xo = last ? nullptr : d_x16[l] + (size_t)p0 * ldo;
+      launch_fit16_fwd(L, x, ldx, (int)n, L.res_mode == 1 ? x : nullptr, ldx, d_y16[l] + (size_t)p0 * ldo, ldo, xo,
+                       ldo, last ? d_eperm + p0 : nullptr, st);
+      x = xo;
+      ldx = ldo;
+    }
+  }
+  launch_energy_out(d_eperm, d_pos, d_type, m.d_shift, ntot, d_atom_e, st);
+  mark(st, stage++);
+
+  for (int t = 0; t < m.ntypes; ++t) {
+    const long long p0 = tstart[t], n = tstart[t + 1] - tstart[t];
+    if (n == 0) continue;
+    const float* g = nullptr;  // dE/dx of the current layer output
+    int ldg = 0;
+    float* bufs[2] = {d_ga, d_gb};
+    int nb = 0;
+    for (int l = H - 1; l >= 0; --l) {
+      const Fit16Layer& L = m.f16[(size_t)t * H + l];
+      const int ldo = m.kpad[l + 1], ldi = m.kpad[l];
+      const bool top = l == H - 1;
+      float* gin = l == 0 ? d_gD_f + (size_t)p0 * k0 : bufs[nb] + (size_t)p0 * ldi;
+      launch_fit16_bwd(L, top ? L.out_w : g, ldg, top, d_y16[l] + (size_t)p0 * ldo, ldo, (int)n, L.res_mode == 1,
+                       gin, ldi, st);
+      g = gin;
+      ldg = ldi;
+      nb ^= 1;
+    }
+  }
+  mark(st, stage++);
+
+  launch_embed_bwd<float, float>(ta, st);
+  mark(st, stage++);
 }
 
 // Stage events are kept per call (up to kMaxTimedCalls) so that a timed loop
@@ -388,107 +657,23 @@ void SeaEngine::compute(const double* coord, const double* box_h, double* energy
   launch_cell_nlist(w, na, st);
   mark(st, stage++);
 
-  EnvArgs ea;
-  ea.coord = d_coord_ext;
-  ea.type = d_type_ext;
-  ea.nlist = d_nlist;
-  ea.row_frame = d_frame_of;
-  ea.row_ext_off = d_frame_ext_off;
-  ea.row_center = d_center;
-  ea.nrows = ntot;
-  ea.nnei = m.nnei;
-  ea.avg = m.d_davg;
-  ea.std_ = m.d_dstd;
-  ea.rmin = (double)m.rcut_smth;
-  ea.rmax = (double)m.rcut;
-  ea.em = d_em;
-  ea.em_deriv = d_em_deriv;
-  ea.rij = d_rij;
-  launch_env_mat_a(ea, st);
-  mark(st, stage++);
-
-  EmbedArgs ta;
-  ta.xyz = d_xyz;
-  ta.D = d_D;
-  ta.gD = d_gD;
-  ta.net_deriv = d_net_deriv;
-  ta.pos = d_pos;
-  ta.tables = m.d_table;
-  ta.tab_off = m.d_tab_off;
-  ta.infos = m.d_infos;
-  ta.net_of = m.d_net_of;
-  ta.em = d_em;
-  ta.row_type = d_type;
-  ta.sec = m.d_sec;
-  ta.ntypes = m.ntypes;
-  ta.nrows = ntot;
-  ta.nnei = m.nnei;
-  ta.M = m.M;
-  ta.A = m.A;
-  ta.inv_nnei = 1.0 / m.nnei;
-  launch_embed_fwd<double, double>(ta, st);
-  mark(st, stage++);
-
-  // fitting forward, one row block per type
-  rocblas_handle h = (rocblas_handle)blas;
-  rb_check(rocblas_set_stream(h, st), "set_stream");
-  const double one = 1.0, zero = 0.0;
-  launch_fill_double(d_atom_e, 0.0, ntot, st);
-  const int L = m.nlayers;
-  for (int t = 0; t < m.ntypes; ++t) {
-    const long long p0 = tstart[t], n = tstart[t + 1] - tstart[t];
-    if (n == 0) continue;
-    const double* x = d_D + (size_t)p0 * m.width[0];
-    for (int l = 0; l < L; ++l) {
-      const int ni = m.width[l], no = m.width[l + 1];
-      const size_t k = (size_t)t * L + l;
-      double* z = d_fz[l] + (size_t)p0 * no;
-      rb_check(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, no, (int)n, ni, &one, m.fit_w[k], no,
-                             x, ni, &zero, z, no),
-               "dgemm fwd");
-      if (l < L - 1) {
-        launch_fit_act(z, m.fit_b[k], m.fit_idt[k], x, m.act[l], m.res_mode[l], n, ni, no, d_fy[l] + (size_t)p0 * no,
-                       d_fx[l] + (size_t)p0 * no, st);
-        x = d_fx[l] + (size_t)p0 * no;
-      } else {
-        launch_fit_out(z, m.out_b_host[t], m.bias_atom_e[t], d_perm, p0, n, d_atom_e, st);
-      }
-    }
-  }
-  mark(st, stage++);
-
-  // fitting backward: dE/dD
-  for (int t = 0; t < m.ntypes; ++t) {
-    const long long p0 = tstart[t], n = tstart[t + 1] - tstart[t];
-    if (n == 0) continue;
-    // g = dE/d(x_{L-1}) = w_out broadcast over rows
-    const int wl = m.width[L - 1];
-    double* g = d_g0;
-    double* gn = d_g1;
-    launch_fill_rows(g, d_outw[t], n, wl, st);
-    for (int l = L - 2; l >= 0; --l) {
-      const int ni = m.width[l], no = m.width[l + 1];
-      const size_t k = (size_t)t * L + l;
-      double* gz = d_fz[l] + (size_t)p0 * no;  // reuse z storage for g_z
-      launch_fit_act_bwd(g, d_fy[l] + (size_t)p0 * no, m.fit_idt[k], m.act[l], n, no, gz, st);
-      double* gin = (l == 0) ? d_gD + (size_t)p0 * ni : gn;
-      rb_check(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, ni, (int)n, no, &one,
-                             m.fit_w[k], no, gz, no, &zero, gin, ni),
-               "dgemm bwd");
-      launch_fit_res_bwd(g, m.res_mode[l], n, ni, no, gin, st);
-      if (l > 0) std::swap(g, gn);
-    }
-  }
-  mark(st, stage++);
-
-  launch_embed_bwd<double, double>(ta, st);
-  mark(st, stage++);
+  if (precision == DPMI_FP64)
+    compute_fp64_descriptor_fitting(st, stage);
+  else
+    compute_mixed_descriptor_fitting(st, stage);
 
   DPMI_CHECK(hipMemsetAsync(d_force_ext, 0, sizeof(double) * (size_t)next_tot * 3, st));
   ForceRowsArgs fa;
-  fa.net_deriv = d_net_deriv;
-  fa.em_deriv = d_em_deriv;
-  fa.rij = d_rij;
+  if (precision == DPMI_FP64) {
+    fa.net_deriv = d_net_deriv;
+    fa.em_deriv = d_em_deriv;
+    fa.rij = d_rij;
+  } else {
+    fa.net_deriv = d_net_deriv_f;
+    fa.em_deriv = d_em_deriv_f;
+    fa.rij = d_rij_f;
+    fa.fp32 = true;
+  }
   fa.nlist = d_nlist;
   fa.row_frame = d_frame_of;
   fa.row_center = d_center;
@@ -539,21 +724,45 @@ long long SeaEngine::fetch(int frame, int what, void* dst) {
     case 2: src = d_type_ext + g.ext_off; n = nall; es = 4; break;
     case 3: src = d_map_ext + g.ext_off; n = nall; es = 4; break;
     case 4: src = d_nlist + r0 * m.nnei; n = (long long)nloc * m.nnei; es = 4; break;
-    case 5: src = d_em + r0 * m.nnei * 4; n = (long long)nloc * m.nnei * 4; es = 8; break;
+    case 5:
+    case 7: {
+      n = (long long)nloc * m.nnei * 4;
+      if (precision == DPMI_FP64) {
+        src = (what == 5 ? d_em : d_net_deriv) + r0 * m.nnei * 4;
+        es = 8;
+        break;
+      }
+      // mixed engine: FP32 storage, returned as double
+      if (dst) {
+        std::vector<float> tmp(n);
+        DPMI_CHECK(hipMemcpy(tmp.data(), (what == 5 ? d_em_f : d_net_deriv_f) + r0 * m.nnei * 4, sizeof(float) * n,
+                             hipMemcpyDeviceToHost));
+        std::copy(tmp.begin(), tmp.end(), static_cast<double*>(dst));
+      }
+      return n;
+    }
     case 6: {
-      // descriptor rows are stored type-permuted
+      // descriptor rows are stored type-permuted (FP16 rows of stride kpad[0]
+      // in the mixed engine)
       const int D = m.M * m.A;
       n = (long long)nloc * D;
       if (dst) {
         std::vector<int> pos(ntot);
         DPMI_CHECK(hipMemcpy(pos.data(), d_pos, sizeof(int) * ntot, hipMemcpyDeviceToHost));
-        for (int i = 0; i < nloc; ++i)
-          DPMI_CHECK(hipMemcpy(static_cast<double*>(dst) + (size_t)i * D, d_D + (size_t)pos[r0 + i] * D,
-                               sizeof(double) * D, hipMemcpyDeviceToHost));
+        std::vector<_Float16> row(D);
+        for (int i = 0; i < nloc; ++i) {
+          double* out = static_cast<double*>(dst) + (size_t)i * D;
+          if (precision == DPMI_FP64) {
+            DPMI_CHECK(hipMemcpy(out, d_D + (size_t)pos[r0 + i] * D, sizeof(double) * D, hipMemcpyDeviceToHost));
+          } else {
+            DPMI_CHECK(hipMemcpy(row.data(), d_Dh + (size_t)pos[r0 + i] * m.kpad[0], sizeof(_Float16) * D,
+                                 hipMemcpyDeviceToHost));
+            for (int k = 0; k < D; ++k) out[k] = (double)row[k];
+          }
+        }
       }
       return n;
     }
-    case 7: src = d_net_deriv + r0 * m.nnei * 4; n = (long long)nloc * m.nnei * 4; es = 8; break;
     case 8: src = d_atom_e + r0; n = nloc; es = 8; break;
     default: throw std::invalid_argument("unknown fetch target");
   }

@@ -3,15 +3,18 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
 #include <vector>
 
 #include "../../include/dpmi.h"
 #include "geometry.h"
+#include "kernels.h"
 
 namespace dpmi {
 
 struct SeaModel {
   explicit SeaModel(const dpmi_sea_model_desc& d);
+  void build_mixed(const dpmi_sea_model_desc& d);
   ~SeaModel();
   int ntypes = 0, nnei = 0, M = 0, A = 0, nnets = 0, nlayers = 0;
   float rcut = 0, rcut_smth = 0;
@@ -29,6 +32,15 @@ struct SeaModel {
   double* d_davg = nullptr;
   double* d_dstd = nullptr;
   std::vector<double*> fit_w, fit_b, fit_idt;  // [ntypes * nlayers]
+  // mixed-precision engine: FP32 tables, FP16 fitting weights padded to
+  // multiples of 64, FP64 energy shift per type (output bias + bias_atom_e)
+  std::string mixed_unsupported;  // empty when DPMI_MIXED can run this model
+  std::vector<int> kpad;          // [nlayers + 1] padded widths
+  float* d_table_f = nullptr;
+  char* d_infos_f = nullptr;
+  double* d_shift = nullptr;
+  std::vector<Fit16Layer> f16;  // [ntypes * (nlayers - 1)] hidden layers
+  std::vector<void*> f16_mem;
 };
 
 struct SeaEngine {
@@ -80,10 +92,20 @@ struct SeaEngine {
   int *d_type_ext = nullptr, *d_map_ext = nullptr, *d_cell_atoms = nullptr, *d_xframe_of_ext = nullptr;
   double *d_coord_ext = nullptr, *d_force_ext = nullptr;
   double *d_em = nullptr, *d_em_deriv = nullptr, *d_rij = nullptr, *d_net_deriv = nullptr;
-  double *d_xyz = nullptr, *d_gxyz = nullptr, *d_D = nullptr, *d_gD = nullptr, *d_atom_e = nullptr,
+  double *d_xyz = nullptr, *d_D = nullptr, *d_gD = nullptr, *d_atom_e = nullptr,
          *d_vpart = nullptr;
   std::vector<double*> d_fz, d_fy, d_fx, d_outw;
   double *d_g0 = nullptr, *d_g1 = nullptr;
+  // mixed engine
+  float *d_em_f = nullptr, *d_em_deriv_f = nullptr, *d_rij_f = nullptr, *d_net_deriv_f = nullptr;
+  float *d_xyz_f = nullptr, *d_gD_f = nullptr, *d_eperm = nullptr, *d_ga = nullptr, *d_gb = nullptr;
+  _Float16* d_Dh = nullptr;
+  std::vector<float*> d_y16;
+  std::vector<_Float16*> d_x16;
+  void compute_mixed_descriptor_fitting(hipStream_t st, int& stage);
+  EnvArgs env_args() const;
+  EmbedArgs embed_args() const;
+  void compute_fp64_descriptor_fitting(hipStream_t st, int& stage);
   int nchunks = 0;
   int *d_chunks = nullptr, *d_chunk_start = nullptr;
   double* d_part = nullptr;

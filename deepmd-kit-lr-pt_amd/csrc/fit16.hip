@@ -5,7 +5,8 @@
 //   forward layer:  z = x W (FP16 x FP16 -> FP32),  y = tanh(z + b),
 //                   x' = y * idt + x (equal widths) -- y kept in FP32 for the
 //                   backward, x' stored in FP16 as the next GEMM input; the
-//                   last hidden layer also accumulates e += x' . w_out
+//                   last hidden layer instead accumulates e += x' . w_out
+//                   from the FP32 x' (its output feeds nothing else)
 //   backward layer: gz = g * idt * (1 - y^2) formed while loading the A tile
 //                   (g from memory, or w_out broadcast for the last hidden
 //                   layer), g_in = gz W^T + g (residual), stored in FP32.
@@ -38,6 +39,7 @@ struct GemmArgs {
   const float* y;     // [rows][ldy] (kAGrad*)
   int ldy;
   const float* a_idt; // [K] or null (kAGrad*)
+  int a_act;          // kAGrad*: 1 = tanh (factor 1 - y^2), 0 = linear
   const _Float16* Bt; // [N][ldb]
   int ldb;
   int rows, N, K;
@@ -90,8 +92,11 @@ __global__ void __launch_bounds__(256) k_fit_gemm(GemmArgs p) {
             const int k = k0 + lk + e;
             float g = (AMODE == kAGradMat) ? p.g[(size_t)r * p.ldg + k] : p.g[k];
             if (p.a_idt) g *= p.a_idt[k];
-            const float yy = p.y[(size_t)r * p.ldy + k];
-            v[e] = (_Float16)(g * (1.f - yy * yy));
+            if (p.a_act) {
+              const float yy = p.y[(size_t)r * p.ldy + k];
+              g *= 1.f - yy * yy;
+            }
+            v[e] = (_Float16)g;
           }
         }
       } else {
@@ -143,9 +148,10 @@ __global__ void __launch_bounds__(256) k_fit_gemm(GemmArgs p) {
           p.y_out[(size_t)r * p.ldyo + n] = v;
           float o = p.idt ? v * p.idt[n] : v;
           if (p.xres) o += (float)p.xres[(size_t)r * p.ldr + n];
-          const _Float16 oh = (_Float16)o;
-          p.x_out[(size_t)r * p.ldxo + n] = oh;
-          if (p.out_w) epart[i][e] += (float)oh * p.out_w[n];
+          if (p.out_w)
+            epart[i][e] += o * p.out_w[n];  // the last hidden output feeds only the energy
+          else
+            p.x_out[(size_t)r * p.ldxo + n] = (_Float16)o;
         } else {
           float o = z;
           if (p.gres) o += p.gres_bcast ? p.gres[n] : p.gres[(size_t)r * p.ldgr + n];
@@ -219,6 +225,7 @@ void launch_fit16_bwd(const Fit16Layer& L, const float* g, int ldg, bool g_bcast
   p.y = y;
   p.ldy = ldy;
   p.a_idt = L.idt;
+  p.a_act = L.act;
   p.Bt = L.w;  // [n_in_pad][n_out_pad]: rows are the outputs of the backward GEMM
   p.ldb = L.n_pad;
   p.rows = rows;

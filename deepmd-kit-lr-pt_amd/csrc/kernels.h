@@ -123,6 +123,7 @@ struct EmbedArgs {
   int nnei = 0;
   int M = 0;
   int A = 0;  // axis_neuron, at most 16
+  int ldD = 0;  // row stride of D and gD (0: M*A)
   double inv_nnei = 0;
 };
 void* make_table_infos(const double* infos_host, int nnets, bool fp32);
@@ -198,7 +199,8 @@ struct Fit16Layer {
   int act = 1, res_mode = 0;
   int k_pad = 0, n_pad = 0;
 };
-// y = act(x W + b) (FP32), xo = y * idt + xres (FP16); e_acc[r] += xo . out_w
+// y = act(x W + b) (FP32), xo = y * idt + xres (FP16); with e_acc (last hidden
+// layer) xo is not stored and e_acc[r] += xo . out_w instead
 void launch_fit16_fwd(const Fit16Layer& L, const _Float16* x, int ldx, int rows, const _Float16* xres, int ldr,
                       float* y, int ldy, _Float16* xo, int ldxo, float* e_acc, hipStream_t st);
 // g_in = (g * idt * (1 - y^2)) W^T + (residual ? g : 0); g_bcast: g is one row [n_pad]

@@ -162,7 +162,10 @@ void dpmi_sea_model_destroy(dpmi_sea_model* model);
 
 /* precision */
 #define DPMI_FP64 0   /* every stage FP64 (the plain path) */
-#define DPMI_MIXED 1  /* FP32 table + FP16/BF16 MFMA contraction and fitting; FP64 image shift, s(r), E/V sums */
+#define DPMI_MIXED 1  /* FP64 image shift and s(r), FP32 env matrix / table embedding / contraction,
+                         FP16 descriptor and fitting GEMMs on MFMA (FP32 accumulate), FP64 force,
+                         virial and energy sums; returns DPMI_ERR_UNSUPPORTED for models with no
+                         hidden fitting layer or width-doubling residual layers */
 
 /* Engine for frames with fixed atom types.  nframes frames, frame f holding
  * natoms_host[f] atoms whose types are atype_host[offset_f ...] (frames

@@ -1,0 +1,151 @@
+"""Mixed-precision engine (DPMI_MIXED).
+
+* Shared FP64 geometry: ghosts and neighbour lists equal the FP64 engine's
+  (which the whole-path tests pin to the oracle) bit for bit.
+* Environment matrix: evaluated in FP64 and stored in FP32, so it equals the
+  FP64 engine's values rounded to FP32 (within one FP32 ulp).
+* Fitting: the atomic energies equal a numpy evaluation of the same
+  arithmetic (FP16 descriptor and weights, FP32/FP64 accumulate, FP16
+  hidden activations) on the engine's own descriptor, which separates the
+  FP16 MFMA GEMMs from the table/contraction stages.
+* End to end against FP64: energy error <= 0.1 meV/atom, force RMSE <= 1
+  meV/A, virial error <= 1 meV/atom (per-component max |dV| / natoms).
+"""
+import numpy as np
+import pytest
+import torch
+
+from dpmi import model as dmodel
+from dpmi import systems
+from dpmi.engine import DeviceModel, Engine
+
+pytestmark = pytest.mark.gpu
+
+E_TOL_MEV_ATOM = 0.1
+F_RMSE_TOL_MEV_A = 1.0
+V_TOL_MEV_ATOM = 1.0
+
+
+@pytest.fixture(scope="module")
+def water():
+    return dmodel.water_model(seed=1)
+
+
+@pytest.fixture(scope="module")
+def water_dev(water):
+    return DeviceModel(water)
+
+
+def _run(dm, natoms, coord, atype, box, precision):
+    eng = Engine(dm, natoms, atype, precision=precision)
+    e, f, v = eng.compute(torch.as_tensor(coord, device="cuda"), box, atom_energy=True)
+    eng.status()
+    return eng, e.cpu().numpy().copy(), f.cpu().numpy().copy(), v.cpu().numpy().copy()
+
+
+def errors(e_m, f_m, v_m, e_d, f_d, v_d, natoms):
+    return {
+        "energy_meV_atom": float(abs(e_m - e_d) / natoms * 1e3),
+        "force_rmse_meV_A": float(np.sqrt(np.mean((f_m - f_d) ** 2)) * 1e3),
+        "virial_meV_atom": float(np.max(np.abs(v_m - v_d)) / natoms * 1e3),
+    }
+
+
+def _fp16(a):
+    return np.asarray(a, np.float64).astype(np.float16).astype(np.float64)
+
+
+def fitting_reference(m, desc16, atype):
+    """Per-atom energy of the mixed fitting arithmetic: x_0 = FP16
+    descriptor, hidden layer y = tanh(x W16 + b), x' = y * idt + x (equal
+    widths), stored in FP16 except the last hidden output, which feeds the
+    output layer in FP32."""
+    fa = m.fitting_arrays()
+    H = len(fa) - 1
+    out = np.zeros(len(atype))
+    for t in range(m.ntypes):
+        rows = np.nonzero(atype == t)[0]
+        if rows.size == 0:
+            continue
+        x = desc16[rows]
+        for li in range(H):
+            W, b, idt, act, res = fa[li]
+            z = x @ _fp16(W[t]) + b[t].astype(np.float32)
+            y = np.tanh(z) if act else z
+            o = y * idt[t].astype(np.float32) if idt is not None else y
+            if res and W.shape[1] == W.shape[2]:
+                o = o + x
+            x = o if li == H - 1 else _fp16(o)
+        W, b, _idt, _act, _res = fa[H]
+        out[rows] = x @ W[t][:, 0].astype(np.float32) + b[t][0] + m.bias_atom_e[t]
+    return out
+
+
+@pytest.mark.parametrize("nmol,seed", [(64, 1), (512, 7)])
+def test_mixed_water_against_fp64(water, water_dev, nmol, seed):
+    print(f"case mixed water nmol={nmol} seed={seed}", flush=True)
+    coord, atype, box = systems.water(nmol, seed=seed)
+    n = len(atype)
+    ed, e_d, f_d, v_d = _run(water_dev, [n], coord, atype, box, "fp64")
+    em, e_m, f_m, v_m = _run(water_dev, [n], coord, atype, box, "mixed")
+    # shared FP64 geometry
+    assert np.array_equal(em.fetch("coord_ext"), ed.fetch("coord_ext"))
+    assert np.array_equal(em.fetch("nlist"), ed.fetch("nlist"))
+    # env matrix: FP64 values rounded once to FP32
+    a, b = em.fetch("em"), ed.fetch("em")
+    assert np.all(np.abs(a - b.astype(np.float32).astype(np.float64)) <= 2.0 ** -23 * np.abs(b) + 1e-30)
+    # descriptor in FP16 (FP32 table and contraction)
+    dm, dd = em.fetch("descriptor"), ed.fetch("descriptor")
+    assert np.max(np.abs(dm - dd)) <= 2e-3 * np.max(np.abs(dd))
+    # fitting GEMMs against the same arithmetic in numpy
+    ref_ae = fitting_reference(water, dm.reshape(n, -1), np.asarray(atype))
+    ae = em.fetch("atom_energy")
+    assert np.max(np.abs(ae - ref_ae)) <= 1e-3 * np.max(np.abs(ref_ae - np.asarray(water.bias_atom_e)[atype])) + 1e-6
+    err = errors(e_m[0], f_m, v_m[0], e_d[0], f_d, v_d[0], n)
+    print("errors vs fp64:", err, flush=True)
+    assert err["energy_meV_atom"] <= E_TOL_MEV_ATOM
+    assert err["force_rmse_meV_A"] <= F_RMSE_TOL_MEV_A
+    assert err["virial_meV_atom"] <= V_TOL_MEV_ATOM
+
+
+def test_mixed_batched_uneven_frames(water, water_dev):
+    frames = systems.uneven_water_frames(6, seed=11, lo=96, hi=900)
+    natoms = [len(a) for (_, a, _) in frames]
+    coord = np.concatenate([c for (c, _, _) in frames])
+    atype = np.concatenate([a for (_, a, _) in frames])
+    box = np.stack([b for (_, _, b) in frames])
+    _, e_d, f_d, v_d = _run(water_dev, natoms, coord, atype, box, "fp64")
+    _, e_m, f_m, v_m = _run(water_dev, natoms, coord, atype, box, "mixed")
+    off = 0
+    for k, n in enumerate(natoms):
+        err = errors(e_m[k], f_m[off:off + n], v_m[k], e_d[k], f_d[off:off + n], v_d[k], n)
+        assert err["energy_meV_atom"] <= E_TOL_MEV_ATOM, (k, err)
+        assert err["force_rmse_meV_A"] <= F_RMSE_TOL_MEV_A, (k, err)
+        assert err["virial_meV_atom"] <= V_TOL_MEV_ATOM, (k, err)
+        off += n
+
+
+def test_mixed_copper_small():
+    m = dmodel.copper_model(seed=2)
+    coord, atype, box = systems.fcc_copper(5, seed=4)
+    dm = DeviceModel(m)
+    n = len(atype)
+    _, e_d, f_d, v_d = _run(dm, [n], coord, atype, box, "fp64")
+    _, e_m, f_m, v_m = _run(dm, [n], coord, atype, box, "mixed")
+    err = errors(e_m[0], f_m, v_m[0], e_d[0], f_d, v_d[0], n)
+    print("copper errors vs fp64:", err, flush=True)
+    assert err["energy_meV_atom"] <= E_TOL_MEV_ATOM
+    assert err["force_rmse_meV_A"] <= F_RMSE_TOL_MEV_A
+    assert err["virial_meV_atom"] <= V_TOL_MEV_ATOM
+
+
+def test_mixed_repeat_is_stable(water, water_dev):
+    """Buffers reused across calls: a second call on the same input gives the
+    same result up to the order of FP32 atomic accumulation."""
+    coord, atype, box = systems.water(64, seed=3)
+    eng = Engine(water_dev, [len(atype)], atype, precision="mixed")
+    x = torch.as_tensor(coord, device="cuda")
+    e1, f1, v1 = [t.cpu().numpy().copy() for t in eng.compute(x, box)]
+    e2, f2, v2 = [t.cpu().numpy().copy() for t in eng.compute(x, box)]
+    assert abs(e1[0] - e2[0]) <= 1e-5 * abs(e1[0])
+    assert np.max(np.abs(f1 - f2)) <= 1e-5 * np.max(np.abs(f1))
