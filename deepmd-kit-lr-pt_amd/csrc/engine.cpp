@@ -49,7 +49,75 @@ void rb_check(rocblas_status s, const char* what) {
   if (s != rocblas_status_success) throw HipError(std::string("rocBLAS error in ") + what + ": " + std::to_string(s));
 }
 
+// (XCD count, compute units) of the current device
+std::pair<int, int> chip_layout() {
+  int dev = 0, n = 1, cus = 1;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeNumberOfXccs, dev) != hipSuccess)
+    n = 1;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 1;
+  return {std::max(n, 1), std::max(cus, 1)};
+}
+
 }  // namespace
+
+// Row tiles of the fused fitting kernel, (type, first row, rows, 0), in launch
+// order.  Workgroups are dealt round-robin to the nx XCDs (block b runs on
+// XCD b % nx), each with its own L2: when a launch fits in one pass, the XCDs
+// are split between types in proportion to their tile counts so that each L2
+// holds the weights of one type; padding slots carry zero rows.
+std::vector<int4> fitting_tile_order(const std::vector<int>& tstart, int ntypes, int rows_per_tile, int nx,
+                                     int cus) {
+  std::vector<std::vector<int4>> per(ntypes);
+  int total = 0, used = 0;
+  for (int t = 0; t < ntypes; ++t) {
+    for (int r = tstart[t]; r < tstart[t + 1]; r += rows_per_tile)
+      per[t].push_back(make_int4(t, r, std::min(rows_per_tile, tstart[t + 1] - r), 0));
+    total += (int)per[t].size();
+    used += per[t].empty() ? 0 : 1;
+  }
+  std::vector<int4> plain;
+  for (auto& v : per) plain.insert(plain.end(), v.begin(), v.end());
+  if (used <= 1 || used > nx || nx <= 1) return plain;
+  // XCDs per type: largest remainder of the proportional share, at least one
+  std::vector<int> nxt(ntypes, 0);
+  int given = 0;
+  for (int t = 0; t < ntypes; ++t)
+    if (!per[t].empty()) {
+      nxt[t] = std::max(1, (int)((long long)nx * per[t].size() / total));
+      given += nxt[t];
+    }
+  while (given > nx) {
+    int best = -1;
+    for (int t = 0; t < ntypes; ++t)
+      if (nxt[t] > 1 && (best < 0 || per[t].size() * nxt[best] < per[best].size() * nxt[t])) best = t;
+    --nxt[best];
+    --given;
+  }
+  while (given < nx) {  // give the next XCD to the type with the most tiles per XCD
+    int best = -1;
+    for (int t = 0; t < ntypes; ++t)
+      if (nxt[t] > 0 && (best < 0 || per[t].size() * nxt[best] > per[best].size() * nxt[t])) best = t;
+    ++nxt[best];
+    ++given;
+  }
+  std::vector<std::vector<int4>> lanes(nx);
+  int x = 0;
+  for (int t = 0; t < ntypes; ++t) {
+    for (size_t k = 0; k < per[t].size(); ++k) lanes[x + (int)(k % nxt[t])].push_back(per[t][k]);
+    x += nxt[t];
+  }
+  size_t depth = 0;
+  for (auto& l : lanes) depth = std::max(depth, l.size());
+  std::vector<int4> out(depth * nx, make_int4(0, 0, 0, 0));
+  for (int xi = 0; xi < nx; ++xi)
+    for (size_t k = 0; k < lanes[xi].size(); ++k) out[k * nx + xi] = lanes[xi][k];
+  // keep the padded order only when it needs no more passes over the
+  // chip (one workgroup per CU) than the plain one
+  const long long per_xcd = std::max(1, cus / nx);
+  const long long passes_out = ((long long)depth + per_xcd - 1) / per_xcd;
+  const long long passes_plain = ((long long)(plain.size() + nx - 1) / nx + per_xcd - 1) / per_xcd;
+  return passes_out <= passes_plain ? out : plain;
+}
 
 // ----------------------------------------------------------------- model --
 SeaModel::SeaModel(const dpmi_sea_model_desc& d) {
@@ -140,8 +208,15 @@ void SeaModel::build_mixed(const dpmi_sea_model_desc& d) {
     mixed_unsupported = "the mixed engine needs at least one hidden fitting layer";
   for (int l = 0; l < nlayers; ++l)
     if (res_mode[l] == 2) mixed_unsupported = "the mixed engine does not support width-doubling residual layers";
+  // fused kernel: at most 4 hidden layers of padded width <= 256 and no
+  // residual on the descriptor layer; its reductions run in steps of 128, so
+  // every width is then padded to a multiple of 128 (else of 64)
+  const int H = nlayers - 1;
+  fused = mixed_unsupported.empty() && H <= 4 && res_mode[0] == 0;
+  for (int l = 1; l <= H && fused; ++l) fused = (width[l] + 127) / 128 * 128 <= 256;
+  const int align = fused ? 128 : 64;
   kpad.resize(nlayers + 1);
-  for (int l = 0; l <= nlayers; ++l) kpad[l] = (width[l] + 63) / 64 * 64;
+  for (int l = 0; l <= nlayers; ++l) kpad[l] = (width[l] + align - 1) / align * align;
   std::vector<float> tf(table_host.begin(), table_host.end());
   d_table_f = dupload(tf.data(), tf.size());
   void* infos = make_table_infos(info_host.data(), nnets, true);
@@ -151,8 +226,8 @@ void SeaModel::build_mixed(const dpmi_sea_model_desc& d) {
   for (int t = 0; t < ntypes; ++t) shift[t] = out_b_host[t] + bias_atom_e[t];
   d_shift = dupload(shift.data(), shift.size());
   if (!mixed_unsupported.empty()) return;
-  const int H = nlayers - 1;
   f16.resize((size_t)ntypes * H);
+  std::vector<std::vector<_Float16>> fw((size_t)ntypes * H), bw((size_t)ntypes * H);
   for (int t = 0; t < ntypes; ++t)
     for (int l = 0; l < H; ++l) {
       const size_t k = (size_t)t * nlayers + l;
@@ -185,7 +260,32 @@ void SeaModel::build_mixed(const dpmi_sea_model_desc& d) {
       for (const void* p : {(const void*)L.wt, (const void*)L.w, (const void*)L.b, (const void*)L.idt,
                             (const void*)L.out_w})
         if (p) f16_mem.push_back(const_cast<void*>(p));
+      fw[(size_t)t * H + l] = std::move(wt);
+      bw[(size_t)t * H + l] = std::move(w);
     }
+  if (!fused) return;
+  std::vector<FusedLayer> fl((size_t)ntypes * H);
+  for (size_t k = 0; k < fl.size(); ++k) {
+    const Fit16Layer& L = f16[k];
+    std::vector<_Float16> sw((size_t)L.n_pad * L.k_pad);
+    swizzle_bfrags(fw[k].data(), L.n_pad, L.k_pad, sw.data());
+    _Float16* wf = dupload(sw.data(), sw.size());
+    swizzle_bfrags(bw[k].data(), L.k_pad, L.n_pad, sw.data());
+    _Float16* wb = dupload(sw.data(), sw.size());
+    f16_mem.push_back(wf);
+    f16_mem.push_back(wb);
+    FusedLayer& F = fl[k];
+    F.wf = wf;
+    F.wb = wb;
+    F.b = L.b;
+    F.idt = L.idt;
+    F.out_w = L.out_w;
+    F.act = L.act;
+    F.res = L.res_mode == 1;
+    F.K = L.k_pad;
+    F.N = L.n_pad;
+  }
+  d_fused = dupload(fl.data(), fl.size());
 }
 
 SeaModel::~SeaModel() {
@@ -202,6 +302,7 @@ SeaModel::~SeaModel() {
   dfree(d_table_f);
   dfree(d_infos_f);
   dfree(d_shift);
+  dfree(d_fused);
   for (auto& p : f16_mem) dfree(p);
 }
 
@@ -294,7 +395,14 @@ SeaEngine::SeaEngine(const SeaModel* m, int nframes_, const int* natoms_h, const
     }
     d_ga = dalloc<float>((size_t)ntot * kmax);
     d_gb = dalloc<float>((size_t)ntot * kmax);
+    const auto chip = chip_layout();
+    const std::vector<int4> tiles = fitting_tile_order(tstart, nt, 64, chip.first, chip.second);
+    ntiles = (int)tiles.size();
+    d_tiles = dupload(tiles.data(), tiles.size());
   }
+  stage_names = precision == DPMI_MIXED && m->fused
+                    ? "geometry;nlist;env_mat;embed_fwd;fitting;embed_bwd;force_virial;reduce"
+                    : "geometry;nlist;env_mat;embed_fwd;fitting_fwd;fitting_bwd;embed_bwd;force_virial;reduce";
   d_atom_e = dalloc<double>(ntot);
   d_vpart = dalloc<double>((size_t)ntot * 9);
   {
@@ -327,6 +435,7 @@ SeaEngine::~SeaEngine() {
   for (auto* p : {&d_em_f, &d_em_deriv_f, &d_rij_f, &d_net_deriv_f, &d_xyz_f, &d_gD_f, &d_eperm, &d_ga, &d_gb})
     dfree(*p);
   dfree(d_Dh);
+  dfree(d_tiles);
   for (auto& p : d_y16) dfree(p);
   for (auto& p : d_x16) dfree(p);
   dfree(d_geom);
@@ -535,6 +644,26 @@ void SeaEngine::compute_mixed_descriptor_fitting(hipStream_t st, int& stage) {
   mark(st, stage++);
 
   const int H = m.nlayers - 1;
+  if (m.fused) {
+    FusedFitArgs fa;
+    fa.D = d_Dh;
+    fa.ldD = k0;
+    fa.gD = d_gD_f;
+    fa.ldG = k0;
+    fa.tiles = d_tiles;
+    fa.layers = m.d_fused;
+    fa.perm = d_perm;
+    fa.atom_e = d_atom_e;
+    fa.shift = m.d_shift;
+    fa.H = H;
+    // atoms of negative type own no tile: their energy is zero
+    if (tstart[m.ntypes + 1] > tstart[m.ntypes]) launch_fill_double(d_atom_e, 0.0, ntot, st);
+    launch_fit_fused(fa, ntiles, st);
+    mark(st, stage++);
+    launch_embed_bwd<float, float>(ta, st);
+    mark(st, stage++);
+    return;
+  }
   DPMI_CHECK(hipMemsetAsync(d_eperm, 0, sizeof(float) * (size_t)ntot, st));
   for (int t = 0; t < m.ntypes; ++t) {
     const long long p0 = tstart[t], n = tstart[t + 1] - tstart[t];
@@ -596,7 +725,6 @@ void SeaEngine::mark(hipStream_t st, int stage) {
   DPMI_CHECK(hipEventRecord(events[k], st));
 }
 
-static const char* kStageNames = "geometry;nlist;env_mat;embed_fwd;fitting_fwd;fitting_bwd;embed_bwd;force_virial;reduce";
 
 void SeaEngine::compute(const double* coord, const double* box_h, double* energy, double* force, double* virial,
                         double* atom_energy, hipStream_t st) {
@@ -786,7 +914,7 @@ int SeaEngine::timing_report(double* ms, int max_stages, char* names, int names_
     ms[s] = acc / timed_calls;
   }
   if (names && names_len > 0) {
-    std::strncpy(names, kStageNames, names_len - 1);
+    std::strncpy(names, stage_names.c_str(), names_len - 1);
     names[names_len - 1] = 0;
   }
   return n;

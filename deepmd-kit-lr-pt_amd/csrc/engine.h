@@ -12,6 +12,9 @@
 
 namespace dpmi {
 
+std::vector<int4> fitting_tile_order(const std::vector<int>& tstart, int ntypes, int rows_per_tile, int nx,
+                                     int cus);
+
 struct SeaModel {
   explicit SeaModel(const dpmi_sea_model_desc& d);
   void build_mixed(const dpmi_sea_model_desc& d);
@@ -41,6 +44,8 @@ struct SeaModel {
   double* d_shift = nullptr;
   std::vector<Fit16Layer> f16;  // [ntypes * (nlayers - 1)] hidden layers
   std::vector<void*> f16_mem;
+  bool fused = false;             // the fused fitting kernel covers this net
+  FusedLayer* d_fused = nullptr;  // [ntypes * (nlayers - 1)]
 };
 
 struct SeaEngine {
@@ -58,6 +63,7 @@ struct SeaEngine {
   double nall_per_atom;
   int ntot = 0;
   std::vector<int> natoms, loc_off, atype, tstart;
+  std::string stage_names;
   bool timing = false;
 
  private:
@@ -100,6 +106,8 @@ struct SeaEngine {
   float *d_em_f = nullptr, *d_em_deriv_f = nullptr, *d_rij_f = nullptr, *d_net_deriv_f = nullptr;
   float *d_xyz_f = nullptr, *d_gD_f = nullptr, *d_eperm = nullptr, *d_ga = nullptr, *d_gb = nullptr;
   _Float16* d_Dh = nullptr;
+  int4* d_tiles = nullptr;
+  int ntiles = 0;
   std::vector<float*> d_y16;
   std::vector<_Float16*> d_x16;
   void compute_mixed_descriptor_fitting(hipStream_t st, int& stage);

@@ -1,0 +1,340 @@
+// Fitting net of the mixed-precision engine as one fused kernel (gfx950).
+//
+// deepmd/pt/model/task/fitting.py:1079-1104 and network/mlp.py:188-219 for
+// one atom type: hidden layers x_{l+1} = act(x_l W_l + b_l) * idt_l (+ x_l
+// when widths are equal), energy e = x_H . w_out + b_out + bias_atom_e, and
+// the adjoint dE/dx_0 back to the descriptor.
+//
+// One workgroup (8 waves) owns a tile of up to 64 descriptor rows of one
+// type and runs every layer forward, the energy, and every layer backward:
+//   * every wave covers all 64 rows and one eighth of the output columns, so
+//     no weight fragment is fetched twice by a workgroup;
+//   * the descriptor tile streams from HBM through a double-buffered LDS
+//     stage; the layer inputs x_l and the backward operands gz_l live in LDS
+//     (FP16); the activation derivatives idt*(1 - y^2) (FP16 pairs) and the
+//     residual gradients stay in registers (a wave owns the same output
+//     positions in every layer);
+//   * weights are pre-swizzled on the host into MFMA fragment order, so one
+//     B fragment (16 columns x 32 reductions) is one 1 KB coalesced read, and
+//     are prefetched kPf k-steps ahead through a register ring;
+//   * products on v_mfma_f32_16x16x32_f16 with FP32 accumulation; bias,
+//     tanh, timestep, residual and the energy dot product in FP32; dE/dD is
+//     written in FP32.
+// Every reduction width is a multiple of 32 kPf (the host pads widths to
+// multiples of 128).  The host orders tiles so that, with workgroups dealt
+// round-robin to the XCDs, each XCD's L2 serves the weights of as few types
+// as possible.
+#include <hip/hip_fp16.h>
+
+#include "dpmi_common.h"
+#include "kernels.h"
+
+namespace dpmi {
+
+namespace {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int kRows = 64;        // rows per workgroup tile
+constexpr int kWaves = 8;
+constexpr int kMaxW = 256;       // widest hidden layer (padded)
+constexpr int kXld = kMaxW + 8;  // LDS row stride of the activation tiles (halves)
+constexpr int kAld = 32 + 8;     // LDS row stride of the descriptor stage (halves)
+constexpr int kPf = 4;           // prefetch distance in k-steps
+
+__device__ __forceinline__ half8 zero8() {
+  half8 v;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = (_Float16)0.f;
+  return v;
+}
+
+// B fragment (k-step kt, 16-column tile ct) of a swizzled weight matrix with
+// NT column tiles: 64 lanes x 8 halves contiguous.
+// (Global address space: a generic load would also count against the LDS
+// wait counter and drain at every LDS wait.)
+typedef __attribute__((address_space(1))) const half8 ghalf8;
+typedef __attribute__((address_space(1))) const half4 ghalf4;
+__device__ __forceinline__ half8 load_bfrag(const _Float16* __restrict__ w, int kt, int NT, int ct, int lane) {
+  return *(ghalf8*)(w + ((size_t)(kt * NT + ct) * 64 + lane) * 8);
+}
+
+// C[64 x (ntw tiles of 16 per wave)] += A[64 x K] B over K = 32 KT, A from an
+// LDS tile (row stride ld), B fragments from a swizzled matrix with NT
+// column tiles; tile j of this wave is column tile ct0 + j (used if < NT).
+// KT is a multiple of kPf.  Loads are unconditional (clamped tile and step
+// indices) so that the prefetch ring is not serialised by the wait counters.
+__device__ __forceinline__ void gemm_lds_a(floatx4 (&acc)[4][2], const _Float16* a_lds, int ld,
+                                           const _Float16* __restrict__ w, int KT, int NT, int ct0, int ntw,
+                                           int lane) {
+  const int lr = lane & 15, lg = lane >> 4;
+  const bool use0 = ntw > 0 && ct0 < NT, use1 = ntw > 1 && ct0 + 1 < NT;
+  const int c0 = min(ct0, NT - 1), c1 = min(ct0 + 1, NT - 1);
+  half8 bq[kPf][2];
+#pragma unroll
+  for (int s = 0; s < kPf; ++s) {
+    bq[s][0] = load_bfrag(w, s, NT, c0, lane);
+    bq[s][1] = load_bfrag(w, s, NT, c1, lane);
+  }
+  for (int kt0 = 0; kt0 < KT; kt0 += kPf) {
+#pragma unroll
+    for (int s = 0; s < kPf; ++s) {
+      const int k = kt0 + s;
+      half8 a[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const half8*>(a_lds + (i * 16 + lr) * ld + k * 32 + lg * 8);
+      if (use0)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][0], acc[i][0], 0, 0, 0);
+      if (use1)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][1], acc[i][1], 0, 0, 0);
+      const int kn = min(k + kPf, KT - 1);
+      bq[s][0] = load_bfrag(w, kn, NT, c0, lane);
+      bq[s][1] = load_bfrag(w, kn, NT, c1, lane);
+    }
+  }
+}
+
+__device__ __forceinline__ void zero_acc(floatx4 (&acc)[4][2]) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+}
+
+template <int H>
+__global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
+  __shared__ _Float16 xs[2][kRows][kXld];
+  __shared__ _Float16 as[2][kRows][kAld];
+  __shared__ float esum[kWaves][kRows];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int4 ti = p.tiles[blockIdx.x];  // type, first row, row count
+  const int t = ti.x, row0 = ti.y, nrows = ti.z;
+  if (nrows <= 0) return;  // padding slot of the XCD-aware tile order
+  const FusedLayer* Ls = p.layers + (size_t)t * H;
+  const int lr = lane & 15, lg = lane >> 4;
+
+  // activation derivative idt * act'(z) of every hidden layer at this wave's
+  // output positions (row tile i, column tile j, rows lg*4 + e), two per register
+  half2v dpk[H][4][2][2];
+  float epart[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) epart[i][e] = 0.f;
+
+  // ------------------------------------------------------------ forward --
+#pragma unroll
+  for (int l = 0; l < H; ++l) {
+    const FusedLayer L = Ls[l];
+    const int KT = L.K / 32, NT = L.N / 16, ntw = (NT + kWaves - 1) / kWaves;
+    const int ct0 = wave * ntw;
+    const int src = (l + 1) & 1, dst = l & 1;  // layer l >= 1 reads xs[src]
+    floatx4 acc[4][2];
+    zero_acc(acc);
+    if (l == 0) {
+      // descriptor rows: 8 bytes per thread per k-step through a register
+      // ring (kPf steps ahead) into the double-buffered LDS stage; rows past
+      // the tile are read from its first row and zeroed
+      const int arow = tid >> 3, acol = (tid & 7) * 4;
+      const bool aval = arow < nrows;
+      const _Float16* dsrc = p.D + (size_t)(row0 + (aval ? arow : 0)) * p.ldD + acol;
+      const bool use0 = ntw > 0 && ct0 < NT, use1 = ntw > 1 && ct0 + 1 < NT;
+      const int c0 = min(ct0, NT - 1), c1 = min(ct0 + 1, NT - 1);
+      const half4 hz = {(_Float16)0.f, (_Float16)0.f, (_Float16)0.f, (_Float16)0.f};
+      half4 ar[kPf];
+      half8 bq[kPf][2];
+#pragma unroll
+      for (int s = 0; s < kPf; ++s) {
+        ar[s] = __builtin_nontemporal_load((ghalf4*)(dsrc + s * 32));
+        bq[s][0] = load_bfrag(L.wf, s, NT, c0, lane);
+        bq[s][1] = load_bfrag(L.wf, s, NT, c1, lane);
+      }
+      *reinterpret_cast<half4*>(&as[0][arow][acol]) = aval ? ar[0] : hz;
+      __syncthreads();
+      for (int kt0 = 0; kt0 < KT; kt0 += kPf) {
+#pragma unroll
+        for (int s = 0; s < kPf; ++s) {
+          const int k = kt0 + s;
+          // stage k + 1 (loaded kPf - 1 steps ago) into the other LDS buffer
+          *reinterpret_cast<half4*>(&as[(k + 1) & 1][arow][acol]) = aval ? ar[(s + 1) % kPf] : hz;
+          half8 a[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const half8*>(&as[k & 1][i * 16 + lr][lg * 8]);
+          if (use0)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+              acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][0], acc[i][0], 0, 0, 0);
+          if (use1)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+              acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][1], acc[i][1], 0, 0, 0);
+          const int kn = min(k + kPf, KT - 1);
+          ar[s] = __builtin_nontemporal_load((ghalf4*)(dsrc + kn * 32));
+          bq[s][0] = load_bfrag(L.wf, kn, NT, c0, lane);
+          bq[s][1] = load_bfrag(L.wf, kn, NT, c1, lane);
+          __syncthreads();
+        }
+      }
+    } else {
+      gemm_lds_a(acc, &xs[src][0][0], kXld, L.wf, KT, NT, ct0, ntw, lane);
+    }
+    // epilogue: lane holds rows lg*4 + e of each 16x16 tile, column lr
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      if (j >= ntw || ct0 + j >= NT) continue;
+      const int n = (ct0 + j) * 16 + lr;
+      const float bn = L.b[n];
+      const float idn = L.idt ? L.idt[n] : 1.f;
+      const float own = (l == H - 1) ? L.out_w[n] : 0.f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        float dv[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int r = i * 16 + lg * 4 + e;
+          const float z = acc[i][j][e] + bn;
+          const float y = L.act ? tanhf(z) : z;
+          dv[e] = L.act ? idn * (1.f - y * y) : idn;
+          float o = y * idn;
+          if (L.res) o += (float)xs[src][r][n];  // equal widths: x_l at the same position
+          if (l == H - 1)
+            epart[i][e] += o * own;
+          else
+            xs[dst][r][n] = (_Float16)o;
+        }
+        dpk[l][i][j][0] = half2v{(_Float16)dv[0], (_Float16)dv[1]};
+        dpk[l][i][j][1] = half2v{(_Float16)dv[2], (_Float16)dv[3]};
+      }
+    }
+    __syncthreads();
+  }
+
+  // energy: row sums over the 16 column lanes, then over the 8 waves
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      float s = epart[i][e];
+      s += __shfl_xor(s, 1, 64);
+      s += __shfl_xor(s, 2, 64);
+      s += __shfl_xor(s, 4, 64);
+      s += __shfl_xor(s, 8, 64);
+      if (lr == 0) esum[wave][i * 16 + lg * 4 + e] = s;
+    }
+  __syncthreads();
+  if (tid < nrows) {
+    float e = 0.f;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) e += esum[w][tid];
+    p.atom_e[p.perm[row0 + tid]] = (double)e + p.shift[t];
+  }
+
+  // ----------------------------------------------------------- backward --
+  // g: dE/d(output of layer l) at this wave's positions; top: w_out
+  float g[4][2][4];
+  {
+    const FusedLayer L = Ls[H - 1];
+    const int NT = L.N / 16, ntw = (NT + kWaves - 1) / kWaves;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int ct = wave * ntw + j;
+      const float w = (j < ntw && ct < NT) ? L.out_w[ct * 16 + lr] : 0.f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) g[i][j][e] = w;
+    }
+  }
+#pragma unroll
+  for (int l = H - 1; l >= 0; --l) {
+    const FusedLayer L = Ls[l];
+    const int NT = L.N / 16, ntw = (NT + kWaves - 1) / kWaves;
+    const int buf = l & 1;
+    // gz = g * idt * act'(z) -> LDS (the A operand, reduction over N)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int ct = wave * ntw + j;
+      if (j >= ntw || ct >= NT) continue;
+      const int n = ct * 16 + lr;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const float d = (float)dpk[l][i][j][e >> 1][e & 1];
+          xs[buf][i * 16 + lg * 4 + e][n] = (_Float16)(g[i][j][e] * d);
+        }
+    }
+    __syncthreads();
+    const int KT = L.N / 32;  // reduction steps
+    const int OT = L.K / 16;  // output column tiles (input width of the layer)
+    if (l > 0) {
+      const int otw = (OT + kWaves - 1) / kWaves;
+      floatx4 acc[4][2];
+      zero_acc(acc);
+      gemm_lds_a(acc, &xs[buf][0][0], kXld, L.wb, KT, OT, wave * otw, otw, lane);
+      // g_l = gz W^T (+ g_{l+1} for residual layers: same positions)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            g[i][j][e] = (j < otw && wave * otw + j < OT ? acc[i][j][e] : 0.f) + (L.res ? g[i][j][e] : 0.f);
+    } else {
+      // descriptor gradient: K0 output columns in chunks of 8 waves x 2 tiles
+      const int nchunk = (OT + 2 * kWaves - 1) / (2 * kWaves);
+      for (int c = 0; c < nchunk; ++c) {
+        const int ot0 = c * 2 * kWaves + wave * 2;
+        floatx4 acc[4][2];
+        zero_acc(acc);
+        gemm_lds_a(acc, &xs[buf][0][0], kXld, L.wb, KT, OT, ot0, 2, lane);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          if (ot0 + j >= OT) continue;
+          const int col = (ot0 + j) * 16 + lr;
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const int r = i * 16 + lg * 4 + e;
+              if (r < nrows) __builtin_nontemporal_store(acc[i][j][e], p.gD + (size_t)(row0 + r) * p.ldG + col);
+            }
+        }
+      }
+    }
+  }
+}
+
+}  // namespace
+
+void launch_fit_fused(const FusedFitArgs& a, int ntiles, hipStream_t st) {
+  if (ntiles <= 0) return;
+  switch (a.H) {
+    case 1: hipLaunchKernelGGL(k_fit_fused<1>, dim3(ntiles), dim3(512), 0, st, a); break;
+    case 2: hipLaunchKernelGGL(k_fit_fused<2>, dim3(ntiles), dim3(512), 0, st, a); break;
+    case 3: hipLaunchKernelGGL(k_fit_fused<3>, dim3(ntiles), dim3(512), 0, st, a); break;
+    case 4: hipLaunchKernelGGL(k_fit_fused<4>, dim3(ntiles), dim3(512), 0, st, a); break;
+    default: throw std::invalid_argument("fused fitting supports 1 to 4 hidden layers");
+  }
+  DPMI_LAUNCH_CHECK();
+}
+
+// Fragment order of B operands: Bt[n][k] (n output column, k reduction) of a
+// [N, K] operand goes to ((k / 32) * (N / 16) + n / 16) * 512 + lane * 8 + e
+// with lane = n % 16 + 16 * ((k % 32) / 8), e = k % 8.
+void swizzle_bfrags(const _Float16* bt, int N, int K, _Float16* out) {
+  const int NT = N / 16;
+  for (int n = 0; n < N; ++n)
+    for (int k = 0; k < K; ++k) {
+      const int lane = n % 16 + 16 * ((k % 32) / 8);
+      out[((size_t)(k / 32) * NT + n / 16) * 512 + lane * 8 + k % 8] = bt[(size_t)n * K + k];
+    }
+}
+
+}  // namespace dpmi

@@ -210,6 +210,34 @@ void launch_fit16_bwd(const Fit16Layer& L, const float* g, int ldg, bool g_bcast
 void launch_energy_out(const float* e_perm, const int* pos, const int* row_type, const double* shift, long long n,
                        double* atom_e, hipStream_t st);
 
+// --------------------------------------------------------- fitfused.hip --
+// Whole fitting net (forward, energy, backward to dE/dD) of the mixed engine
+// in one launch, one workgroup per tile of <= 64 rows of one type.
+struct FusedLayer {
+  const _Float16* wf = nullptr;  // forward B fragments: Bt = W^T [N][K], swizzle_bfrags order
+  const _Float16* wb = nullptr;  // backward B fragments: Bt = W [K][N]
+  const float* b = nullptr;      // [N]
+  const float* idt = nullptr;    // [N] or null
+  const float* out_w = nullptr;  // [N] output weights (last hidden layer)
+  int act = 1, res = 0;
+  int K = 0, N = 0;              // padded input / output widths (multiples of 64, N <= 256)
+};
+struct FusedFitArgs {
+  const _Float16* D = nullptr;  // [rows, ldD] descriptor (FP16), rows in type order
+  int ldD = 0;
+  float* gD = nullptr;          // [rows, ldG] dE/dD out
+  int ldG = 0;
+  const int4* tiles = nullptr;  // [ntiles] (type, first row, row count, 0)
+  const FusedLayer* layers = nullptr;  // [ntypes * H] device
+  const int* perm = nullptr;    // descriptor row -> atom
+  double* atom_e = nullptr;     // [atoms] out
+  const double* shift = nullptr;  // [ntypes] output bias + bias_atom_e
+  int H = 0;                    // hidden layers, 1..4
+};
+void launch_fit_fused(const FusedFitArgs& a, int ntiles, hipStream_t st);
+// host: Bt [N][K] (row-major) -> MFMA B-fragment order (N % 16 == 0, K % 32 == 0)
+void swizzle_bfrags(const _Float16* bt, int N, int K, _Float16* out);
+
 void launch_fill_int(int* p, int v, long long n, hipStream_t st);
 void launch_fill_double(double* p, double v, long long n, hipStream_t st);
 

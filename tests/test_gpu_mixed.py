@@ -149,3 +149,56 @@ def test_mixed_repeat_is_stable(water, water_dev):
     e2, f2, v2 = [t.cpu().numpy().copy() for t in eng.compute(x, box)]
     assert abs(e1[0] - e2[0]) <= 1e-5 * abs(e1[0])
     assert np.max(np.abs(f1 - f2)) <= 1e-5 * np.max(np.abs(f1))
+
+
+def _perturbed(m, seed):
+    """Non-zero fitting biases and atom-energy biases (the random models
+    start with zeros)."""
+    rng = np.random.default_rng(seed)
+    for layers in m.fitting:
+        for layer in layers:
+            layer.b = rng.normal(0.0, 0.1, size=layer.b.shape)
+    m.bias_atom_e = np.array([-93.5, -187.2][: m.ntypes])
+    return m
+
+
+@pytest.mark.parametrize("fit_neuron,resnet_dt,one_side", [
+    ([240, 240, 240], True, False),   # fused kernel, timestep layers
+    ([64, 96], False, True),          # fused kernel, unequal widths (no residual)
+    ([320, 320], True, False),        # wider than the fused kernel: per-layer GEMMs
+    ([100, 100, 100, 100], False, False),  # fused kernel, four hidden layers, padded widths
+])
+def test_mixed_fitting_variants(fit_neuron, resnet_dt, one_side):
+    print(f"case fit={fit_neuron} resnet_dt={resnet_dt} one_side={one_side}", flush=True)
+    m = dmodel.SeAModel.random(2, [46, 92], 6.0, 0.5, [25, 50, 100], 16, fit_neuron, seed=5,
+                               type_one_side=one_side, resnet_dt=resnet_dt)
+    m = _perturbed(m, 6)
+    m.build_tables(0.5)
+    coord, atype, box = systems.water(64, seed=12)
+    n = len(atype)
+    dm = DeviceModel(m)
+    ed, e_d, f_d, v_d = _run(dm, [n], coord, atype, box, "fp64")
+    em, e_m, f_m, v_m = _run(dm, [n], coord, atype, box, "mixed")
+    ref_ae = fitting_reference(m, em.fetch("descriptor").reshape(n, -1), np.asarray(atype))
+    ae = em.fetch("atom_energy")
+    scale = np.max(np.abs(ref_ae - np.asarray(m.bias_atom_e)[atype]))
+    assert np.max(np.abs(ae - ref_ae)) <= 1e-3 * scale + 1e-6
+    err = errors(e_m[0], f_m, v_m[0], e_d[0], f_d, v_d[0], n)
+    print("errors vs fp64:", err, flush=True)
+    assert err["energy_meV_atom"] <= E_TOL_MEV_ATOM
+    assert err["force_rmse_meV_A"] <= F_RMSE_TOL_MEV_A
+    assert err["virial_meV_atom"] <= V_TOL_MEV_ATOM
+
+
+def test_mixed_rejects_width_doubling_residual():
+    """A residual layer of doubled width (concatenated input) is an FP64-only
+    feature: the mixed engine refuses the model instead of mis-evaluating."""
+    from dpmi.lib import DpmiError
+
+    m = dmodel.SeAModel.random(2, [46, 92], 6.0, 0.5, [25, 50, 100], 16, [64, 128], seed=5)
+    m.build_tables(0.5)
+    coord, atype, box = systems.water(8, seed=1)
+    dm = DeviceModel(m)
+    _run(dm, [len(atype)], coord, atype, box, "fp64")
+    with pytest.raises(DpmiError, match="UNSUPPORTED"):
+        Engine(dm, [len(atype)], atype, precision="mixed")
