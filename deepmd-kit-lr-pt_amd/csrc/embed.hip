@@ -12,10 +12,9 @@
 //   dy = dE/dxyz = dE/dT / nnei
 //   net_deriv[g, j, :] = dE/d(em[g, j, :]) through the table (tabulate.cc
 //   :234-341) with the em_x gradient added to component 0.
-// The adjoint loop uses a transposed lane layout: 16 neighbours x 4 column
-// groups per wave, each lane walking every fourth column of its neighbour's
-// table row, so the reduction over columns is one 2-step butterfly per 16
-// neighbours instead of a 6-step butterfly per neighbour.
+// The adjoint keeps the forward's lane layout (lanes over columns, one
+// coalesced table row per neighbour) and reduces over columns with a
+// butterfly transpose over groups of neighbours.
 #include <hip/hip_fp16.h>
 
 #include "dpmi_common.h"
@@ -93,16 +92,45 @@ __global__ void __launch_bounds__(256) k_embed_fwd(T* __restrict__ xyz, TD* __re
   }
 }
 
-// Adjoint of one neighbour block (nb slots) of row g.  dyk: [M][4] in LDS.
-template <typename T>
-__device__ void embed_bwd_block(T* __restrict__ nd, const T* dyk, const T* __restrict__ table, const Info<T>& f,
-                                const T* __restrict__ em, int nb, int M) {
-  if (nb <= 0) return;
+// Sum over the 64 lanes of NV per-lane values w[v] (NV a power of two,
+// 4..64): butterfly transpose -- at step s each lane keeps the half of its
+// values whose index bit s equals its lane bit s and adds the partner's
+// copy of that half -- so lane l ends with the total of value l % NV after
+// log2(NV) exchanges of NV/2, NV/4, ... values instead of NV full reductions.
+template <typename T, int NV>
+__device__ __forceinline__ T transpose_sum(T (&w)[NV]) {
   const int lane = lane_id();
-  const int jl = lane & 15, kg = lane >> 4;
+#pragma unroll
+  for (int s = 0; (1 << s) < NV; ++s) {
+    const bool hi = (lane >> s) & 1;
+#pragma unroll
+    for (int i = 0; i < (NV >> (s + 1)); ++i) {
+      const T keep = hi ? w[2 * i + 1] : w[2 * i];
+      const T send = hi ? w[2 * i] : w[2 * i + 1];
+      w[i] = keep + __shfl_xor(send, 1 << s, kWave);
+    }
+  }
+  T r = w[0];
+#pragma unroll
+  for (int s = 0; (NV << s) < kWave; ++s) r += __shfl_xor(r, NV << s, kWave);
+  return r;
+}
+
+// Adjoint of one neighbour block (nb slots) of row g, lanes over columns k
+// (as the forward, so each neighbour's table row is one coalesced read):
+//   nd[j, c] = sum_k dy[k, c] P_k(x_j) + [c = 0] sum_k P'_k(x_j) sum_c' dy[k, c'] em[j, c']
+// per-lane partial sums of QG neighbours x 4 components are reduced across
+// the wave with transpose_sum, which leaves neighbour (l / 4) component
+// (l % 4) on lane l for one coalesced store.  Sorted-padding tail as the
+// forward: the first tail slot carries weight (nb - jt), later slots zero.
+template <typename T, int KQ>
+__device__ void embed_bwd_block(T* __restrict__ nd, const T (&dy)[KQ][4], const T* __restrict__ table,
+                                const Info<T>& f, const T* __restrict__ em, int nb, int M) {
+  if (nb <= 0) return;
+  constexpr int QG = sizeof(T) == 8 ? 8 : 16;  // neighbours per reduction group
+  const int lane = lane_id();
   const T last = em[(size_t)(nb - 1) * 4];
-  // first neighbour of the sorted zero-padding tail (tabulate.cc:188-201)
-  int jt = nb;
+  int jt = nb;  // first neighbour of the sorted zero-padding tail (tabulate.cc:188-201)
   for (int c0 = 0; c0 < nb; c0 += kWave) {
     const int j = c0 + lane;
     bool tail = false;
@@ -117,41 +145,48 @@ __device__ void embed_bwd_block(T* __restrict__ nd, const T* dyk, const T* __res
     }
   }
   const int jend = jt < nb ? jt + 1 : nb;
-  for (int j0 = 0; j0 < jend; j0 += 16) {
-    const int j = j0 + jl;
-    const bool valid = j < jend;
-    T ll[4] = {0, 0, 0, 0};
-    if (valid)
+  for (int c0 = 0; c0 < jend; c0 += kWave) {
+    Nbr<T, 4> r;
+    load_nbr(r, f, em + (size_t)c0 * 4, 4, em + (size_t)c0 * 4, lane, jend - c0, last, false);
+    const int cnt = min(kWave, jend - c0);
+    for (int q0 = 0; q0 < cnt; q0 += QG) {
+      T w[QG * 4];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) ll[c] = em[(size_t)j * 4 + c];
-    T x = ll[0], delta = 0;
-    int idx = 0;
-    if (valid) locate(f, x, idx, delta);
-    const T* row = table + (size_t)idx * M * 6;
-    T px = 0, pm[4] = {0, 0, 0, 0};
-    for (int k = kg; k < M; k += 4) {
-      T a[6];
-      load6(row + 6 * k, a);
-      T v, d;
-      quintic(a, x, delta, v, d);
-      const T* dy = dyk + 4 * k;
-      const T proj = ll[0] * dy[0] + ll[1] * dy[1] + ll[2] * dy[2] + ll[3] * dy[3];
-      px += d * proj;
+      for (int qq = 0; qq < QG; ++qq) {
+        T p[4] = {0, 0, 0, 0};
+        const int q = q0 + qq;
+        if (q < cnt) {
+          const int idx = bcast(r.idx, q);
+          const T x = bcast(r.x, q), delta = bcast(r.delta, q);
+          T ll[4];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) pm[c] += v * dy[c];
-    }
-    px += __shfl_xor(px, 16, kWave);
-    px += __shfl_xor(px, 32, kWave);
+          for (int c = 0; c < 4; ++c) ll[c] = bcast(r.ll[c], q);
+          const T* row = table + (size_t)idx * M * 6;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      pm[c] += __shfl_xor(pm[c], 16, kWave);
-      pm[c] += __shfl_xor(pm[c], 32, kWave);
-    }
-    if (kg == 0 && valid) {
-      const T scale = (j == jt) ? (T)(nb - jt) : (T)1;
-      nd[(size_t)j * 4 + 0] = (pm[0] + px) * scale;
+          for (int kq = 0; kq < KQ; ++kq) {
+            const int k = lane + kq * kWave;
+            if (k < M) {
+              T a[6];
+              load6(row + 6 * k, a);
+              T v, d;
+              quintic(a, x, delta, v, d);
+              const T proj = ll[0] * dy[kq][0] + ll[1] * dy[kq][1] + ll[2] * dy[kq][2] + ll[3] * dy[kq][3];
+              p[0] += v * dy[kq][0] + d * proj;
 #pragma unroll
-      for (int c = 1; c < 4; ++c) nd[(size_t)j * 4 + c] = pm[c] * scale;
+              for (int c = 1; c < 4; ++c) p[c] += v * dy[kq][c];
+            }
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) w[qq * 4 + c] = p[c];
+      }
+      const T tot = transpose_sum<T, QG * 4>(w);
+      const int qq = (lane >> 2) & (QG - 1);
+      const int j = c0 + q0 + qq;
+      if (lane < QG * 4 && q0 + qq < cnt) {
+        const T scale = (j == jt) ? (T)(nb - jt) : (T)1;
+        nd[(size_t)j * 4 + (lane & 3)] = tot * scale;
+      }
     }
   }
   for (int j = jend + lane; j < nb; j += kWave)
@@ -167,7 +202,6 @@ __global__ void __launch_bounds__(64) k_embed_bwd(T* __restrict__ net_deriv, con
                                                   const T* __restrict__ em, const int* __restrict__ row_type,
                                                   const int* __restrict__ sec, int ntypes, long long nrows, int nnei,
                                                   int M, int A, int ldD, T inv_nnei) {
-  __shared__ T dyk[KQ * kWave * 4];
   const long long g = blockIdx.x;
   const int lane = lane_id();
   if (g >= nrows) return;
@@ -184,54 +218,53 @@ __global__ void __launch_bounds__(64) k_embed_bwd(T* __restrict__ net_deriv, con
 #pragma unroll
     for (int c = 0; c < 4; ++c) Tk[kq][c] = k < M ? xyz[((size_t)g * 4 + c) * M + k] * inv_nnei : (T)0;
   }
-  const TG* G = gD + (size_t)pos[g] * ldD;
-  // sum_a G[k, a] T[c, a]
-  T dT[KQ][4];
+  T ta[16][4];  // T[c, a] for a < A, broadcast
 #pragma unroll
-  for (int kq = 0; kq < KQ; ++kq)
+  for (int a = 0; a < 16; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) ta[a][c] = a < A ? bcast(Tk[0][c], a) : (T)0;
+  const TG* G = gD + (size_t)pos[g] * ldD;
+  // dT[c, k] = sum_a G[k, a] T[c, a];  per-lane terms of S[c, a] = sum_m G[m, a] T[c, m]
+  T dT[KQ][4];
+  T sp[64];
+#pragma unroll
+  for (int v = 0; v < 64; ++v) sp[v] = 0;
+#pragma unroll
+  for (int kq = 0; kq < KQ; ++kq) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) dT[kq][c] = 0;
-  for (int a = 0; a < A; ++a) {
-    T ta[4];
+    const int k = lane + kq * kWave;
+    if (k < M) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) ta[c] = bcast(Tk[0][c], a);
+      for (int a = 0; a < 16; ++a) {
+        if (a < A) {
+          const T gk = (T)G[(size_t)k * A + a];
 #pragma unroll
-    for (int kq = 0; kq < KQ; ++kq) {
-      const int k = lane + kq * kWave;
-      if (k < M) {
-        const T gk = (T)G[(size_t)k * A + a];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) dT[kq][c] += gk * ta[c];
+          for (int c = 0; c < 4; ++c) {
+            dT[kq][c] += gk * ta[a][c];
+            sp[c * 16 + a] += gk * Tk[kq][c];
+          }
+        }
       }
     }
   }
-  // S[c, a] = sum_m G[m, a] T[c, m] on lane c*A + a (A <= 16)
-  T S = 0;
-  {
-    const int c = lane / A, a = lane - (lane / A) * A;
-    if (c < 4) {
-      const T* xr = xyz + ((size_t)g * 4 + c) * M;
-      for (int m = 0; m < M; ++m) S += (T)G[(size_t)m * A + a] * xr[m];
-      S *= inv_nnei;
-    }
-  }
+  // S[c, a] lands on lane c*16 + a; lane a < A adds S[c, a] to dT[c, a]
+  const T S = transpose_sum<T, 64>(sp);  // Tk already carries 1 / nnei
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    const T s = __shfl(S, c * A + (lane < A ? lane : 0), kWave);
-    if (lane < A) dT[0][c] += s;
+    const T sv = __shfl(S, c * 16 + (lane & 15), kWave);
+    if (lane < A) dT[0][c] += sv;
   }
+  T dy[KQ][4];
 #pragma unroll
-  for (int kq = 0; kq < KQ; ++kq) {
-    const int k = lane + kq * kWave;
+  for (int kq = 0; kq < KQ; ++kq)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) dyk[k * 4 + c] = dT[kq][c] * inv_nnei;
-  }
-  __syncthreads();
+    for (int c = 0; c < 4; ++c) dy[kq][c] = dT[kq][c] * inv_nnei;
   for (int tj = 0; tj < ntypes; ++tj) {
     const int net = net_of[ti * ntypes + tj];
     const int b = sec[tj], nb = sec[tj + 1] - sec[tj];
-    embed_bwd_block<T>(net_deriv + ((size_t)g * nnei + b) * 4, dyk, tables + tab_off[net], infos[net],
-                       em + ((size_t)g * nnei + b) * 4, nb, M);
+    embed_bwd_block<T, KQ>(net_deriv + ((size_t)g * nnei + b) * 4, dy, tables + tab_off[net], infos[net],
+                           em + ((size_t)g * nnei + b) * 4, nb, M);
   }
 }
 
