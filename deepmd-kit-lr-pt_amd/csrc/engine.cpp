@@ -810,6 +810,8 @@ void SeaEngine::compute(const double* coord, const double* box_h, double* energy
   fa.nnei = m.nnei;
   fa.force_ext = d_force_ext;
   fa.vpart = d_vpart;
+  fa.sorted_loc = d_sorted_loc;
+  fa.geom = d_geom;
   launch_force_virial_rows(fa, st);
   DPMI_CHECK(hipMemsetAsync(force, 0, sizeof(double) * (size_t)ntot * 3, st));
   launch_fold_ghost_forces(d_force_ext, d_map_ext, d_xframe_of_ext, d_geom, d_nall, next_tot, force, st);

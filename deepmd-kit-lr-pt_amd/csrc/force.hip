@@ -9,6 +9,8 @@
 // 3-wide rows read as contiguous spans); the scatter to neighbours uses FP64
 // atomics, per-row sums use a wave butterfly, and totals are reduced per row
 // into a partial buffer that a second kernel sums with compensation.
+#include <algorithm>
+
 #include "dpmi_common.h"
 #include "kernels.h"
 
@@ -238,6 +240,99 @@ __global__ void __launch_bounds__(256) k_force_virial_rows(const TI* __restrict_
   }
 }
 
+// Same products with the neighbour scatter pre-summed in LDS.  Device-scope
+// FP64 atomics on a multi-XCD chip are resolved beyond the per-XCD L2s, so
+// their rate, not bandwidth, bounds the direct scatter.  Here one workgroup
+// takes R centre rows in cell order (positions p of the cell-sorted atom
+// list, so the rows' neighbour sets overlap), accumulates every force term
+// into an LDS hash keyed by extended atom (FP64 LDS atomics, linear
+// probing), and then issues one global atomic per distinct atom and
+// component.  R (nnei + 1) <= kHashSize distinct keys at most, so every probe
+// sequence ends.
+constexpr int kHashBits = 12, kHashSize = 1 << kHashBits;
+
+__device__ __forceinline__ int hash_slot(int* keys, int key) {
+  unsigned h = ((unsigned)key * 2654435761u) >> (32 - kHashBits);
+  while (true) {
+    const int prev = atomicCAS(&keys[h], -1, key);
+    if (prev == -1 || prev == key) return (int)h;
+    h = (h + 1) & (kHashSize - 1);
+  }
+}
+
+template <typename TI>
+__global__ void __launch_bounds__(1024) k_force_virial_hash(
+    const TI* __restrict__ nd, const TI* __restrict__ ed, const TI* __restrict__ rij, const int* __restrict__ nlist,
+    const int* __restrict__ row_frame, const int* __restrict__ row_center, const int* __restrict__ frame_ext_off,
+    const FrameGeom* __restrict__ geom, const int* __restrict__ sorted_loc, long long nrows, int nnei, int R,
+    double* __restrict__ force_ext, double* __restrict__ vpart) {
+  __shared__ int hkey[kHashSize];
+  __shared__ double hval[kHashSize][3];
+  for (int h = threadIdx.x; h < kHashSize; h += blockDim.x) {
+    hkey[h] = -1;
+    hval[h][0] = hval[h][1] = hval[h][2] = 0.;
+  }
+  __syncthreads();
+  const int lane = lane_id(), wave = threadIdx.x / kWave;
+  const long long p = (long long)blockIdx.x * R + wave;
+  if (wave < R && p < nrows) {
+    const int f = row_frame[p];
+    const long long g = geom[f].loc_off + sorted_loc[p];  // packed row of the p-th atom in cell order
+    const int eo = frame_ext_off[f];
+    const int i = row_center[g];
+    double fi[3] = {0., 0., 0.};
+    double vs[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) vs[k] = 0.;
+    for (int s = lane; s < nnei; s += kWave) {
+      const size_t b = (size_t)g * nnei + s;
+      const int j = nlist[b];
+      if (j < 0) continue;  // padding slots have zero em_deriv
+      TI n4[4], e12[12], r3[3], fs[3], v[9];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) n4[a] = nd[b * 4 + a];
+#pragma unroll
+      for (int k = 0; k < 12; ++k) e12[k] = ed[b * 12 + k];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) r3[d] = rij[b * 3 + d];
+      slot_force(n4, e12, fs);
+      slot_virial(n4, e12, r3, v);
+      const int h = hash_slot(hkey, eo + j);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        fi[d] -= fs[d];
+        atomicAdd(&hval[h][d], (double)fs[d]);
+      }
+#pragma unroll
+      for (int k = 0; k < 9; ++k) vs[k] += v[k];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) fi[d] = wave_sum(fi[d]);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) vs[k] = wave_sum(vs[k]);
+    if (lane == 0) {
+      const int h = hash_slot(hkey, eo + i);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) atomicAdd(&hval[h][d], fi[d]);
+    }
+    if (lane < 9) {
+      double out = 0.;
+#pragma unroll
+      for (int k = 0; k < 9; ++k)
+        if (lane == k) out = vs[k];
+      vpart[(size_t)g * 9 + lane] = out;
+    }
+  }
+  __syncthreads();
+  for (int h = threadIdx.x; h < kHashSize; h += blockDim.x) {
+    const int key = hkey[h];
+    if (key >= 0) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) atomicAdd(force_ext + (size_t)key * 3 + d, hval[h][d]);
+    }
+  }
+}
+
 __global__ void k_fold_ghosts(const double* __restrict__ force_ext, const int* __restrict__ map_ext,
                               const int* __restrict__ xframe_of_ext, const FrameGeom* __restrict__ geom,
                               const int* __restrict__ nall, long long next_tot, double* __restrict__ force) {
@@ -341,6 +436,20 @@ void launch_prod_virial_grad_a(double* grad_net, const double* grad, const doubl
 
 void launch_force_virial_rows(const ForceRowsArgs& a, hipStream_t st) {
   if (a.nrows <= 0) return;
+  if (a.sorted_loc && a.geom && a.nnei + 1 <= kHashSize) {
+    const int R = std::min(16, kHashSize / (a.nnei + 1));
+    const dim3 grid(ceil_div(a.nrows, R)), block(R * kWave);
+    if (a.fp32)
+      hipLaunchKernelGGL(k_force_virial_hash<float>, grid, block, 0, st, (const float*)a.net_deriv,
+                         (const float*)a.em_deriv, (const float*)a.rij, a.nlist, a.row_frame, a.row_center,
+                         a.frame_ext_off, a.geom, a.sorted_loc, a.nrows, a.nnei, R, a.force_ext, a.vpart);
+    else
+      hipLaunchKernelGGL(k_force_virial_hash<double>, grid, block, 0, st, (const double*)a.net_deriv,
+                         (const double*)a.em_deriv, (const double*)a.rij, a.nlist, a.row_frame, a.row_center,
+                         a.frame_ext_off, a.geom, a.sorted_loc, a.nrows, a.nnei, R, a.force_ext, a.vpart);
+    DPMI_LAUNCH_CHECK();
+    return;
+  }
   if (a.fp32)
     hipLaunchKernelGGL(k_force_virial_rows<float>, dim3(ceil_div(a.nrows * kWave, 256)), dim3(256), 0, st,
                        (const float*)a.net_deriv, (const float*)a.em_deriv, (const float*)a.rij, a.nlist,

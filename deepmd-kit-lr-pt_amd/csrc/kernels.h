@@ -160,6 +160,10 @@ struct ForceRowsArgs {
   int nnei = 0;
   double* force_ext = nullptr;        // [sum ext_cap, 3] (zeroed by caller)
   double* vpart = nullptr;            // [nrows, 9]
+  // cell order of the rows (per-frame local indices, packed like the rows)
+  // and the frame layout; when set, the LDS-aggregated kernel is used
+  const int* sorted_loc = nullptr;
+  const FrameGeom* geom = nullptr;
 };
 void launch_force_virial_rows(const ForceRowsArgs& a, hipStream_t st);
 // force[loc] += force_ext[e] for every extended atom e < nall of each frame
