@@ -151,34 +151,42 @@ __device__ void embed_bwd_block(T* __restrict__ nd, const T (&dy)[KQ][4], const 
     const int cnt = min(kWave, jend - c0);
     for (int q0 = 0; q0 < cnt; q0 += QG) {
       T w[QG * 4];
+      // rows of U neighbours requested before use (clamped, zero-weighted
+      // beyond cnt), as in the forward
+      constexpr int U = KQ <= 2 ? 4 : (KQ <= 4 ? 2 : 1);
 #pragma unroll
-      for (int qq = 0; qq < QG; ++qq) {
-        T p[4] = {0, 0, 0, 0};
-        const int q = q0 + qq;
-        if (q < cnt) {
-          const int idx = bcast(r.idx, q);
+      for (int u0 = 0; u0 < QG; u0 += U) {
+        T a[U][KQ][6];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int q = min(q0 + u0 + u, cnt - 1);
+          const T* row = table + (size_t)bcast(r.idx, q) * M * 6;
+#pragma unroll
+          for (int kq = 0; kq < KQ; ++kq) load6(row + 6 * min(lane + kq * kWave, M - 1), a[u][kq]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int q = min(q0 + u0 + u, cnt - 1);
           const T x = bcast(r.x, q), delta = bcast(r.delta, q);
           T ll[4];
 #pragma unroll
           for (int c = 0; c < 4; ++c) ll[c] = bcast(r.ll[c], q);
-          const T* row = table + (size_t)idx * M * 6;
+          T p[4] = {0, 0, 0, 0};
 #pragma unroll
           for (int kq = 0; kq < KQ; ++kq) {
-            const int k = lane + kq * kWave;
-            if (k < M) {
-              T a[6];
-              load6(row + 6 * k, a);
-              T v, d;
-              quintic(a, x, delta, v, d);
-              const T proj = ll[0] * dy[kq][0] + ll[1] * dy[kq][1] + ll[2] * dy[kq][2] + ll[3] * dy[kq][3];
-              p[0] += v * dy[kq][0] + d * proj;
+            const bool kin = lane + kq * kWave < M;
+            T v, d;
+            quintic(a[u][kq], x, delta, v, d);
+            v = kin ? v : (T)0;
+            d = kin ? d : (T)0;
+            const T proj = ll[0] * dy[kq][0] + ll[1] * dy[kq][1] + ll[2] * dy[kq][2] + ll[3] * dy[kq][3];
+            p[0] += v * dy[kq][0] + d * proj;
 #pragma unroll
-              for (int c = 1; c < 4; ++c) p[c] += v * dy[kq][c];
-            }
+            for (int c = 1; c < 4; ++c) p[c] += v * dy[kq][c];
           }
-        }
 #pragma unroll
-        for (int c = 0; c < 4; ++c) w[qq * 4 + c] = p[c];
+          for (int c = 0; c < 4; ++c) w[(u0 + u) * 4 + c] = p[c];
+        }
       }
       const T tot = transpose_sum<T, QG * 4>(w);
       const int qq = (lane >> 2) & (QG - 1);

@@ -23,7 +23,8 @@ struct SeaModel {
   float rcut = 0, rcut_smth = 0;
   bool type_one_side = false;
   std::vector<int> sel, sec, net_of, width, act, res_mode, has_idt;
-  std::vector<double> davg, dstd, table_host, info_host, bias_atom_e, out_b_host;
+  std::vector<double> davg, dstd, info_host, bias_atom_e, out_b_host;
+  std::vector<double> table_host;  // all nets, rows [M][6] as given
   std::vector<std::vector<double>> out_w_host;
   std::vector<long long> tab_off;
   // device

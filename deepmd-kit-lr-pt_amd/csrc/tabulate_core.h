@@ -58,14 +58,6 @@ __device__ __forceinline__ void load6(const T* p, T* a) {
   for (int q = 0; q < 6; ++q) a[q] = p[q];
 }
 
-template <typename T>
-__device__ __forceinline__ void quintic(const T* a, T x, T delta, T& v, T& d) {
-  // tabulate.cc:122-158
-  d = a[1] + ((T)2 * a[2] + ((T)3 * a[3] + ((T)4 * a[4] + (T)5 * a[5] * x) * x) * x) * x;
-  v = a[0] + (a[1] + (a[2] + (a[3] + (a[4] + a[5] * x) * x) * x) * x) * x;
-  v += d * delta;
-}
-
 // Per-lane neighbour record of a 64-neighbour chunk.
 template <typename T, int ND>
 struct Nbr {
@@ -108,22 +100,32 @@ __device__ __forceinline__ void fwd_block(T (&acc)[KQ][ND], const T* __restrict_
     const unsigned long long tm = __ballot(tail);
     int jend = min(kWave, nnei - c0);
     if (tm) jend = __ffsll((long long)tm);  // up to and including the first tail neighbour
-    for (int q = 0; q < jend; ++q) {
-      const int idx = bcast(r.idx, q);
-      const T x = bcast(r.x, q), delta = bcast(r.delta, q);
-      T ll[ND];
+    // U neighbours per step: every table row of the step is requested before
+    // any is used (addresses clamped to valid rows and columns, out-of-range
+    // terms weighted by zero), so the row loads overlap instead of forming
+    // a load-use chain per neighbour
+    constexpr int U = KQ <= 2 ? 4 : (KQ <= 4 ? 2 : 1);
+    for (int q0 = 0; q0 < jend; q0 += U) {
+      T a[U][KQ][6];
 #pragma unroll
-      for (int m = 0; m < ND; ++m) ll[m] = bcast(r.ll[m], q);
-      const T scale = (tm && q == jend - 1) ? (T)(nnei - (c0 + q)) : (T)1;
-      const T* row = table + (size_t)idx * M * 6;
+      for (int u = 0; u < U; ++u) {
+        const int q = min(q0 + u, jend - 1);
+        const T* row = table + (size_t)bcast(r.idx, q) * M * 6;
 #pragma unroll
-      for (int kq = 0; kq < KQ; ++kq) {
-        const int k = lane + kq * kWave;
-        if (k < M) {
-          T a[6];
-          load6(row + 6 * k, a);
+        for (int kq = 0; kq < KQ; ++kq) load6(row + 6 * min(lane + kq * kWave, M - 1), a[u][kq]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int q = min(q0 + u, jend - 1);
+        const T x = bcast(r.x, q), delta = bcast(r.delta, q);
+        T ll[ND];
+#pragma unroll
+        for (int m = 0; m < ND; ++m) ll[m] = bcast(r.ll[m], q);
+        const T scale = q0 + u >= jend ? (T)0 : ((tm && q == jend - 1) ? (T)(nnei - (c0 + q)) : (T)1);
+#pragma unroll
+        for (int kq = 0; kq < KQ; ++kq) {
           T v, d;
-          quintic(a, x, delta, v, d);
+          quintic(a[u][kq], x, delta, v, d);
           const T w = v * scale;
 #pragma unroll
           for (int m = 0; m < ND; ++m) acc[kq][m] += w * ll[m];
