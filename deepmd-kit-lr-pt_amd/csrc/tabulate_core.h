@@ -58,6 +58,14 @@ __device__ __forceinline__ void load6(const T* p, T* a) {
   for (int q = 0; q < 6; ++q) a[q] = p[q];
 }
 
+template <typename T>
+__device__ __forceinline__ void quintic(const T* a, T x, T delta, T& v, T& d) {
+  // tabulate.cc:122-158
+  d = a[1] + ((T)2 * a[2] + ((T)3 * a[3] + ((T)4 * a[4] + (T)5 * a[5] * x) * x) * x) * x;
+  v = a[0] + (a[1] + (a[2] + (a[3] + (a[4] + a[5] * x) * x) * x) * x) * x;
+  v += d * delta;
+}
+
 // Per-lane neighbour record of a 64-neighbour chunk.
 template <typename T, int ND>
 struct Nbr {
