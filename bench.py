@@ -30,13 +30,18 @@ METRIC = "atom-steps/s (E+F+V), 12k-atom water, mixed vs FP64 GPU; force RMSE vs
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 FP16_PEAK_TFLOPS = 2500.0
+FP32_PEAK_TFLOPS = 157.3
 
 
 def stage_model(m, nloc, nall, precision):
-    """Algorithmic bytes or flops per launch of each engine stage: per local
-    atom figures times nloc.  N = sum(sel), M = embedding width, A = axis,
+    """Algorithmic work per launch of each engine stage: per local atom
+    figures times nloc.  N = sum(sel), M = embedding width, A = axis,
     D = M*A, s = bytes per stored float (8 FP64, 4 mixed), h = bytes per
-    descriptor element (8 FP64, 2 mixed), rho = nall / nloc."""
+    descriptor element (8 FP64, 2 mixed), rho = nall / nloc.  Entries are
+    (bound, amount, peak): "hbm" stages are bytes against HBM bandwidth,
+    "flop" stages are flops against the arithmetic peak of the path (table
+    stages: FP32 / FP64 vector; fitting: FP16 / FP64 matrix).  The table
+    stages also carry their algorithmic bytes."""
     N, M, A = m.nnei, m.M, m.axis_neuron
     D = M * A
     mixed = precision == "mixed"
@@ -44,26 +49,40 @@ def stage_model(m, nloc, nall, precision):
     h = 2 if mixed else 8
     rho = nall / nloc
     fit = [D] + list(m.fit_neuron) + [1]
-    fit_flops = 2 * sum(fit[i] * fit[i + 1] for i in range(len(fit) - 1))
+    fit_fwd = 2 * sum(fit[i] * fit[i + 1] for i in range(len(fit) - 1))
+    fit_bwd = fit_fwd - 2 * fit[-2]  # no product for the output layer
     mm_peak = FP16_PEAK_TFLOPS if mixed else FP64_PEAK_TFLOPS
+    vec_peak = FP32_PEAK_TFLOPS if mixed else FP64_PEAK_TFLOPS
     per_atom = {
         # reads nlist (4N) and extended coordinates, writes em/em_deriv/rij (19 N s)
-        "env_mat": ("hbm", 4 * N + 19 * N * s + 24 * rho, None),
-        # reads em (4 N s), writes xyz (4 M s) and the descriptor (D h)
-        "embed_fwd": ("hbm", 4 * N * s + 4 * M * s + D * h, None),
-        # reads em, xyz, dE/dD (D s); writes net_deriv (4 N s)
-        "embed_bwd": ("hbm", 8 * N * s + 4 * M * s + D * (4 if mixed else 8), None),
+        "env_mat": ("hbm", 4 * N + 19 * N * s + 24 * rho, None, None),
+        # quintic + contraction (18 N M), descriptor (8 M A); reads em, writes xyz and D
+        "embed_fwd": ("flop", 18 * N * M + 8 * M * A, vec_peak, 4 * N * s + 4 * M * s + D * h),
+        # dT from dE/dD (16 M A), table adjoint (36 N M); reads em, xyz, dE/dD, writes net_deriv
+        "embed_bwd": ("flop", 36 * N * M + 16 * M * A, vec_peak, 8 * N * s + 4 * M * s + D * s),
         # reads net_deriv, em_deriv, rij, nlist; scatters forces (FP64)
-        "force_virial": ("hbm", 19 * N * s + 4 * N + 24 * rho + 72, None),
+        "force_virial": ("hbm", 19 * N * s + 4 * N + 24 * rho + 72, None, None),
         # reads the cell lists and ghost coordinates, writes the sorted rows
-        "nlist": ("hbm", 8 * N, None),
-        "fitting_fwd": ("flop", fit_flops, mm_peak),
-        "fitting_bwd": ("flop", fit_flops - 2 * fit[-2], mm_peak),
+        "nlist": ("hbm", 8 * N, None, None),
+        "fitting": ("flop", fit_fwd + fit_bwd, mm_peak, None),
+        "fitting_fwd": ("flop", fit_fwd, mm_peak, None),
+        "fitting_bwd": ("flop", fit_bwd, mm_peak, None),
     }
-    return {k: (b, amt * nloc, pk) for k, (b, amt, pk) in per_atom.items()}
+    return {k: (b, amt * nloc, pk, None if by is None else by * nloc) for k, (b, amt, pk, by) in per_atom.items()}
 
 
-def cpu_baseline(m, coord, atype, box, max_seconds=25.0):
+# HBM bytes per launch (read MiB, write MiB) measured with rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE on this workload, scaled by a 256 MiB copy
+# (profiles/water12k_kernels_and_traffic.txt).
+MEASURED_TRAFFIC_MIB = {
+    "mixed": {"embed_bwd": (129.15, 26.00), "embed_fwd": (30.26, 56.25), "fitting": (57.95, 94.87),
+              "force_virial": (91.00, 19.75), "env_mat": (15.12, 122.91), "nlist": (8.15, 7.12)},
+    "fp64": {"embed_bwd": (264.69, 51.89), "embed_fwd": (67.16, 205.97), "force_virial": (166.85, 19.75),
+             "env_mat": (16.32, 263.49), "nlist": (8.15, 7.12)},
+}
+
+
+def cpu_baseline(m, coord, atype, box, max_seconds=15.0):
     """Reference CPU path (the compiled reference lib where present, else the
     oracle restatement) plus numpy glue for the fitting, on the host cores."""
     from oracle import oracle_lib, ref_available, ref_lib
@@ -74,7 +93,7 @@ def cpu_baseline(m, coord, atype, box, max_seconds=25.0):
     t0 = time.perf_counter()
     r = evaluate(m, coord, atype, box, lib)  # warm-up, also the CPU result
     first = time.perf_counter() - t0
-    n = max(1, min(3, int(max_seconds / max(first, 1e-3))))
+    n = max(1, min(8, int(max_seconds / max(first, 1e-3))))
     t0 = time.perf_counter()
     for _ in range(n):
         evaluate(m, coord, atype, box, lib)
@@ -109,7 +128,7 @@ def time_engine(eng, x, box, steps, warmup):
 def roofline(m, nloc, nall, precision, stages):
     model = stage_model(m, nloc, nall, precision)
     dom = max((k for k in stages if k in model), key=lambda k: stages[k])
-    kind, amount, peak = model[dom]
+    kind, amount, peak, nbytes = model[dom]
     t_ms = stages[dom]
     if kind == "hbm":
         roof = {"kernel": dom, "bound": "hbm", "achieved": amount / (t_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
@@ -118,8 +137,13 @@ def roofline(m, nloc, nall, precision, stages):
         roof = {"kernel": dom, "bound": "mfma", "achieved": amount / (t_ms * 1e-3) / 1e12, "peak": peak,
                 "unit": "TFLOP/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
+    tr = MEASURED_TRAFFIC_MIB.get(precision, {}).get(dom)
+    roof["traffic"] = None if tr is None else (tr[0] + tr[1]) * 1048576.0
     roof["avg_ms"] = t_ms
+    if nbytes is not None:
+        roof["algorithmic_hbm_GBps"] = nbytes / (t_ms * 1e-3) / 1e9
+    if roof["traffic"] is not None:
+        roof["measured_hbm_GBps"] = roof["traffic"] / (t_ms * 1e-3) / 1e9
     return roof
 
 
@@ -132,11 +156,24 @@ def main() -> int:
                     help="path reported as value (the other one is timed beside it)")
     ap.add_argument("--nmol", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run 5 plain device-to-device copies of this many MiB (the reference kernel "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)
         return 2
+    if args.copy_mib > 0:
+        import torch
+
+        src = torch.ones(args.copy_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_mib": args.copy_mib, "copies": 5}), flush=True)
+        return 0
 
     import numpy as np
     import torch
