@@ -287,44 +287,87 @@ int dpmi_prod_env_mat_a_f64(double* em, double* em_deriv, double* rij, int* nlis
 DPMI_TAB_ENTRY(f64, double)
 DPMI_TAB_ENTRY(f32, float)
 
-int dpmi_prod_force_a_f64(double* force, const double* net_deriv, const double* in_deriv, const int* nlist, int nloc,
-                          int nall, int nnei, int nframes, void* stream) {
+extern "C++" {
+namespace {
+
+template <typename T>
+int prod_force_impl(T* force, const T* net_deriv, const T* in_deriv, const int* nlist, int nloc, int nall, int nnei,
+                    int nframes, void* stream) {
   return guarded([&] {
-    dpmi::launch_prod_force_a(force, net_deriv, in_deriv, nlist, nloc, nall, nnei, nframes, S(stream));
+    dpmi::launch_prod_force_a<T>(force, net_deriv, in_deriv, nlist, nloc, nall, nnei, nframes, S(stream));
     return kOk;
   });
 }
 
-int dpmi_prod_virial_a_f64(double* virial, double* atom_virial, const double* net_deriv, const double* env_deriv,
-                           const double* rij, const int* nlist, int nloc, int nall, int nnei, void* stream) {
+template <typename T>
+int prod_virial_impl(T* virial, T* atom_virial, const T* net_deriv, const T* env_deriv, const T* rij,
+                     const int* nlist, int nloc, int nall, int nnei, void* stream) {
   return guarded([&] {
     hipStream_t st = S(stream);
     double* vpart = static_cast<double*>(scratch().get("virial_part", sizeof(double) * 9 * (size_t)std::max(nloc, 1)));
-    dpmi::launch_prod_virial_a(vpart, atom_virial, net_deriv, env_deriv, rij, nlist, nloc, nall, nnei, st);
+    dpmi::launch_prod_virial_a<T>(vpart, atom_virial, net_deriv, env_deriv, rij, nlist, nloc, nall, nnei, st);
     if (nloc > 0)
-      dpmi::launch_reduce_rows9(virial, vpart, nloc, st);
+      dpmi::launch_reduce_rows9<T>(virial, vpart, nloc, st);
     else
-      DPMI_CHECK(hipMemsetAsync(virial, 0, sizeof(double) * 9, st));
+      DPMI_CHECK(hipMemsetAsync(virial, 0, sizeof(T) * 9, st));
     // the shared partial buffer is reused by the next call: finish here
     DPMI_CHECK(hipStreamSynchronize(st));
     return kOk;
   });
 }
 
-int dpmi_prod_force_grad_a_f64(double* grad_net, const double* grad, const double* env_deriv, const int* nlist,
-                               int nloc, int nnei, int nframes, void* stream) {
+template <typename T>
+int prod_force_grad_impl(T* grad_net, const T* grad, const T* env_deriv, const int* nlist, int nloc, int nnei,
+                         int nframes, void* stream) {
   return guarded([&] {
-    dpmi::launch_prod_force_grad_a(grad_net, grad, env_deriv, nlist, nloc, nnei, nframes, S(stream));
+    dpmi::launch_prod_force_grad_a<T>(grad_net, grad, env_deriv, nlist, nloc, nnei, nframes, S(stream));
     return kOk;
   });
 }
 
-int dpmi_prod_virial_grad_a_f64(double* grad_net, const double* grad, const double* env_deriv, const double* rij,
-                                const int* nlist, int nloc, int nnei, void* stream) {
+template <typename T>
+int prod_virial_grad_impl(T* grad_net, const T* grad, const T* env_deriv, const T* rij, const int* nlist, int nloc,
+                          int nnei, void* stream) {
   return guarded([&] {
-    dpmi::launch_prod_virial_grad_a(grad_net, grad, env_deriv, rij, nlist, nloc, nnei, S(stream));
+    dpmi::launch_prod_virial_grad_a<T>(grad_net, grad, env_deriv, rij, nlist, nloc, nnei, S(stream));
     return kOk;
   });
+}
+
+}  // namespace
+}  // extern "C++"
+
+int dpmi_prod_force_a_f64(double* force, const double* net_deriv, const double* in_deriv, const int* nlist, int nloc,
+                          int nall, int nnei, int nframes, void* stream) {
+  return prod_force_impl(force, net_deriv, in_deriv, nlist, nloc, nall, nnei, nframes, stream);
+}
+int dpmi_prod_force_a_f32(float* force, const float* net_deriv, const float* in_deriv, const int* nlist, int nloc,
+                          int nall, int nnei, int nframes, void* stream) {
+  return prod_force_impl(force, net_deriv, in_deriv, nlist, nloc, nall, nnei, nframes, stream);
+}
+int dpmi_prod_virial_a_f64(double* virial, double* atom_virial, const double* net_deriv, const double* env_deriv,
+                           const double* rij, const int* nlist, int nloc, int nall, int nnei, void* stream) {
+  return prod_virial_impl(virial, atom_virial, net_deriv, env_deriv, rij, nlist, nloc, nall, nnei, stream);
+}
+int dpmi_prod_virial_a_f32(float* virial, float* atom_virial, const float* net_deriv, const float* env_deriv,
+                           const float* rij, const int* nlist, int nloc, int nall, int nnei, void* stream) {
+  return prod_virial_impl(virial, atom_virial, net_deriv, env_deriv, rij, nlist, nloc, nall, nnei, stream);
+}
+int dpmi_prod_force_grad_a_f64(double* grad_net, const double* grad, const double* env_deriv, const int* nlist,
+                               int nloc, int nnei, int nframes, void* stream) {
+  return prod_force_grad_impl(grad_net, grad, env_deriv, nlist, nloc, nnei, nframes, stream);
+}
+int dpmi_prod_force_grad_a_f32(float* grad_net, const float* grad, const float* env_deriv, const int* nlist,
+                               int nloc, int nnei, int nframes, void* stream) {
+  return prod_force_grad_impl(grad_net, grad, env_deriv, nlist, nloc, nnei, nframes, stream);
+}
+int dpmi_prod_virial_grad_a_f64(double* grad_net, const double* grad, const double* env_deriv, const double* rij,
+                                const int* nlist, int nloc, int nnei, void* stream) {
+  return prod_virial_grad_impl(grad_net, grad, env_deriv, rij, nlist, nloc, nnei, stream);
+}
+int dpmi_prod_virial_grad_a_f32(float* grad_net, const float* grad, const float* env_deriv, const float* rij,
+                                const int* nlist, int nloc, int nnei, void* stream) {
+  return prod_virial_grad_impl(grad_net, grad, env_deriv, rij, nlist, nloc, nnei, stream);
 }
 
 // ---------------------------------------------------------------- engine --

@@ -252,34 +252,34 @@ void prod_env_mat_a_gpu<double>(double* em, double* em_deriv, double* rij, int* 
 DPMI_TAB_TEMPLATES(double, f64)
 DPMI_TAB_TEMPLATES(float, f32)
 
-template <>
-void prod_force_a_gpu<double>(double* force, const double* net_deriv, const double* in_deriv, const int* nlist,
-                              const int nloc, const int nall, const int nnei, const int nframes) {
-  ok_or_throw(dpmi_prod_force_a_f64(force, net_deriv, in_deriv, nlist, nloc, nall, nnei, nframes, nullptr));
-  sync_or_throw();
-}
+#define DPMI_FORCE_TEMPLATES(T, SUF)                                                                            \
+  template <>                                                                                                   \
+  void prod_force_a_gpu<T>(T * force, const T* net_deriv, const T* in_deriv, const int* nlist, const int nloc,  \
+                           const int nall, const int nnei, const int nframes) {                                 \
+    ok_or_throw(dpmi_prod_force_a_##SUF(force, net_deriv, in_deriv, nlist, nloc, nall, nnei, nframes, nullptr)); \
+    sync_or_throw();                                                                                            \
+  }                                                                                                             \
+  template <>                                                                                                   \
+  void prod_virial_a_gpu<T>(T * virial, T * atom_virial, const T* net_deriv, const T* env_deriv, const T* rij,  \
+                            const int* nlist, const int nloc, const int nall, const int nnei) {                 \
+    ok_or_throw(                                                                                                \
+        dpmi_prod_virial_a_##SUF(virial, atom_virial, net_deriv, env_deriv, rij, nlist, nloc, nall, nnei, nullptr)); \
+    sync_or_throw();                                                                                            \
+  }                                                                                                             \
+  template <>                                                                                                   \
+  void prod_force_grad_a_gpu<T>(T * grad_net, const T* grad, const T* env_deriv, const int* nlist,              \
+                                const int nloc, const int nnei, const int nframes) {                            \
+    ok_or_throw(dpmi_prod_force_grad_a_##SUF(grad_net, grad, env_deriv, nlist, nloc, nnei, nframes, nullptr));  \
+    sync_or_throw();                                                                                            \
+  }                                                                                                             \
+  template <>                                                                                                   \
+  void prod_virial_grad_a_gpu<T>(T * grad_net, const T* grad, const T* env_deriv, const T* rij,                 \
+                                 const int* nlist, const int nloc, const int nnei) {                            \
+    ok_or_throw(dpmi_prod_virial_grad_a_##SUF(grad_net, grad, env_deriv, rij, nlist, nloc, nnei, nullptr));     \
+    sync_or_throw();                                                                                            \
+  }
 
-template <>
-void prod_virial_a_gpu<double>(double* virial, double* atom_virial, const double* net_deriv,
-                               const double* env_deriv, const double* rij, const int* nlist, const int nloc,
-                               const int nall, const int nnei) {
-  ok_or_throw(dpmi_prod_virial_a_f64(virial, atom_virial, net_deriv, env_deriv, rij, nlist, nloc, nall, nnei,
-                                     nullptr));
-  sync_or_throw();
-}
-
-template <>
-void prod_force_grad_a_gpu<double>(double* grad_net, const double* grad, const double* env_deriv, const int* nlist,
-                                   const int nloc, const int nnei, const int nframes) {
-  ok_or_throw(dpmi_prod_force_grad_a_f64(grad_net, grad, env_deriv, nlist, nloc, nnei, nframes, nullptr));
-  sync_or_throw();
-}
-
-template <>
-void prod_virial_grad_a_gpu<double>(double* grad_net, const double* grad, const double* env_deriv, const double* rij,
-                                    const int* nlist, const int nloc, const int nnei) {
-  ok_or_throw(dpmi_prod_virial_grad_a_f64(grad_net, grad, env_deriv, rij, nlist, nloc, nnei, nullptr));
-  sync_or_throw();
-}
+DPMI_FORCE_TEMPLATES(double, f64)
+DPMI_FORCE_TEMPLATES(float, f32)
 
 }  // namespace deepmd

@@ -40,8 +40,9 @@ __device__ __forceinline__ void slot_virial(const T* nd, const T* ed, const T* r
     }
 }
 
-__global__ void __launch_bounds__(256) k_prod_force(double* __restrict__ force, const double* __restrict__ nd,
-                                                    const double* __restrict__ ed, const int* __restrict__ nlist,
+template <typename T>
+__global__ void __launch_bounds__(256) k_prod_force(T* __restrict__ force, const T* __restrict__ nd,
+                                                    const T* __restrict__ ed, const int* __restrict__ nlist,
                                                     int nloc, int nall, int nnei, int nframes) {
   const int g = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int lane = lane_id();
@@ -50,7 +51,7 @@ __global__ void __launch_bounds__(256) k_prod_force(double* __restrict__ force, 
   double fi[3] = {0., 0., 0.};
   for (int s = lane; s < nnei; s += kWave) {
     const size_t b = (size_t)g * nnei + s;
-    double n4[4], e12[12], fs[3];
+    T n4[4], e12[12], fs[3];
 #pragma unroll
     for (int a = 0; a < 4; ++a) n4[a] = nd[b * 4 + a];
 #pragma unroll
@@ -61,7 +62,7 @@ __global__ void __launch_bounds__(256) k_prod_force(double* __restrict__ force, 
     fi[2] -= fs[2];
     const int j = nlist[b];
     if (j >= 0) {
-      double* fj = force + ((size_t)f * nall + j) * 3;
+      T* fj = force + ((size_t)f * nall + j) * 3;
       atomicAdd(fj + 0, fs[0]);
       atomicAdd(fj + 1, fs[1]);
       atomicAdd(fj + 2, fs[2]);
@@ -69,16 +70,17 @@ __global__ void __launch_bounds__(256) k_prod_force(double* __restrict__ force, 
   }
   for (int d = 0; d < 3; ++d) fi[d] = wave_sum(fi[d]);
   if (lane == 0) {
-    double* fo = force + ((size_t)f * nall + i) * 3;
-    atomicAdd(fo + 0, fi[0]);
-    atomicAdd(fo + 1, fi[1]);
-    atomicAdd(fo + 2, fi[2]);
+    T* fo = force + ((size_t)f * nall + i) * 3;
+    atomicAdd(fo + 0, (T)fi[0]);
+    atomicAdd(fo + 1, (T)fi[1]);
+    atomicAdd(fo + 2, (T)fi[2]);
   }
 }
 
-__global__ void __launch_bounds__(256) k_prod_virial(double* __restrict__ vpart, double* __restrict__ atom_virial,
-                                                     const double* __restrict__ nd, const double* __restrict__ ed,
-                                                     const double* __restrict__ rij, const int* __restrict__ nlist,
+template <typename T>
+__global__ void __launch_bounds__(256) k_prod_virial(double* __restrict__ vpart, T* __restrict__ atom_virial,
+                                                     const T* __restrict__ nd, const T* __restrict__ ed,
+                                                     const T* __restrict__ rij, const int* __restrict__ nlist,
                                                      int nloc, int nnei) {
   const int i = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int lane = lane_id();
@@ -90,7 +92,7 @@ __global__ void __launch_bounds__(256) k_prod_virial(double* __restrict__ vpart,
     const size_t b = (size_t)i * nnei + s;
     const int j = nlist[b];
     if (j < 0) continue;
-    double n4[4], e12[12], r3[3], v[9];
+    T n4[4], e12[12], r3[3], v[9];
 #pragma unroll
     for (int a = 0; a < 4; ++a) n4[a] = nd[b * 4 + a];
 #pragma unroll
@@ -116,7 +118,8 @@ __global__ void __launch_bounds__(256) k_prod_virial(double* __restrict__ vpart,
 }
 
 // Sum of rows [nrows, 9] -> out9 (one workgroup, compensated).
-__global__ void __launch_bounds__(256) k_reduce_rows9(double* __restrict__ out, const double* __restrict__ part,
+template <typename T>
+__global__ void __launch_bounds__(256) k_reduce_rows9(T* __restrict__ out, const double* __restrict__ part,
                                                       int nrows) {
   __shared__ double sh[256][9];
   KahanD acc[9];
@@ -129,13 +132,14 @@ __global__ void __launch_bounds__(256) k_reduce_rows9(double* __restrict__ out, 
   if (threadIdx.x < 9) {
     KahanD t;
     for (int r = 0; r < (int)blockDim.x; ++r) t.add(sh[r][threadIdx.x]);
-    out[threadIdx.x] = t.value();
+    out[threadIdx.x] = (T)t.value();
   }
 }
 
 // prod_force_grad.cc:23-75 (ghost index folded with j % nloc)
-__global__ void k_prod_force_grad(double* __restrict__ gn, const double* __restrict__ grad,
-                                  const double* __restrict__ ed, const int* __restrict__ nlist, int nloc, int nnei,
+template <typename T>
+__global__ void k_prod_force_grad(T* __restrict__ gn, const T* __restrict__ grad,
+                                  const T* __restrict__ ed, const int* __restrict__ nlist, int nloc, int nnei,
                                   int nframes) {
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= (long long)nframes * nloc * nnei) return;
@@ -144,14 +148,14 @@ __global__ void k_prod_force_grad(double* __restrict__ gn, const double* __restr
   const int f = (int)(g / nloc);
   int j = nlist[g * nnei + s];
   if (j >= nloc) j = j % nloc;
-  const double* e = ed + ((size_t)g * nnei + s) * 12;
-  const double gi[3] = {grad[g * 3 + 0], grad[g * 3 + 1], grad[g * 3 + 2]};
-  double gj[3] = {0., 0., 0.};
+  const T* e = ed + ((size_t)g * nnei + s) * 12;
+  const T gi[3] = {grad[g * 3 + 0], grad[g * 3 + 1], grad[g * 3 + 2]};
+  T gj[3] = {0, 0, 0};
   if (j >= 0)
     for (int d = 0; d < 3; ++d) gj[d] = grad[((size_t)f * nloc + j) * 3 + d];
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
-    double v = 0.;
+    T v = 0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) v -= gi[d] * e[a * 3 + d];
     if (j >= 0)
@@ -162,17 +166,18 @@ __global__ void k_prod_force_grad(double* __restrict__ gn, const double* __restr
 }
 
 // prod_virial_grad.cc:22-66
-__global__ void k_prod_virial_grad(double* __restrict__ gn, const double* __restrict__ grad,
-                                   const double* __restrict__ ed, const double* __restrict__ rij,
+template <typename T>
+__global__ void k_prod_virial_grad(T* __restrict__ gn, const T* __restrict__ grad,
+                                   const T* __restrict__ ed, const T* __restrict__ rij,
                                    const int* __restrict__ nlist, int nloc, int nnei) {
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= (long long)nloc * nnei) return;
   const int j = nlist[tid];
-  const double* e = ed + (size_t)tid * 12;
-  const double* r = rij + (size_t)tid * 3;
+  const T* e = ed + (size_t)tid * 12;
+  const T* r = rij + (size_t)tid * 3;
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
-    double v = 0.;
+    T v = 0;
     if (j >= 0)
 #pragma unroll
       for (int p = 0; p < 3; ++p)
@@ -392,47 +397,62 @@ __global__ void k_frame_combine(const double* __restrict__ part, const int* __re
 
 }  // namespace
 
-void launch_prod_force_a(double* force, const double* net_deriv, const double* env_deriv, const int* nlist, int nloc,
-                         int nall, int nnei, int nframes, hipStream_t st) {
-  DPMI_CHECK(hipMemsetAsync(force, 0, sizeof(double) * (size_t)nframes * nall * 3, st));
+template <typename T>
+void launch_prod_force_a(T* force, const T* net_deriv, const T* env_deriv, const int* nlist, int nloc, int nall,
+                         int nnei, int nframes, hipStream_t st) {
+  DPMI_CHECK(hipMemsetAsync(force, 0, sizeof(T) * (size_t)nframes * nall * 3, st));
   const long long rows = (long long)nframes * nloc;
   if (rows <= 0 || nnei <= 0) return;
-  hipLaunchKernelGGL(k_prod_force, dim3(ceil_div(rows * kWave, 256)), dim3(256), 0, st, force, net_deriv, env_deriv,
+  hipLaunchKernelGGL(k_prod_force<T>, dim3(ceil_div(rows * kWave, 256)), dim3(256), 0, st, force, net_deriv, env_deriv,
                      nlist, nloc, nall, nnei, nframes);
   DPMI_LAUNCH_CHECK();
 }
 
-void launch_prod_virial_a(double* vpart, double* atom_virial, const double* net_deriv, const double* env_deriv,
-                          const double* rij, const int* nlist, int nloc, int nall, int nnei, hipStream_t st) {
-  if (atom_virial) DPMI_CHECK(hipMemsetAsync(atom_virial, 0, sizeof(double) * (size_t)nall * 9, st));
+template <typename T>
+void launch_prod_virial_a(double* vpart, T* atom_virial, const T* net_deriv, const T* env_deriv, const T* rij,
+                          const int* nlist, int nloc, int nall, int nnei, hipStream_t st) {
+  if (atom_virial) DPMI_CHECK(hipMemsetAsync(atom_virial, 0, sizeof(T) * (size_t)nall * 9, st));
   if (nloc <= 0) return;
-  hipLaunchKernelGGL(k_prod_virial, dim3(ceil_div((long long)nloc * kWave, 256)), dim3(256), 0, st, vpart,
+  hipLaunchKernelGGL(k_prod_virial<T>, dim3(ceil_div((long long)nloc * kWave, 256)), dim3(256), 0, st, vpart,
                      atom_virial, net_deriv, env_deriv, rij, nlist, nloc, nnei);
   DPMI_LAUNCH_CHECK();
 }
 
-void launch_reduce_rows9(double* out9, const double* part, int nrows, hipStream_t st) {
-  hipLaunchKernelGGL(k_reduce_rows9, dim3(1), dim3(256), 0, st, out9, part, nrows);
+template <typename T>
+void launch_reduce_rows9(T* out9, const double* part, int nrows, hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce_rows9<T>, dim3(1), dim3(256), 0, st, out9, part, nrows);
   DPMI_LAUNCH_CHECK();
 }
 
-void launch_prod_force_grad_a(double* grad_net, const double* grad, const double* env_deriv, const int* nlist,
-                              int nloc, int nnei, int nframes, hipStream_t st) {
+template <typename T>
+void launch_prod_force_grad_a(T* grad_net, const T* grad, const T* env_deriv, const int* nlist, int nloc, int nnei,
+                              int nframes, hipStream_t st) {
   const long long n = (long long)nframes * nloc * nnei;
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_prod_force_grad, dim3(ceil_div(n, 256)), dim3(256), 0, st, grad_net, grad, env_deriv, nlist,
-                     nloc, nnei, nframes);
+  hipLaunchKernelGGL(k_prod_force_grad<T>, dim3(ceil_div(n, 256)), dim3(256), 0, st, grad_net, grad, env_deriv,
+                     nlist, nloc, nnei, nframes);
   DPMI_LAUNCH_CHECK();
 }
 
-void launch_prod_virial_grad_a(double* grad_net, const double* grad, const double* env_deriv, const double* rij,
-                               const int* nlist, int nloc, int nnei, hipStream_t st) {
+template <typename T>
+void launch_prod_virial_grad_a(T* grad_net, const T* grad, const T* env_deriv, const T* rij, const int* nlist,
+                               int nloc, int nnei, hipStream_t st) {
   const long long n = (long long)nloc * nnei;
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_prod_virial_grad, dim3(ceil_div(n, 256)), dim3(256), 0, st, grad_net, grad, env_deriv, rij,
-                     nlist, nloc, nnei);
+  hipLaunchKernelGGL(k_prod_virial_grad<T>, dim3(ceil_div(n, 256)), dim3(256), 0, st, grad_net, grad, env_deriv,
+                     rij, nlist, nloc, nnei);
   DPMI_LAUNCH_CHECK();
 }
+
+#define DPMI_FORCE_INST(T)                                                                                          \
+  template void launch_prod_force_a<T>(T*, const T*, const T*, const int*, int, int, int, int, hipStream_t);        \
+  template void launch_prod_virial_a<T>(double*, T*, const T*, const T*, const T*, const int*, int, int, int,      \
+                                        hipStream_t);                                                               \
+  template void launch_reduce_rows9<T>(T*, const double*, int, hipStream_t);                                        \
+  template void launch_prod_force_grad_a<T>(T*, const T*, const T*, const int*, int, int, int, hipStream_t);        \
+  template void launch_prod_virial_grad_a<T>(T*, const T*, const T*, const T*, const int*, int, int, hipStream_t);
+DPMI_FORCE_INST(double)
+DPMI_FORCE_INST(float)
 
 void launch_force_virial_rows(const ForceRowsArgs& a, hipStream_t st) {
   if (a.nrows <= 0) return;

@@ -135,16 +135,21 @@ template <typename T, typename TG>
 void launch_embed_bwd(const EmbedArgs& a, hipStream_t st);
 
 // ------------------------------------------------------------- force.hip --
-void launch_prod_force_a(double* force, const double* net_deriv, const double* env_deriv, const int* nlist,
-                         int nloc, int nall, int nnei, int nframes, hipStream_t st);
-// per-row virial partials [nloc, 9] (caller reduces) and atom_virial
-void launch_prod_virial_a(double* vpart, double* atom_virial, const double* net_deriv, const double* env_deriv,
-                          const double* rij, const int* nlist, int nloc, int nall, int nnei, hipStream_t st);
-void launch_reduce_rows9(double* out9, const double* part, int nrows, hipStream_t st);
-void launch_prod_force_grad_a(double* grad_net, const double* grad, const double* env_deriv, const int* nlist,
-                              int nloc, int nnei, int nframes, hipStream_t st);
-void launch_prod_virial_grad_a(double* grad_net, const double* grad, const double* env_deriv, const double* rij,
-                               const int* nlist, int nloc, int nnei, hipStream_t st);
+template <typename T>
+void launch_prod_force_a(T* force, const T* net_deriv, const T* env_deriv, const int* nlist, int nloc, int nall,
+                         int nnei, int nframes, hipStream_t st);
+// per-row virial partials [nloc, 9] (FP64, caller reduces) and atom_virial
+template <typename T>
+void launch_prod_virial_a(double* vpart, T* atom_virial, const T* net_deriv, const T* env_deriv, const T* rij,
+                          const int* nlist, int nloc, int nall, int nnei, hipStream_t st);
+template <typename T>
+void launch_reduce_rows9(T* out9, const double* part, int nrows, hipStream_t st);
+template <typename T>
+void launch_prod_force_grad_a(T* grad_net, const T* grad, const T* env_deriv, const int* nlist, int nloc, int nnei,
+                              int nframes, hipStream_t st);
+template <typename T>
+void launch_prod_virial_grad_a(T* grad_net, const T* grad, const T* env_deriv, const T* rij, const int* nlist,
+                               int nloc, int nnei, hipStream_t st);
 
 // engine: force/virial over packed rows into extended force buffers
 struct ForceRowsArgs {

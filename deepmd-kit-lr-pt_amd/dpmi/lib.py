@@ -42,6 +42,10 @@ SIGNATURES = {
     "dpmi_prod_virial_a_f64": ([_P, _P, _P, _P, _P, _P, _I, _I, _I, _P], _I),
     "dpmi_prod_force_grad_a_f64": ([_P, _P, _P, _P, _I, _I, _I, _P], _I),
     "dpmi_prod_virial_grad_a_f64": ([_P, _P, _P, _P, _P, _I, _I, _P], _I),
+    "dpmi_prod_force_a_f32": ([_P, _P, _P, _P, _I, _I, _I, _I, _P], _I),
+    "dpmi_prod_virial_a_f32": ([_P, _P, _P, _P, _P, _P, _I, _I, _I, _P], _I),
+    "dpmi_prod_force_grad_a_f32": ([_P, _P, _P, _P, _I, _I, _I, _P], _I),
+    "dpmi_prod_virial_grad_a_f32": ([_P, _P, _P, _P, _P, _I, _I, _P], _I),
     "dpmi_sea_model_create": ([_P, _P], _I),
     "dpmi_sea_model_destroy": ([_P], None),
     "dpmi_sea_engine_create": ([_P, _P, _I, _P, _P, _I, _D], _I),

@@ -140,33 +140,37 @@ def tabulate_fusion_se_a_grad_grad(table, table_info, em_x, em, dz_dy_dem_x, dz_
 
 
 def prod_force_a(net_deriv, em_deriv, nlist, nloc, nall, nframes=1):
+    """prod_force.h prod_force_a_gpu -> force [nframes, nall, 3] (dtype of net_deriv)."""
     nnei = nlist.shape[-1]
-    f = torch.empty((nframes, nall, 3), dtype=torch.float64, device=net_deriv.device)
-    check(load().dpmi_prod_force_a_f64(_p(f), _p(net_deriv), _p(em_deriv), _p(nlist), nloc, nall, nnei, nframes,
-                                       _stream()))
+    f = torch.empty((nframes, nall, 3), dtype=net_deriv.dtype, device=net_deriv.device)
+    fn = getattr(load(), "dpmi_prod_force_a_" + _suf(net_deriv))
+    check(fn(_p(f), _p(net_deriv), _p(em_deriv), _p(nlist), nloc, nall, nnei, nframes, _stream()))
     return f
 
 
 def prod_virial_a(net_deriv, em_deriv, rij, nlist, nloc, nall):
+    """prod_virial.h prod_virial_a_gpu (one frame) -> (virial [9], atom_virial [nall, 9])."""
     nnei = nlist.shape[-1]
-    v = torch.empty(9, dtype=torch.float64, device=net_deriv.device)
-    av = torch.empty((nall, 9), dtype=torch.float64, device=net_deriv.device)
-    check(load().dpmi_prod_virial_a_f64(_p(v), _p(av), _p(net_deriv), _p(em_deriv), _p(rij), _p(nlist), nloc, nall,
-                                        nnei, _stream()))
+    v = torch.empty(9, dtype=net_deriv.dtype, device=net_deriv.device)
+    av = torch.empty((nall, 9), dtype=net_deriv.dtype, device=net_deriv.device)
+    fn = getattr(load(), "dpmi_prod_virial_a_" + _suf(net_deriv))
+    check(fn(_p(v), _p(av), _p(net_deriv), _p(em_deriv), _p(rij), _p(nlist), nloc, nall, nnei, _stream()))
     return v, av
 
 
 def prod_force_grad_a(grad, em_deriv, nlist, nloc, nframes=1):
+    """prod_force_grad.h: d(loss)/d(net_deriv) [nframes*nloc, 4*nnei] from d(loss)/d(force) [nframes, nloc, 3]."""
     nnei = nlist.shape[-1]
-    out = torch.empty((nframes * nloc, 4 * nnei), dtype=torch.float64, device=grad.device)
-    check(load().dpmi_prod_force_grad_a_f64(_p(out), _p(grad), _p(em_deriv), _p(nlist), nloc, nnei, nframes,
-                                            _stream()))
+    out = torch.empty((nframes * nloc, 4 * nnei), dtype=grad.dtype, device=grad.device)
+    fn = getattr(load(), "dpmi_prod_force_grad_a_" + _suf(grad))
+    check(fn(_p(out), _p(grad), _p(em_deriv), _p(nlist), nloc, nnei, nframes, _stream()))
     return out
 
 
 def prod_virial_grad_a(grad, em_deriv, rij, nlist, nloc):
+    """prod_virial_grad.h (one frame): d(loss)/d(net_deriv) [nloc, 4*nnei] from d(loss)/d(virial) [9]."""
     nnei = nlist.shape[-1]
-    out = torch.empty((nloc, 4 * nnei), dtype=torch.float64, device=grad.device)
-    check(load().dpmi_prod_virial_grad_a_f64(_p(out), _p(grad), _p(em_deriv), _p(rij), _p(nlist), nloc, nnei,
-                                             _stream()))
+    out = torch.empty((nloc, 4 * nnei), dtype=grad.dtype, device=grad.device)
+    fn = getattr(load(), "dpmi_prod_virial_grad_a_" + _suf(grad))
+    check(fn(_p(out), _p(grad), _p(em_deriv), _p(rij), _p(nlist), nloc, nnei, _stream()))
     return out

@@ -122,6 +122,16 @@ int dpmi_prod_force_grad_a_f64(double* grad_net, const double* grad, const doubl
 /* prod_virial_grad.h:27-34 prod_virial_grad_a_gpu<double> (one frame). */
 int dpmi_prod_virial_grad_a_f64(double* grad_net, const double* grad, const double* env_deriv,
                                 const double* rij, const int* nlist, int nloc, int nnei, void* stream);
+/* The same four for float (prod_force.h / prod_virial.h / _grad.h float
+ * instantiations; per-row virial partials are summed in FP64). */
+int dpmi_prod_force_a_f32(float* force, const float* net_deriv, const float* in_deriv, const int* nlist,
+                          int nloc, int nall, int nnei, int nframes, void* stream);
+int dpmi_prod_virial_a_f32(float* virial, float* atom_virial, const float* net_deriv, const float* env_deriv,
+                           const float* rij, const int* nlist, int nloc, int nall, int nnei, void* stream);
+int dpmi_prod_force_grad_a_f32(float* grad_net, const float* grad, const float* env_deriv, const int* nlist,
+                               int nloc, int nnei, int nframes, void* stream);
+int dpmi_prod_virial_grad_a_f32(float* grad_net, const float* grad, const float* env_deriv, const float* rij,
+                                const int* nlist, int nloc, int nnei, void* stream);
 
 /* ===================================================================== */
 /* 2. whole-path engine                                                  */

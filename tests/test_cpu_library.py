@@ -43,6 +43,8 @@ def test_deepmd_cpp_templates_exported():
                  "deepmd::tabulate_fusion_se_a_grad_grad_gpu<double>", "deepmd::prod_env_mat_a_gpu<double>",
                  "deepmd::prod_force_a_gpu<double>", "deepmd::prod_virial_a_gpu<double>",
                  "deepmd::prod_force_grad_a_gpu<double>", "deepmd::prod_virial_grad_a_gpu<double>",
+                 "deepmd::prod_force_a_gpu<float>", "deepmd::prod_virial_a_gpu<float>",
+                 "deepmd::prod_force_grad_a_gpu<float>", "deepmd::prod_virial_grad_a_gpu<float>",
                  "deepmd::build_nlist_gpu<double>", "deepmd::copy_coord_gpu<double>",
                  "deepmd::normalize_coord_gpu<double>", "deepmd::tabulate_fusion_se_a_gpu<float>"):
         assert name in out, name
