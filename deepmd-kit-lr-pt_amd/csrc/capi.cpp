@@ -423,6 +423,44 @@ int dpmi_sea_engine_compute(dpmi_sea_engine* engine, const double* coord, const 
   });
 }
 
+int dpmi_sea_engine_graph_capture(dpmi_sea_engine* engine, const double* coord, const double* box_host,
+                                  double* energy, double* force, double* virial, double* atom_energy, void* stream) {
+  return guarded([&] {
+    if (!engine || !coord || !box_host || !energy || !force || !virial) throw std::invalid_argument("null argument");
+    engine->impl->capture(coord, box_host, energy, force, virial, atom_energy, S(stream));
+    return kOk;
+  });
+}
+
+int dpmi_sea_engine_graph_launch(dpmi_sea_engine* engine, void* stream) {
+  return guarded([&] {
+    if (!engine) throw std::invalid_argument("null engine");
+    engine->impl->launch_graph(S(stream));
+    return kOk;
+  });
+}
+
+int dpmi_sea_engine_autotune(dpmi_sea_engine* engine, const double* coord, const double* box_host, int reps,
+                             void* stream) {
+  return guarded([&] {
+    if (!engine || !coord || !box_host) throw std::invalid_argument("null argument");
+    engine->impl->autotune(coord, box_host, reps, S(stream));
+    return kOk;
+  });
+}
+
+int dpmi_sea_engine_tuning(dpmi_sea_engine* engine, int* knobs_host) {
+  return guarded([&] {
+    if (!engine || !knobs_host) throw std::invalid_argument("null argument");
+    const dpmi::Tune& t = engine->impl->tune;
+    knobs_host[0] = t.env_tpb;
+    knobs_host[1] = t.fwd_waves;
+    knobs_host[2] = t.bwd_waves;
+    knobs_host[3] = t.force_rows;
+    return kOk;
+  });
+}
+
 int dpmi_sea_engine_status(dpmi_sea_engine* engine) {
   return guarded([&] {
     const int s = engine->impl->status();

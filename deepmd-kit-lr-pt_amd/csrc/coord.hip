@@ -256,7 +256,7 @@ __global__ void k_emit_ghosts(const int* __restrict__ xcell_frame, const FrameGe
 }  // namespace
 
 void launch_build_extended(const CellWork& w, hipStream_t st) {
-  DPMI_CHECK(hipMemsetAsync(w.cell_cnt, 0, sizeof(int) * w.ncells_tot, st));
+  launch_fill_int(w.cell_cnt, 0, w.ncells_tot, st);  // a kernel, not a memset node, under graph capture
   const int tpb = 256;
   hipLaunchKernelGGL(k_wrap_bin, dim3(ceil_div(w.ntot, tpb)), dim3(tpb), 0, st, w.coord_in, w.type_in, w.frame_of,
                      w.geom, w.ntot, w.coord_ext, w.type_ext, w.map_ext, w.cell_of, w.cell_cnt, w.do_wrap);

@@ -203,14 +203,14 @@ __device__ void embed_bwd_block(T* __restrict__ nd, const T (&dy)[KQ][4], const 
 }
 
 template <typename T, typename TG, int KQ>
-__global__ void __launch_bounds__(64) k_embed_bwd(T* __restrict__ net_deriv, const T* __restrict__ xyz,
+__global__ void __launch_bounds__(256) k_embed_bwd(T* __restrict__ net_deriv, const T* __restrict__ xyz,
                                                   const TG* __restrict__ gD, const int* __restrict__ pos,
                                                   const T* __restrict__ tables, const long long* __restrict__ tab_off,
                                                   const Info<T>* __restrict__ infos, const int* __restrict__ net_of,
                                                   const T* __restrict__ em, const int* __restrict__ row_type,
                                                   const int* __restrict__ sec, int ntypes, long long nrows, int nnei,
                                                   int M, int A, int ldD, T inv_nnei) {
-  const long long g = blockIdx.x;
+  const long long g = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int lane = lane_id();
   if (g >= nrows) return;
   const int ti = row_type[g];
@@ -303,8 +303,9 @@ void free_table_infos(void* p, bool fp32) {
 template <typename T, typename TD>
 void launch_embed_fwd(const EmbedArgs& a, hipStream_t st) {
   if (a.nrows <= 0) return;
-  const dim3 grid(ceil_div(a.nrows * kWave, 256));
-  DPMI_KQ_DISPATCH(a.M, hipLaunchKernelGGL((k_embed_fwd<T, TD, KQ>), grid, dim3(256), 0, st, (T*)a.xyz, (TD*)a.D,
+  const int tpb = kWave * (a.waves > 0 ? a.waves : 4);
+  const dim3 grid(ceil_div(a.nrows * kWave, tpb));
+  DPMI_KQ_DISPATCH(a.M, hipLaunchKernelGGL((k_embed_fwd<T, TD, KQ>), grid, dim3(tpb), 0, st, (T*)a.xyz, (TD*)a.D,
                                            a.pos, (const T*)a.tables, a.tab_off, (const Info<T>*)a.infos, a.net_of,
                                            (const T*)a.em, a.row_type, a.sec, a.ntypes, a.nrows, a.nnei, a.M, a.A,
                                            a.ldD > 0 ? a.ldD : a.M * a.A, (T)a.inv_nnei));
@@ -314,7 +315,8 @@ void launch_embed_fwd(const EmbedArgs& a, hipStream_t st) {
 template <typename T, typename TG>
 void launch_embed_bwd(const EmbedArgs& a, hipStream_t st) {
   if (a.nrows <= 0) return;
-  DPMI_KQ_DISPATCH(a.M, hipLaunchKernelGGL((k_embed_bwd<T, TG, KQ>), dim3(a.nrows), dim3(kWave), 0, st,
+  const int tpb = kWave * (a.waves_bwd > 0 ? a.waves_bwd : 1);
+  DPMI_KQ_DISPATCH(a.M, hipLaunchKernelGGL((k_embed_bwd<T, TG, KQ>), dim3(ceil_div(a.nrows * kWave, tpb)), dim3(tpb), 0, st,
                                            (T*)a.net_deriv, (const T*)a.xyz, (const TG*)a.gD, a.pos,
                                            (const T*)a.tables, a.tab_off, (const Info<T>*)a.infos, a.net_of,
                                            (const T*)a.em, a.row_type, a.sec, a.ntypes, a.nrows, a.nnei, a.M, a.A,

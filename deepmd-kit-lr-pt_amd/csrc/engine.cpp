@@ -12,6 +12,8 @@
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -419,9 +421,12 @@ SeaEngine::SeaEngine(const SeaModel* m, int nframes_, const int* natoms_h, const
   rb_check(rocblas_create_handle(&h), "create_handle");
   blas = h;
   nlist_cap = nlist_row_capacity(std::max(128, 2 * m->nnei));
+  load_cached_tune();
 }
 
 SeaEngine::~SeaEngine() {
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  if (graph) (void)hipGraphDestroy(graph);
   if (blas) rocblas_destroy_handle((rocblas_handle)blas);
   if (geom_pinned) (void)hipHostFree(geom_pinned);
   for (auto* p : {&d_frame_of, &d_center, &d_type, &d_pos, &d_perm, &d_flag, &d_nall, &d_frame_ext_off, &d_nlist,
@@ -524,6 +529,7 @@ EnvArgs SeaEngine::env_args() const {
   ea.std_ = m.d_dstd;
   ea.rmin = (double)m.rcut_smth;
   ea.rmax = (double)m.rcut;
+  ea.tpb = tune.env_tpb;
   return ea;
 }
 
@@ -541,6 +547,8 @@ EmbedArgs SeaEngine::embed_args() const {
   ta.M = m.M;
   ta.A = m.A;
   ta.inv_nnei = 1.0 / m.nnei;
+  ta.waves = tune.fwd_waves;
+  ta.waves_bwd = tune.bwd_waves;
   return ta;
 }
 
@@ -664,7 +672,7 @@ void SeaEngine::compute_mixed_descriptor_fitting(hipStream_t st, int& stage) {
     mark(st, stage++);
     return;
   }
-  DPMI_CHECK(hipMemsetAsync(d_eperm, 0, sizeof(float) * (size_t)ntot, st));
+  launch_fill_float(d_eperm, 0.f, ntot, st);
   for (int t = 0; t < m.ntypes; ++t) {
     const long long p0 = tstart[t], n = tstart[t + 1] - tstart[t];
     if (n == 0) continue;
@@ -741,7 +749,7 @@ void SeaEngine::compute(const double* coord, const double* box_h, double* energy
     DPMI_CHECK(hipMemcpyAsync(d_geom, geom_pinned, sizeof(FrameGeom) * nframes, hipMemcpyHostToDevice, st));
     geom_uploaded = true;
   }
-  DPMI_CHECK(hipMemsetAsync(d_flag, 0, sizeof(int), st));
+  launch_fill_int(d_flag, 0, 1, st);
 
   CellWork w;
   w.nframes = nframes;
@@ -790,7 +798,7 @@ void SeaEngine::compute(const double* coord, const double* box_h, double* energy
   else
     compute_mixed_descriptor_fitting(st, stage);
 
-  DPMI_CHECK(hipMemsetAsync(d_force_ext, 0, sizeof(double) * (size_t)next_tot * 3, st));
+  launch_fill_double(d_force_ext, 0.0, next_tot * 3, st);
   ForceRowsArgs fa;
   if (precision == DPMI_FP64) {
     fa.net_deriv = d_net_deriv;
@@ -812,16 +820,135 @@ void SeaEngine::compute(const double* coord, const double* box_h, double* energy
   fa.vpart = d_vpart;
   fa.sorted_loc = d_sorted_loc;
   fa.geom = d_geom;
+  fa.rows = tune.force_rows;
   launch_force_virial_rows(fa, st);
-  DPMI_CHECK(hipMemsetAsync(force, 0, sizeof(double) * (size_t)ntot * 3, st));
+  launch_fill_double(force, 0.0, (long long)ntot * 3, st);
   launch_fold_ghost_forces(d_force_ext, d_map_ext, d_xframe_of_ext, d_geom, d_nall, next_tot, force, st);
   mark(st, stage++);
   launch_frame_reduce(d_atom_e, d_vpart, d_chunks, nchunks, d_chunk_start, nframes, d_part, energy, virial, st);
   if (atom_energy)
-    DPMI_CHECK(hipMemcpyAsync(atom_energy, d_atom_e, sizeof(double) * ntot, hipMemcpyDeviceToDevice, st));
+    launch_copy_double(atom_energy, d_atom_e, ntot, st);
   mark(st, stage++);
   nstages = stage - 1;
   if (timing && timed_calls < kMaxTimedCalls) ++timed_calls;
+}
+
+// Tuned launch shapes per (precision, log2 of the packed atom count).
+namespace {
+std::mutex g_tune_mu;
+std::map<long long, Tune>& tune_cache() {
+  static auto* m = new std::map<long long, Tune>();
+  return *m;
+}
+long long tune_key(int precision, int ntot) {
+  int b = 0;
+  while ((1LL << (b + 1)) <= ntot) ++b;
+  return (long long)precision * 64 + b;
+}
+}  // namespace
+
+void SeaEngine::load_cached_tune() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = tune_cache().find(tune_key(precision, ntot));
+  if (it != tune_cache().end()) tune = it->second;
+}
+
+// Times `reps` steps for each of three candidate values of every knob (the
+// knobs act on different stages, so all are varied together) and keeps,
+// per stage, the fastest value; the result is cached for the size bucket.
+void SeaEngine::autotune(const double* coord, const double* box_h, int reps, hipStream_t st) {
+  const int env_c[3] = {128, 256, 512}, fwd_c[3] = {1, 2, 4}, bwd_c[3] = {1, 2, 4}, frc_c[3] = {4, 8, 16};
+  double t_env[3], t_fwd[3], t_bwd[3], t_frc[3];
+  double *e = nullptr, *f = nullptr, *v = nullptr;
+  DPMI_CHECK(hipMalloc(&e, sizeof(double) * nframes));
+  DPMI_CHECK(hipMalloc(&f, sizeof(double) * 3 * (size_t)ntot));
+  DPMI_CHECK(hipMalloc(&v, sizeof(double) * 9 * nframes));
+  const bool was_timing = timing;
+  std::vector<std::string> names;
+  for (size_t a = 0, b; a <= stage_names.size(); a = b + 1) {
+    b = stage_names.find(';', a);
+    if (b == std::string::npos) b = stage_names.size();
+    names.push_back(stage_names.substr(a, b - a));
+  }
+  auto idx = [&](const char* n) {
+    for (size_t k = 0; k < names.size(); ++k)
+      if (names[k] == n) return (int)k;
+    return -1;
+  };
+  try {
+    compute(coord, box_h, e, f, v, nullptr, st);  // geometry and buffers
+    for (int c = 0; c < 3; ++c) {
+      tune.env_tpb = env_c[c];
+      tune.fwd_waves = fwd_c[c];
+      tune.bwd_waves = bwd_c[c];
+      tune.force_rows = frc_c[c];
+      timing = true;
+      reset_timing();
+      for (int r = 0; r < std::max(reps, 1); ++r) compute(coord, box_h, e, f, v, nullptr, st);
+      std::vector<double> ms(names.size(), 0.0);
+      timing_report(ms.data(), (int)ms.size(), nullptr, 0);
+      t_env[c] = ms[idx("env_mat")];
+      t_fwd[c] = ms[idx("embed_fwd")];
+      t_bwd[c] = ms[idx("embed_bwd")];
+      t_frc[c] = ms[idx("force_virial")];
+    }
+  } catch (...) {
+    timing = was_timing;
+    (void)hipFree(e);
+    (void)hipFree(f);
+    (void)hipFree(v);
+    throw;
+  }
+  timing = was_timing;
+  reset_timing();
+  DPMI_CHECK(hipStreamSynchronize(st));
+  (void)hipFree(e);
+  (void)hipFree(f);
+  (void)hipFree(v);
+  auto best = [](const double* t) { return t[0] <= t[1] ? (t[0] <= t[2] ? 0 : 2) : (t[1] <= t[2] ? 1 : 2); };
+  tune.env_tpb = env_c[best(t_env)];
+  tune.fwd_waves = fwd_c[best(t_fwd)];
+  tune.bwd_waves = bwd_c[best(t_bwd)];
+  tune.force_rows = frc_c[best(t_frc)];
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  tune_cache()[tune_key(precision, ntot)] = tune;
+}
+
+// The captured step holds only kernel nodes (every clear is a fill kernel,
+// no library call is made by the mixed engine with the fused fitting), so
+// replays are exact repeats of an eager step on the current coordinates.
+void SeaEngine::capture(const double* coord, const double* box_h, double* energy, double* force, double* virial,
+                        double* atom_energy, hipStream_t st) {
+  if (precision != DPMI_MIXED || !model->fused)
+    throw Unsupported("graph capture needs the mixed engine with the fused fitting kernel");
+  if (!st) throw std::invalid_argument("graph capture needs a non-default stream");
+  compute(coord, box_h, energy, force, virial, atom_energy, st);  // geometry, buffers, uploads for these boxes
+  DPMI_CHECK(hipStreamSynchronize(st));
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  graph_exec = nullptr;
+  graph = nullptr;
+  const bool was_timing = timing;
+  timing = false;
+  DPMI_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  try {
+    compute(coord, box_h, energy, force, virial, atom_energy, st);
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(st, &g);
+    if (g) (void)hipGraphDestroy(g);
+    timing = was_timing;
+    throw;
+  }
+  timing = was_timing;
+  DPMI_CHECK(hipStreamEndCapture(st, &graph));
+  DPMI_CHECK(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+}
+
+void SeaEngine::launch_graph(hipStream_t st) {
+  if (!graph_exec) throw std::invalid_argument("no captured step: capture one first");
+  DPMI_CHECK(hipGraphLaunch(graph_exec, st));
+  last_stream = st;
 }
 
 int SeaEngine::status() {

@@ -57,6 +57,15 @@ struct SeaEngine {
                double* atom_energy, hipStream_t st);
   int status();
   long long fetch(int frame, int what, void* dst);
+  // one step (eager) then the same step captured into a hipGraph; replays
+  // read the coordinates at `coord` and use the boxes given here
+  void capture(const double* coord, const double* box_h, double* energy, double* force, double* virial,
+               double* atom_energy, hipStream_t st);
+  void launch_graph(hipStream_t st);
+  // picks the launch-shape knobs for this engine's size bucket (and caches
+  // them for later engines of the same bucket)
+  void autotune(const double* coord, const double* box_h, int reps, hipStream_t st);
+  Tune tune;
   int timing_report(double* ms, int max_stages, char* names, int names_len);
 
   const SeaModel* model;
@@ -69,6 +78,7 @@ struct SeaEngine {
 
  private:
   void prepare_geometry(const double* box_h);
+  void load_cached_tune();
   void mark(hipStream_t st, int stage);
 
   std::vector<FrameGeom> geom_h;
@@ -79,6 +89,8 @@ struct SeaEngine {
   int nlist_cap = 256;
   void* blas = nullptr;
   hipStream_t last_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
   std::vector<hipEvent_t> events;
   int nstages = 0;
   int timed_calls = 0;

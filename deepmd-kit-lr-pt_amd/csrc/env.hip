@@ -116,7 +116,8 @@ __global__ void k_env_mat_a(const double* __restrict__ coord, const int* __restr
 void launch_env_mat_a(const EnvArgs& a, hipStream_t st) {
   const long long n = a.nrows * (long long)a.nnei;
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_env_mat_a<double>, dim3(ceil_div(n, 256)), dim3(256), 0, st, a.coord, a.type, a.nlist, a.row_frame,
+  const int tpb = a.tpb > 0 ? a.tpb : 256;
+  hipLaunchKernelGGL(k_env_mat_a<double>, dim3(ceil_div(n, tpb)), dim3(tpb), 0, st, a.coord, a.type, a.nlist, a.row_frame,
                      a.row_ext_off, a.row_center, a.strided_nloc, a.nrows, a.nnei, a.nall_stride, a.avg, a.std_, a.rmin, a.rmax,
                      a.em, a.em_deriv, a.rij);
   DPMI_LAUNCH_CHECK();
@@ -125,7 +126,8 @@ void launch_env_mat_a(const EnvArgs& a, hipStream_t st) {
 void launch_env_mat_a_f32(const EnvArgs& a, float* em, float* em_deriv, float* rij, hipStream_t st) {
   const long long n = a.nrows * (long long)a.nnei;
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_env_mat_a<float>, dim3(ceil_div(n, 256)), dim3(256), 0, st, a.coord, a.type, a.nlist,
+  const int tpb = a.tpb > 0 ? a.tpb : 256;
+  hipLaunchKernelGGL(k_env_mat_a<float>, dim3(ceil_div(n, tpb)), dim3(tpb), 0, st, a.coord, a.type, a.nlist,
                      a.row_frame, a.row_ext_off, a.row_center, a.strided_nloc, a.nrows, a.nnei, a.nall_stride, a.avg,
                      a.std_, a.rmin, a.rmax, em, em_deriv, rij);
   DPMI_LAUNCH_CHECK();

@@ -78,6 +78,16 @@ __global__ void k_fill_double(double* __restrict__ p, double v, long long n) {
   if (i < n) p[i] = v;
 }
 
+__global__ void k_fill_float(float* __restrict__ p, float v, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_copy_double(double* __restrict__ dst, const double* __restrict__ src, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
 }  // namespace
 
 void launch_fit_act(const double* z, const double* b, const double* idt, const double* x, int act, int res_mode,
@@ -128,6 +138,18 @@ void launch_fill_int(int* p, int v, long long n, hipStream_t st) {
 void launch_fill_double(double* p, double v, long long n, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_fill_double, dim3(ceil_div(n, 256)), dim3(256), 0, st, p, v, n);
+  DPMI_LAUNCH_CHECK();
+}
+
+void launch_fill_float(float* p, float v, long long n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_float, dim3(ceil_div(n, 256)), dim3(256), 0, st, p, v, n);
+  DPMI_LAUNCH_CHECK();
+}
+
+void launch_copy_double(double* dst, const double* src, long long n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_copy_double, dim3(ceil_div(n, 256)), dim3(256), 0, st, dst, src, n);
   DPMI_LAUNCH_CHECK();
 }
 

@@ -457,7 +457,7 @@ DPMI_FORCE_INST(float)
 void launch_force_virial_rows(const ForceRowsArgs& a, hipStream_t st) {
   if (a.nrows <= 0) return;
   if (a.sorted_loc && a.geom && a.nnei + 1 <= kHashSize) {
-    const int R = std::min(16, kHashSize / (a.nnei + 1));
+    const int R = std::max(1, std::min(a.rows > 0 ? a.rows : 16, kHashSize / (a.nnei + 1)));
     const dim3 grid(ceil_div(a.nrows, R)), block(R * kWave);
     if (a.fp32)
       hipLaunchKernelGGL(k_force_virial_hash<float>, grid, block, 0, st, (const float*)a.net_deriv,

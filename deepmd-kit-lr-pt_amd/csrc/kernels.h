@@ -85,6 +85,7 @@ struct EnvArgs {
   double* em = nullptr;
   double* em_deriv = nullptr;
   double* rij = nullptr;
+  int tpb = 0;  // threads per block (0: 256)
 };
 void launch_env_mat_a(const EnvArgs& a, hipStream_t st);
 // same, storing em / em_deriv / rij in FP32 (em, em_deriv, rij of a are ignored)
@@ -125,6 +126,8 @@ struct EmbedArgs {
   int A = 0;  // axis_neuron, at most 16
   int ldD = 0;  // row stride of D and gD (0: M*A)
   double inv_nnei = 0;
+  int waves = 0;      // forward: waves (rows) per workgroup (0: 4)
+  int waves_bwd = 0;  // backward: waves (rows) per workgroup (0: 1)
 };
 void* make_table_infos(const double* infos_host, int nnets, bool fp32);
 size_t table_info_bytes(bool fp32);
@@ -169,6 +172,16 @@ struct ForceRowsArgs {
   // and the frame layout; when set, the LDS-aggregated kernel is used
   const int* sorted_loc = nullptr;
   const FrameGeom* geom = nullptr;
+  int rows = 0;  // rows per workgroup of the LDS-aggregated kernel (0: 16)
+};
+
+// Launch-shape knobs of the engine stages, picked per size bucket by
+// SeaEngine::autotune.
+struct Tune {
+  int env_tpb = 256;
+  int fwd_waves = 4;
+  int bwd_waves = 1;
+  int force_rows = 16;
 };
 void launch_force_virial_rows(const ForceRowsArgs& a, hipStream_t st);
 // force[loc] += force_ext[e] for every extended atom e < nall of each frame
@@ -249,5 +262,7 @@ void swizzle_bfrags(const _Float16* bt, int N, int K, _Float16* out);
 
 void launch_fill_int(int* p, int v, long long n, hipStream_t st);
 void launch_fill_double(double* p, double v, long long n, hipStream_t st);
+void launch_fill_float(float* p, float v, long long n, hipStream_t st);
+void launch_copy_double(double* dst, const double* src, long long n, hipStream_t st);
 
 }  // namespace dpmi

@@ -124,6 +124,40 @@ class Engine:
                                              C.c_void_p(self.atom_energy.data_ptr()) if atom_energy else None, st))
         return self.energy, self.force, self.virial
 
+    def capture(self, coord: torch.Tensor, box, atom_energy: bool = False):
+        """One eager step, then the same step captured into a hipGraph on the
+        current (non-default) stream.  replay() re-runs it on the values
+        stored in `coord` at that time (same tensor, same boxes)."""
+        assert coord.dtype == torch.float64 and coord.is_cuda and coord.is_contiguous()
+        b = _arr(box, np.float64).reshape(self.nframes, 9)
+        st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        check(load().dpmi_sea_engine_graph_capture(
+            self.handle, C.c_void_p(coord.data_ptr()), b.ctypes.data_as(C.c_void_p),
+            C.c_void_p(self.energy.data_ptr()), C.c_void_p(self.force.data_ptr()),
+            C.c_void_p(self.virial.data_ptr()), C.c_void_p(self.atom_energy.data_ptr()) if atom_energy else None, st))
+        self._graph_coord = coord  # the captured graph reads this storage
+        return self.energy, self.force, self.virial
+
+    def replay(self):
+        st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        check(load().dpmi_sea_engine_graph_launch(self.handle, st))
+        return self.energy, self.force, self.virial
+
+    def autotune(self, coord: torch.Tensor, box, reps: int = 5) -> dict:
+        """Picks the stage launch shapes for this size bucket (cached for
+        later engines of the same bucket); returns the chosen knobs."""
+        b = _arr(box, np.float64).reshape(self.nframes, 9)
+        st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        check(load().dpmi_sea_engine_autotune(self.handle, C.c_void_p(coord.data_ptr()),
+                                              b.ctypes.data_as(C.c_void_p), int(reps), st))
+        return self.tuning()
+
+    def tuning(self) -> dict:
+        k = (C.c_int * 4)()
+        check(load().dpmi_sea_engine_tuning(self.handle, k))
+        return {"env_threads_per_block": k[0], "embed_fwd_waves": k[1], "embed_bwd_waves": k[2],
+                "force_rows_per_workgroup": k[3]}
+
     def status(self) -> None:
         check(load().dpmi_sea_engine_status(self.handle))
 

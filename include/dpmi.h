@@ -196,6 +196,30 @@ int dpmi_sea_engine_compute(dpmi_sea_engine* engine, const double* coord, const 
 
 /* Synchronises the stream of the last compute and returns its device-side
  * status (DPMI_OK, DPMI_ERR_NLIST_CAPACITY, DPMI_ERR_MEMORY, DPMI_ERR_RANGE). */
+/* hipGraph of one step (no counterpart in the reference; the batched-launch
+ * pattern): runs one eager step, then captures the same step on `stream`
+ * (must not be the NULL stream).  dpmi_sea_engine_graph_launch replays it:
+ * the whole path (wrap, ghosts, cell list, neighbour list, ... , force,
+ * virial, energies) on the coordinates currently stored at `coord`, with the
+ * boxes given at capture, into the same output buffers.  A new box needs a
+ * new capture.  DPMI_MIXED engines with the fused fitting only
+ * (DPMI_ERR_UNSUPPORTED otherwise). */
+int dpmi_sea_engine_graph_capture(dpmi_sea_engine* engine, const double* coord, const double* box_host,
+                                  double* energy, double* force, double* virial, double* atom_energy,
+                                  void* stream);
+int dpmi_sea_engine_graph_launch(dpmi_sea_engine* engine, void* stream);
+
+/* Autotuner (no counterpart in the reference): times `reps` steps for each
+ * candidate launch shape of the env-mat, embedding and force stages and
+ * keeps the fastest per stage; the choice is cached per (precision, size
+ * bucket = floor(log2 of the packed atom count)) and applied to engines
+ * created later for the same bucket.  dpmi_sea_engine_tuning returns the
+ * current knobs: {env threads per block, forward embedding waves per
+ * workgroup, adjoint waves per workgroup, force rows per workgroup}. */
+int dpmi_sea_engine_autotune(dpmi_sea_engine* engine, const double* coord, const double* box_host, int reps,
+                             void* stream);
+int dpmi_sea_engine_tuning(dpmi_sea_engine* engine, int* knobs_host);
+
 int dpmi_sea_engine_status(dpmi_sea_engine* engine);
 
 /* Copies an intermediate of the last compute of frame `frame` to host memory

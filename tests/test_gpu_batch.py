@@ -1,0 +1,93 @@
+"""Batched uneven frames: hipGraph capture/replay and the autotuner.
+
+* A captured step replayed on changed coordinates equals an eager step of a
+  second engine on the same coordinates, replay after replay (eager work and
+  synchronisations in between).
+* Every tuned launch shape gives the same physics; the tuned knobs come from
+  the candidate sets and are reused by a later engine of the same size
+  bucket.
+"""
+import numpy as np
+import pytest
+import torch
+
+from dpmi import model as dmodel
+from dpmi import systems
+from dpmi.engine import DeviceModel, Engine
+from dpmi.lib import DpmiError
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def water():
+    return dmodel.water_model(seed=1)
+
+
+@pytest.fixture(scope="module")
+def batch():
+    frames = systems.uneven_water_frames(8, seed=5, lo=96, hi=900)
+    natoms = [len(a) for (_, a, _) in frames]
+    coord = np.concatenate([c for (c, _, _) in frames])
+    atype = np.concatenate([a for (_, a, _) in frames])
+    box = np.stack([b for (_, _, b) in frames])
+    return natoms, coord, atype, box
+
+
+def _close(a, b, tol):
+    a, b = np.asarray(a).ravel(), np.asarray(b).ravel()
+    return np.max(np.abs(a - b)) <= tol * max(np.max(np.abs(b)), 1e-30)
+
+
+def test_graph_replay_matches_eager(water, batch):
+    natoms, coord, atype, box = batch
+    dm = DeviceModel(water)
+    eg = Engine(dm, natoms, atype, precision="mixed")
+    ee = Engine(dm, natoms, atype, precision="mixed")
+    x = torch.as_tensor(coord, device="cuda")
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        eg.capture(x, box)
+    s.synchronize()
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    for it in range(3):
+        print(f"replay {it}", flush=True)
+        with torch.cuda.stream(s):
+            x.add_(0.02 * torch.randn(x.shape, dtype=x.dtype, device=x.device, generator=gen))
+            e, f, v = [t.clone() for t in eg.replay()]
+        s.synchronize()
+        eg.status()
+        e1, f1, v1 = ee.compute(x.clone(), box)
+        torch.cuda.synchronize()
+        ee.status()
+        assert _close(e.cpu().numpy(), e1.cpu().numpy(), 1e-10)
+        assert _close(f.cpu().numpy(), f1.cpu().numpy(), 1e-9)
+        assert _close(v.cpu().numpy(), v1.cpu().numpy(), 1e-9)
+
+
+def test_graph_capture_rejects_fp64(water, batch):
+    natoms, coord, atype, box = batch
+    eng = Engine(DeviceModel(water), natoms, atype, precision="fp64")
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s), pytest.raises(DpmiError, match="UNSUPPORTED"):
+        eng.capture(torch.as_tensor(coord, device="cuda"), box)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_autotune_keeps_results_and_caches(water, batch, precision):
+    natoms, coord, atype, box = batch
+    dm = DeviceModel(water)
+    x = torch.as_tensor(coord, device="cuda")
+    ref = Engine(dm, natoms, atype, precision=precision)
+    e0, f0, v0 = [t.cpu().numpy().copy() for t in ref.compute(x, box)]
+    eng = Engine(dm, natoms, atype, precision=precision)
+    knobs = eng.autotune(x, box, reps=2)
+    print(precision, knobs, flush=True)
+    assert knobs["env_threads_per_block"] in (128, 256, 512)
+    assert knobs["embed_fwd_waves"] in (1, 2, 4) and knobs["embed_bwd_waves"] in (1, 2, 4)
+    assert knobs["force_rows_per_workgroup"] in (4, 8, 16)
+    e, f, v = [t.cpu().numpy().copy() for t in eng.compute(x, box)]
+    tol = 1e-12 if precision == "fp64" else 1e-9
+    assert _close(e, e0, tol) and _close(f, f0, tol) and _close(v, v0, tol)
+    later = Engine(dm, natoms, atype, precision=precision)
+    assert later.tuning() == knobs
