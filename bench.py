@@ -3,7 +3,10 @@
 Workload (the 12,288-atom water configuration): 4096 H2O in a 49.68 A cubic
 box, se_e2_a rcut 6.0 / rcut_smth 0.5, sel [46, 92], embedding [25, 50, 100]
 compressed at strides 0.01/0.1, axis 16, fitting [240, 240, 240] tanh, seeded
-random weights; synthetic seeded coordinates.  One step = one full E+F+V
+random weights, davg/dstd from the environment statistics of a water frame
+(so the normalised inputs and the energies are of order one, as in a trained
+model); synthetic seeded coordinates.  Launch shapes are autotuned per engine
+before timing.  One step = one full E+F+V
 evaluation from raw coordinates (wrap, ghosts, cell-list neighbour list,
 environment matrix, table embedding + contraction, fitting forward and
 backward, force/virial, ghost fold, per-frame sums) with the coordinates
@@ -82,7 +85,7 @@ MEASURED_TRAFFIC_MIB = {
 }
 
 
-def cpu_baseline(m, coord, atype, box, max_seconds=15.0):
+def cpu_baseline(m, coord, atype, box, max_seconds=15.0, what="the same 12,288-atom water frame"):
     """Reference CPU path (the compiled reference lib where present, else the
     oracle restatement) plus numpy glue for the fitting, on the host cores."""
     from oracle import oracle_lib, ref_available, ref_lib
@@ -100,7 +103,7 @@ def cpu_baseline(m, coord, atype, box, max_seconds=15.0):
     dt = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return r, {"value": len(atype) * n / dt, "unit": "atom-steps/s", "cores": cores, "kind": kind,
-               "sample": f"{n} E+F+V evaluations of the same 12,288-atom water frame "
+               "sample": f"{n} E+F+V evaluations of {what} "
                          f"(reference lib + numpy glue, OMP threads {cores})"}
 
 
@@ -229,6 +232,96 @@ def run_batched(args) -> int:
     return 0
 
 
+def run_copper(args) -> int:
+    """1,048,576-atom fcc Cu (64^3 cells, a = 3.615 A, 0.05 A Gaussian
+    displacements), one frame: se_e2_a rcut 8.0 / rcut_smth 2.0, sel [200],
+    embedding [32, 64, 128] compressed, axis 16, fitting [240, 240, 240];
+    mixed engine timed over `steps` evaluations, the FP64 engine over a few
+    beside it for the error and the ratio.  Per-stage device time, algorithmic
+    HBM GB/s of every stage and the fitting's MFMA utilisation."""
+    import numpy as np
+    import torch
+
+    from dpmi import model as dmodel
+    from dpmi import systems
+    from dpmi.engine import DeviceModel, Engine
+
+    m = dmodel.copper_model(seed=1)
+    coord, atype, box = systems.fcc_copper(args.ncell, seed=7)
+    nloc = len(atype)
+    dm = DeviceModel(m)
+    x = torch.as_tensor(coord, device="cuda")
+    res, knobs = {}, {}
+    for prec, steps in (("mixed", args.steps), ("fp64", max(2, args.steps // 4))):
+        print(f"copper {nloc} atoms: {prec} engine, {steps} steps", file=sys.stderr, flush=True)
+        eng = Engine(dm, [nloc], atype, precision=prec)
+        knobs[prec] = eng.autotune(x, box, reps=2)
+        print(f"  tuned {knobs[prec]}", file=sys.stderr, flush=True)
+        dt, stages = time_engine(eng, x, box, steps, min(args.warmup, 3))
+        e, f, v = eng.compute(x, box)
+        torch.cuda.synchronize()
+        nall = int(eng.fetch("nall")[0])
+        free, total = torch.cuda.mem_get_info()
+        res[prec] = {"dt": dt / steps, "stages": stages, "nall": nall, "e": e.cpu().numpy().copy(),
+                     "f": f.cpu().numpy().copy(), "v": v.cpu().numpy().copy(), "hbm_gib": (total - free) / 2**30}
+        del eng, e, f, v
+        torch.cuda.empty_cache()
+    rm, rd = res["mixed"], res["fp64"]
+    model = stage_model(m, nloc, rm["nall"], "mixed")
+    per_stage = {}
+    for k, t_ms in rm["stages"].items():
+        row = {"ms": round(t_ms, 4)}
+        if k in model:
+            kind, amount, peak, nbytes = model[k]
+            if kind == "hbm":
+                row["hbm_GBps"] = round(amount / (t_ms * 1e-3) / 1e9, 1)
+            else:
+                row["TFLOPs"] = round(amount / (t_ms * 1e-3) / 1e12, 2)
+                row["frac_of_peak"] = round(amount / (t_ms * 1e-3) / 1e12 / peak, 4)
+                if nbytes is not None:
+                    row["hbm_GBps"] = round(nbytes / (t_ms * 1e-3) / 1e9, 1)
+        per_stage[k] = row
+    err = {
+        "energy_meV_atom": float(abs(rm["e"][0] - rd["e"][0]) / nloc * 1e3),
+        "force_rmse_meV_A": float(np.sqrt(np.mean((rm["f"] - rd["f"]) ** 2)) * 1e3),
+        "force_max_meV_A": float(np.max(np.abs(rm["f"] - rd["f"])) * 1e3),
+        "virial_meV_atom": float(np.max(np.abs(rm["v"][0] - rd["v"][0])) / nloc * 1e3),
+    }
+    out = {
+        "metric": "atom-steps/s (E+F+V), 1M-atom fcc Cu, mixed engine",
+        "value": nloc / rm["dt"],
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": min(args.warmup, 3),
+        "ms_per_step": rm["dt"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp16-mfma/fp32 (fp64 geometry, s(r), sums)",
+        "data": "synthetic",
+        "config": {"workload": f"fcc Cu {args.ncell}^3 cells ({nloc} atoms, box {box[0]:.2f} A), rcut 8.0, "
+                               f"sel [200], embedding [32, 64, 128] compressed, one frame, E+F+V per step",
+                   "nloc": nloc, "nall": rm["nall"], "parallelism": "single-gpu"},
+        "roofline": roofline(m, nloc, rm["nall"], "mixed", rm["stages"]),
+        "stages": per_stage,
+        "fp64_value": nloc / rd["dt"],
+        "mixed_over_fp64": rd["dt"] / rm["dt"],
+        "fp64_stages_ms": {k: round(vv, 4) for k, vv in rd["stages"].items()},
+        "mixed_error_vs_fp64": err,
+        "hbm_in_use_GiB": {"mixed": round(rm["hbm_gib"], 2), "fp64": round(rd["hbm_gib"], 2)},
+        "tuned_knobs": knobs,
+    }
+    out["roofline"]["traffic"] = None
+    out["roofline"].pop("measured_hbm_GBps", None)
+    if not args.no_cpu_baseline:
+        cc, ca, cbx = systems.fcc_copper(6, seed=7)
+        _, cb = cpu_baseline(m, cc, ca, cbx, max_seconds=10.0, what=f"a {len(ca)}-atom fcc Cu frame (6^3 cells)")
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,8 +331,10 @@ def main() -> int:
                     help="path reported as value (the other one is timed beside it)")
     ap.add_argument("--nmol", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="water12k", choices=["water12k", "batched"],
-                    help="water12k: the 12,288-atom water frame (default); batched: uneven water frames")
+    ap.add_argument("--workload", default="water12k", choices=["water12k", "batched", "copper1m"],
+                    help="water12k: the 12,288-atom water frame (default); batched: uneven water frames; "
+                         "copper1m: 1M-atom fcc copper")
+    ap.add_argument("--ncell", type=int, default=64, help="fcc cells per edge of the copper workload")
     ap.add_argument("--frames", type=int, default=256, help="frames of the batched workload")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run 5 plain device-to-device copies of this many MiB (the reference kernel "
@@ -251,6 +346,8 @@ def main() -> int:
         return 2
     if args.workload == "batched":
         return run_batched(args)
+    if args.workload == "copper1m":
+        return run_copper(args)
     if args.copy_mib > 0:
         import torch
 
@@ -277,8 +374,10 @@ def main() -> int:
     x = torch.as_tensor(coord, device="cuda")
     other = "fp64" if args.precision == "mixed" else "mixed"
     res = {}
+    knobs = {}
     for prec in (args.precision, other):
         eng = Engine(dm, [nloc], atype, precision=prec)
+        knobs[prec] = eng.autotune(x, box, reps=3)
         dt, stages = time_engine(eng, x, box, args.steps, args.warmup)
         e, f, v = eng.compute(x, box)
         torch.cuda.synchronize()
@@ -322,6 +421,7 @@ def main() -> int:
         "mixed_over_fp64": mixed_value / fp64_value,
         "mixed_error_vs_fp64": mixed_err,
         "other_stages_ms": {k: round(vv, 5) for k, vv in res[other]["stages"].items()},
+        "tuned_knobs": knobs,
     }
     if not args.no_cpu_baseline:
         ref, cb = cpu_baseline(m, coord, atype, box)

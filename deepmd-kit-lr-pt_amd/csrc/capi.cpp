@@ -457,6 +457,36 @@ int dpmi_sea_engine_tuning(dpmi_sea_engine* engine, int* knobs_host) {
     knobs_host[1] = t.fwd_waves;
     knobs_host[2] = t.bwd_waves;
     knobs_host[3] = t.force_rows;
+    knobs_host[4] = t.embed_mfma;
+    return kOk;
+  });
+}
+
+int dpmi_sea_engine_set_tuning(dpmi_sea_engine* engine, const int* knobs_host) {
+  return guarded([&] {
+    if (!engine || !knobs_host) throw std::invalid_argument("null argument");
+    dpmi::Tune& t = engine->impl->tune;
+    const int* k = knobs_host;
+    if (k[0] > 0) {
+      if (k[0] % 64 || k[0] > 1024) throw std::invalid_argument("env threads per block: a multiple of 64 up to 1024");
+      t.env_tpb = k[0];
+    }
+    if (k[1] > 0) {
+      if (k[1] > 4) throw std::invalid_argument("forward embedding waves per workgroup: 1 to 4");
+      t.fwd_waves = k[1];
+    }
+    if (k[2] > 0) {
+      if (k[2] > 4) throw std::invalid_argument("adjoint embedding waves per workgroup: 1 to 4");
+      t.bwd_waves = k[2];
+    }
+    if (k[3] > 0) {
+      if (k[3] > 16) throw std::invalid_argument("force rows per workgroup: 1 to 16");
+      t.force_rows = k[3];
+    }
+    if (k[4] >= 0) {
+      if (k[4] > 3) throw std::invalid_argument("matrix-core embedding mask: 0 to 3");
+      t.embed_mfma = k[4];
+    }
     return kOk;
   });
 }

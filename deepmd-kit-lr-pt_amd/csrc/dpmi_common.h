@@ -22,6 +22,7 @@ enum DeviceFlag : int {
   kFlagIndexRange = 4,        // an extended index does not fit the 24-bit sort key
   kFlagTypeRange = 8,         // atom type >= 255 (sort key)
   kFlagCellCapacity = 16,     // a cell holds more atoms than the binning assumes
+  kFlagHalfRange = 32,        // a mixed-precision matrix-core operand exceeds the FP16 range
 };
 
 struct HipError : std::runtime_error {

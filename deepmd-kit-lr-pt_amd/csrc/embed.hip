@@ -17,6 +17,9 @@
 // butterfly transpose over groups of neighbours.
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
+#include <cmath>
+
 #include "dpmi_common.h"
 #include "kernels.h"
 #include "tabulate_core.h"
@@ -202,36 +205,26 @@ __device__ void embed_bwd_block(T* __restrict__ nd, const T (&dy)[KQ][4], const 
     for (int c = 0; c < 4; ++c) nd[(size_t)j * 4 + c] = 0;
 }
 
+// dy = dE/dxyz of row g in the lanes-over-columns layout (lane k holds
+// column k + 64 kq):  dE/dT[c, k] = sum_a G[k, a] T[c, a] + [k < A] sum_m
+// G[m, k] T[c, m] with G = dE/dD (row pos[g]) and T = xyz / nnei, then
+// dy = dE/dT / nnei.  Columns k >= M come out zero.
 template <typename T, typename TG, int KQ>
-__global__ void __launch_bounds__(256) k_embed_bwd(T* __restrict__ net_deriv, const T* __restrict__ xyz,
-                                                  const TG* __restrict__ gD, const int* __restrict__ pos,
-                                                  const T* __restrict__ tables, const long long* __restrict__ tab_off,
-                                                  const Info<T>* __restrict__ infos, const int* __restrict__ net_of,
-                                                  const T* __restrict__ em, const int* __restrict__ row_type,
-                                                  const int* __restrict__ sec, int ntypes, long long nrows, int nnei,
-                                                  int M, int A, int ldD, T inv_nnei) {
-  const long long g = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+__device__ __forceinline__ void bwd_dy(T (&dy)[KQ][4], const T* __restrict__ xyz_g, const TG* __restrict__ G, int M,
+                                       int A, T inv_nnei) {
   const int lane = lane_id();
-  if (g >= nrows) return;
-  const int ti = row_type[g];
-  if (ti < 0) {
-    for (int s = lane; s < nnei * 4; s += kWave) net_deriv[(size_t)g * nnei * 4 + s] = 0;
-    return;
-  }
-  // T[c, k] for the lane's columns
   T Tk[KQ][4];
 #pragma unroll
   for (int kq = 0; kq < KQ; ++kq) {
     const int k = lane + kq * kWave;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) Tk[kq][c] = k < M ? xyz[((size_t)g * 4 + c) * M + k] * inv_nnei : (T)0;
+    for (int c = 0; c < 4; ++c) Tk[kq][c] = k < M ? xyz_g[(size_t)c * M + k] * inv_nnei : (T)0;
   }
   T ta[16][4];  // T[c, a] for a < A, broadcast
 #pragma unroll
   for (int a = 0; a < 16; ++a)
 #pragma unroll
     for (int c = 0; c < 4; ++c) ta[a][c] = a < A ? bcast(Tk[0][c], a) : (T)0;
-  const TG* G = gD + (size_t)pos[g] * ldD;
   // dT[c, k] = sum_a G[k, a] T[c, a];  per-lane terms of S[c, a] = sum_m G[m, a] T[c, m]
   T dT[KQ][4];
   T sp[64];
@@ -263,16 +256,560 @@ __global__ void __launch_bounds__(256) k_embed_bwd(T* __restrict__ net_deriv, co
     const T sv = __shfl(S, c * 16 + (lane & 15), kWave);
     if (lane < A) dT[0][c] += sv;
   }
-  T dy[KQ][4];
 #pragma unroll
   for (int kq = 0; kq < KQ; ++kq)
 #pragma unroll
     for (int c = 0; c < 4; ++c) dy[kq][c] = dT[kq][c] * inv_nnei;
+}
+
+template <typename T, typename TG, int KQ>
+__global__ void __launch_bounds__(256) k_embed_bwd(T* __restrict__ net_deriv, const T* __restrict__ xyz,
+                                                  const TG* __restrict__ gD, const int* __restrict__ pos,
+                                                  const T* __restrict__ tables, const long long* __restrict__ tab_off,
+                                                  const Info<T>* __restrict__ infos, const int* __restrict__ net_of,
+                                                  const T* __restrict__ em, const int* __restrict__ row_type,
+                                                  const int* __restrict__ sec, int ntypes, long long nrows, int nnei,
+                                                  int M, int A, int ldD, T inv_nnei) {
+  const long long g = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = lane_id();
+  if (g >= nrows) return;
+  const int ti = row_type[g];
+  if (ti < 0) {
+    for (int s = lane; s < nnei * 4; s += kWave) net_deriv[(size_t)g * nnei * 4 + s] = 0;
+    return;
+  }
+  T dy[KQ][4];
+  bwd_dy<T, TG, KQ>(dy, xyz + (size_t)g * 4 * M, gD + (size_t)pos[g] * ldD, M, A, inv_nnei);
   for (int tj = 0; tj < ntypes; ++tj) {
     const int net = net_of[ti * ntypes + tj];
     const int b = sec[tj], nb = sec[tj + 1] - sec[tj];
     embed_bwd_block<T, KQ>(net_deriv + ((size_t)g * nnei + b) * 4, dy, tables + tab_off[net], infos[net],
                            em + ((size_t)g * nnei + b) * 4, nb, M);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Matrix-core kernels of the mixed engine: the table embedding in table space.
+//
+// With t' = t / s and delta' = delta / s (s the stride of the row's table
+// segment) the quintic of neighbour j at table row r is
+//   G_k(x_j) = sum_p a'_p[r, k] T_j[p],  a'_p = a_p s^p,
+//   T_j[p] = t'^p + delta' p t'^(p-1),
+// so the contraction becomes a product over (row, p):
+//   xyz[c, k] = sum_{r, p} Q[r, p, c] a'_p[r, k],
+//   Q[r, p, c] = sum of em[j, c] T_j[p] over the neighbours j at row r.
+// A row block's neighbours are sorted by distance, so their table rows do
+// not increase along the list.  A wave takes 4 atoms of one type (N = 4
+// atoms x 4 components = 16), walks their neighbour lists together through
+// windows of 16 table rows (top row first), accumulates the window's Q in
+// LDS and multiplies it with the table on the matrix cores
+// (v_mfma_f32_16x16x32_f16, K = 4 rows x 8 p).  Both operands are split
+// into FP16 hi + lo parts and hi*hi + hi*lo + lo*hi are accumulated in FP32
+// (about 22 bits of each operand).
+// The adjoint walks the same windows with the unscaled coefficients (the
+// em_x derivative would multiply the errors of a'_p by 1 / s), each power p
+// scaled by a power of two sigma_p that brings the largest |a_p| of the net
+// near the top of the FP16 range:  H[r, p, (a, c)] = sum_k sigma_p a_p[r, k]
+// dy_a[k, c] for the row pairs the window's neighbours use, then per
+// neighbour (t, delta unscaled)
+//   nd[j, c] = sum_p (t^p + delta p t^(p-1)) / sigma_p  H[r_j, p, c],
+//   nd[j, 0] += sum_p p t^(p-1) / sigma_p  sum_c em[j, c] H[r_j, p, c].
+// The sorted-padding tail of a block (first padded slot jt, weight nb - jt,
+// tabulate.cc:188-201) is evaluated per atom with the FP32 table.
+typedef _Float16 mhalf8 __attribute__((ext_vector_type(8)));
+typedef float mfloat4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const mhalf8 gmhalf8;
+
+constexpr int kTA = kEmbedTileAtoms;  // atoms per wave
+constexpr int kWinRows = 16;          // table rows per window
+constexpr int kRS = 36;               // floats per (atom, row): [c][8 p] + 4 pad
+constexpr int kAS = kWinRows * kRS;   // floats per atom
+constexpr int kWinF = kTA * kAS;      // floats of one wave's window
+constexpr int kEW = 4;                // waves per workgroup
+constexpr float kHalfMax = 32768.f;   // operands at or above this magnitude are flagged
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uni(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
+
+__device__ __forceinline__ void split8(const float4& v0, const float4& v1, mhalf8& hi, mhalf8& lo) {
+  const float v[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const _Float16 h = (_Float16)v[i];
+    hi[i] = h;
+    lo[i] = (_Float16)(v[i] - (float)h);
+  }
+}
+
+__device__ __forceinline__ bool is_tail(const float4& r, float last) {
+  return r.x == last && r.y == 0.f && r.z == 0.f && r.w == 0.f;
+}
+
+// First slot of the sorted zero-padding tail of a block of nb slots (nb if
+// there is none).
+__device__ __forceinline__ int find_tail(const float* __restrict__ e, int nb, float last) {
+  const int lane = lane_id();
+  for (int c0 = 0; c0 < nb; c0 += kWave) {
+    const int j = c0 + lane;
+    bool tail = false;
+    if (j < nb) tail = is_tail(*reinterpret_cast<const float4*>(e + (size_t)j * 4), last);
+    const unsigned long long m = __ballot(tail);
+    if (m) return c0 + __ffsll((long long)m) - 1;
+  }
+  return nb;
+}
+
+// Powers of one neighbour: T[p] = t'^p + delta' p t'^(p-1) and, for the
+// adjoint, Td[p] = p t'^(p-1) / s.
+__device__ __forceinline__ void powers(float tp, float dp, float (&T)[6]) {
+  const float t2 = tp * tp, t3 = t2 * tp, t4 = t3 * tp;
+  T[0] = 1.f;
+  T[1] = tp + dp;
+  T[2] = t2 + 2.f * dp * tp;
+  T[3] = t3 + 3.f * dp * t2;
+  T[4] = t4 + 4.f * dp * t3;
+  T[5] = t4 * tp + 5.f * dp * t4;
+}
+
+// Walk state of the tile's atoms in one row block.
+struct Walk {
+  const float* e[kTA];
+  float last[kTA];
+  int cur[kTA];
+  bool done[kTA];
+};
+
+__device__ __forceinline__ void walk_init(Walk& w, const float* em, const long long (&g)[kTA], int na, int nnei,
+                                          int b, int nb) {
+#pragma unroll
+  for (int i = 0; i < kTA; ++i) {
+    w.done[i] = i >= na;
+    w.cur[i] = 0;
+    w.e[i] = w.done[i] ? em : em + ((size_t)g[i] * nnei + b) * 4;
+    w.last[i] = w.done[i] ? 0.f : uni(w.e[i][(size_t)(nb - 1) * 4]);
+  }
+}
+
+// Row of the neighbour at each atom's cursor; atoms at the end of their list
+// or at the padding tail are done.  Returns the top row (-1: all done).
+__device__ __forceinline__ int walk_top(Walk& w, const Info<float>& f, int nb) {
+  int rtop = -1;
+#pragma unroll
+  for (int i = 0; i < kTA; ++i) {
+    if (w.done[i]) continue;
+    if (w.cur[i] >= nb) {
+      w.done[i] = true;
+      continue;
+    }
+    const float4 r = *reinterpret_cast<const float4*>(w.e[i] + (size_t)w.cur[i] * 4);
+    if (uni((int)is_tail(r, w.last[i]))) {
+      w.done[i] = true;
+      continue;
+    }
+    float x = r.x, d;
+    int row;
+    locate(f, x, row, d);
+    rtop = max(rtop, uni(row));
+  }
+  return rtop;
+}
+
+// Neighbour (lane) of atom i from position `from`: record, row, and whether
+// it lies in the window rows [R0, R0 + kWinRows) (not padding).
+struct NbrW {
+  float4 ll;
+  float t, d;
+  int row;
+  bool in;
+};
+__device__ __forceinline__ NbrW load_w(const Walk& w, int i, int from, int nb, const Info<float>& f, int R0) {
+  NbrW n;
+  const int j = from + lane_id();
+  n.ll = make_float4(0.f, 0.f, 0.f, 0.f);
+  const bool ok = j < nb;
+  if (ok) n.ll = *reinterpret_cast<const float4*>(w.e[i] + (size_t)j * 4);
+  n.t = n.ll.x;
+  locate(f, n.t, n.row, n.d);
+  n.in = ok && !is_tail(n.ll, w.last[i]) && n.row >= R0 && n.row < R0 + kWinRows;
+  return n;
+}
+
+template <int MT>
+__global__ void __launch_bounds__(256) k_embed_fwd_mfma(const EmbedArgs a) {
+  __shared__ __attribute__((aligned(16))) float s_win[kEW][kWinF];
+  const int lane = lane_id();
+  const int wave = uni((int)(threadIdx.x / kWave));
+  const int tile = blockIdx.x * kEW + wave;
+  if (tile >= a.netiles) return;
+  const int4 tl = a.etiles[tile];
+  const int ti = uni(tl.x), p0 = uni(tl.y), na = uni(tl.z);
+  float* win = s_win[wave];
+  const float* em = (const float*)a.em;
+  const Info<float>* infos = (const Info<float>*)a.infos;
+  const int M = a.M, nnei = a.nnei;
+  long long g[kTA];
+#pragma unroll
+  for (int i = 0; i < kTA; ++i) g[i] = i < na ? uni(a.perm[p0 + i]) : 0;
+  const int fn = lane & 15, fa = fn >> 2, fc = fn & 3, fk = lane >> 4;  // fragment coordinates
+  mfloat4 acc[MT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) acc[mt] = mfloat4{0.f, 0.f, 0.f, 0.f};
+  bool range_ok = true;
+  if (ti >= 0) {
+    for (int tj = 0; tj < a.ntypes; ++tj) {
+      const int net = a.net_of[ti * a.ntypes + tj];
+      const int b = a.sec[tj], nb = a.sec[tj + 1] - b;
+      if (nb <= 0) continue;
+      const Info<float> f = infos[net];
+      const float is0 = 1.f / f.s0, is1 = 1.f / f.s1;
+      const _Float16* thi = a.tab16 + a.tab16_off[4 * net];
+      const _Float16* tlo = a.tab16 + a.tab16_off[4 * net + 1];
+      Walk w;
+      walk_init(w, em, g, na, nnei, b, nb);
+      while (true) {
+        const int R1 = walk_top(w, f, nb);
+        if (R1 < 0) break;
+        const int R0 = max(R1 - (kWinRows - 1), 0);
+        for (int q = lane; q < kWinF / 4; q += kWave) reinterpret_cast<float4*>(win)[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+        unsigned kbits = 0;
+#pragma unroll
+        for (int i = 0; i < kTA; ++i) {
+          if (w.done[i]) continue;
+          while (true) {
+            const NbrW n = load_w(w, i, w.cur[i], nb, f, R0);
+            const unsigned long long m = __ballot(n.in);
+            const int cnt = ~m == 0ull ? kWave : (int)__builtin_ctzll(~m);  // leading in-window slots
+            if (lane < cnt) {
+              const float is = n.row < f.n0 ? is0 : is1;
+              float T[6];
+              powers(n.t * is, n.d * is, T);
+              float* dst = win + i * kAS + (n.row - R0) * kRS;
+              const float ll[4] = {n.ll.x, n.ll.y, n.ll.z, n.ll.w};
+#pragma unroll
+              for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int p = 0; p < 6; ++p) {
+                  const float q = T[p] * ll[c];
+                  range_ok = range_ok && fabsf(q) < kHalfMax;
+                  atomicAdd(dst + c * 8 + p, q);
+                }
+              kbits |= 1u << ((n.row - R0) >> 2);
+            }
+            w.cur[i] += cnt;
+            if (cnt < kWave) break;
+          }
+        }
+        unsigned kb = 0;
+#pragma unroll
+        for (int ks = 0; ks < kWinRows / 4; ++ks)
+          if (__ballot((kbits >> ks) & 1u)) kb |= 1u << ks;
+#pragma unroll
+        for (int ks = 0; ks < kWinRows / 4; ++ks) {
+          if (!((kb >> ks) & 1u)) continue;
+          const float* src = win + fa * kAS + (4 * ks + fk) * kRS + fc * 8;
+          mhalf8 bh, bl;
+          split8(*reinterpret_cast<const float4*>(src), *reinterpret_cast<const float4*>(src + 4), bh, bl);
+          const size_t ro = ((size_t)(R0 + 4 * ks + fk) * (MT * 16) + fn) * 8;
+#pragma unroll
+          for (int mt = 0; mt < MT; ++mt) {
+            const mhalf8 ah = *(gmhalf8*)(thi + ro + mt * 128);
+            const mhalf8 al = *(gmhalf8*)(tlo + ro + mt * 128);
+            acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc[mt], 0, 0, 0);
+            acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc[mt], 0, 0, 0);
+            acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc[mt], 0, 0, 0);
+          }
+        }
+      }
+    }
+  }
+  if (!range_ok) atomicOr(a.flag, kFlagHalfRange);
+  // stage xyz as [atom][c][16 MT] in LDS: lane (fa, fc) holds columns 16 mt + 4 fk .. +3
+  constexpr int LD = MT * 16;
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+    *reinterpret_cast<float4*>(win + fn * LD + mt * 16 + fk * 4) = make_float4(acc[mt][0], acc[mt][1], acc[mt][2], acc[mt][3]);
+  // per atom, lanes over columns: padding-tail term (FP32 table), xyz, descriptor
+  float* xyz = (float*)a.xyz;
+  _Float16* D = (_Float16*)a.D;
+  const float* tab32 = (const float*)a.tables;
+  const float inv = (float)a.inv_nnei;
+  const int A = a.A;
+  constexpr int KQ = (MT * 16 + kWave - 1) / kWave;
+  for (int i = 0; i < na; ++i) {
+    const long long gi = uni(a.perm[p0 + i]);
+    float X[KQ][4];
+#pragma unroll
+    for (int kq = 0; kq < KQ; ++kq) {
+      const int k = min(lane + kq * kWave, LD - 1);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) X[kq][c] = win[(i * 4 + c) * LD + k];
+    }
+    if (ti >= 0) {
+      for (int tj = 0; tj < a.ntypes; ++tj) {
+        const int b = a.sec[tj], nb = a.sec[tj + 1] - b;
+        if (nb <= 0) continue;
+        const int net = a.net_of[ti * a.ntypes + tj];
+        const float* e = em + ((size_t)gi * nnei + b) * 4;
+        const float last = uni(e[(size_t)(nb - 1) * 4]);
+        const int jt = uni(find_tail(e, nb, last));
+        if (jt >= nb) continue;
+        const float4 r = *reinterpret_cast<const float4*>(e + (size_t)jt * 4);
+        float x = r.x, d;
+        int idx;
+        locate(infos[net], x, idx, d);
+        const float wgt = (float)(nb - jt);
+        const float* row = tab32 + a.tab_off[net] + (size_t)idx * M * 6;
+#pragma unroll
+        for (int kq = 0; kq < KQ; ++kq) {
+          float c6[6], v, dv;
+          load6(row + 6 * min(lane + kq * kWave, M - 1), c6);
+          quintic(c6, x, d, v, dv);
+          v *= wgt;
+          X[kq][0] += v * r.x;
+          X[kq][1] += v * r.y;
+          X[kq][2] += v * r.z;
+          X[kq][3] += v * r.w;
+        }
+      }
+    }
+    float ta[16][4];
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) ta[q][c] = q < A ? bcast(X[0][c], q) * inv : 0.f;
+    _Float16* out = D + (size_t)(p0 + i) * a.ldD;
+#pragma unroll
+    for (int kq = 0; kq < KQ; ++kq) {
+      const int k = lane + kq * kWave;
+      if (k >= M) continue;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) xyz[((size_t)gi * 4 + c) * M + k] = X[kq][c];
+      float v[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        v[q] = 0.f;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[q] += (X[kq][c] * inv) * ta[q][c];
+      }
+      if (A == 16) {
+        mhalf8 h0, h1;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          h0[q] = (_Float16)v[q];
+          h1[q] = (_Float16)v[q + 8];
+        }
+        *reinterpret_cast<mhalf8*>(out + (size_t)k * 16) = h0;
+        *reinterpret_cast<mhalf8*>(out + (size_t)k * 16 + 8) = h1;
+      } else {
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+          if (q < A) out[(size_t)k * A + q] = (_Float16)v[q];
+      }
+    }
+  }
+}
+
+template <int KS>
+__global__ void __launch_bounds__(256) k_embed_bwd_mfma(const EmbedArgs a) {
+  __shared__ __attribute__((aligned(16))) float s_win[kEW][kWinF];
+  const int lane = lane_id();
+  const int wave = uni((int)(threadIdx.x / kWave));
+  const int tile = blockIdx.x * kEW + wave;
+  if (tile >= a.netiles) return;
+  const int4 tl = a.etiles[tile];
+  const int ti = uni(tl.x), p0 = uni(tl.y), na = uni(tl.z);
+  float* win = s_win[wave];
+  const float* em = (const float*)a.em;
+  const Info<float>* infos = (const Info<float>*)a.infos;
+  const int M = a.M, nnei = a.nnei;
+  float* ndv = (float*)a.net_deriv;
+  long long g[kTA];
+#pragma unroll
+  for (int i = 0; i < kTA; ++i) g[i] = i < na ? uni(a.perm[p0 + i]) : 0;
+  if (ti < 0) {
+    for (int i = 0; i < na; ++i) {
+      const long long gi = uni(a.perm[p0 + i]);
+      for (int s = lane; s < nnei * 4; s += kWave) ndv[(size_t)gi * nnei * 4 + s] = 0.f;
+    }
+    return;
+  }
+  constexpr int KP = KS * 32;                     // padded columns
+  constexpr int KQ = (KP + kWave - 1) / kWave;    // columns per lane in the dy layout
+  const float* tab32 = (const float*)a.tables;
+  const float inv = (float)a.inv_nnei;
+  // dy of each atom (lanes over columns), the padding-tail slots, and dy
+  // staged as [atom][c][KP]
+  float dmax = 0.f;
+  for (int i = 0; i < kTA; ++i) {
+    float dy[KQ][4];
+    if (i < na) {
+      const long long gi = uni(a.perm[p0 + i]);
+      bwd_dy<float, float, KQ>(dy, (const float*)a.xyz + (size_t)gi * 4 * M,
+                               (const float*)a.gD + (size_t)(p0 + i) * a.ldD, M, a.A, inv);
+      for (int tj = 0; tj < a.ntypes; ++tj) {
+        const int b = a.sec[tj], nb = a.sec[tj + 1] - b;
+        if (nb <= 0) continue;
+        const int net = a.net_of[ti * a.ntypes + tj];
+        const float* e = em + ((size_t)gi * nnei + b) * 4;
+        const float last = uni(e[(size_t)(nb - 1) * 4]);
+        const int jt = uni(find_tail(e, nb, last));
+        if (jt >= nb) continue;
+        float* nd = ndv + ((size_t)gi * nnei + b) * 4;
+        const float4 r = *reinterpret_cast<const float4*>(e + (size_t)jt * 4);
+        float x = r.x, d;
+        int idx;
+        locate(infos[net], x, idx, d);
+        const float* row = tab32 + a.tab_off[net] + (size_t)idx * M * 6;
+        float pp[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kq = 0; kq < KQ; ++kq) {
+          float c6[6], v, dv;
+          load6(row + 6 * min(lane + kq * kWave, M - 1), c6);
+          quintic(c6, x, d, v, dv);
+          const bool kin = lane + kq * kWave < M;
+          v = kin ? v : 0.f;
+          dv = kin ? dv : 0.f;
+          const float proj = r.x * dy[kq][0] + r.y * dy[kq][1] + r.z * dy[kq][2] + r.w * dy[kq][3];
+          pp[0] += v * dy[kq][0] + dv * proj;
+          pp[1] += v * dy[kq][1];
+          pp[2] += v * dy[kq][2];
+          pp[3] += v * dy[kq][3];
+        }
+        const float wgt = (float)(nb - jt);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) pp[c] = wave_sum(pp[c]) * wgt;
+        if (lane == 0) *reinterpret_cast<float4*>(nd + (size_t)jt * 4) = make_float4(pp[0], pp[1], pp[2], pp[3]);
+        for (int j = jt + 1 + lane; j < nb; j += kWave)
+          *reinterpret_cast<float4*>(nd + (size_t)j * 4) = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    } else {
+#pragma unroll
+      for (int kq = 0; kq < KQ; ++kq)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) dy[kq][c] = 0.f;
+    }
+#pragma unroll
+    for (int kq = 0; kq < KQ; ++kq) {
+      const int k = lane + kq * kWave;
+      if (k < KP)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const float v = k < M ? dy[kq][c] : 0.f;
+          dmax = fmaxf(dmax, fabsf(v));
+          win[(i * 4 + c) * KP + k] = v;
+        }
+    }
+  }
+  // power-of-two scale that brings the largest |dy| to [2^13, 2^14)
+  for (int off = 32; off > 0; off >>= 1) dmax = fmaxf(dmax, __shfl_xor(dmax, off, kWave));
+  int ex = 0;
+  frexpf(uni(dmax), &ex);
+  const float sc = dmax > 0.f ? ldexpf(1.f, 14 - ex) : 1.f, isc = 1.f / sc;
+  const int fn = lane & 15, fa = fn >> 2, fc = fn & 3, fk = lane >> 4;
+  mhalf8 bh[KS], bl[KS];
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks) {
+    const float* src = win + fn * KP + ks * 32 + fk * 8;
+    float4 v0 = *reinterpret_cast<const float4*>(src), v1 = *reinterpret_cast<const float4*>(src + 4);
+    v0.x *= sc; v0.y *= sc; v0.z *= sc; v0.w *= sc;
+    v1.x *= sc; v1.y *= sc; v1.z *= sc; v1.w *= sc;
+    split8(v0, v1, bh[ks], bl[ks]);
+  }
+  for (int tj = 0; tj < a.ntypes; ++tj) {
+    const int net = a.net_of[ti * a.ntypes + tj];
+    const int b = a.sec[tj], nb = a.sec[tj + 1] - b;
+    if (nb <= 0) continue;
+    const Info<float> f = infos[net];
+    float isg[6];  // 1 / sigma_p
+#pragma unroll
+    for (int q = 0; q < 6; ++q) isg[q] = a.tab16_pscale[8 * net + q];
+    const _Float16* thi = a.tab16 + a.tab16_off[4 * net + 2];
+    const _Float16* tlo = a.tab16 + a.tab16_off[4 * net + 3];
+    Walk w;
+    walk_init(w, em, g, na, nnei, b, nb);
+    while (true) {
+      const int R1 = walk_top(w, f, nb);
+      if (R1 < 0) break;
+      const int R0 = max(R1 - (kWinRows - 1), 0);
+      // pass 1: in-window extent of each atom and the row pairs used
+      int cnt[kTA];
+      unsigned pbits = 0;
+#pragma unroll
+      for (int i = 0; i < kTA; ++i) {
+        cnt[i] = 0;
+        if (w.done[i]) continue;
+        while (true) {
+          const NbrW n = load_w(w, i, w.cur[i] + cnt[i], nb, f, R0);
+          const unsigned long long m = __ballot(n.in);
+          const int c = ~m == 0ull ? kWave : (int)__builtin_ctzll(~m);
+          if (lane < c) pbits |= 1u << ((n.row - R0) >> 1);
+          cnt[i] += c;
+          if (c < kWave) break;
+        }
+      }
+      unsigned pb = 0;
+#pragma unroll
+      for (int q = 0; q < kWinRows / 2; ++q)
+        if (__ballot((pbits >> q) & 1u)) pb |= 1u << q;
+      // H of the used row pairs: A row fn = (row R0 + 2 mt + fn / 8, p = fn % 8)
+#pragma unroll
+      for (int mt = 0; mt < kWinRows / 2; ++mt) {
+        if (!((pb >> mt) & 1u)) continue;
+        mfloat4 h = mfloat4{0.f, 0.f, 0.f, 0.f};
+        const size_t ao = ((size_t)(R0 + 2 * mt + (fn >> 3)) * 8 + (fn & 7)) * KP + fk * 8;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+          const mhalf8 ah = *(gmhalf8*)(thi + ao + ks * 32);
+          const mhalf8 al = *(gmhalf8*)(tlo + ao + ks * 32);
+          h = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[ks], h, 0, 0, 0);
+          h = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[ks], h, 0, 0, 0);
+          h = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[ks], h, 0, 0, 0);
+        }
+        // lane holds rows 4 fk + e: row R0 + 2 mt + fk / 2, p = 4 (fk % 2) + e
+        *reinterpret_cast<float4*>(win + fa * kAS + (2 * mt + (fk >> 1)) * kRS + fc * 8 + 4 * (fk & 1)) =
+            make_float4(h[0] * isc, h[1] * isc, h[2] * isc, h[3] * isc);
+      }
+      // pass 2: net_deriv of the window's neighbours
+#pragma unroll
+      for (int i = 0; i < kTA; ++i) {
+        if (w.done[i] || cnt[i] == 0) continue;
+        float* nd = ndv + ((size_t)g[i] * nnei + b) * 4;
+        for (int c0 = 0; c0 < cnt[i]; c0 += kWave) {
+          const NbrW n = load_w(w, i, w.cur[i] + c0, nb, f, R0);
+          if (c0 + lane < cnt[i]) {
+            const float tp = n.t;
+            float T[6];
+            powers(tp, n.d, T);
+            const float t2 = tp * tp;
+            float Td[6] = {0.f, 1.f, 2.f * tp, 3.f * t2, 4.f * t2 * tp, 5.f * t2 * t2};
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+              T[q] *= isg[q];
+              Td[q] *= isg[q];
+            }
+            const float* hs = win + i * kAS + (n.row - R0) * kRS;
+            const float ll[4] = {n.ll.x, n.ll.y, n.ll.z, n.ll.w};
+            float o[4] = {0.f, 0.f, 0.f, 0.f};
+            float pr[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+              const float4 h0 = *reinterpret_cast<const float4*>(hs + c * 8);
+              const float2 h1 = *reinterpret_cast<const float2*>(hs + c * 8 + 4);
+              const float hv[6] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y};
+#pragma unroll
+              for (int p = 0; p < 6; ++p) {
+                o[c] += T[p] * hv[p];
+                pr[p] += ll[c] * hv[p];
+              }
+            }
+#pragma unroll
+            for (int p = 1; p < 6; ++p) o[0] += Td[p] * pr[p];
+            *reinterpret_cast<float4*>(nd + (size_t)(w.cur[i] + c0 + lane) * 4) = make_float4(o[0], o[1], o[2], o[3]);
+          }
+        }
+        w.cur[i] += cnt[i];
+      }
+    }
   }
 }
 
@@ -321,6 +858,93 @@ void launch_embed_bwd(const EmbedArgs& a, hipStream_t st) {
                                            (const T*)a.tables, a.tab_off, (const Info<T>*)a.infos, a.net_of,
                                            (const T*)a.em, a.row_type, a.sec, a.ntypes, a.nrows, a.nnei, a.M, a.A,
                                            a.ldD > 0 ? a.ldD : a.M * a.A, (T)a.inv_nnei));
+  DPMI_LAUNCH_CHECK();
+}
+
+bool embed_mfma_supported(int M) { return M > 0 && M <= 128 && M % 4 == 0; }
+
+void build_split_tables(const double* table, const long long* tab_off, const int* nrows, const double* infos,
+                        int nnets, int M, std::vector<_Float16>& out, std::vector<long long>& off,
+                        std::vector<float>& pscale) {
+  const int LDf = (M + 15) / 16 * 16, KP = (M + 31) / 32 * 32;
+  off.assign(4 * (size_t)nnets, 0);
+  long long tot = 0;
+  for (int n = 0; n < nnets; ++n) {
+    const long long R = nrows[n] + kWinRows;
+    off[4 * n] = tot;
+    tot += R * LDf * 8;
+    off[4 * n + 1] = tot;
+    tot += R * LDf * 8;
+    off[4 * n + 2] = tot;
+    tot += R * 8 * KP;
+    off[4 * n + 3] = tot;
+    tot += R * 8 * KP;
+  }
+  out.assign((size_t)tot, (_Float16)0.f);
+  pscale.assign(8 * (size_t)nnets, 1.f);
+  auto put = [&](long long o_hi, long long o_lo, size_t at, double v) {
+    const _Float16 hi = (_Float16)v;
+    out[o_hi + at] = hi;
+    out[o_lo + at] = (_Float16)(v - (double)hi);
+  };
+  for (int n = 0; n < nnets; ++n) {
+    float fi[6];
+    for (int q = 0; q < 6; ++q) fi[q] = (float)infos[n * 6 + q];
+    const Info<float> f = make_info(fi);  // the segment rule of the kernels
+    if (f.ntail >= nrows[n]) throw std::invalid_argument("table has fewer rows than its info addresses");
+    const double* T = table + tab_off[n];
+    // adjoint scale per power: 2^e with 2^13 <= sigma_p max|a_p| < 2^14
+    double sig[6];
+    for (int p = 0; p < 6; ++p) {
+      double mx = 0.0;
+      for (size_t q = 0; q < (size_t)nrows[n] * M; ++q) mx = std::max(mx, std::fabs(T[q * 6 + p]));
+      int ex = 0;
+      if (mx > 0.0) std::frexp(mx, &ex);
+      sig[p] = mx > 0.0 ? std::ldexp(1.0, 14 - ex) : 1.0;
+      pscale[8 * n + p] = (float)(1.0 / sig[p]);
+    }
+    for (int r = 0; r < nrows[n]; ++r) {
+      const double s = r < f.n0 ? (double)f.s0 : (double)f.s1;
+      for (int k = 0; k < M; ++k) {
+        double sp = 1.0;
+        for (int p = 0; p < 6; ++p) {
+          const double a = T[((size_t)r * M + k) * 6 + p];
+          put(off[4 * n], off[4 * n + 1], ((size_t)r * LDf + k) * 8 + p, a * sp);
+          put(off[4 * n + 2], off[4 * n + 3], ((size_t)r * 8 + p) * KP + k, a * sig[p]);
+          sp *= s;
+        }
+      }
+    }
+  }
+}
+
+void launch_embed_fwd_mfma(const EmbedArgs& a, hipStream_t st) {
+  if (a.netiles <= 0) return;
+  if (!embed_mfma_supported(a.M)) throw std::invalid_argument("matrix-core embedding needs M <= 128, M % 4 == 0");
+  const dim3 grid(ceil_div(a.netiles, kEW)), block(kEW * kWave);
+  switch ((a.M + 15) / 16) {
+    case 1: hipLaunchKernelGGL(k_embed_fwd_mfma<1>, grid, block, 0, st, a); break;
+    case 2: hipLaunchKernelGGL(k_embed_fwd_mfma<2>, grid, block, 0, st, a); break;
+    case 3: hipLaunchKernelGGL(k_embed_fwd_mfma<3>, grid, block, 0, st, a); break;
+    case 4: hipLaunchKernelGGL(k_embed_fwd_mfma<4>, grid, block, 0, st, a); break;
+    case 5: hipLaunchKernelGGL(k_embed_fwd_mfma<5>, grid, block, 0, st, a); break;
+    case 6: hipLaunchKernelGGL(k_embed_fwd_mfma<6>, grid, block, 0, st, a); break;
+    case 7: hipLaunchKernelGGL(k_embed_fwd_mfma<7>, grid, block, 0, st, a); break;
+    default: hipLaunchKernelGGL(k_embed_fwd_mfma<8>, grid, block, 0, st, a); break;
+  }
+  DPMI_LAUNCH_CHECK();
+}
+
+void launch_embed_bwd_mfma(const EmbedArgs& a, hipStream_t st) {
+  if (a.netiles <= 0) return;
+  if (!embed_mfma_supported(a.M)) throw std::invalid_argument("matrix-core embedding needs M <= 128, M % 4 == 0");
+  const dim3 grid(ceil_div(a.netiles, kEW)), block(kEW * kWave);
+  switch ((a.M + 31) / 32) {
+    case 1: hipLaunchKernelGGL(k_embed_bwd_mfma<1>, grid, block, 0, st, a); break;
+    case 2: hipLaunchKernelGGL(k_embed_bwd_mfma<2>, grid, block, 0, st, a); break;
+    case 3: hipLaunchKernelGGL(k_embed_bwd_mfma<3>, grid, block, 0, st, a); break;
+    default: hipLaunchKernelGGL(k_embed_bwd_mfma<4>, grid, block, 0, st, a); break;
+  }
   DPMI_LAUNCH_CHECK();
 }
 

@@ -142,6 +142,7 @@ SeaModel::SeaModel(const dpmi_sea_model_desc& d) {
   davg.assign(d.davg, d.davg + (size_t)ntypes * nnei * 4);
   dstd.assign(d.dstd, d.dstd + (size_t)ntypes * nnei * 4);
   tab_off.assign(nnets, 0);
+  nspline.assign(d.nspline, d.nspline + nnets);
   long long tot = 0;
   for (int n = 0; n < nnets; ++n) {
     tab_off[n] = tot;
@@ -224,22 +225,38 @@ void SeaModel::build_mixed(const dpmi_sea_model_desc& d) {
   void* infos = make_table_infos(info_host.data(), nnets, true);
   d_infos_f = dupload(static_cast<const char*>(infos), table_info_bytes(true) * nnets);
   free_table_infos(infos, true);
+  if (embed_mfma_supported(M)) {
+    std::vector<_Float16> t16;
+    std::vector<long long> off16;
+    std::vector<float> psc;
+    build_split_tables(table_host.data(), tab_off.data(), nspline.data(), info_host.data(), nnets, M, t16, off16,
+                       psc);
+    d_tab16 = dupload(t16.data(), t16.size());
+    d_tab16_off = dupload(off16.data(), off16.size());
+    d_tab16_pscale = dupload(psc.data(), psc.size());
+    embed_mfma = true;
+  }
   std::vector<double> shift(ntypes);
   for (int t = 0; t < ntypes; ++t) shift[t] = out_b_host[t] + bias_atom_e[t];
   d_shift = dupload(shift.data(), shift.size());
   if (!mixed_unsupported.empty()) return;
   f16.resize((size_t)ntypes * H);
-  std::vector<std::vector<_Float16>> fw((size_t)ntypes * H), bw((size_t)ntypes * H);
+  std::vector<std::vector<_Float16>> fw((size_t)ntypes * H), bw((size_t)ntypes * H), fwl((size_t)ntypes * H),
+      bwl((size_t)ntypes * H);
   for (int t = 0; t < ntypes; ++t)
     for (int l = 0; l < H; ++l) {
       const size_t k = (size_t)t * nlayers + l;
       const int ni = width[l], no = width[l + 1], kp = kpad[l], np = kpad[l + 1];
       std::vector<_Float16> wt((size_t)np * kp, (_Float16)0.f), w((size_t)kp * np, (_Float16)0.f);
+      std::vector<_Float16> wtl((size_t)np * kp, (_Float16)0.f), wl((size_t)kp * np, (_Float16)0.f);
       for (int i = 0; i < ni; ++i)
         for (int o = 0; o < no; ++o) {
-          const _Float16 v = (_Float16)d.fit_w[k][(size_t)i * no + o];
+          const double x = d.fit_w[k][(size_t)i * no + o];
+          const _Float16 v = (_Float16)x, vl = (_Float16)(x - (double)v);
           wt[(size_t)o * kp + i] = v;
           w[(size_t)i * np + o] = v;
+          wtl[(size_t)o * kp + i] = vl;
+          wl[(size_t)i * np + o] = vl;
         }
       std::vector<float> b(np, 0.f), idt(np, 0.f), ow(np, 0.f);
       for (int o = 0; o < no; ++o) {
@@ -249,6 +266,8 @@ void SeaModel::build_mixed(const dpmi_sea_model_desc& d) {
       Fit16Layer& L = f16[(size_t)t * H + l];
       L.wt = dupload(wt.data(), wt.size());
       L.w = dupload(w.data(), w.size());
+      L.wt_lo = dupload(wtl.data(), wtl.size());
+      L.w_lo = dupload(wl.data(), wl.size());
       L.b = dupload(b.data(), b.size());
       L.idt = d.fit_idt[k] ? dupload(idt.data(), idt.size()) : nullptr;
       if (l == H - 1) {
@@ -259,11 +278,13 @@ void SeaModel::build_mixed(const dpmi_sea_model_desc& d) {
       L.res_mode = res_mode[l];
       L.k_pad = kp;
       L.n_pad = np;
-      for (const void* p : {(const void*)L.wt, (const void*)L.w, (const void*)L.b, (const void*)L.idt,
-                            (const void*)L.out_w})
+      for (const void* p : {(const void*)L.wt, (const void*)L.w, (const void*)L.wt_lo, (const void*)L.w_lo,
+                            (const void*)L.b, (const void*)L.idt, (const void*)L.out_w})
         if (p) f16_mem.push_back(const_cast<void*>(p));
       fw[(size_t)t * H + l] = std::move(wt);
       bw[(size_t)t * H + l] = std::move(w);
+      fwl[(size_t)t * H + l] = std::move(wtl);
+      bwl[(size_t)t * H + l] = std::move(wl);
     }
   if (!fused) return;
   std::vector<FusedLayer> fl((size_t)ntypes * H);
@@ -274,11 +295,16 @@ void SeaModel::build_mixed(const dpmi_sea_model_desc& d) {
     _Float16* wf = dupload(sw.data(), sw.size());
     swizzle_bfrags(bw[k].data(), L.k_pad, L.n_pad, sw.data());
     _Float16* wb = dupload(sw.data(), sw.size());
-    f16_mem.push_back(wf);
-    f16_mem.push_back(wb);
+    swizzle_bfrags(fwl[k].data(), L.n_pad, L.k_pad, sw.data());
+    _Float16* wfl = dupload(sw.data(), sw.size());
+    swizzle_bfrags(bwl[k].data(), L.k_pad, L.n_pad, sw.data());
+    _Float16* wbl = dupload(sw.data(), sw.size());
+    for (_Float16* q : {wf, wb, wfl, wbl}) f16_mem.push_back(q);
     FusedLayer& F = fl[k];
     F.wf = wf;
     F.wb = wb;
+    F.wf_lo = wfl;
+    F.wb_lo = wbl;
     F.b = L.b;
     F.idt = L.idt;
     F.out_w = L.out_w;
@@ -305,6 +331,9 @@ SeaModel::~SeaModel() {
   dfree(d_infos_f);
   dfree(d_shift);
   dfree(d_fused);
+  dfree(d_tab16);
+  dfree(d_tab16_off);
+  dfree(d_tab16_pscale);
   for (auto& p : f16_mem) dfree(p);
 }
 
@@ -401,6 +430,14 @@ SeaEngine::SeaEngine(const SeaModel* m, int nframes_, const int* natoms_h, const
     const std::vector<int4> tiles = fitting_tile_order(tstart, nt, 64, chip.first, chip.second);
     ntiles = (int)tiles.size();
     d_tiles = dupload(tiles.data(), tiles.size());
+    // embedding tiles: groups of kEmbedTileAtoms descriptor positions of one
+    // type (atoms of negative type last, as type -1)
+    std::vector<int4> et;
+    for (int t = 0; t <= nt; ++t)
+      for (int p = tstart[t]; p < tstart[t + 1]; p += kEmbedTileAtoms)
+        et.push_back(make_int4(t < nt ? t : -1, p, std::min(kEmbedTileAtoms, tstart[t + 1] - p), 0));
+    netiles = (int)et.size();
+    if (netiles) d_etiles = dupload(et.data(), et.size());
   }
   stage_names = precision == DPMI_MIXED && m->fused
                     ? "geometry;nlist;env_mat;embed_fwd;fitting;embed_bwd;force_virial;reduce"
@@ -441,6 +478,7 @@ SeaEngine::~SeaEngine() {
     dfree(*p);
   dfree(d_Dh);
   dfree(d_tiles);
+  dfree(d_etiles);
   for (auto& p : d_y16) dfree(p);
   for (auto& p : d_x16) dfree(p);
   dfree(d_geom);
@@ -549,6 +587,13 @@ EmbedArgs SeaEngine::embed_args() const {
   ta.inv_nnei = 1.0 / m.nnei;
   ta.waves = tune.fwd_waves;
   ta.waves_bwd = tune.bwd_waves;
+  ta.perm = d_perm;
+  ta.tab16 = m.d_tab16;
+  ta.tab16_off = m.d_tab16_off;
+  ta.tab16_pscale = m.d_tab16_pscale;
+  ta.etiles = d_etiles;
+  ta.netiles = netiles;
+  ta.flag = d_flag;
   return ta;
 }
 
@@ -648,7 +693,11 @@ void SeaEngine::compute_mixed_descriptor_fitting(hipStream_t st, int& stage) {
   ta.infos = m.d_infos_f;
   ta.em = d_em_f;
   ta.ldD = k0;
-  launch_embed_fwd<float, _Float16>(ta, st);
+  const bool mfma_fwd = m.embed_mfma && (tune.embed_mfma & 1), mfma_bwd = m.embed_mfma && (tune.embed_mfma & 2);
+  if (mfma_fwd)
+    launch_embed_fwd_mfma(ta, st);
+  else
+    launch_embed_fwd<float, _Float16>(ta, st);
   mark(st, stage++);
 
   const int H = m.nlayers - 1;
@@ -668,7 +717,10 @@ void SeaEngine::compute_mixed_descriptor_fitting(hipStream_t st, int& stage) {
     if (tstart[m.ntypes + 1] > tstart[m.ntypes]) launch_fill_double(d_atom_e, 0.0, ntot, st);
     launch_fit_fused(fa, ntiles, st);
     mark(st, stage++);
-    launch_embed_bwd<float, float>(ta, st);
+    if (mfma_bwd)
+      launch_embed_bwd_mfma(ta, st);
+    else
+      launch_embed_bwd<float, float>(ta, st);
     mark(st, stage++);
     return;
   }
@@ -713,7 +765,10 @@ void SeaEngine::compute_mixed_descriptor_fitting(hipStream_t st, int& stage) {
   }
   mark(st, stage++);
 
-  launch_embed_bwd<float, float>(ta, st);
+  if (mfma_bwd)
+    launch_embed_bwd_mfma(ta, st);
+  else
+    launch_embed_bwd<float, float>(ta, st);
   mark(st, stage++);
 }
 
@@ -858,7 +913,12 @@ void SeaEngine::load_cached_tune() {
 // per stage, the fastest value; the result is cached for the size bucket.
 void SeaEngine::autotune(const double* coord, const double* box_h, int reps, hipStream_t st) {
   const int env_c[3] = {128, 256, 512}, fwd_c[3] = {1, 2, 4}, bwd_c[3] = {1, 2, 4}, frc_c[3] = {4, 8, 16};
-  double t_env[3], t_fwd[3], t_bwd[3], t_frc[3];
+  // candidates 0..2: per-neighbour embedding kernels with the shapes above;
+  // candidate 3 (mixed engine with matrix-core tables): the table-space
+  // embedding kernels, whose launch shape is fixed
+  const bool try_mfma = precision == DPMI_MIXED && model->embed_mfma;
+  const int ncand = try_mfma ? 4 : 3;
+  double t_env[4], t_fwd[4], t_bwd[4], t_frc[4];
   double *e = nullptr, *f = nullptr, *v = nullptr;
   DPMI_CHECK(hipMalloc(&e, sizeof(double) * nframes));
   DPMI_CHECK(hipMalloc(&f, sizeof(double) * 3 * (size_t)ntot));
@@ -877,11 +937,13 @@ void SeaEngine::autotune(const double* coord, const double* box_h, int reps, hip
   };
   try {
     compute(coord, box_h, e, f, v, nullptr, st);  // geometry and buffers
-    for (int c = 0; c < 3; ++c) {
-      tune.env_tpb = env_c[c];
-      tune.fwd_waves = fwd_c[c];
-      tune.bwd_waves = bwd_c[c];
-      tune.force_rows = frc_c[c];
+    for (int c = 0; c < ncand; ++c) {
+      const int k = c < 3 ? c : 1;
+      tune.env_tpb = env_c[k];
+      tune.fwd_waves = fwd_c[k];
+      tune.bwd_waves = bwd_c[k];
+      tune.force_rows = frc_c[k];
+      tune.embed_mfma = c < 3 ? 0 : 3;
       timing = true;
       reset_timing();
       for (int r = 0; r < std::max(reps, 1); ++r) compute(coord, box_h, e, f, v, nullptr, st);
@@ -910,6 +972,11 @@ void SeaEngine::autotune(const double* coord, const double* box_h, int reps, hip
   tune.fwd_waves = fwd_c[best(t_fwd)];
   tune.bwd_waves = bwd_c[best(t_bwd)];
   tune.force_rows = frc_c[best(t_frc)];
+  tune.embed_mfma = 0;
+  if (try_mfma) {
+    if (t_fwd[3] < t_fwd[best(t_fwd)]) tune.embed_mfma |= 1;
+    if (t_bwd[3] < t_bwd[best(t_bwd)]) tune.embed_mfma |= 2;
+  }
   std::lock_guard<std::mutex> lk(g_tune_mu);
   tune_cache()[tune_key(precision, ntot)] = tune;
 }
@@ -957,7 +1024,7 @@ int SeaEngine::status() {
   DPMI_CHECK(hipMemcpy(&flag, d_flag, sizeof(int), hipMemcpyDeviceToHost));
   if (flag & kFlagNlistCapacity) return DPMI_ERR_NLIST_CAPACITY;
   if (flag & kFlagGhostCapacity) return DPMI_ERR_MEMORY;
-  if (flag & (kFlagIndexRange | kFlagTypeRange)) return DPMI_ERR_RANGE;
+  if (flag & (kFlagIndexRange | kFlagTypeRange | kFlagHalfRange)) return DPMI_ERR_RANGE;
   return DPMI_OK;
 }
 

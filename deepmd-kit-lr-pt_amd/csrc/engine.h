@@ -47,6 +47,12 @@ struct SeaModel {
   std::vector<void*> f16_mem;
   bool fused = false;             // the fused fitting kernel covers this net
   FusedLayer* d_fused = nullptr;  // [ntypes * (nlayers - 1)]
+  // matrix-core embedding: split FP16 tables (forward and adjoint layouts)
+  std::vector<int> nspline;  // table rows per net
+  bool embed_mfma = false;
+  _Float16* d_tab16 = nullptr;
+  long long* d_tab16_off = nullptr;
+  float* d_tab16_pscale = nullptr;
 };
 
 struct SeaEngine {
@@ -121,6 +127,8 @@ struct SeaEngine {
   _Float16* d_Dh = nullptr;
   int4* d_tiles = nullptr;
   int ntiles = 0;
+  int4* d_etiles = nullptr;  // embedding tiles: 4 descriptor positions of one type
+  int netiles = 0;
   std::vector<float*> d_y16;
   std::vector<_Float16*> d_x16;
   void compute_mixed_descriptor_fitting(hipStream_t st, int& stage);

@@ -11,8 +11,9 @@
 //                   (g from memory, or w_out broadcast for the last hidden
 //                   layer), g_in = gz W^T + g (residual), stored in FP32.
 // Tiles: 64 x 64 outputs per 256-thread workgroup, 4 waves of 32 x 32, each
-// 2 x 2 v_mfma_f32_16x16x32_f16 per 32-wide K step; A and B tiles are staged
-// through LDS with padded rows.  Widths are padded to multiples of 64 with
+// 2 x 2 x 2 v_mfma_f32_16x16x32_f16 per 32-wide K step (weights split into
+// FP16 hi + lo parts, both multiplied); A and B tiles are staged through LDS
+// with padded rows.  Widths are padded to multiples of 64 with
 // zero weights (padded activations stay exactly zero).
 #include <hip/hip_fp16.h>
 
@@ -40,7 +41,8 @@ struct GemmArgs {
   int ldy;
   const float* a_idt; // [K] or null (kAGrad*)
   int a_act;          // kAGrad*: 1 = tanh (factor 1 - y^2), 0 = linear
-  const _Float16* Bt; // [N][ldb]
+  const _Float16* Bt; // [N][ldb] (hi part)
+  const _Float16* Btl; // [N][ldb] (lo part)
   int ldb;
   int rows, N, K;
   // forward epilogue
@@ -67,6 +69,7 @@ template <int AMODE, int EMODE>
 __global__ void __launch_bounds__(256) k_fit_gemm(GemmArgs p) {
   __shared__ _Float16 As[BM][BK + LDS_PAD];
   __shared__ _Float16 Bs[BN][BK + LDS_PAD];
+  __shared__ _Float16 Bsl[BN][BK + LDS_PAD];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
@@ -108,21 +111,28 @@ __global__ void __launch_bounds__(256) k_fit_gemm(GemmArgs p) {
     // ---- B tile (rows of Bt are output columns)
     {
       const int n = col0 + lr;
-      half8 v = *reinterpret_cast<const half8*>(p.Bt + (size_t)n * p.ldb + k0 + lk);
+      const half8 v = *reinterpret_cast<const half8*>(p.Bt + (size_t)n * p.ldb + k0 + lk);
+      const half8 vl = *reinterpret_cast<const half8*>(p.Btl + (size_t)n * p.ldb + k0 + lk);
       *reinterpret_cast<half8*>(&Bs[lr][lk]) = v;
+      *reinterpret_cast<half8*>(&Bsl[lr][lk]) = vl;
     }
     __syncthreads();
-    half8 af[2], bf[2];
+    half8 af[2], bf[2], bl[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
       af[i] = *reinterpret_cast<const half8*>(&As[wr * 32 + i * 16 + (lane & 15)][(lane >> 4) * 8]);
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < 2; ++j) {
       bf[j] = *reinterpret_cast<const half8*>(&Bs[wc * 32 + j * 16 + (lane & 15)][(lane >> 4) * 8]);
+      bl[j] = *reinterpret_cast<const half8*>(&Bsl[wc * 32 + j * 16 + (lane & 15)][(lane >> 4) * 8]);
+    }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(af[i], bf[j], acc[i][j], 0, 0, 0);
+      for (int j = 0; j < 2; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(af[i], bf[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(af[i], bl[j], acc[i][j], 0, 0, 0);
+      }
     __syncthreads();
   }
 
@@ -196,6 +206,7 @@ void launch_fit16_fwd(const Fit16Layer& L, const _Float16* x, int ldx, int rows,
   p.A = x;
   p.lda = ldx;
   p.Bt = L.wt;
+  p.Btl = L.wt_lo;
   p.ldb = L.k_pad;
   p.rows = rows;
   p.N = L.n_pad;
@@ -227,6 +238,7 @@ void launch_fit16_bwd(const Fit16Layer& L, const float* g, int ldg, bool g_bcast
   p.a_idt = L.idt;
   p.a_act = L.act;
   p.Bt = L.w;  // [n_in_pad][n_out_pad]: rows are the outputs of the backward GEMM
+  p.Btl = L.w_lo;
   p.ldb = L.n_pad;
   p.rows = rows;
   p.N = L.k_pad;

@@ -62,22 +62,25 @@ __device__ __forceinline__ half8 load_bfrag(const _Float16* __restrict__ w, int 
   return *(ghalf8*)(w + ((size_t)(kt * NT + ct) * 64 + lane) * 8);
 }
 
-// C[64 x (ntw tiles of 16 per wave)] += A[64 x K] B over K = 32 KT, A from an
-// LDS tile (row stride ld), B fragments from a swizzled matrix with NT
-// column tiles; tile j of this wave is column tile ct0 + j (used if < NT).
-// KT is a multiple of kPf.  Loads are unconditional (clamped tile and step
-// indices) so that the prefetch ring is not serialised by the wait counters.
+// C[64 x (ntw tiles of 16 per wave)] += A[64 x K] (B hi + B lo) over
+// K = 32 KT, A from an LDS tile (row stride ld), B fragments from swizzled
+// matrices with NT column tiles; tile j of this wave is column tile ct0 + j
+// (used if < NT).  KT is a multiple of kPf.  Loads are unconditional
+// (clamped tile and step indices) so that the prefetch ring is not
+// serialised by the wait counters.
 __device__ __forceinline__ void gemm_lds_a(floatx4 (&acc)[4][2], const _Float16* a_lds, int ld,
-                                           const _Float16* __restrict__ w, int KT, int NT, int ct0, int ntw,
-                                           int lane) {
+                                           const _Float16* __restrict__ w, const _Float16* __restrict__ wl, int KT,
+                                           int NT, int ct0, int ntw, int lane) {
   const int lr = lane & 15, lg = lane >> 4;
   const bool use0 = ntw > 0 && ct0 < NT, use1 = ntw > 1 && ct0 + 1 < NT;
   const int c0 = min(ct0, NT - 1), c1 = min(ct0 + 1, NT - 1);
-  half8 bq[kPf][2];
+  half8 bq[kPf][2], bl[kPf][2];
 #pragma unroll
   for (int s = 0; s < kPf; ++s) {
     bq[s][0] = load_bfrag(w, s, NT, c0, lane);
     bq[s][1] = load_bfrag(w, s, NT, c1, lane);
+    bl[s][0] = load_bfrag(wl, s, NT, c0, lane);
+    bl[s][1] = load_bfrag(wl, s, NT, c1, lane);
   }
   for (int kt0 = 0; kt0 < KT; kt0 += kPf) {
 #pragma unroll
@@ -88,13 +91,21 @@ __device__ __forceinline__ void gemm_lds_a(floatx4 (&acc)[4][2], const _Float16*
       for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const half8*>(a_lds + (i * 16 + lr) * ld + k * 32 + lg * 8);
       if (use0)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][0], acc[i][0], 0, 0, 0);
+        for (int i = 0; i < 4; ++i) {
+          acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][0], acc[i][0], 0, 0, 0);
+          acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bl[s][0], acc[i][0], 0, 0, 0);
+        }
       if (use1)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][1], acc[i][1], 0, 0, 0);
+        for (int i = 0; i < 4; ++i) {
+          acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][1], acc[i][1], 0, 0, 0);
+          acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bl[s][1], acc[i][1], 0, 0, 0);
+        }
       const int kn = min(k + kPf, KT - 1);
       bq[s][0] = load_bfrag(w, kn, NT, c0, lane);
       bq[s][1] = load_bfrag(w, kn, NT, c1, lane);
+      bl[s][0] = load_bfrag(wl, kn, NT, c0, lane);
+      bl[s][1] = load_bfrag(wl, kn, NT, c1, lane);
     }
   }
 }
@@ -148,12 +159,14 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
       const int c0 = min(ct0, NT - 1), c1 = min(ct0 + 1, NT - 1);
       const half4 hz = {(_Float16)0.f, (_Float16)0.f, (_Float16)0.f, (_Float16)0.f};
       half4 ar[kPf];
-      half8 bq[kPf][2];
+      half8 bq[kPf][2], bl[kPf][2];
 #pragma unroll
       for (int s = 0; s < kPf; ++s) {
         ar[s] = __builtin_nontemporal_load((ghalf4*)(dsrc + s * 32));
         bq[s][0] = load_bfrag(L.wf, s, NT, c0, lane);
         bq[s][1] = load_bfrag(L.wf, s, NT, c1, lane);
+        bl[s][0] = load_bfrag(L.wf_lo, s, NT, c0, lane);
+        bl[s][1] = load_bfrag(L.wf_lo, s, NT, c1, lane);
       }
       *reinterpret_cast<half4*>(&as[0][arow][acol]) = aval ? ar[0] : hz;
       __syncthreads();
@@ -168,21 +181,27 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
           for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const half8*>(&as[k & 1][i * 16 + lr][lg * 8]);
           if (use0)
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < 4; ++i) {
               acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][0], acc[i][0], 0, 0, 0);
+              acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bl[s][0], acc[i][0], 0, 0, 0);
+            }
           if (use1)
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < 4; ++i) {
               acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][1], acc[i][1], 0, 0, 0);
+              acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bl[s][1], acc[i][1], 0, 0, 0);
+            }
           const int kn = min(k + kPf, KT - 1);
           ar[s] = __builtin_nontemporal_load((ghalf4*)(dsrc + kn * 32));
           bq[s][0] = load_bfrag(L.wf, kn, NT, c0, lane);
           bq[s][1] = load_bfrag(L.wf, kn, NT, c1, lane);
+          bl[s][0] = load_bfrag(L.wf_lo, kn, NT, c0, lane);
+          bl[s][1] = load_bfrag(L.wf_lo, kn, NT, c1, lane);
           __syncthreads();
         }
       }
     } else {
-      gemm_lds_a(acc, &xs[src][0][0], kXld, L.wf, KT, NT, ct0, ntw, lane);
+      gemm_lds_a(acc, &xs[src][0][0], kXld, L.wf, L.wf_lo, KT, NT, ct0, ntw, lane);
     }
     // epilogue: lane holds rows lg*4 + e of each 16x16 tile, column lr
 #pragma unroll
@@ -277,7 +296,7 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
       const int otw = (OT + kWaves - 1) / kWaves;
       floatx4 acc[4][2];
       zero_acc(acc);
-      gemm_lds_a(acc, &xs[buf][0][0], kXld, L.wb, KT, OT, wave * otw, otw, lane);
+      gemm_lds_a(acc, &xs[buf][0][0], kXld, L.wb, L.wb_lo, KT, OT, wave * otw, otw, lane);
       // g_l = gz W^T (+ g_{l+1} for residual layers: same positions)
 #pragma unroll
       for (int i = 0; i < 4; ++i)
@@ -293,7 +312,7 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
         const int ot0 = c * 2 * kWaves + wave * 2;
         floatx4 acc[4][2];
         zero_acc(acc);
-        gemm_lds_a(acc, &xs[buf][0][0], kXld, L.wb, KT, OT, ot0, 2, lane);
+        gemm_lds_a(acc, &xs[buf][0][0], kXld, L.wb, L.wb_lo, KT, OT, ot0, 2, lane);
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
           if (ot0 + j >= OT) continue;

@@ -128,6 +128,14 @@ struct EmbedArgs {
   double inv_nnei = 0;
   int waves = 0;      // forward: waves (rows) per workgroup (0: 4)
   int waves_bwd = 0;  // backward: waves (rows) per workgroup (0: 1)
+  // matrix-core (table-space) kernels of the mixed engine
+  const int* perm = nullptr;             // [nrows] descriptor position -> row
+  const _Float16* tab16 = nullptr;       // split FP16 tables of every net
+  const long long* tab16_off = nullptr;  // [4 nnets]: forward hi, forward lo, adjoint hi, adjoint lo
+  const float* tab16_pscale = nullptr;   // [8 nnets]: 1 / sigma_p of the adjoint tables
+  const int4* etiles = nullptr;          // [netiles] (type or -1, first position, count <= 4, 0)
+  int netiles = 0;
+  int* flag = nullptr;                   // device status word (kFlagHalfRange)
 };
 void* make_table_infos(const double* infos_host, int nnets, bool fp32);
 size_t table_info_bytes(bool fp32);
@@ -136,6 +144,20 @@ template <typename T, typename TD>
 void launch_embed_fwd(const EmbedArgs& a, hipStream_t st);
 template <typename T, typename TG>
 void launch_embed_bwd(const EmbedArgs& a, hipStream_t st);
+// Mixed engine, matrix cores: forward (xyz, D) and adjoint (net_deriv) of the
+// table embedding computed in table space; M <= 128 and M % 4 == 0.
+// Split tables: per net, rows [rows + 16] as FP16 hi + lo parts (p < 6;
+// zero for p = 6, 7 and padded rows / columns): forward a_p s^p (s the
+// stride of the row's segment) in layout [row][16 ceil(M/16)][8 p]; adjoint
+// sigma_p a_p (sigma_p a power of two per net and p, 1 / sigma_p returned
+// in pscale[8 net + p]) in layout [row][8 p][32 ceil(M/32)].
+constexpr int kEmbedTileAtoms = 4;
+bool embed_mfma_supported(int M);
+void build_split_tables(const double* table, const long long* tab_off, const int* nrows, const double* infos,
+                        int nnets, int M, std::vector<_Float16>& out, std::vector<long long>& off,
+                        std::vector<float>& pscale);
+void launch_embed_fwd_mfma(const EmbedArgs& a, hipStream_t st);
+void launch_embed_bwd_mfma(const EmbedArgs& a, hipStream_t st);
 
 // ------------------------------------------------------------- force.hip --
 template <typename T>
@@ -182,6 +204,10 @@ struct Tune {
   int fwd_waves = 4;
   int bwd_waves = 1;
   int force_rows = 16;
+  // mixed engine: bit 0 forward, bit 1 adjoint of the table embedding on the
+  // matrix cores (table space); pays off when many neighbours of an atom
+  // share table rows, so it is off until the autotuner measures it faster
+  int embed_mfma = 0;
 };
 void launch_force_virial_rows(const ForceRowsArgs& a, hipStream_t st);
 // force[loc] += force_ext[e] for every extended atom e < nall of each frame
@@ -213,8 +239,10 @@ void launch_fill_rows(double* g, const double* w, long long n, int width, hipStr
 // ------------------------------------------------------------- fit16.hip --
 // One fitting layer of the mixed engine, widths padded to multiples of 64.
 struct Fit16Layer {
-  const _Float16* wt = nullptr;  // [n_pad][k_pad] forward operand (W^T)
-  const _Float16* w = nullptr;   // [k_pad][n_pad] backward operand (W)
+  const _Float16* wt = nullptr;  // [n_pad][k_pad] forward operand (W^T), FP16 hi part
+  const _Float16* w = nullptr;   // [k_pad][n_pad] backward operand (W), FP16 hi part
+  const _Float16* wt_lo = nullptr;  // lo parts (W - hi in FP16), same layouts
+  const _Float16* w_lo = nullptr;
   const float* b = nullptr;      // [n_pad]
   const float* idt = nullptr;    // [n_pad] or null
   const float* out_w = nullptr;  // [n_pad] output-layer weights (last hidden layer) or null
@@ -235,9 +263,14 @@ void launch_energy_out(const float* e_perm, const int* pos, const int* row_type,
 // --------------------------------------------------------- fitfused.hip --
 // Whole fitting net (forward, energy, backward to dE/dD) of the mixed engine
 // in one launch, one workgroup per tile of <= 64 rows of one type.
+// Weights are split into FP16 hi + lo parts (W = hi + lo to about 22 bits):
+// each product is x hi + x lo with FP32 accumulation, so rounding the
+// weights to FP16 leaves no systematic error in the energy.
 struct FusedLayer {
-  const _Float16* wf = nullptr;  // forward B fragments: Bt = W^T [N][K], swizzle_bfrags order
-  const _Float16* wb = nullptr;  // backward B fragments: Bt = W [K][N]
+  const _Float16* wf = nullptr;     // forward B fragments: Bt = W^T [N][K], swizzle_bfrags order (hi)
+  const _Float16* wb = nullptr;     // backward B fragments: Bt = W [K][N] (hi)
+  const _Float16* wf_lo = nullptr;  // lo parts, same layouts
+  const _Float16* wb_lo = nullptr;
   const float* b = nullptr;      // [N]
   const float* idt = nullptr;    // [N] or null
   const float* out_w = nullptr;  // [N] output weights (last hidden layer)

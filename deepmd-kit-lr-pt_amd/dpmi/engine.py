@@ -152,11 +152,28 @@ class Engine:
                                               b.ctypes.data_as(C.c_void_p), int(reps), st))
         return self.tuning()
 
+    TUNING_KEYS = ("env_threads_per_block", "embed_fwd_waves", "embed_bwd_waves", "force_rows_per_workgroup",
+                   "embed_matrix_cores")
+
     def tuning(self) -> dict:
-        k = (C.c_int * 4)()
+        """Current launch knobs; embed_matrix_cores is a mask (bit 0 forward,
+        bit 1 adjoint of the table embedding on the matrix cores, mixed
+        engine only)."""
+        k = (C.c_int * 5)()
         check(load().dpmi_sea_engine_tuning(self.handle, k))
-        return {"env_threads_per_block": k[0], "embed_fwd_waves": k[1], "embed_bwd_waves": k[2],
-                "force_rows_per_workgroup": k[3]}
+        return dict(zip(self.TUNING_KEYS, list(k)))
+
+    def set_tuning(self, **knobs) -> None:
+        """Sets launch knobs by name (keys of tuning()); the others keep
+        their values."""
+        bad = set(knobs) - set(self.TUNING_KEYS)
+        if bad:
+            raise KeyError(f"unknown knobs {sorted(bad)}")
+        k = (C.c_int * 5)(0, 0, 0, 0, -1)
+        for i, name in enumerate(self.TUNING_KEYS):
+            if name in knobs:
+                k[i] = int(knobs[name])
+        check(load().dpmi_sea_engine_set_tuning(self.handle, k))
 
     def status(self) -> None:
         check(load().dpmi_sea_engine_status(self.handle))

@@ -55,6 +55,7 @@ SIGNATURES = {
     "dpmi_sea_engine_graph_launch": ([_P, _P], _I),
     "dpmi_sea_engine_autotune": ([_P, _P, _P, _I, _P], _I),
     "dpmi_sea_engine_tuning": ([_P, _P], _I),
+    "dpmi_sea_engine_set_tuning": ([_P, _P], _I),
     "dpmi_sea_engine_status": ([_P], _I),
     "dpmi_sea_engine_fetch": ([_P, _I, _I, _P, _P], _I),
     "dpmi_sea_engine_set_timing": ([_P, _I], _I),

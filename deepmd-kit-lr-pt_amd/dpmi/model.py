@@ -154,6 +154,60 @@ class SeAModel:
             bias_atom_e=np.zeros(ntypes),
         )
 
+    # ------------------------------------------------- input statistics
+    def set_env_stat(self, frames, protection: float = 1e-2) -> None:
+        """davg / dstd from the environment matrices of `frames` (list of
+        (coord [n, 3], atype [n], box [9]) with orthorhombic boxes longer
+        than 2 rcut), as deepmd/dpmodel/utils/env_mat_stat.py:300-423 does:
+        per centre type, over every slot (empty slots count as zeros) of the
+        unnormalised matrix, avg_r = mean of s/r, std_r and std_a (pooled
+        over the three s x/r^2 components) with `protection` as the floor;
+        davg = [avg_r, 0, 0, 0], dstd = [std_r, std_a, std_a, std_a] in
+        every slot."""
+        from scipy.spatial import cKDTree
+
+        nt, nnei, sec = self.ntypes, self.nnei, self.sec
+        acc = np.zeros((nt, 5))  # count, sum r, sum r^2, sum a, sum a^2
+        for coord, atype, box in frames:
+            box = np.asarray(box, np.float64).reshape(3, 3)
+            if np.any(np.abs(box - np.diag(np.diag(box))) > 0) or np.any(np.diag(box) <= 2 * self.rcut):
+                raise ValueError("set_env_stat needs orthorhombic boxes longer than 2 rcut")
+            L = np.diag(box)
+            x = np.mod(np.asarray(coord, np.float64), L)
+            atype = np.asarray(atype)
+            tree = cKDTree(x, boxsize=L)
+            for i, nb in enumerate(tree.query_ball_point(x, self.rcut)):
+                nb = np.array([j for j in nb if j != i], dtype=np.int64)
+                rij = x[nb] - x[i]
+                rij -= L * np.round(rij / L)
+                r = np.linalg.norm(rij, axis=1)
+                env = np.zeros((nnei, 4))
+                for tj in range(nt):
+                    sel = np.nonzero(atype[nb] == tj)[0]
+                    sel = sel[np.argsort(r[sel], kind="stable")][: sec[tj + 1] - sec[tj]]
+                    rr, dd = r[sel], rij[sel]
+                    sw = np.array([_switch(v, self.rcut_smth, self.rcut) for v in rr])
+                    blk = env[sec[tj]: sec[tj] + len(sel)]
+                    blk[:, 0] = sw / rr
+                    blk[:, 1:] = (sw / rr ** 2)[:, None] * dd
+                ti = atype[i]
+                acc[ti] += [nnei, env[:, 0].sum(), (env[:, 0] ** 2).sum(), env[:, 1:].sum(), (env[:, 1:] ** 2).sum()]
+        davg = np.zeros((nt, nnei, 4))
+        dstd = np.ones((nt, nnei, 4))
+        for t in range(nt):
+            n, sr, sr2, sa, sa2 = acc[t]
+            if n == 0:
+                dstd[t] = 0.1
+                continue
+            avg_r = sr / n
+            std_r = max(np.sqrt(max(sr2 / n - avg_r * avg_r, 0.0)), protection)
+            na = 3 * n
+            std_a = max(np.sqrt(max(sa2 / na - (sa / na) ** 2, 0.0)), protection)
+            davg[t, :, 0] = avg_r
+            dstd[t, :, 0] = std_r
+            dstd[t, :, 1:] = std_a
+        self.davg, self.dstd = davg, dstd
+
     # ------------------------------------------------------ embedding net
     def embed_eval(self, net: int, x: np.ndarray):
         """Embedding net output and its first and second derivative w.r.t.
@@ -271,19 +325,32 @@ def _switch(x: float, rmin: float, rmax: float) -> float:
     return 0.0
 
 
-def water_model(seed: int = 1, min_nbor_dist: float = 0.5, type_one_side: bool = False) -> SeAModel:
+def water_model(seed: int = 1, min_nbor_dist: float = 0.5, type_one_side: bool = False,
+                env_stat: bool = True) -> SeAModel:
     """The water se_e2_a model of the benchmark: rcut 6.0, rcut_smth 0.5,
     sel [46, 92], embedding [25, 50, 100], axis 16, fitting [240]*3, tanh,
-    N(0, 1/fan_in) weights, zero biases, compressed at strides 0.01/0.1."""
+    N(0, 1/fan_in) weights, zero biases, compressed at strides 0.01/0.1.
+    With env_stat, davg/dstd come from a 768-atom water frame (as training
+    sets them), so the normalised environment matrix and the energies are
+    of order one; else davg 0, dstd 1."""
+    from .systems import water
+
     m = SeAModel.random(2, [46, 92], 6.0, 0.5, [25, 50, 100], 16, [240, 240, 240],
                         seed=seed, type_one_side=type_one_side)
+    if env_stat:
+        m.set_env_stat([water(256, seed=90001)])
     m.build_tables(min_nbor_dist)
     return m
 
 
-def copper_model(seed: int = 1, min_nbor_dist: float = 1.0) -> SeAModel:
+def copper_model(seed: int = 1, min_nbor_dist: float = 1.0, env_stat: bool = True) -> SeAModel:
     """Single-type fcc Cu model: rcut 8.0, rcut_smth 2.0, sel [200],
-    embedding [32, 64, 128], axis 16, fitting [240]*3."""
+    embedding [32, 64, 128], axis 16, fitting [240]*3; davg/dstd from a
+    500-atom fcc frame with env_stat (else davg 0, dstd 1)."""
+    from .systems import fcc_copper
+
     m = SeAModel.random(1, [200], 8.0, 2.0, [32, 64, 128], 16, [240, 240, 240], seed=seed)
+    if env_stat:
+        m.set_env_stat([fcc_copper(5, seed=90002)])
     m.build_tables(min_nbor_dist)
     return m

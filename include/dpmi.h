@@ -33,7 +33,8 @@ typedef enum dpmi_status {
   DPMI_ERR_HIP = 2,              /* HIP runtime error */
   DPMI_ERR_NLIST_CAPACITY = 3,   /* a neighbour row exceeds its capacity (deepmd_exception_nlist_capacity) */
   DPMI_ERR_MEMORY = 4,           /* nall > mem_nall or allocation failure (reference returns 1 / deepmd_exception_oom) */
-  DPMI_ERR_RANGE = 5,            /* index/type outside the sort-key range (reference traps) */
+  DPMI_ERR_RANGE = 5,            /* index/type outside the sort-key range (reference traps), or a
+                                    mixed-precision matrix-core operand outside the FP16 range */
   DPMI_ERR_UNSUPPORTED = 6       /* ndescrpt not in {4,9,16,25}, last_layer_size outside (0,1024] */
 } dpmi_status;
 
@@ -214,11 +215,16 @@ int dpmi_sea_engine_graph_launch(dpmi_sea_engine* engine, void* stream);
  * keeps the fastest per stage; the choice is cached per (precision, size
  * bucket = floor(log2 of the packed atom count)) and applied to engines
  * created later for the same bucket.  dpmi_sea_engine_tuning returns the
- * current knobs: {env threads per block, forward embedding waves per
- * workgroup, adjoint waves per workgroup, force rows per workgroup}. */
+ * current knobs (5 ints): {env threads per block, forward embedding waves
+ * per workgroup, adjoint waves per workgroup, force rows per workgroup,
+ * matrix-core embedding mask (DPMI_MIXED: bit 0 forward, bit 1 adjoint in
+ * table space on the matrix cores; else the per-neighbour kernels)}.
+ * dpmi_sea_engine_set_tuning sets them (a knob <= 0, or < 0 for the mask,
+ * keeps its value; out-of-range values give DPMI_ERR_ARGUMENT). */
 int dpmi_sea_engine_autotune(dpmi_sea_engine* engine, const double* coord, const double* box_host, int reps,
                              void* stream);
 int dpmi_sea_engine_tuning(dpmi_sea_engine* engine, int* knobs_host);
+int dpmi_sea_engine_set_tuning(dpmi_sea_engine* engine, const int* knobs_host);
 
 int dpmi_sea_engine_status(dpmi_sea_engine* engine);
 

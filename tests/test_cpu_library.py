@@ -81,7 +81,40 @@ def test_torch_op_rejects_cpu_tensors_loudly():
 def test_model_table_layout():
     from dpmi import model
 
-    m = model.water_model(seed=3)
+    m = model.water_model(seed=3, env_stat=False)
     assert m.nnets == 4 and m.tables[0].shape == (280, 600)
     np.testing.assert_allclose(m.table_info[0], [0.0, 2.0, 10.0, 0.01, 0.1, -1.0])
     assert [m.net_index(ti, tj) for ti in range(2) for tj in range(2)] == [0, 2, 1, 3]
+
+
+def test_env_stat_of_a_frame():
+    """davg/dstd as the reference's EnvMatStatSe sets them: [avg_r, 0, 0, 0]
+    and [std_r, std_a x 3] per centre type, over every slot (empty slots as
+    zeros); checked on a two-atom-type frame against a direct evaluation."""
+    from dpmi import model, systems
+
+    m = model.SeAModel.random(2, [46, 92], 6.0, 0.5, [8, 16], 4, [16], seed=1)
+    coord, atype, box = systems.water(64, seed=5)
+    m.set_env_stat([(coord, atype, box)])
+    L = box[0]
+    x = np.mod(coord, L)
+    per_type = {0: [], 1: []}
+    for i in range(len(atype)):
+        d = x - x[i]
+        d -= L * np.round(d / L)
+        r = np.linalg.norm(d, axis=1)
+        env = np.zeros((m.nnei, 4))
+        for tj in range(2):
+            js = [j for j in np.argsort(r, kind="stable") if j != i and atype[j] == tj and r[j] < m.rcut]
+            js = js[: m.sel[tj]]
+            for s, j in enumerate(js):
+                u = min(max((r[j] - m.rcut_smth) / (m.rcut - m.rcut_smth), 0.0), 1.0)
+                sw = 1.0 if r[j] < m.rcut_smth else u ** 3 * (-6 * u * u + 15 * u - 10) + 1
+                env[m.sec[tj] + s] = [sw / r[j], *(sw / r[j] ** 2 * d[j])]
+        per_type[atype[i]].append(env)
+    for t in range(2):
+        e = np.concatenate(per_type[t])
+        np.testing.assert_allclose(m.davg[t, :, 0], e[:, 0].mean(), rtol=1e-12)
+        assert np.all(m.davg[t, :, 1:] == 0.0)
+        np.testing.assert_allclose(m.dstd[t, :, 0], e[:, 0].std(), rtol=1e-10)
+        np.testing.assert_allclose(m.dstd[t, :, 1:], e[:, 1:].std(), rtol=1e-10)

@@ -55,11 +55,18 @@ def _fp16(a):
     return np.asarray(a, np.float64).astype(np.float16).astype(np.float64)
 
 
+def _split16(a):
+    """FP16 hi + lo parts of the weights, as the mixed fitting multiplies them."""
+    a = np.asarray(a, np.float64)
+    hi = _fp16(a)
+    return hi + _fp16(a - hi)
+
+
 def fitting_reference(m, desc16, atype):
     """Per-atom energy of the mixed fitting arithmetic: x_0 = FP16
-    descriptor, hidden layer y = tanh(x W16 + b), x' = y * idt + x (equal
-    widths), stored in FP16 except the last hidden output, which feeds the
-    output layer in FP32."""
+    descriptor, hidden layer y = tanh(x (W16hi + W16lo) + b), x' = y * idt + x
+    (equal widths), stored in FP16 except the last hidden output, which feeds
+    the output layer in FP32."""
     fa = m.fitting_arrays()
     H = len(fa) - 1
     out = np.zeros(len(atype))
@@ -70,7 +77,7 @@ def fitting_reference(m, desc16, atype):
         x = desc16[rows]
         for li in range(H):
             W, b, idt, act, res = fa[li]
-            z = x @ _fp16(W[t]) + b[t].astype(np.float32)
+            z = x @ _split16(W[t]) + b[t].astype(np.float32)
             y = np.tanh(z) if act else z
             o = y * idt[t].astype(np.float32) if idt is not None else y
             if res and W.shape[1] == W.shape[2]:
