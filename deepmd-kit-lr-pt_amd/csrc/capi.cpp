@@ -458,6 +458,7 @@ int dpmi_sea_engine_tuning(dpmi_sea_engine* engine, int* knobs_host) {
     knobs_host[2] = t.bwd_waves;
     knobs_host[3] = t.force_rows;
     knobs_host[4] = t.embed_mfma;
+    knobs_host[5] = t.fit_rows;
     return kOk;
   });
 }
@@ -486,6 +487,10 @@ int dpmi_sea_engine_set_tuning(dpmi_sea_engine* engine, const int* knobs_host) {
     if (k[4] >= 0) {
       if (k[4] > 3) throw std::invalid_argument("matrix-core embedding mask: 0 to 3");
       t.embed_mfma = k[4];
+    }
+    if (k[5] > 0) {
+      if (k[5] != 32 && k[5] != 64) throw std::invalid_argument("fused fitting rows per tile: 32 or 64");
+      t.fit_rows = k[5];
     }
     return kOk;
   });

@@ -45,7 +45,7 @@ __device__ __forceinline__ TD cvt(T v) {
   return (TD)v;
 }
 
-template <typename T, typename TD, int KQ>
+template <typename T, typename TD, int KQ, bool PK>
 __global__ void __launch_bounds__(256) k_embed_fwd(T* __restrict__ xyz, TD* __restrict__ D, const int* __restrict__ pos,
                                                    const T* __restrict__ tables, const long long* __restrict__ tab_off,
                                                    const Info<T>* __restrict__ infos, const int* __restrict__ net_of,
@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(256) k_embed_fwd(T* __restrict__ xyz, TD* __re
       const int net = net_of[ti * ntypes + tj];
       const int b = sec[tj], nb = sec[tj + 1] - sec[tj];
       const T* e = em + ((size_t)g * nnei + b) * 4;
-      fwd_block<T, 4, KQ>(acc, tables + tab_off[net], infos[net], e, 4, e, nb, M, true);
+      fwd_block<T, 4, KQ, PK>(acc, tables + tab_off[net], infos[net], e, 4, e, nb, M, true);
     }
   }
 #pragma unroll
@@ -126,7 +126,7 @@ __device__ __forceinline__ T transpose_sum(T (&w)[NV]) {
 // the wave with transpose_sum, which leaves neighbour (l / 4) component
 // (l % 4) on lane l for one coalesced store.  Sorted-padding tail as the
 // forward: the first tail slot carries weight (nb - jt), later slots zero.
-template <typename T, int KQ>
+template <typename T, int KQ, bool PK>
 __device__ void embed_bwd_block(T* __restrict__ nd, const T (&dy)[KQ][4], const T* __restrict__ table,
                                 const Info<T>& f, const T* __restrict__ em, int nb, int M) {
   if (nb <= 0) return;
@@ -163,9 +163,9 @@ __device__ void embed_bwd_block(T* __restrict__ nd, const T (&dy)[KQ][4], const 
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const int q = min(q0 + u0 + u, cnt - 1);
-          const T* row = table + (size_t)bcast(r.idx, q) * M * 6;
+          const int idx = bcast(r.idx, q);
 #pragma unroll
-          for (int kq = 0; kq < KQ; ++kq) load6(row + 6 * min(lane + kq * kWave, M - 1), a[u][kq]);
+          for (int kq = 0; kq < KQ; ++kq) load_coef<T, PK>(table, idx, min(lane + kq * kWave, M - 1), M, a[u][kq]);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -262,7 +262,7 @@ __device__ __forceinline__ void bwd_dy(T (&dy)[KQ][4], const T* __restrict__ xyz
     for (int c = 0; c < 4; ++c) dy[kq][c] = dT[kq][c] * inv_nnei;
 }
 
-template <typename T, typename TG, int KQ>
+template <typename T, typename TG, int KQ, bool PK>
 __global__ void __launch_bounds__(256) k_embed_bwd(T* __restrict__ net_deriv, const T* __restrict__ xyz,
                                                   const TG* __restrict__ gD, const int* __restrict__ pos,
                                                   const T* __restrict__ tables, const long long* __restrict__ tab_off,
@@ -283,7 +283,7 @@ __global__ void __launch_bounds__(256) k_embed_bwd(T* __restrict__ net_deriv, co
   for (int tj = 0; tj < ntypes; ++tj) {
     const int net = net_of[ti * ntypes + tj];
     const int b = sec[tj], nb = sec[tj + 1] - sec[tj];
-    embed_bwd_block<T, KQ>(net_deriv + ((size_t)g * nnei + b) * 4, dy, tables + tab_off[net], infos[net],
+    embed_bwd_block<T, KQ, PK>(net_deriv + ((size_t)g * nnei + b) * 4, dy, tables + tab_off[net], infos[net],
                            em + ((size_t)g * nnei + b) * 4, nb, M);
   }
 }
@@ -842,10 +842,22 @@ void launch_embed_fwd(const EmbedArgs& a, hipStream_t st) {
   if (a.nrows <= 0) return;
   const int tpb = kWave * (a.waves > 0 ? a.waves : 4);
   const dim3 grid(ceil_div(a.nrows * kWave, tpb));
-  DPMI_KQ_DISPATCH(a.M, hipLaunchKernelGGL((k_embed_fwd<T, TD, KQ>), grid, dim3(tpb), 0, st, (T*)a.xyz, (TD*)a.D,
-                                           a.pos, (const T*)a.tables, a.tab_off, (const Info<T>*)a.infos, a.net_of,
-                                           (const T*)a.em, a.row_type, a.sec, a.ntypes, a.nrows, a.nnei, a.M, a.A,
-                                           a.ldD > 0 ? a.ldD : a.M * a.A, (T)a.inv_nnei));
+  const T* tables = (const T*)(a.packed ? a.packed : a.tables);
+  const long long* off = a.packed ? a.packed_off : a.tab_off;
+#define DPMI_FWD_LAUNCH(PKV)                                                                                        \
+  DPMI_KQ_DISPATCH(a.M, hipLaunchKernelGGL((k_embed_fwd<T, TD, KQ, PKV>), grid, dim3(tpb), 0, st, (T*)a.xyz, (TD*)a.D, \
+                                           a.pos, tables, off, (const Info<T>*)a.infos, a.net_of, (const T*)a.em,     \
+                                           a.row_type, a.sec, a.ntypes, a.nrows, a.nnei, a.M, a.A,                     \
+                                           a.ldD > 0 ? a.ldD : a.M * a.A, (T)a.inv_nnei))
+  if constexpr (sizeof(T) == 4) {
+    if (a.packed)
+      DPMI_FWD_LAUNCH(true);
+    else
+      DPMI_FWD_LAUNCH(false);
+  } else {
+    DPMI_FWD_LAUNCH(false);
+  }
+#undef DPMI_FWD_LAUNCH
   DPMI_LAUNCH_CHECK();
 }
 
@@ -853,15 +865,49 @@ template <typename T, typename TG>
 void launch_embed_bwd(const EmbedArgs& a, hipStream_t st) {
   if (a.nrows <= 0) return;
   const int tpb = kWave * (a.waves_bwd > 0 ? a.waves_bwd : 1);
-  DPMI_KQ_DISPATCH(a.M, hipLaunchKernelGGL((k_embed_bwd<T, TG, KQ>), dim3(ceil_div(a.nrows * kWave, tpb)), dim3(tpb), 0, st,
-                                           (T*)a.net_deriv, (const T*)a.xyz, (const TG*)a.gD, a.pos,
-                                           (const T*)a.tables, a.tab_off, (const Info<T>*)a.infos, a.net_of,
-                                           (const T*)a.em, a.row_type, a.sec, a.ntypes, a.nrows, a.nnei, a.M, a.A,
-                                           a.ldD > 0 ? a.ldD : a.M * a.A, (T)a.inv_nnei));
+  const dim3 grid(ceil_div(a.nrows * kWave, tpb));
+  const T* tables = (const T*)(a.packed ? a.packed : a.tables);
+  const long long* off = a.packed ? a.packed_off : a.tab_off;
+#define DPMI_BWD_LAUNCH(PKV)                                                                                        \
+  DPMI_KQ_DISPATCH(a.M, hipLaunchKernelGGL((k_embed_bwd<T, TG, KQ, PKV>), grid, dim3(tpb), 0, st, (T*)a.net_deriv,    \
+                                           (const T*)a.xyz, (const TG*)a.gD, a.pos, tables, off,                      \
+                                           (const Info<T>*)a.infos, a.net_of, (const T*)a.em, a.row_type, a.sec,      \
+                                           a.ntypes, a.nrows, a.nnei, a.M, a.A, a.ldD > 0 ? a.ldD : a.M * a.A,        \
+                                           (T)a.inv_nnei))
+  if constexpr (sizeof(T) == 4) {
+    if (a.packed)
+      DPMI_BWD_LAUNCH(true);
+    else
+      DPMI_BWD_LAUNCH(false);
+  } else {
+    DPMI_BWD_LAUNCH(false);
+  }
+#undef DPMI_BWD_LAUNCH
   DPMI_LAUNCH_CHECK();
 }
 
 bool embed_mfma_supported(int M) { return M > 0 && M <= 128 && M % 4 == 0; }
+
+void build_packed_tables(const double* table, const long long* tab_off, const int* nrows, int nnets, int M,
+                         std::vector<float>& out, std::vector<long long>& off) {
+  off.assign(nnets, 0);
+  long long tot = 0;
+  for (int n = 0; n < nnets; ++n) {
+    off[n] = tot;
+    tot += (long long)nrows[n] * M * 4;
+  }
+  out.assign((size_t)tot, 0.f);
+  for (int n = 0; n < nnets; ++n)
+    for (long long q = 0; q < (long long)nrows[n] * M; ++q) {
+      const double* a = table + tab_off[n] + q * 6;
+      float* r = out.data() + off[n] + q * 4;
+      r[0] = (float)a[0];
+      r[1] = (float)a[1];
+      const tab_half2 h23 = {(_Float16)a[2], (_Float16)a[3]}, h45 = {(_Float16)a[4], (_Float16)a[5]};
+      r[2] = __builtin_bit_cast(float, h23);
+      r[3] = __builtin_bit_cast(float, h45);
+    }
+}
 
 void build_split_tables(const double* table, const long long* tab_off, const int* nrows, const double* infos,
                         int nnets, int M, std::vector<_Float16>& out, std::vector<long long>& off,

@@ -222,6 +222,13 @@ void SeaModel::build_mixed(const dpmi_sea_model_desc& d) {
   for (int l = 0; l <= nlayers; ++l) kpad[l] = (width[l] + align - 1) / align * align;
   std::vector<float> tf(table_host.begin(), table_host.end());
   d_table_f = dupload(tf.data(), tf.size());
+  {
+    std::vector<float> pk;
+    std::vector<long long> pko;
+    build_packed_tables(table_host.data(), tab_off.data(), nspline.data(), nnets, M, pk, pko);
+    d_table_pk = dupload(pk.data(), pk.size());
+    d_tab_off_pk = dupload(pko.data(), pko.size());
+  }
   void* infos = make_table_infos(info_host.data(), nnets, true);
   d_infos_f = dupload(static_cast<const char*>(infos), table_info_bytes(true) * nnets);
   free_table_infos(infos, true);
@@ -328,6 +335,8 @@ SeaModel::~SeaModel() {
   for (auto& p : fit_b) dfree(p);
   for (auto& p : fit_idt) dfree(p);
   dfree(d_table_f);
+  dfree(d_table_pk);
+  dfree(d_tab_off_pk);
   dfree(d_infos_f);
   dfree(d_shift);
   dfree(d_fused);
@@ -430,6 +439,11 @@ SeaEngine::SeaEngine(const SeaModel* m, int nframes_, const int* natoms_h, const
     const std::vector<int4> tiles = fitting_tile_order(tstart, nt, 64, chip.first, chip.second);
     ntiles = (int)tiles.size();
     d_tiles = dupload(tiles.data(), tiles.size());
+    const std::vector<int4> tiles32 = fitting_tile_order(tstart, nt, 32, chip.first, chip.second);
+    ntiles32 = (int)tiles32.size();
+    d_tiles32 = dupload(tiles32.data(), tiles32.size());
+    // 64-row tiles unless they would leave compute units without a tile
+    fit_rows_default = ntiles < chip.second ? 32 : 64;
     // embedding tiles: groups of kEmbedTileAtoms descriptor positions of one
     // type (atoms of negative type last, as type -1)
     std::vector<int4> et;
@@ -478,6 +492,7 @@ SeaEngine::~SeaEngine() {
     dfree(*p);
   dfree(d_Dh);
   dfree(d_tiles);
+  dfree(d_tiles32);
   dfree(d_etiles);
   for (auto& p : d_y16) dfree(p);
   for (auto& p : d_x16) dfree(p);
@@ -693,6 +708,8 @@ void SeaEngine::compute_mixed_descriptor_fitting(hipStream_t st, int& stage) {
   ta.infos = m.d_infos_f;
   ta.em = d_em_f;
   ta.ldD = k0;
+  ta.packed = m.d_table_pk;
+  ta.packed_off = m.d_tab_off_pk;
   const bool mfma_fwd = m.embed_mfma && (tune.embed_mfma & 1), mfma_bwd = m.embed_mfma && (tune.embed_mfma & 2);
   if (mfma_fwd)
     launch_embed_fwd_mfma(ta, st);
@@ -707,7 +724,8 @@ void SeaEngine::compute_mixed_descriptor_fitting(hipStream_t st, int& stage) {
     fa.ldD = k0;
     fa.gD = d_gD_f;
     fa.ldG = k0;
-    fa.tiles = d_tiles;
+    const int rows = tune.fit_rows ? tune.fit_rows : fit_rows_default;
+    fa.tiles = rows == 32 ? d_tiles32 : d_tiles;
     fa.layers = m.d_fused;
     fa.perm = d_perm;
     fa.atom_e = d_atom_e;
@@ -715,7 +733,7 @@ void SeaEngine::compute_mixed_descriptor_fitting(hipStream_t st, int& stage) {
     fa.H = H;
     // atoms of negative type own no tile: their energy is zero
     if (tstart[m.ntypes + 1] > tstart[m.ntypes]) launch_fill_double(d_atom_e, 0.0, ntot, st);
-    launch_fit_fused(fa, ntiles, st);
+    launch_fit_fused(fa, rows == 32 ? ntiles32 : ntiles, rows, st);
     mark(st, stage++);
     if (mfma_bwd)
       launch_embed_bwd_mfma(ta, st);
@@ -912,13 +930,14 @@ void SeaEngine::load_cached_tune() {
 // knobs act on different stages, so all are varied together) and keeps,
 // per stage, the fastest value; the result is cached for the size bucket.
 void SeaEngine::autotune(const double* coord, const double* box_h, int reps, hipStream_t st) {
-  const int env_c[3] = {128, 256, 512}, fwd_c[3] = {1, 2, 4}, bwd_c[3] = {1, 2, 4}, frc_c[3] = {4, 8, 16};
+  const int env_c[3] = {128, 256, 512}, fwd_c[3] = {1, 2, 4}, bwd_c[3] = {1, 2, 4}, frc_c[3] = {4, 8, 16},
+            fit_c[3] = {32, 64, 64};
   // candidates 0..2: per-neighbour embedding kernels with the shapes above;
   // candidate 3 (mixed engine with matrix-core tables): the table-space
   // embedding kernels, whose launch shape is fixed
   const bool try_mfma = precision == DPMI_MIXED && model->embed_mfma;
   const int ncand = try_mfma ? 4 : 3;
-  double t_env[4], t_fwd[4], t_bwd[4], t_frc[4];
+  double t_env[4], t_fwd[4], t_bwd[4], t_frc[4], t_fit[4];
   double *e = nullptr, *f = nullptr, *v = nullptr;
   DPMI_CHECK(hipMalloc(&e, sizeof(double) * nframes));
   DPMI_CHECK(hipMalloc(&f, sizeof(double) * 3 * (size_t)ntot));
@@ -943,6 +962,7 @@ void SeaEngine::autotune(const double* coord, const double* box_h, int reps, hip
       tune.fwd_waves = fwd_c[k];
       tune.bwd_waves = bwd_c[k];
       tune.force_rows = frc_c[k];
+      tune.fit_rows = fit_c[k];
       tune.embed_mfma = c < 3 ? 0 : 3;
       timing = true;
       reset_timing();
@@ -953,6 +973,7 @@ void SeaEngine::autotune(const double* coord, const double* box_h, int reps, hip
       t_fwd[c] = ms[idx("embed_fwd")];
       t_bwd[c] = ms[idx("embed_bwd")];
       t_frc[c] = ms[idx("force_virial")];
+      t_fit[c] = idx("fitting") >= 0 ? ms[idx("fitting")] : 0.0;
     }
   } catch (...) {
     timing = was_timing;
@@ -972,6 +993,7 @@ void SeaEngine::autotune(const double* coord, const double* box_h, int reps, hip
   tune.fwd_waves = fwd_c[best(t_fwd)];
   tune.bwd_waves = bwd_c[best(t_bwd)];
   tune.force_rows = frc_c[best(t_frc)];
+  tune.fit_rows = idx("fitting") >= 0 ? fit_c[best(t_fit)] : 0;
   tune.embed_mfma = 0;
   if (try_mfma) {
     if (t_fwd[3] < t_fwd[best(t_fwd)]) tune.embed_mfma |= 1;

@@ -41,6 +41,8 @@ struct SeaModel {
   std::string mixed_unsupported;  // empty when DPMI_MIXED can run this model
   std::vector<int> kpad;          // [nlayers + 1] padded widths
   float* d_table_f = nullptr;
+  float* d_table_pk = nullptr;  // packed records (a0, a1 FP32; a2..a5 FP16) per column
+  long long* d_tab_off_pk = nullptr;
   char* d_infos_f = nullptr;
   double* d_shift = nullptr;
   std::vector<Fit16Layer> f16;  // [ntypes * (nlayers - 1)] hidden layers
@@ -125,8 +127,11 @@ struct SeaEngine {
   float *d_em_f = nullptr, *d_em_deriv_f = nullptr, *d_rij_f = nullptr, *d_net_deriv_f = nullptr;
   float *d_xyz_f = nullptr, *d_gD_f = nullptr, *d_eperm = nullptr, *d_ga = nullptr, *d_gb = nullptr;
   _Float16* d_Dh = nullptr;
-  int4* d_tiles = nullptr;
+  int4* d_tiles = nullptr;  // fused fitting tiles of 64 rows
   int ntiles = 0;
+  int4* d_tiles32 = nullptr;  // and of 32 rows
+  int ntiles32 = 0;
+  int fit_rows_default = 64;
   int4* d_etiles = nullptr;  // embedding tiles: 4 descriptor positions of one type
   int netiles = 0;
   std::vector<float*> d_y16;

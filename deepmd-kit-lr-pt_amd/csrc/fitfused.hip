@@ -5,10 +5,11 @@
 // when widths are equal), energy e = x_H . w_out + b_out + bias_atom_e, and
 // the adjoint dE/dx_0 back to the descriptor.
 //
-// One workgroup (8 waves) owns a tile of up to 64 descriptor rows of one
-// type and runs every layer forward, the energy, and every layer backward:
-//   * every wave covers all 64 rows and one eighth of the output columns, so
-//     no weight fragment is fetched twice by a workgroup;
+// One workgroup (8 waves) owns a tile of up to 16 RT descriptor rows of one
+// type (RT = 4, or 2 when 64-row tiles would leave compute units idle) and
+// runs every layer forward, the energy, and every layer backward:
+//   * every wave covers all rows of the tile and one eighth of the output
+//     columns, so no weight fragment is fetched twice by a workgroup;
 //   * the descriptor tile streams from HBM through a double-buffered LDS
 //     stage; the layer inputs x_l and the backward operands gz_l live in LDS
 //     (FP16); the activation derivatives idt*(1 - y^2) (FP16 pairs) and the
@@ -34,11 +35,11 @@ namespace dpmi {
 namespace {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
-typedef _Float16 half4 __attribute__((ext_vector_type(4)));
 typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+template <int N>
+using halfv = _Float16 __attribute__((ext_vector_type(N)));
 
-constexpr int kRows = 64;        // rows per workgroup tile
 constexpr int kWaves = 8;
 constexpr int kMaxW = 256;       // widest hidden layer (padded)
 constexpr int kXld = kMaxW + 8;  // LDS row stride of the activation tiles (halves)
@@ -57,7 +58,6 @@ __device__ __forceinline__ half8 zero8() {
 // (Global address space: a generic load would also count against the LDS
 // wait counter and drain at every LDS wait.)
 typedef __attribute__((address_space(1))) const half8 ghalf8;
-typedef __attribute__((address_space(1))) const half4 ghalf4;
 __device__ __forceinline__ half8 load_bfrag(const _Float16* __restrict__ w, int kt, int NT, int ct, int lane) {
   return *(ghalf8*)(w + ((size_t)(kt * NT + ct) * 64 + lane) * 8);
 }
@@ -68,7 +68,8 @@ __device__ __forceinline__ half8 load_bfrag(const _Float16* __restrict__ w, int 
 // (used if < NT).  KT is a multiple of kPf.  Loads are unconditional
 // (clamped tile and step indices) so that the prefetch ring is not
 // serialised by the wait counters.
-__device__ __forceinline__ void gemm_lds_a(floatx4 (&acc)[4][2], const _Float16* a_lds, int ld,
+template <int RT>
+__device__ __forceinline__ void gemm_lds_a(floatx4 (&acc)[RT][2], const _Float16* a_lds, int ld,
                                            const _Float16* __restrict__ w, const _Float16* __restrict__ wl, int KT,
                                            int NT, int ct0, int ntw, int lane) {
   const int lr = lane & 15, lg = lane >> 4;
@@ -86,18 +87,18 @@ __device__ __forceinline__ void gemm_lds_a(floatx4 (&acc)[4][2], const _Float16*
 #pragma unroll
     for (int s = 0; s < kPf; ++s) {
       const int k = kt0 + s;
-      half8 a[4];
+      half8 a[RT];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const half8*>(a_lds + (i * 16 + lr) * ld + k * 32 + lg * 8);
+      for (int i = 0; i < RT; ++i) a[i] = *reinterpret_cast<const half8*>(a_lds + (i * 16 + lr) * ld + k * 32 + lg * 8);
       if (use0)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < RT; ++i) {
           acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][0], acc[i][0], 0, 0, 0);
           acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bl[s][0], acc[i][0], 0, 0, 0);
         }
       if (use1)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < RT; ++i) {
           acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][1], acc[i][1], 0, 0, 0);
           acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bl[s][1], acc[i][1], 0, 0, 0);
         }
@@ -110,15 +111,18 @@ __device__ __forceinline__ void gemm_lds_a(floatx4 (&acc)[4][2], const _Float16*
   }
 }
 
-__device__ __forceinline__ void zero_acc(floatx4 (&acc)[4][2]) {
+template <int RT>
+__device__ __forceinline__ void zero_acc(floatx4 (&acc)[RT][2]) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < RT; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 }
 
-template <int H>
+template <int H, int RT>
 __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
+  constexpr int kRows = 16 * RT;  // rows per workgroup tile
+  constexpr int kLdT = 32 / RT;   // descriptor loaders per row (RT halves each)
   __shared__ _Float16 xs[2][kRows][kXld];
   __shared__ _Float16 as[2][kRows][kAld];
   __shared__ float esum[kWaves][kRows];
@@ -132,10 +136,10 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
 
   // activation derivative idt * act'(z) of every hidden layer at this wave's
   // output positions (row tile i, column tile j, rows lg*4 + e), two per register
-  half2v dpk[H][4][2][2];
-  float epart[4][4];
+  half2v dpk[H][RT][2][2];
+  float epart[RT][4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < RT; ++i)
 #pragma unroll
     for (int e = 0; e < 4; ++e) epart[i][e] = 0.f;
 
@@ -146,53 +150,56 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
     const int KT = L.K / 32, NT = L.N / 16, ntw = (NT + kWaves - 1) / kWaves;
     const int ct0 = wave * ntw;
     const int src = (l + 1) & 1, dst = l & 1;  // layer l >= 1 reads xs[src]
-    floatx4 acc[4][2];
+    floatx4 acc[RT][2];
     zero_acc(acc);
     if (l == 0) {
-      // descriptor rows: 8 bytes per thread per k-step through a register
+      // descriptor rows: 2 RT bytes per thread per k-step through a register
       // ring (kPf steps ahead) into the double-buffered LDS stage; rows past
       // the tile are read from its first row and zeroed
-      const int arow = tid >> 3, acol = (tid & 7) * 4;
+      typedef __attribute__((address_space(1))) const halfv<RT> ghv;
+      const int arow = tid / kLdT, acol = (tid % kLdT) * RT;
       const bool aval = arow < nrows;
       const _Float16* dsrc = p.D + (size_t)(row0 + (aval ? arow : 0)) * p.ldD + acol;
       const bool use0 = ntw > 0 && ct0 < NT, use1 = ntw > 1 && ct0 + 1 < NT;
       const int c0 = min(ct0, NT - 1), c1 = min(ct0 + 1, NT - 1);
-      const half4 hz = {(_Float16)0.f, (_Float16)0.f, (_Float16)0.f, (_Float16)0.f};
-      half4 ar[kPf];
+      halfv<RT> hz;
+#pragma unroll
+      for (int e = 0; e < RT; ++e) hz[e] = (_Float16)0.f;
+      halfv<RT> ar[kPf];
       half8 bq[kPf][2], bl[kPf][2];
 #pragma unroll
       for (int s = 0; s < kPf; ++s) {
-        ar[s] = __builtin_nontemporal_load((ghalf4*)(dsrc + s * 32));
+        ar[s] = __builtin_nontemporal_load((ghv*)(dsrc + s * 32));
         bq[s][0] = load_bfrag(L.wf, s, NT, c0, lane);
         bq[s][1] = load_bfrag(L.wf, s, NT, c1, lane);
         bl[s][0] = load_bfrag(L.wf_lo, s, NT, c0, lane);
         bl[s][1] = load_bfrag(L.wf_lo, s, NT, c1, lane);
       }
-      *reinterpret_cast<half4*>(&as[0][arow][acol]) = aval ? ar[0] : hz;
+      *reinterpret_cast<halfv<RT>*>(&as[0][arow][acol]) = aval ? ar[0] : hz;
       __syncthreads();
       for (int kt0 = 0; kt0 < KT; kt0 += kPf) {
 #pragma unroll
         for (int s = 0; s < kPf; ++s) {
           const int k = kt0 + s;
           // stage k + 1 (loaded kPf - 1 steps ago) into the other LDS buffer
-          *reinterpret_cast<half4*>(&as[(k + 1) & 1][arow][acol]) = aval ? ar[(s + 1) % kPf] : hz;
-          half8 a[4];
+          *reinterpret_cast<halfv<RT>*>(&as[(k + 1) & 1][arow][acol]) = aval ? ar[(s + 1) % kPf] : hz;
+          half8 a[RT];
 #pragma unroll
-          for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const half8*>(&as[k & 1][i * 16 + lr][lg * 8]);
+          for (int i = 0; i < RT; ++i) a[i] = *reinterpret_cast<const half8*>(&as[k & 1][i * 16 + lr][lg * 8]);
           if (use0)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
+            for (int i = 0; i < RT; ++i) {
               acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][0], acc[i][0], 0, 0, 0);
               acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bl[s][0], acc[i][0], 0, 0, 0);
             }
           if (use1)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
+            for (int i = 0; i < RT; ++i) {
               acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][1], acc[i][1], 0, 0, 0);
               acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bl[s][1], acc[i][1], 0, 0, 0);
             }
           const int kn = min(k + kPf, KT - 1);
-          ar[s] = __builtin_nontemporal_load((ghalf4*)(dsrc + kn * 32));
+          ar[s] = __builtin_nontemporal_load((ghv*)(dsrc + kn * 32));
           bq[s][0] = load_bfrag(L.wf, kn, NT, c0, lane);
           bq[s][1] = load_bfrag(L.wf, kn, NT, c1, lane);
           bl[s][0] = load_bfrag(L.wf_lo, kn, NT, c0, lane);
@@ -212,7 +219,7 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
       const float idn = L.idt ? L.idt[n] : 1.f;
       const float own = (l == H - 1) ? L.out_w[n] : 0.f;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
+      for (int i = 0; i < RT; ++i) {
         float dv[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -236,7 +243,7 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
 
   // energy: row sums over the 16 column lanes, then over the 8 waves
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < RT; ++i)
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       float s = epart[i][e];
@@ -256,7 +263,7 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
 
   // ----------------------------------------------------------- backward --
   // g: dE/d(output of layer l) at this wave's positions; top: w_out
-  float g[4][2][4];
+  float g[RT][2][4];
   {
     const FusedLayer L = Ls[H - 1];
     const int NT = L.N / 16, ntw = (NT + kWaves - 1) / kWaves;
@@ -265,7 +272,7 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
       const int ct = wave * ntw + j;
       const float w = (j < ntw && ct < NT) ? L.out_w[ct * 16 + lr] : 0.f;
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < RT; ++i)
 #pragma unroll
         for (int e = 0; e < 4; ++e) g[i][j][e] = w;
     }
@@ -282,7 +289,7 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
       if (j >= ntw || ct >= NT) continue;
       const int n = ct * 16 + lr;
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < RT; ++i)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const float d = (float)dpk[l][i][j][e >> 1][e & 1];
@@ -294,12 +301,12 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
     const int OT = L.K / 16;  // output column tiles (input width of the layer)
     if (l > 0) {
       const int otw = (OT + kWaves - 1) / kWaves;
-      floatx4 acc[4][2];
+      floatx4 acc[RT][2];
       zero_acc(acc);
       gemm_lds_a(acc, &xs[buf][0][0], kXld, L.wb, L.wb_lo, KT, OT, wave * otw, otw, lane);
       // g_l = gz W^T (+ g_{l+1} for residual layers: same positions)
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < RT; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
@@ -310,7 +317,7 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
       const int nchunk = (OT + 2 * kWaves - 1) / (2 * kWaves);
       for (int c = 0; c < nchunk; ++c) {
         const int ot0 = c * 2 * kWaves + wave * 2;
-        floatx4 acc[4][2];
+        floatx4 acc[RT][2];
         zero_acc(acc);
         gemm_lds_a(acc, &xs[buf][0][0], kXld, L.wb, L.wb_lo, KT, OT, ot0, 2, lane);
 #pragma unroll
@@ -318,7 +325,7 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
           if (ot0 + j >= OT) continue;
           const int col = (ot0 + j) * 16 + lr;
 #pragma unroll
-          for (int i = 0; i < 4; ++i)
+          for (int i = 0; i < RT; ++i)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
               const int r = i * 16 + lg * 4 + e;
@@ -332,15 +339,25 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
 
 }  // namespace
 
-void launch_fit_fused(const FusedFitArgs& a, int ntiles, hipStream_t st) {
-  if (ntiles <= 0) return;
+template <int RT>
+void launch_fit_fused_rt(const FusedFitArgs& a, int ntiles, hipStream_t st) {
   switch (a.H) {
-    case 1: hipLaunchKernelGGL(k_fit_fused<1>, dim3(ntiles), dim3(512), 0, st, a); break;
-    case 2: hipLaunchKernelGGL(k_fit_fused<2>, dim3(ntiles), dim3(512), 0, st, a); break;
-    case 3: hipLaunchKernelGGL(k_fit_fused<3>, dim3(ntiles), dim3(512), 0, st, a); break;
-    case 4: hipLaunchKernelGGL(k_fit_fused<4>, dim3(ntiles), dim3(512), 0, st, a); break;
+    case 1: hipLaunchKernelGGL((k_fit_fused<1, RT>), dim3(ntiles), dim3(512), 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_fit_fused<2, RT>), dim3(ntiles), dim3(512), 0, st, a); break;
+    case 3: hipLaunchKernelGGL((k_fit_fused<3, RT>), dim3(ntiles), dim3(512), 0, st, a); break;
+    case 4: hipLaunchKernelGGL((k_fit_fused<4, RT>), dim3(ntiles), dim3(512), 0, st, a); break;
     default: throw std::invalid_argument("fused fitting supports 1 to 4 hidden layers");
   }
+}
+
+void launch_fit_fused(const FusedFitArgs& a, int ntiles, int rows_per_tile, hipStream_t st) {
+  if (ntiles <= 0) return;
+  if (rows_per_tile == 64)
+    launch_fit_fused_rt<4>(a, ntiles, st);
+  else if (rows_per_tile == 32)
+    launch_fit_fused_rt<2>(a, ntiles, st);
+  else
+    throw std::invalid_argument("fused fitting tiles hold 32 or 64 rows");
   DPMI_LAUNCH_CHECK();
 }
 

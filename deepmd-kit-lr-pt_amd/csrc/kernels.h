@@ -128,6 +128,11 @@ struct EmbedArgs {
   double inv_nnei = 0;
   int waves = 0;      // forward: waves (rows) per workgroup (0: 4)
   int waves_bwd = 0;  // backward: waves (rows) per workgroup (0: 1)
+  // packed FP32 tables of the per-neighbour FP32 kernels (one 16-byte record
+  // per column: a0, a1 FP32, a2..a5 FP16) and their per-net offsets in
+  // floats; null: the plain tables above
+  const void* packed = nullptr;
+  const long long* packed_off = nullptr;
   // matrix-core (table-space) kernels of the mixed engine
   const int* perm = nullptr;             // [nrows] descriptor position -> row
   const _Float16* tab16 = nullptr;       // split FP16 tables of every net
@@ -153,6 +158,9 @@ void launch_embed_bwd(const EmbedArgs& a, hipStream_t st);
 // in pscale[8 net + p]) in layout [row][8 p][32 ceil(M/32)].
 constexpr int kEmbedTileAtoms = 4;
 bool embed_mfma_supported(int M);
+// packed FP32 tables (records of 4 floats per column) and per-net offsets
+void build_packed_tables(const double* table, const long long* tab_off, const int* nrows, int nnets, int M,
+                         std::vector<float>& out, std::vector<long long>& off);
 void build_split_tables(const double* table, const long long* tab_off, const int* nrows, const double* infos,
                         int nnets, int M, std::vector<_Float16>& out, std::vector<long long>& off,
                         std::vector<float>& pscale);
@@ -208,6 +216,7 @@ struct Tune {
   // matrix cores (table space); pays off when many neighbours of an atom
   // share table rows, so it is off until the autotuner measures it faster
   int embed_mfma = 0;
+  int fit_rows = 0;  // fused fitting rows per workgroup tile (32 or 64; 0: chosen from the size)
 };
 void launch_force_virial_rows(const ForceRowsArgs& a, hipStream_t st);
 // force[loc] += force_ext[e] for every extended atom e < nall of each frame
@@ -289,7 +298,8 @@ struct FusedFitArgs {
   const double* shift = nullptr;  // [ntypes] output bias + bias_atom_e
   int H = 0;                    // hidden layers, 1..4
 };
-void launch_fit_fused(const FusedFitArgs& a, int ntiles, hipStream_t st);
+// tiles of rows_per_tile (32 or 64) descriptor rows
+void launch_fit_fused(const FusedFitArgs& a, int ntiles, int rows_per_tile, hipStream_t st);
 // host: Bt [N][K] (row-major) -> MFMA B-fragment order (N % 16 == 0, K % 32 == 0)
 void swizzle_bfrags(const _Float16* bt, int N, int K, _Float16* out);
 

@@ -58,6 +58,29 @@ __device__ __forceinline__ void load6(const T* p, T* a) {
   for (int q = 0; q < 6; ++q) a[q] = p[q];
 }
 
+// Coefficients a0..a5 of column `col` of table row `idx`.  Plain tables hold
+// 6 values per column ([row][M][6]); packed FP32 tables (PK) hold one 16-byte
+// record per column, a0 and a1 in FP32 and a2..a5 in FP16 ([row][M] records;
+// the terms a_p t^p, p >= 2, are at most a_p stride^2 in size, so FP16 keeps
+// them to FP32-level absolute accuracy of the value), read with one load.
+typedef _Float16 tab_half2 __attribute__((ext_vector_type(2)));
+template <typename T, bool PK>
+__device__ __forceinline__ void load_coef(const T* __restrict__ table, int idx, int col, int M, T* a) {
+  if constexpr (PK) {
+    static_assert(sizeof(T) == 4, "packed tables are FP32");
+    const float4 r = reinterpret_cast<const float4*>(table)[(size_t)idx * M + col];
+    const tab_half2 h23 = __builtin_bit_cast(tab_half2, r.z), h45 = __builtin_bit_cast(tab_half2, r.w);
+    a[0] = r.x;
+    a[1] = r.y;
+    a[2] = (T)h23.x;
+    a[3] = (T)h23.y;
+    a[4] = (T)h45.x;
+    a[5] = (T)h45.y;
+  } else {
+    load6(table + ((size_t)idx * M + col) * 6, a);
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ void quintic(const T* a, T x, T delta, T& v, T& d) {
   // tabulate.cc:122-158
@@ -95,7 +118,7 @@ __device__ __forceinline__ bool load_nbr(Nbr<T, ND>& r, const Info<T>& f, const 
 }
 
 // Accumulates one row block (nnei neighbours) into acc[KQ][ND].
-template <typename T, int ND, int KQ>
+template <typename T, int ND, int KQ, bool PK = false>
 __device__ __forceinline__ void fwd_block(T (&acc)[KQ][ND], const T* __restrict__ table, const Info<T>& f,
                                           const T* __restrict__ em_x, int xs, const T* __restrict__ em, int nnei,
                                           int M, bool sorted) {
@@ -118,9 +141,9 @@ __device__ __forceinline__ void fwd_block(T (&acc)[KQ][ND], const T* __restrict_
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int q = min(q0 + u, jend - 1);
-        const T* row = table + (size_t)bcast(r.idx, q) * M * 6;
+        const int idx = bcast(r.idx, q);
 #pragma unroll
-        for (int kq = 0; kq < KQ; ++kq) load6(row + 6 * min(lane + kq * kWave, M - 1), a[u][kq]);
+        for (int kq = 0; kq < KQ; ++kq) load_coef<T, PK>(table, idx, min(lane + kq * kWave, M - 1), M, a[u][kq]);
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {

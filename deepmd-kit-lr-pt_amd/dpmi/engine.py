@@ -153,13 +153,13 @@ class Engine:
         return self.tuning()
 
     TUNING_KEYS = ("env_threads_per_block", "embed_fwd_waves", "embed_bwd_waves", "force_rows_per_workgroup",
-                   "embed_matrix_cores")
+                   "embed_matrix_cores", "fitting_rows_per_tile")
 
     def tuning(self) -> dict:
         """Current launch knobs; embed_matrix_cores is a mask (bit 0 forward,
         bit 1 adjoint of the table embedding on the matrix cores, mixed
         engine only)."""
-        k = (C.c_int * 5)()
+        k = (C.c_int * 6)()
         check(load().dpmi_sea_engine_tuning(self.handle, k))
         return dict(zip(self.TUNING_KEYS, list(k)))
 
@@ -169,7 +169,7 @@ class Engine:
         bad = set(knobs) - set(self.TUNING_KEYS)
         if bad:
             raise KeyError(f"unknown knobs {sorted(bad)}")
-        k = (C.c_int * 5)(0, 0, 0, 0, -1)
+        k = (C.c_int * 6)(0, 0, 0, 0, -1, 0)
         for i, name in enumerate(self.TUNING_KEYS):
             if name in knobs:
                 k[i] = int(knobs[name])

@@ -215,10 +215,11 @@ int dpmi_sea_engine_graph_launch(dpmi_sea_engine* engine, void* stream);
  * keeps the fastest per stage; the choice is cached per (precision, size
  * bucket = floor(log2 of the packed atom count)) and applied to engines
  * created later for the same bucket.  dpmi_sea_engine_tuning returns the
- * current knobs (5 ints): {env threads per block, forward embedding waves
+ * current knobs (6 ints): {env threads per block, forward embedding waves
  * per workgroup, adjoint waves per workgroup, force rows per workgroup,
  * matrix-core embedding mask (DPMI_MIXED: bit 0 forward, bit 1 adjoint in
- * table space on the matrix cores; else the per-neighbour kernels)}.
+ * table space on the matrix cores; else the per-neighbour kernels), fused
+ * fitting rows per tile (32 or 64; 0 = chosen from the atom count)}.
  * dpmi_sea_engine_set_tuning sets them (a knob <= 0, or < 0 for the mask,
  * keeps its value; out-of-range values give DPMI_ERR_ARGUMENT). */
 int dpmi_sea_engine_autotune(dpmi_sea_engine* engine, const double* coord, const double* box_host, int reps,

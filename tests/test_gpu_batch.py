@@ -87,6 +87,7 @@ def test_autotune_keeps_results_and_caches(water, batch, precision):
     assert knobs["embed_fwd_waves"] in (1, 2, 4) and knobs["embed_bwd_waves"] in (1, 2, 4)
     assert knobs["force_rows_per_workgroup"] in (4, 8, 16)
     assert knobs["embed_matrix_cores"] in ((0, 1, 2, 3) if precision == "mixed" else (0,))
+    assert knobs["fitting_rows_per_tile"] in ((32, 64) if precision == "mixed" else (0,))
     e, f, v = [t.cpu().numpy().copy() for t in eng.compute(x, box)]
     tol = 1e-12 if precision == "fp64" else 1e-9
     assert _close(e, e0, tol) and _close(f, f0, tol) and _close(v, v0, tol)

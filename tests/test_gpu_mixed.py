@@ -209,3 +209,20 @@ def test_mixed_rejects_width_doubling_residual():
     _run(dm, [len(atype)], coord, atype, box, "fp64")
     with pytest.raises(DpmiError, match="UNSUPPORTED"):
         Engine(dm, [len(atype)], atype, precision="mixed")
+
+
+def test_fitting_tile_rows_give_identical_rows(water_dev):
+    """Fused fitting tiles of 32 and 64 rows compute every row with the same
+    arithmetic, so atomic energies and net_deriv agree bit for bit."""
+    coord, atype, box = systems.water(100, seed=13)
+    n = len(atype)
+    out = {}
+    for rows in (32, 64):
+        eng = Engine(water_dev, [n], atype, precision="mixed")
+        eng.set_tuning(fitting_rows_per_tile=rows)
+        assert eng.tuning()["fitting_rows_per_tile"] == rows
+        eng.compute(torch.as_tensor(coord, device="cuda"), box, atom_energy=True)
+        eng.status()
+        out[rows] = (eng.fetch("atom_energy"), eng.fetch("net_deriv"))
+    assert np.array_equal(out[32][0], out[64][0])
+    assert np.array_equal(out[32][1], out[64][1])
