@@ -130,7 +130,7 @@ template <typename T, int KQ, bool PK>
 __device__ void embed_bwd_block(T* __restrict__ nd, const T (&dy)[KQ][4], const T* __restrict__ table,
                                 const Info<T>& f, const T* __restrict__ em, int nb, int M) {
   if (nb <= 0) return;
-  constexpr int QG = sizeof(T) == 8 ? 8 : 16;  // neighbours per reduction group
+  constexpr int QG = 8;  // neighbours per reduction group
   const int lane = lane_id();
   const T last = em[(size_t)(nb - 1) * 4];
   int jt = nb;  // first neighbour of the sorted zero-padding tail (tabulate.cc:188-201)
@@ -177,11 +177,9 @@ __device__ void embed_bwd_block(T* __restrict__ nd, const T (&dy)[KQ][4], const 
           T p[4] = {0, 0, 0, 0};
 #pragma unroll
           for (int kq = 0; kq < KQ; ++kq) {
-            const bool kin = lane + kq * kWave < M;
+            // columns k >= M: clamped (finite) coefficients times dy = 0
             T v, d;
             quintic(a[u][kq], x, delta, v, d);
-            v = kin ? v : (T)0;
-            d = kin ? d : (T)0;
             const T proj = ll[0] * dy[kq][0] + ll[1] * dy[kq][1] + ll[2] * dy[kq][2] + ll[3] * dy[kq][3];
             p[0] += v * dy[kq][0] + d * proj;
 #pragma unroll
