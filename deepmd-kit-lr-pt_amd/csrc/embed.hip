@@ -119,6 +119,90 @@ __device__ __forceinline__ T transpose_sum(T (&w)[NV]) {
   return r;
 }
 
+// FP32 variant of transpose_sum with the exchanges across lane bits 3..0 as
+// DPP row_mirror / row_half_mirror / quad_perm moves fused into the adds
+// (each pairs every lane with one on the other side of that bit; no LDS
+// traffic), those across bits 5 and 4 as shuffles.  Bits are taken from the
+// top, so lane l ends with the total of value tvalue<NV>(l), whose bit k is
+// lane bit 5 - k.
+template <int NV>
+__device__ __forceinline__ int tvalue(int lane) {
+  int v = 0;
+#pragma unroll
+  for (int k = 0; (1 << k) < NV; ++k) v |= ((lane >> (5 - k)) & 1) << k;
+  return v;
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int B>
+__device__ __forceinline__ float dpp_partner(float v) {
+  // partner across lane bit B (B < 4): 15 - l, 7 - l (within 16 / 8 lanes),
+  // l ^ 2, l ^ 1
+  if constexpr (B == 3) return dpp_move<0x140>(v);
+  else if constexpr (B == 2) return dpp_move<0x141>(v);
+  else if constexpr (B == 1) return dpp_move<0x4E>(v);
+  else return dpp_move<0xB1>(v);
+}
+template <int B>
+__device__ __forceinline__ float pair_step(float lo_val, float hi_val, bool hi) {
+  // lanes with bit B clear keep the total of lo_val over the pair, the
+  // others that of hi_val
+  if constexpr (B >= 4) {
+    const float keep = hi ? hi_val : lo_val, send = hi ? lo_val : hi_val;
+    return keep + __shfl_xor(send, 1 << B, kWave);
+  } else {
+    const float t0 = lo_val + dpp_partner<B>(lo_val), t1 = hi_val + dpp_partner<B>(hi_val);
+    return hi ? t1 : t0;
+  }
+}
+template <int NV, int S>
+__device__ __forceinline__ void transpose_steps(float (&w)[NV], int lane) {
+  if constexpr ((1 << S) < NV) {
+    constexpr int B = 5 - S;
+    const bool hi = (lane >> B) & 1;
+#pragma unroll
+    for (int i = 0; i < (NV >> (S + 1)); ++i) w[i] = pair_step<B>(w[2 * i], w[2 * i + 1], hi);
+    transpose_steps<NV, S + 1>(w, lane);
+  }
+}
+template <int NV>
+__device__ __forceinline__ float transpose_sum_f32(float (&w)[NV]) {
+  static_assert(NV >= 4 && NV <= 64 && (NV & (NV - 1)) == 0, "4 <= NV <= 64, a power of two");
+  transpose_steps<NV, 0>(w, lane_id());
+  // lane bits below the transposed ones (bits < 6 - log2 NV): butterflies
+  float r = w[0];
+  if constexpr (NV <= 32) r += dpp_partner<0>(r);
+  if constexpr (NV <= 16) r += dpp_partner<1>(r);
+  if constexpr (NV <= 8) r += dpp_partner<2>(r);
+  if constexpr (NV <= 4) r += dpp_partner<3>(r);
+  return r;
+}
+
+// Transposed sum with its lane layout: value_of(lane) is the value a lane
+// ends with, owner(lane) picks one lane per value, lane_of(v) a lane that
+// holds value v.
+template <typename T, int NV>
+struct TSum {
+  static __device__ __forceinline__ T sum(T (&w)[NV]) { return transpose_sum<T, NV>(w); }
+  static __device__ __forceinline__ int value_of(int lane) { return lane % NV; }
+  static __device__ __forceinline__ bool owner(int lane) { return lane < NV; }
+  static __device__ __forceinline__ int lane_of(int v) { return v; }
+};
+template <int NV>
+struct TSum<float, NV> {
+  static __device__ __forceinline__ float sum(float (&w)[NV]) { return transpose_sum_f32<NV>(w); }
+  static __device__ __forceinline__ int value_of(int lane) { return tvalue<NV>(lane); }
+  static __device__ __forceinline__ bool owner(int lane) { return (lane & (kWave / NV - 1)) == 0; }
+  static __device__ __forceinline__ int lane_of(int v) {
+    int l = 0;
+#pragma unroll
+    for (int k = 0; (1 << k) < NV; ++k) l |= ((v >> k) & 1) << (5 - k);
+    return l;
+  }
+};
+
 // Adjoint of one neighbour block (nb slots) of row g, lanes over columns k
 // (as the forward, so each neighbour's table row is one coalesced read):
 //   nd[j, c] = sum_k dy[k, c] P_k(x_j) + [c = 0] sum_k P'_k(x_j) sum_c' dy[k, c'] em[j, c']
@@ -189,12 +273,13 @@ __device__ void embed_bwd_block(T* __restrict__ nd, const T (&dy)[KQ][4], const 
           for (int c = 0; c < 4; ++c) w[(u0 + u) * 4 + c] = p[c];
         }
       }
-      const T tot = transpose_sum<T, QG * 4>(w);
-      const int qq = (lane >> 2) & (QG - 1);
+      using TS = TSum<T, QG * 4>;
+      const T tot = TS::sum(w);
+      const int val = TS::value_of(lane), qq = val >> 2;
       const int j = c0 + q0 + qq;
-      if (lane < QG * 4 && q0 + qq < cnt) {
+      if (TS::owner(lane) && q0 + qq < cnt) {
         const T scale = (j == jt) ? (T)(nb - jt) : (T)1;
-        nd[(size_t)j * 4 + (lane & 3)] = tot * scale;
+        nd[(size_t)j * 4 + (val & 3)] = tot * scale;
       }
     }
   }
@@ -247,11 +332,13 @@ __device__ __forceinline__ void bwd_dy(T (&dy)[KQ][4], const T* __restrict__ xyz
       }
     }
   }
-  // S[c, a] lands on lane c*16 + a; lane a < A adds S[c, a] to dT[c, a]
-  const T S = transpose_sum<T, 64>(sp);  // Tk already carries 1 / nnei
+  // value c*16 + a of the transposed sum is S[c, a]; lane a < A adds S[c, a]
+  // to dT[c, a]
+  using TS = TSum<T, 64>;
+  const T S = TS::sum(sp);  // Tk already carries 1 / nnei
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    const T sv = __shfl(S, c * 16 + (lane & 15), kWave);
+    const T sv = __shfl(S, TS::lane_of(c * 16 + (lane & 15)), kWave);
     if (lane < A) dT[0][c] += sv;
   }
 #pragma unroll

@@ -289,27 +289,40 @@ __global__ void __launch_bounds__(1024) k_force_virial_hash(
     double vs[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) vs[k] = 0.;
-    for (int s = lane; s < nnei; s += kWave) {
-      const size_t b = (size_t)g * nnei + s;
-      const int j = nlist[b];
-      if (j < 0) continue;  // padding slots have zero em_deriv
-      TI n4[4], e12[12], r3[3], fs[3], v[9];
+    // SU slots per lane are loaded before any is used (out-of-range slots
+    // read slot 0 and are dropped), so their loads overlap
+    constexpr int SU = sizeof(TI) == 8 ? 1 : 3;
+    for (int s0 = 0; s0 < nnei; s0 += SU * kWave) {
+      TI n4[SU][4], e12[SU][12], r3[SU][3];
+      int jj[SU];
 #pragma unroll
-      for (int a = 0; a < 4; ++a) n4[a] = nd[b * 4 + a];
+      for (int u = 0; u < SU; ++u) {
+        const int s = s0 + u * kWave + lane;
+        const bool ok = s < nnei;
+        const size_t b = (size_t)g * nnei + (ok ? s : 0);
+        jj[u] = ok ? nlist[b] : -1;
 #pragma unroll
-      for (int k = 0; k < 12; ++k) e12[k] = ed[b * 12 + k];
+        for (int a = 0; a < 4; ++a) n4[u][a] = nd[b * 4 + a];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) r3[d] = rij[b * 3 + d];
-      slot_force(n4, e12, fs);
-      slot_virial(n4, e12, r3, v);
-      const int h = hash_slot(hkey, eo + j);
+        for (int k = 0; k < 12; ++k) e12[u][k] = ed[b * 12 + k];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        fi[d] -= fs[d];
-        atomicAdd(&hval[h][d], (double)fs[d]);
+        for (int d = 0; d < 3; ++d) r3[u][d] = rij[b * 3 + d];
       }
 #pragma unroll
-      for (int k = 0; k < 9; ++k) vs[k] += v[k];
+      for (int u = 0; u < SU; ++u) {
+        if (jj[u] < 0) continue;  // padding slots have zero em_deriv
+        TI fs[3], v[9];
+        slot_force(n4[u], e12[u], fs);
+        slot_virial(n4[u], e12[u], r3[u], v);
+        const int h = hash_slot(hkey, eo + jj[u]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          fi[d] -= fs[d];
+          atomicAdd(&hval[h][d], (double)fs[d]);
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) vs[k] += v[k];
+      }
     }
 #pragma unroll
     for (int d = 0; d < 3; ++d) fi[d] = wave_sum(fi[d]);

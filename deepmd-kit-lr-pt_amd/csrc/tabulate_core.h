@@ -89,6 +89,17 @@ __device__ __forceinline__ void quintic(const T* a, T x, T delta, T& v, T& d) {
   v += d * delta;
 }
 
+// Value and derivative of the quintic separately (the forward needs the
+// derivative only for the linear extension outside the table, delta != 0).
+template <typename T>
+__device__ __forceinline__ T quintic_value(const T* a, T x) {
+  return a[0] + (a[1] + (a[2] + (a[3] + (a[4] + a[5] * x) * x) * x) * x) * x;
+}
+template <typename T>
+__device__ __forceinline__ T quintic_slope(const T* a, T x) {
+  return a[1] + ((T)2 * a[2] + ((T)3 * a[3] + ((T)4 * a[4] + (T)5 * a[5] * x) * x) * x) * x;
+}
+
 // Per-lane neighbour record of a 64-neighbour chunk.
 template <typename T, int ND>
 struct Nbr {
@@ -117,6 +128,32 @@ __device__ __forceinline__ bool load_nbr(Nbr<T, ND>& r, const Info<T>& f, const 
   return tail;
 }
 
+// acc += w P(x_q) ll_q for neighbour q of the chunk record r (the slope
+// term only outside the table, delta != 0: a uniform branch).
+template <typename T, int ND, int KQ>
+__device__ __forceinline__ void fwd_neighbour(T (&acc)[KQ][ND], const T (&a)[KQ][6], const Nbr<T, ND>& r, int q,
+                                              T w) {
+  const T x = bcast(r.x, q), delta = bcast(r.delta, q);
+  T ll[ND];
+#pragma unroll
+  for (int m = 0; m < ND; ++m) ll[m] = bcast(r.ll[m], q);
+  T v[KQ];
+#pragma unroll
+  for (int kq = 0; kq < KQ; ++kq) v[kq] = quintic_value(a[kq], x);
+  if (delta != (T)0) {
+#pragma unroll
+    for (int kq = 0; kq < KQ; ++kq) v[kq] += quintic_slope(a[kq], x) * delta;
+  }
+  if (w != (T)1) {
+#pragma unroll
+    for (int kq = 0; kq < KQ; ++kq) v[kq] *= w;
+  }
+#pragma unroll
+  for (int kq = 0; kq < KQ; ++kq)
+#pragma unroll
+    for (int m = 0; m < ND; ++m) acc[kq][m] += v[kq] * ll[m];
+}
+
 // Accumulates one row block (nnei neighbours) into acc[KQ][ND].
 template <typename T, int ND, int KQ, bool PK = false>
 __device__ __forceinline__ void fwd_block(T (&acc)[KQ][ND], const T* __restrict__ table, const Info<T>& f,
@@ -132,36 +169,30 @@ __device__ __forceinline__ void fwd_block(T (&acc)[KQ][ND], const T* __restrict_
     int jend = min(kWave, nnei - c0);
     if (tm) jend = __ffsll((long long)tm);  // up to and including the first tail neighbour
     // U neighbours per step: every table row of the step is requested before
-    // any is used (addresses clamped to valid rows and columns, out-of-range
-    // terms weighted by zero), so the row loads overlap instead of forming
-    // a load-use chain per neighbour
+    // any is used, so the row loads overlap instead of forming a load-use
+    // chain per neighbour.  Full steps hold unit-weight neighbours only; the
+    // remaining ones and the first tail neighbour (weight nnei - jt) follow
+    // one at a time.  The order of accumulation is the neighbour order.
     constexpr int U = KQ <= 2 ? 4 : (KQ <= 4 ? 2 : 1);
-    for (int q0 = 0; q0 < jend; q0 += U) {
+    const int jmain = tm ? jend - 1 : jend;
+    int q0 = 0;
+    for (; q0 + U <= jmain; q0 += U) {
       T a[U][KQ][6];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int q = min(q0 + u, jend - 1);
-        const int idx = bcast(r.idx, q);
+        const int idx = bcast(r.idx, q0 + u);
 #pragma unroll
         for (int kq = 0; kq < KQ; ++kq) load_coef<T, PK>(table, idx, min(lane + kq * kWave, M - 1), M, a[u][kq]);
       }
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int q = min(q0 + u, jend - 1);
-        const T x = bcast(r.x, q), delta = bcast(r.delta, q);
-        T ll[ND];
+      for (int u = 0; u < U; ++u) fwd_neighbour<T, ND, KQ>(acc, a[u], r, q0 + u, (T)1);
+    }
+    for (int q = q0; q < jend; ++q) {
+      T a[KQ][6];
+      const int idx = bcast(r.idx, q);
 #pragma unroll
-        for (int m = 0; m < ND; ++m) ll[m] = bcast(r.ll[m], q);
-        const T scale = q0 + u >= jend ? (T)0 : ((tm && q == jend - 1) ? (T)(nnei - (c0 + q)) : (T)1);
-#pragma unroll
-        for (int kq = 0; kq < KQ; ++kq) {
-          T v, d;
-          quintic(a[u][kq], x, delta, v, d);
-          const T w = v * scale;
-#pragma unroll
-          for (int m = 0; m < ND; ++m) acc[kq][m] += w * ll[m];
-        }
-      }
+      for (int kq = 0; kq < KQ; ++kq) load_coef<T, PK>(table, idx, min(lane + kq * kWave, M - 1), M, a[kq]);
+      fwd_neighbour<T, ND, KQ>(acc, a, r, q, (tm && q == jend - 1) ? (T)(nnei - (c0 + q)) : (T)1);
     }
     if (tm) break;
   }
