@@ -78,10 +78,10 @@ def stage_model(m, nloc, nall, precision):
 # FETCH_SIZE / WRITE_SIZE on this workload, scaled by a 256 MiB copy
 # (profiles/water12k_kernels_and_traffic.txt).
 MEASURED_TRAFFIC_MIB = {
-    "mixed": {"embed_bwd": (129.15, 26.00), "embed_fwd": (30.26, 56.25), "fitting": (57.95, 94.87),
-              "force_virial": (91.00, 19.75), "env_mat": (15.12, 122.91), "nlist": (8.15, 7.12)},
-    "fp64": {"embed_bwd": (264.69, 51.89), "embed_fwd": (67.16, 205.97), "force_virial": (166.85, 19.75),
-             "env_mat": (16.32, 263.49), "nlist": (8.15, 7.12)},
+    "mixed": {"embed_bwd": (154.26, 26.28), "embed_fwd": (56.75, 56.29), "fitting": (78.11, 100.79),
+              "force_virial": (131.20, 26.33), "env_mat": (15.13, 122.91), "nlist": (8.15, 7.12)},
+    "fp64": {"embed_bwd": (513.48, 52.38), "embed_fwd": (295.91, 258.70), "force_virial": (166.52, 25.88),
+             "env_mat": (16.41, 264.96), "nlist": (8.15, 7.12)},
 }
 
 
