@@ -73,10 +73,19 @@ __global__ void k_env_mat_a(const double* __restrict__ coord, const int* __restr
     const double inr3 = inr4 * nr;
     double sw, dsw;
     smooth_switch(sw, dsw, nr, rmin, rmax);
-    val[0] = 1. / nr;
-    val[1] = rr[0] / nr2;
-    val[2] = rr[1] / nr2;
-    val[3] = rr[2] / nr2;
+    if constexpr (sizeof(TO) == 4) {
+      // stored in FP32: reciprocal products differ from the quotients by
+      // about one FP64 ulp, far below the FP32 rounding that follows
+      val[0] = inr;
+      val[1] = rr[0] * inr2;
+      val[2] = rr[1] * inr2;
+      val[3] = rr[2] * inr2;
+    } else {
+      val[0] = 1. / nr;
+      val[1] = rr[0] / nr2;
+      val[2] = rr[1] / nr2;
+      val[3] = rr[2] / nr2;
+    }
 #pragma unroll
     for (int d = 0; d < 3; ++d) der[d] = rr[d] * inr3 * sw - val[0] * dsw * rr[d] * inr;
 #pragma unroll
@@ -97,9 +106,16 @@ __global__ void k_env_mat_a(const double* __restrict__ coord, const int* __restr
     const double* sd = stdv + (size_t)ti * nnei * 4 + s * 4;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
-      em[base * 4 + a] = (TO)((val[a] - av[a]) / sd[a]);
+      if constexpr (sizeof(TO) == 4) {
+        const double isd = 1. / sd[a];
+        em[base * 4 + a] = (TO)((val[a] - av[a]) * isd);
 #pragma unroll
-      for (int d = 0; d < 3; ++d) em_deriv[base * 12 + a * 3 + d] = (TO)(der[a * 3 + d] / sd[a]);
+        for (int d = 0; d < 3; ++d) em_deriv[base * 12 + a * 3 + d] = (TO)(der[a * 3 + d] * isd);
+      } else {
+        em[base * 4 + a] = (TO)((val[a] - av[a]) / sd[a]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) em_deriv[base * 12 + a * 3 + d] = (TO)(der[a * 3 + d] / sd[a]);
+      }
     }
   } else {
 #pragma unroll
