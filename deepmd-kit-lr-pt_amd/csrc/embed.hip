@@ -258,16 +258,23 @@ __device__ void embed_bwd_block(T* __restrict__ nd, const T (&dy)[KQ][4], const 
           T ll[4];
 #pragma unroll
           for (int c = 0; c < 4; ++c) ll[c] = bcast(r.ll[c], q);
-          T p[4] = {0, 0, 0, 0};
+          // columns k >= M: clamped (finite) coefficients times dy = 0
+          T v[KQ], d[KQ];
+#pragma unroll
+          for (int kq = 0; kq < KQ; ++kq) quintic(a[u][kq], x, delta, v[kq], d[kq]);
+          T p[4];
 #pragma unroll
           for (int kq = 0; kq < KQ; ++kq) {
-            // columns k >= M: clamped (finite) coefficients times dy = 0
-            T v, d;
-            quintic(a[u][kq], x, delta, v, d);
             const T proj = ll[0] * dy[kq][0] + ll[1] * dy[kq][1] + ll[2] * dy[kq][2] + ll[3] * dy[kq][3];
-            p[0] += v * dy[kq][0] + d * proj;
+            if (kq == 0) {
+              p[0] = v[kq] * dy[kq][0] + d[kq] * proj;
 #pragma unroll
-            for (int c = 1; c < 4; ++c) p[c] += v * dy[kq][c];
+              for (int c = 1; c < 4; ++c) p[c] = v[kq] * dy[kq][c];
+            } else {
+              p[0] += v[kq] * dy[kq][0] + d[kq] * proj;
+#pragma unroll
+              for (int c = 1; c < 4; ++c) p[c] += v[kq] * dy[kq][c];
+            }
           }
 #pragma unroll
           for (int c = 0; c < 4; ++c) w[(u0 + u) * 4 + c] = p[c];

@@ -326,13 +326,14 @@ def _switch(x: float, rmin: float, rmax: float) -> float:
 
 
 def water_model(seed: int = 1, min_nbor_dist: float = 0.5, type_one_side: bool = False,
-                env_stat: bool = True) -> SeAModel:
+                env_stat: bool = False) -> SeAModel:
     """The water se_e2_a model of the benchmark: rcut 6.0, rcut_smth 0.5,
     sel [46, 92], embedding [25, 50, 100], axis 16, fitting [240]*3, tanh,
-    N(0, 1/fan_in) weights, zero biases, compressed at strides 0.01/0.1.
-    With env_stat, davg/dstd come from a 768-atom water frame (as training
-    sets them), so the normalised environment matrix and the energies are
-    of order one; else davg 0, dstd 1."""
+    N(0, 1/fan_in) weights, zero biases, davg 0, dstd 1, compressed at
+    strides 0.01/0.1.  With env_stat, davg/dstd come from a 768-atom water
+    frame instead (as training sets them): the normalised environment
+    matrix and the energies are then of order one, and the tables span
+    about 2000 rows."""
     from .systems import water
 
     m = SeAModel.random(2, [46, 92], 6.0, 0.5, [25, 50, 100], 16, [240, 240, 240],
@@ -343,10 +344,10 @@ def water_model(seed: int = 1, min_nbor_dist: float = 0.5, type_one_side: bool =
     return m
 
 
-def copper_model(seed: int = 1, min_nbor_dist: float = 1.0, env_stat: bool = True) -> SeAModel:
+def copper_model(seed: int = 1, min_nbor_dist: float = 1.0, env_stat: bool = False) -> SeAModel:
     """Single-type fcc Cu model: rcut 8.0, rcut_smth 2.0, sel [200],
-    embedding [32, 64, 128], axis 16, fitting [240]*3; davg/dstd from a
-    500-atom fcc frame with env_stat (else davg 0, dstd 1)."""
+    embedding [32, 64, 128], axis 16, fitting [240]*3, davg 0, dstd 1; with
+    env_stat, davg/dstd from a 500-atom fcc frame."""
     from .systems import fcc_copper
 
     m = SeAModel.random(1, [200], 8.0, 2.0, [32, 64, 128], 16, [240, 240, 240], seed=seed)

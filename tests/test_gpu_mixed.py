@@ -90,6 +90,18 @@ def fitting_reference(m, desc16, atype):
 
 @pytest.mark.parametrize("nmol,seed", [(64, 1), (512, 7)])
 def test_mixed_water_against_fp64(water, water_dev, nmol, seed):
+    _check_water_against_fp64(water, water_dev, nmol, seed)
+
+
+def test_mixed_water_with_data_statistics_against_fp64():
+    """davg/dstd from the environment statistics of a water frame: energies
+    of order one (the random model with davg 0 / dstd 1 gives ~1e-4
+    eV/atom), tables of ~2000 rows."""
+    m = dmodel.water_model(seed=1, env_stat=True)
+    _check_water_against_fp64(m, DeviceModel(m), 512, 7)
+
+
+def _check_water_against_fp64(water, water_dev, nmol, seed):
     print(f"case mixed water nmol={nmol} seed={seed}", flush=True)
     coord, atype, box = systems.water(nmol, seed=seed)
     n = len(atype)
@@ -132,8 +144,9 @@ def test_mixed_batched_uneven_frames(water, water_dev):
         off += n
 
 
-def test_mixed_copper_small():
-    m = dmodel.copper_model(seed=2)
+@pytest.mark.parametrize("env_stat", [False, True])
+def test_mixed_copper_small(env_stat):
+    m = dmodel.copper_model(seed=2, env_stat=env_stat)
     coord, atype, box = systems.fcc_copper(5, seed=4)
     dm = DeviceModel(m)
     n = len(atype)
