@@ -129,6 +129,28 @@ def time_engine(eng, x, box, steps, warmup):
     return dt, stages
 
 
+def time_graph(eng, x, box, steps, warmup):
+    """The step captured into a hipGraph on a side stream, `steps` timed
+    replays bracketed by device syncs; returns seconds."""
+    import torch
+
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        eng.capture(x, box)
+        for _ in range(warmup):
+            eng.replay()
+    torch.cuda.synchronize()
+    eng.status()
+    t0 = time.perf_counter()
+    with torch.cuda.stream(s):
+        for _ in range(steps):
+            eng.replay()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    eng.status()
+    return dt
+
+
 def roofline(m, nloc, nall, precision, stages):
     model = stage_model(m, nloc, nall, precision)
     dom = max((k for k in stages if k in model), key=lambda k: stages[k])
@@ -365,6 +387,7 @@ def main() -> int:
                     help="path reported as value (the other one is timed beside it)")
     ap.add_argument("--nmol", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--eager", action="store_true", help="time the mixed engine with eager launches only")
     ap.add_argument("--workload", default="water12k", choices=["water12k", "batched", "copper1m"],
                     help="water12k: the 12,288-atom water frame (default); batched: uneven water frames; "
                          "copper1m: 1M-atom fcc copper")
@@ -412,11 +435,16 @@ def main() -> int:
     for prec in (args.precision, other):
         eng = Engine(dm, [nloc], atype, precision=prec)
         knobs[prec] = eng.autotune(x, box, reps=3)
-        dt, stages = time_engine(eng, x, box, args.steps, args.warmup)
+        dt_eager, stages = time_engine(eng, x, box, args.steps, args.warmup)
+        dt = dt_eager
+        if prec == "mixed" and not args.eager:
+            # the same step captured once into a hipGraph and replayed (the
+            # coordinates are re-read from x on every replay)
+            dt = time_graph(eng, x, box, args.steps, args.warmup)
         e, f, v = eng.compute(x, box)
         torch.cuda.synchronize()
         nall = int(eng.fetch("nall")[0])
-        res[prec] = {"dt": dt, "stages": stages, "nall": nall, "e": e.cpu().numpy().copy(),
+        res[prec] = {"dt": dt, "dt_eager": dt_eager, "stages": stages, "nall": nall, "e": e.cpu().numpy().copy(),
                      "f": f.cpu().numpy().copy(), "v": v.cpu().numpy().copy()}
         del eng
     r = res[args.precision]
@@ -452,6 +480,8 @@ def main() -> int:
         "stages_ms": {k: round(vv, 5) for k, vv in r["stages"].items()},
         "fp64_value": fp64_value,
         "mixed_value": mixed_value,
+        "mixed_launch": "eager" if args.eager else "hipGraph replay (stages_ms from the eager run)",
+        "mixed_eager_value": nloc * args.steps / res["mixed"]["dt_eager"],
         "mixed_over_fp64": mixed_value / fp64_value,
         "mixed_error_vs_fp64": mixed_err,
         "other_stages_ms": {k: round(vv, 5) for k, vv in res[other]["stages"].items()},
