@@ -56,6 +56,11 @@ __device__ __forceinline__ float wave_sum(float v) {
   return v;
 }
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+// Global index of the calling wave, marked wave-uniform (readfirstlane) so
+// that addresses derived from it go through scalar registers and loads.
+__device__ __forceinline__ long long wave_index() {
+  return (long long)blockIdx.x * (blockDim.x / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+}
 
 // Neumaier-compensated accumulation (sum, compensation).
 struct KahanD {

@@ -52,7 +52,7 @@ __global__ void __launch_bounds__(256) k_embed_fwd(T* __restrict__ xyz, TD* __re
                                                    const T* __restrict__ em, const int* __restrict__ row_type,
                                                    const int* __restrict__ sec, int ntypes, long long nrows, int nnei,
                                                    int M, int A, int ldD, T inv_nnei) {
-  const long long g = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const long long g = wave_index();
   if (g >= nrows) return;
   const int lane = lane_id();
   const int ti = row_type[g];
@@ -232,6 +232,10 @@ __device__ void embed_bwd_block(T* __restrict__ nd, const T (&dy)[KQ][4], const 
     }
   }
   const int jend = jt < nb ? jt + 1 : nb;
+  const __amdgpu_buffer_rsrc_t rs = pk_rsrc(table);
+  int coff[KQ];  // byte offset of the lane's record in a packed row
+#pragma unroll
+  for (int kq = 0; kq < KQ; ++kq) coff[kq] = min(lane + kq * kWave, M - 1) * 16;
   for (int c0 = 0; c0 < jend; c0 += kWave) {
     Nbr<T, 4> r;
     load_nbr(r, f, em + (size_t)c0 * 4, 4, em + (size_t)c0 * 4, lane, jend - c0, last, false);
@@ -248,16 +252,21 @@ __device__ void embed_bwd_block(T* __restrict__ nd, const T (&dy)[KQ][4], const 
         for (int u = 0; u < U; ++u) {
           const int q = min(q0 + u0 + u, cnt - 1);
           const int idx = bcast(r.idx, q);
+          if constexpr (PK) {
 #pragma unroll
-          for (int kq = 0; kq < KQ; ++kq) load_coef<T, PK>(table, idx, min(lane + kq * kWave, M - 1), M, a[u][kq]);
+            for (int kq = 0; kq < KQ; ++kq) load_coef_pk(rs, idx * M * 16, coff[kq], a[u][kq]);
+          } else {
+#pragma unroll
+            for (int kq = 0; kq < KQ; ++kq) load_coef<T, PK>(table, idx, min(lane + kq * kWave, M - 1), M, a[u][kq]);
+          }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const int q = min(q0 + u0 + u, cnt - 1);
           const T x = bcast(r.x, q), delta = bcast(r.delta, q);
-          T ll[4];
+          T ll[4];  // the neighbour's em row, read as uniform (scalar) loads
 #pragma unroll
-          for (int c = 0; c < 4; ++c) ll[c] = bcast(r.ll[c], q);
+          for (int c = 0; c < 4; ++c) ll[c] = em[(size_t)(c0 + q) * 4 + c];
           // columns k >= M: clamped (finite) coefficients times dy = 0
           T v[KQ], d[KQ];
 #pragma unroll
@@ -362,7 +371,7 @@ __global__ void __launch_bounds__(256) k_embed_bwd(T* __restrict__ net_deriv, co
                                                   const T* __restrict__ em, const int* __restrict__ row_type,
                                                   const int* __restrict__ sec, int ntypes, long long nrows, int nnei,
                                                   int M, int A, int ldD, T inv_nnei) {
-  const long long g = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const long long g = wave_index();
   const int lane = lane_id();
   if (g >= nrows) return;
   const int ti = row_type[g];

@@ -69,7 +69,8 @@ __device__ __forceinline__ void load_coef(const T* __restrict__ table, int idx, 
   if constexpr (PK) {
     static_assert(sizeof(T) == 4, "packed tables are FP32");
     const float4 r = reinterpret_cast<const float4*>(table)[(size_t)idx * M + col];
-    const tab_half2 h23 = __builtin_bit_cast(tab_half2, r.z), h45 = __builtin_bit_cast(tab_half2, r.w);
+    const float w2 = r.z, w3 = r.w;  // scalars for the bit casts (as load_coef_pk)
+    const tab_half2 h23 = __builtin_bit_cast(tab_half2, w2), h45 = __builtin_bit_cast(tab_half2, w3);
     a[0] = r.x;
     a[1] = r.y;
     a[2] = (T)h23.x;
@@ -81,11 +82,45 @@ __device__ __forceinline__ void load_coef(const T* __restrict__ table, int idx, 
   }
 }
 
+// Packed records through a buffer resource: the row offset idx * M * 16 is
+// wave-uniform and goes in the scalar offset, the lane's column offset
+// col * 16 in the vector offset, so a record read needs no per-lane 64-bit
+// address arithmetic.  The resource spans the whole 32-bit offset range (the
+// row index is already clamped to the table by locate()).
+typedef unsigned int tab_u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pk_rsrc(const void* table) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(table), (short)0, -1, 0x00020000);
+}
+__device__ __forceinline__ void load_coef_pk(__amdgpu_buffer_rsrc_t rs, int row_off, int col_off, float* a) {
+  const tab_u32x4 r = __builtin_amdgcn_raw_buffer_load_b128(rs, col_off, row_off, 0);
+  // elements copied to scalars first: clang's __builtin_bit_cast of an
+  // ext_vector element expression reads from the start of the vector
+  const unsigned int w0 = r.x, w1 = r.y, w2 = r.z, w3 = r.w;
+  const tab_half2 h23 = __builtin_bit_cast(tab_half2, w2), h45 = __builtin_bit_cast(tab_half2, w3);
+  a[0] = __uint_as_float(w0);
+  a[1] = __uint_as_float(w1);
+  a[2] = (float)h23.x;
+  a[3] = (float)h23.y;
+  a[4] = (float)h45.x;
+  a[5] = (float)h45.y;
+}
+
 template <typename T>
 __device__ __forceinline__ void quintic(const T* a, T x, T delta, T& v, T& d) {
-  // tabulate.cc:122-158
-  d = a[1] + ((T)2 * a[2] + ((T)3 * a[3] + ((T)4 * a[4] + (T)5 * a[5] * x) * x) * x) * x;
-  v = a[0] + (a[1] + (a[2] + (a[3] + (a[4] + a[5] * x) * x) * x) * x) * x;
+  // tabulate.cc:122-158, evaluated as Horner's scheme with the derivative
+  // carried along (synthetic division): b_i = b_(i+1) x + a_i gives the
+  // value b_0, c_i = c_(i+1) x + b_(i+1) the slope c_0 -- nine FMAs and no
+  // multiplies by the constants 2..5
+  T b = a[5] * x + a[4];
+  T c = a[5] * x + b;
+  b = b * x + a[3];
+  c = c * x + b;
+  b = b * x + a[2];
+  c = c * x + b;
+  b = b * x + a[1];
+  c = c * x + b;
+  d = c;
+  v = b * x + a[0];
   v += d * delta;
 }
 
