@@ -232,10 +232,7 @@ __device__ void embed_bwd_block(T* __restrict__ nd, const T (&dy)[KQ][4], const 
     }
   }
   const int jend = jt < nb ? jt + 1 : nb;
-  const __amdgpu_buffer_rsrc_t rs = pk_rsrc(table);
-  int coff[KQ];  // byte offset of the lane's record in a packed row
-#pragma unroll
-  for (int kq = 0; kq < KQ; ++kq) coff[kq] = min(lane + kq * kWave, M - 1) * 16;
+  const RowReader<T, KQ, PK> rows(table, M, lane);
   for (int c0 = 0; c0 < jend; c0 += kWave) {
     Nbr<T, 4> r;
     load_nbr(r, f, em + (size_t)c0 * 4, 4, em + (size_t)c0 * 4, lane, jend - c0, last, false);
@@ -251,14 +248,7 @@ __device__ void embed_bwd_block(T* __restrict__ nd, const T (&dy)[KQ][4], const 
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const int q = min(q0 + u0 + u, cnt - 1);
-          const int idx = bcast(r.idx, q);
-          if constexpr (PK) {
-#pragma unroll
-            for (int kq = 0; kq < KQ; ++kq) load_coef_pk(rs, idx * M * 16, coff[kq], a[u][kq]);
-          } else {
-#pragma unroll
-            for (int kq = 0; kq < KQ; ++kq) load_coef<T, PK>(table, idx, min(lane + kq * kWave, M - 1), M, a[u][kq]);
-          }
+          rows.load(bcast(r.idx, q), a[u]);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -267,7 +257,7 @@ __device__ void embed_bwd_block(T* __restrict__ nd, const T (&dy)[KQ][4], const 
           T ll[4];  // the neighbour's em row, read as uniform (scalar) loads
 #pragma unroll
           for (int c = 0; c < 4; ++c) ll[c] = em[(size_t)(c0 + q) * 4 + c];
-          // columns k >= M: clamped (finite) coefficients times dy = 0
+          // columns k >= M: zero coefficients times dy = 0
           T v[KQ], d[KQ];
 #pragma unroll
           for (int kq = 0; kq < KQ; ++kq) quintic(a[u][kq], x, delta, v[kq], d[kq]);

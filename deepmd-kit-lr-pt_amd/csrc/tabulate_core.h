@@ -85,11 +85,13 @@ __device__ __forceinline__ void load_coef(const T* __restrict__ table, int idx, 
 // Packed records through a buffer resource: the row offset idx * M * 16 is
 // wave-uniform and goes in the scalar offset, the lane's column offset
 // col * 16 in the vector offset, so a record read needs no per-lane 64-bit
-// address arithmetic.  The resource spans the whole 32-bit offset range (the
-// row index is already clamped to the table by locate()).
+// address arithmetic.  The resource's record count is 2^31 - 1 bytes; a lane
+// with vector offset kDeadOffset (2^31) is out of range and reads zeros
+// (buffer range checking), which masks lanes without a branch.
 typedef unsigned int tab_u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kDeadOffset = (int)0x80000000u;
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t pk_rsrc(const void* table) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(table), (short)0, -1, 0x00020000);
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(table), (short)0, 0x7fffffff, 0x00020000);
 }
 __device__ __forceinline__ void load_coef_pk(__amdgpu_buffer_rsrc_t rs, int row_off, int col_off, float* a) {
   const tab_u32x4 r = __builtin_amdgcn_raw_buffer_load_b128(rs, col_off, row_off, 0);
@@ -104,6 +106,32 @@ __device__ __forceinline__ void load_coef_pk(__amdgpu_buffer_rsrc_t rs, int row_
   a[4] = (float)h45.x;
   a[5] = (float)h45.y;
 }
+
+// Reads the coefficients of a lane's KQ columns (k = lane + 64 kq) of one
+// table row.  Packed tables: lanes past the last column (k >= M) read zeros
+// through an out-of-range offset.  Plain tables: such lanes read column M - 1
+// (finite values; callers give those columns zero weight).
+template <typename T, int KQ, bool PK>
+struct RowReader {
+  const T* table;
+  __amdgpu_buffer_rsrc_t rs;
+  int M, col[KQ];
+  __device__ __forceinline__ RowReader(const T* t, int M_, int lane) : table(t), M(M_) {
+    rs = pk_rsrc(t);
+#pragma unroll
+    for (int kq = 0; kq < KQ; ++kq) {
+      const int k = lane + kq * kWave;
+      col[kq] = PK ? (k < M ? k * 16 : kDeadOffset) : min(k, M - 1);
+    }
+  }
+  __device__ __forceinline__ void load(int idx, T (&a)[KQ][6]) const {
+#pragma unroll
+    for (int kq = 0; kq < KQ; ++kq) {
+      if constexpr (PK) load_coef_pk(rs, idx * M * 16, col[kq], a[kq]);
+      else load_coef<T, PK>(table, idx, col[kq], M, a[kq]);
+    }
+  }
+};
 
 template <typename T>
 __device__ __forceinline__ void quintic(const T* a, T x, T delta, T& v, T& d) {
@@ -197,6 +225,7 @@ __device__ __forceinline__ void fwd_block(T (&acc)[KQ][ND], const T* __restrict_
   if (nnei <= 0) return;
   const int lane = lane_id();
   const T last = em_x[(size_t)(nnei - 1) * xs];
+  const RowReader<T, KQ, PK> rows(table, M, lane);
   for (int c0 = 0; c0 < nnei; c0 += kWave) {
     Nbr<T, ND> r;
     const bool tail = load_nbr(r, f, em_x + (size_t)c0 * xs, xs, em + (size_t)c0 * ND, lane, nnei - c0, last, sorted);
@@ -214,19 +243,13 @@ __device__ __forceinline__ void fwd_block(T (&acc)[KQ][ND], const T* __restrict_
     for (; q0 + U <= jmain; q0 += U) {
       T a[U][KQ][6];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int idx = bcast(r.idx, q0 + u);
-#pragma unroll
-        for (int kq = 0; kq < KQ; ++kq) load_coef<T, PK>(table, idx, min(lane + kq * kWave, M - 1), M, a[u][kq]);
-      }
+      for (int u = 0; u < U; ++u) rows.load(bcast(r.idx, q0 + u), a[u]);
 #pragma unroll
       for (int u = 0; u < U; ++u) fwd_neighbour<T, ND, KQ>(acc, a[u], r, q0 + u, (T)1);
     }
     for (int q = q0; q < jend; ++q) {
       T a[KQ][6];
-      const int idx = bcast(r.idx, q);
-#pragma unroll
-      for (int kq = 0; kq < KQ; ++kq) load_coef<T, PK>(table, idx, min(lane + kq * kWave, M - 1), M, a[kq]);
+      rows.load(bcast(r.idx, q), a);
       fwd_neighbour<T, ND, KQ>(acc, a, r, q, (tm && q == jend - 1) ? (T)(nnei - (c0 + q)) : (T)1);
     }
     if (tm) break;
