@@ -18,6 +18,10 @@
 //   * weights are pre-swizzled on the host into MFMA fragment order, so one
 //     B fragment (16 columns x 32 reductions) is one 1 KB coalesced read, and
 //     are prefetched kPf k-steps ahead through a register ring;
+//   * forward products with the weights split into FP16 hi + lo parts (the
+//     energy sees FP32-level weights), backward products with the hi parts
+//     (relative gradient error 2^-12, far inside the force tolerance, for
+//     half the backward weight traffic);
 //   * products on v_mfma_f32_16x16x32_f16 with FP32 accumulation; bias,
 //     tanh, timestep, residual and the energy dot product in FP32; dE/dD is
 //     written in FP32.
@@ -62,16 +66,20 @@ __device__ __forceinline__ half8 load_bfrag(const _Float16* __restrict__ w, int 
   return *(ghalf8*)(w + ((size_t)(kt * NT + ct) * 64 + lane) * 8);
 }
 
-// C[64 x (ntw tiles of 16 per wave)] += A[64 x K] (B hi + B lo) over
-// K = 32 KT, A from an LDS tile (row stride ld), B fragments from swizzled
-// matrices with NT column tiles; tile j of this wave is column tile ct0 + j
-// (used if < NT).  KT is a multiple of kPf.  Loads are unconditional
-// (clamped tile and step indices) so that the prefetch ring is not
-// serialised by the wait counters.
-template <int RT>
-__device__ __forceinline__ void gemm_lds_a(floatx4 (&acc)[RT][2], const _Float16* a_lds, int ld,
-                                           const _Float16* __restrict__ w, const _Float16* __restrict__ wl, int KT,
-                                           int NT, int ct0, int ntw, int lane) {
+// C[64 x (ntw tiles of 16 per wave)] += A[64 x K] (B hi [+ B lo when LO])
+// over K = 32 KT, A from an LDS tile (row stride ld), B fragments from
+// swizzled matrices with NT column tiles; tile j of this wave is column tile
+// ct0 + j (used if < NT).  KT is a multiple of kPf.  Loads are
+// unconditional (clamped tile and step indices) so that the prefetch ring is
+// not serialised by the wait counters.
+template <int RT, bool LO, int KTC>
+__device__ __forceinline__ void gemm_lds_a_kt(floatx4 (&acc)[RT][2], const _Float16* a_lds, int ld,
+                                              const _Float16* __restrict__ w, const _Float16* __restrict__ wl,
+                                              int KT_, int NT, int ct0, int ntw, int lane) {
+  // KTC > 0: the step count is a compile-time constant, the loop is unrolled
+  // completely (no loop-carried ring, so the wait counters stay exact) and
+  // the last kPf steps issue no refills
+  const int KT = KTC > 0 ? KTC : KT_;
   const int lr = lane & 15, lg = lane >> 4;
   const bool use0 = ntw > 0 && ct0 < NT, use1 = ntw > 1 && ct0 + 1 < NT;
   const int c0 = min(ct0, NT - 1), c1 = min(ct0 + 1, NT - 1);
@@ -80,35 +88,56 @@ __device__ __forceinline__ void gemm_lds_a(floatx4 (&acc)[RT][2], const _Float16
   for (int s = 0; s < kPf; ++s) {
     bq[s][0] = load_bfrag(w, s, NT, c0, lane);
     bq[s][1] = load_bfrag(w, s, NT, c1, lane);
-    bl[s][0] = load_bfrag(wl, s, NT, c0, lane);
-    bl[s][1] = load_bfrag(wl, s, NT, c1, lane);
+    if constexpr (LO) {
+      bl[s][0] = load_bfrag(wl, s, NT, c0, lane);
+      bl[s][1] = load_bfrag(wl, s, NT, c1, lane);
+    }
   }
-  for (int kt0 = 0; kt0 < KT; kt0 += kPf) {
+  auto step = [&](int k, int s) {
+    half8 a[RT];
 #pragma unroll
-    for (int s = 0; s < kPf; ++s) {
-      const int k = kt0 + s;
-      half8 a[RT];
+    for (int i = 0; i < RT; ++i) a[i] = *reinterpret_cast<const half8*>(a_lds + (i * 16 + lr) * ld + k * 32 + lg * 8);
+    if (use0)
 #pragma unroll
-      for (int i = 0; i < RT; ++i) a[i] = *reinterpret_cast<const half8*>(a_lds + (i * 16 + lr) * ld + k * 32 + lg * 8);
-      if (use0)
+      for (int i = 0; i < RT; ++i) {
+        acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][0], acc[i][0], 0, 0, 0);
+        if constexpr (LO) acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bl[s][0], acc[i][0], 0, 0, 0);
+      }
+    if (use1)
 #pragma unroll
-        for (int i = 0; i < RT; ++i) {
-          acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][0], acc[i][0], 0, 0, 0);
-          acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bl[s][0], acc[i][0], 0, 0, 0);
-        }
-      if (use1)
-#pragma unroll
-        for (int i = 0; i < RT; ++i) {
-          acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][1], acc[i][1], 0, 0, 0);
-          acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bl[s][1], acc[i][1], 0, 0, 0);
-        }
-      const int kn = min(k + kPf, KT - 1);
-      bq[s][0] = load_bfrag(w, kn, NT, c0, lane);
-      bq[s][1] = load_bfrag(w, kn, NT, c1, lane);
+      for (int i = 0; i < RT; ++i) {
+        acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][1], acc[i][1], 0, 0, 0);
+        if constexpr (LO) acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bl[s][1], acc[i][1], 0, 0, 0);
+      }
+    if (KTC > 0 && k + kPf >= KTC) return;  // folds away after unrolling
+    const int kn = min(k + kPf, KT - 1);
+    bq[s][0] = load_bfrag(w, kn, NT, c0, lane);
+    bq[s][1] = load_bfrag(w, kn, NT, c1, lane);
+    if constexpr (LO) {
       bl[s][0] = load_bfrag(wl, kn, NT, c0, lane);
       bl[s][1] = load_bfrag(wl, kn, NT, c1, lane);
     }
+  };
+  if constexpr (KTC > 0) {
+#pragma unroll
+    for (int kt0 = 0; kt0 < KTC; kt0 += kPf)
+#pragma unroll
+      for (int s = 0; s < kPf; ++s) step(kt0 + s, s);
+  } else {
+    for (int kt0 = 0; kt0 < KT; kt0 += kPf)
+#pragma unroll
+      for (int s = 0; s < kPf; ++s) step(kt0 + s, s);
   }
+}
+
+template <int RT, bool LO>
+__device__ __forceinline__ void gemm_lds_a(floatx4 (&acc)[RT][2], const _Float16* a_lds, int ld,
+                                           const _Float16* __restrict__ w, const _Float16* __restrict__ wl, int KT,
+                                           int NT, int ct0, int ntw, int lane) {
+  if (KT == 8)  // 256-wide hidden layers
+    gemm_lds_a_kt<RT, LO, 8>(acc, a_lds, ld, w, wl, KT, NT, ct0, ntw, lane);
+  else
+    gemm_lds_a_kt<RT, LO, 0>(acc, a_lds, ld, w, wl, KT, NT, ct0, ntw, lane);
 }
 
 template <int RT>
@@ -208,7 +237,7 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
         }
       }
     } else {
-      gemm_lds_a(acc, &xs[src][0][0], kXld, L.wf, L.wf_lo, KT, NT, ct0, ntw, lane);
+      gemm_lds_a<RT, true>(acc, &xs[src][0][0], kXld, L.wf, L.wf_lo, KT, NT, ct0, ntw, lane);
     }
     // epilogue: lane holds rows lg*4 + e of each 16x16 tile, column lr
 #pragma unroll
@@ -303,7 +332,7 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
       const int otw = (OT + kWaves - 1) / kWaves;
       floatx4 acc[RT][2];
       zero_acc(acc);
-      gemm_lds_a(acc, &xs[buf][0][0], kXld, L.wb, L.wb_lo, KT, OT, wave * otw, otw, lane);
+      gemm_lds_a<RT, false>(acc, &xs[buf][0][0], kXld, L.wb, L.wb, KT, OT, wave * otw, otw, lane);
       // g_l = gz W^T (+ g_{l+1} for residual layers: same positions)
 #pragma unroll
       for (int i = 0; i < RT; ++i)
@@ -319,7 +348,7 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
         const int ot0 = c * 2 * kWaves + wave * 2;
         floatx4 acc[RT][2];
         zero_acc(acc);
-        gemm_lds_a(acc, &xs[buf][0][0], kXld, L.wb, L.wb_lo, KT, OT, ot0, 2, lane);
+        gemm_lds_a<RT, false>(acc, &xs[buf][0][0], kXld, L.wb, L.wb, KT, OT, ot0, 2, lane);
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
           if (ot0 + j >= OT) continue;
