@@ -9,12 +9,12 @@
 // produced with counting sorts and prefix sums instead of the reference's
 // per-cell vectors:
 //   1. k_wrap_bin      one thread per local atom: wrap, bin, count per cell
-//   2. k_scan_cells    one workgroup per frame: exclusive scan of the counts
+//   2. k_scan_cells    one workgroup per frame: exclusive scan of the counts,
+//                      then the prefix over extended cells (all atoms, and
+//                      ghosts only) -> nall
 //   3. k_scatter_cells one thread per local atom: place into its cell
-//   4. k_sort_cells    one thread per cell: ascending index inside the cell
-//   5. k_scan_xcells   one workgroup per frame: prefix over extended cells
-//                      (all atoms, and ghosts only) -> nall
-//   6. k_emit_ghosts   one wave per extended cell: write ghost images and
+//   4. k_sort_cells    one wave per cell: ascending index inside the cell
+//   5. k_emit_ghosts   one wave per extended cell: write ghost images and
 //                      the extended cell -> atom index lists
 // Floating-point expressions that decide cell membership and image
 // coordinates are evaluated without contraction, in the reference's order.
@@ -81,26 +81,39 @@ __global__ void k_wrap_bin(const double* __restrict__ coord_in, const int* __res
   atomicAdd(&cell_cnt[g.cell_off + c], 1);
 }
 
-// Block-wide exclusive scan of n ints (one workgroup, blockDim = 1024).
+// Block-wide exclusive scan of n ints (one workgroup, blockDim = 1024): each
+// thread sums a contiguous chunk, the chunk sums are scanned within each
+// wave (shuffles) and across the 16 wave totals, then each thread writes
+// its chunk.  Returns the total.
 __device__ int block_exclusive_scan(const int* __restrict__ in, int* __restrict__ out, int n, int* sh) {
   const int t = threadIdx.x;
   const int nt = blockDim.x;
+  const int lane = t & (kWave - 1), w = t / kWave, nw = nt / kWave;
   const int chunk = (n + nt - 1) / nt;
   const int b = t * chunk;
   const int e = min(n, b + chunk);
   int s = 0;
   for (int k = b; k < e; ++k) s += in[k];
-  sh[t] = s;
-  __syncthreads();
-  // Hillis-Steele over thread totals
-  for (int off = 1; off < nt; off <<= 1) {
-    int v = (t >= off) ? sh[t - off] : 0;
-    __syncthreads();
-    sh[t] += v;
-    __syncthreads();
+  int inc = s;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const int v = __shfl_up(inc, off, kWave);
+    if (lane >= off) inc += v;
   }
-  int run = (t == 0) ? 0 : sh[t - 1];
-  const int total = sh[nt - 1];
+  if (lane == kWave - 1) sh[w] = inc;
+  __syncthreads();
+  if (w == 0) {
+    int tot = lane < nw ? sh[lane] : 0;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const int v = __shfl_up(tot, off, kWave);
+      if (lane >= off) tot += v;
+    }
+    if (lane < nw) sh[kWave + lane] = tot;  // inclusive prefix of the wave totals
+  }
+  __syncthreads();
+  int run = inc - s + (w > 0 ? sh[kWave + w - 1] : 0);
+  const int total = sh[kWave + nw - 1];
   for (int k = b; k < e; ++k) {
     const int v = in[k];
     out[k] = run;
@@ -108,15 +121,6 @@ __device__ int block_exclusive_scan(const int* __restrict__ in, int* __restrict_
   }
   __syncthreads();
   return total;
-}
-
-__global__ void __launch_bounds__(1024) k_scan_cells(const FrameGeom* __restrict__ geom, const int* __restrict__ cell_cnt,
-                                                     int* __restrict__ cell_start, int* __restrict__ cell_fill) {
-  __shared__ int sh[1024];
-  const FrameGeom& g = geom[blockIdx.x];
-  const int n = g.nc[0] * g.nc[1] * g.nc[2];
-  block_exclusive_scan(cell_cnt + g.cell_off, cell_start + g.cell_off, n, sh);
-  for (int k = threadIdx.x; k < n; k += blockDim.x) cell_fill[g.cell_off + k] = 0;
 }
 
 __global__ void k_scatter_cells(const int* __restrict__ frame_of, const FrameGeom* __restrict__ geom, int ntot,
@@ -183,13 +187,20 @@ __device__ __forceinline__ int src_cell(const FrameGeom& g, int e, int* cx, bool
   return (s[0] * g.nc[1] + s[1]) * g.nc[2] + s[2];
 }
 
-__global__ void __launch_bounds__(1024) k_scan_xcells(const FrameGeom* __restrict__ geom, const int* __restrict__ cell_cnt,
-                                                      int* __restrict__ xcnt, int* __restrict__ xstart,
-                                                      int* __restrict__ xgcnt, int* __restrict__ xgstart,
-                                                      int* __restrict__ nall_out, int* __restrict__ flag) {
-  __shared__ int sh[1024];
+// One workgroup per frame: exclusive scan of the cell counts (cell_start,
+// cell_fill cleared), then the per-extended-cell counts (all atoms, and
+// ghosts only) and their scans -> nall.
+__global__ void __launch_bounds__(1024) k_scan_cells(const FrameGeom* __restrict__ geom, const int* __restrict__ cell_cnt,
+                                                     int* __restrict__ cell_start, int* __restrict__ cell_fill,
+                                                     int* __restrict__ xcnt, int* __restrict__ xstart,
+                                                     int* __restrict__ xgcnt, int* __restrict__ xgstart,
+                                                     int* __restrict__ nall_out, int* __restrict__ flag) {
+  __shared__ int sh[2 * kWave];
   const int f = blockIdx.x;
   const FrameGeom& g = geom[f];
+  const int nc = g.nc[0] * g.nc[1] * g.nc[2];
+  block_exclusive_scan(cell_cnt + g.cell_off, cell_start + g.cell_off, nc, sh);
+  for (int k = threadIdx.x; k < nc; k += blockDim.x) cell_fill[g.cell_off + k] = 0;
   const int n = g.nx[0] * g.nx[1] * g.nx[2];
   const int off = g.xcell_off;
   for (int e = threadIdx.x; e < n; e += blockDim.x) {
@@ -261,7 +272,8 @@ void launch_build_extended(const CellWork& w, hipStream_t st) {
   hipLaunchKernelGGL(k_wrap_bin, dim3(ceil_div(w.ntot, tpb)), dim3(tpb), 0, st, w.coord_in, w.type_in, w.frame_of,
                      w.geom, w.ntot, w.coord_ext, w.type_ext, w.map_ext, w.cell_of, w.cell_cnt, w.do_wrap);
   DPMI_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_scan_cells, dim3(w.nframes), dim3(1024), 0, st, w.geom, w.cell_cnt, w.cell_start, w.cell_fill);
+  hipLaunchKernelGGL(k_scan_cells, dim3(w.nframes), dim3(1024), 0, st, w.geom, w.cell_cnt, w.cell_start, w.cell_fill,
+                     w.xcnt, w.xstart, w.xgcnt, w.xgstart, w.nall, w.flag);
   DPMI_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_scatter_cells, dim3(ceil_div(w.ntot, tpb)), dim3(tpb), 0, st, w.frame_of, w.geom, w.ntot,
                      w.cell_of, w.cell_start, w.cell_fill, w.sorted_loc);
@@ -269,9 +281,6 @@ void launch_build_extended(const CellWork& w, hipStream_t st) {
   hipLaunchKernelGGL(k_sort_cells, dim3(ceil_div((long long)w.ncells_tot * kWave, tpb)), dim3(tpb), 0, st,
                      w.cell_frame, w.geom,
                      w.ncells_tot, w.cell_cnt, w.cell_start, w.sorted_loc);
-  DPMI_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_scan_xcells, dim3(w.nframes), dim3(1024), 0, st, w.geom, w.cell_cnt, w.xcnt, w.xstart, w.xgcnt,
-                     w.xgstart, w.nall, w.flag);
   DPMI_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_emit_ghosts, dim3(ceil_div((long long)w.nx_tot * kWave, tpb)), dim3(tpb), 0, st,
                      w.xcell_frame, w.geom, w.nx_tot, w.cell_start, w.sorted_loc, w.xcnt, w.xstart, w.xgstart,

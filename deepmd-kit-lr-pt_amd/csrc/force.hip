@@ -388,11 +388,15 @@ __global__ void __launch_bounds__(256) k_chunk_reduce(const double* __restrict__
 #pragma unroll
   for (int k = 0; k < 10; ++k) sh[k][threadIdx.x] = acc[k].value();
   __syncthreads();
-  if (threadIdx.x < 10) {
-    KahanD t;
-    for (int r = 0; r < (int)blockDim.x; ++r) t.add(sh[threadIdx.x][r]);
-    part[(size_t)blockIdx.x * 10 + threadIdx.x] = t.value();
+  // pairwise tree over the 256 thread sums (rounding error grows with
+  // log2(256), not with the count; the order is fixed, so results repeat)
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h)
+#pragma unroll
+      for (int k = 0; k < 10; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + h];
+    __syncthreads();
   }
+  if (threadIdx.x < 10) part[(size_t)blockIdx.x * 10 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
 __global__ void k_frame_combine(const double* __restrict__ part, const int* __restrict__ chunk_start, int nframes,
