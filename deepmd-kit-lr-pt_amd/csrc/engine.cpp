@@ -416,8 +416,13 @@ SeaEngine::SeaEngine(const SeaModel* m, int nframes_, const int* natoms_h, const
     d_g1 = dalloc<double>((size_t)ntot * wmax);
   } else {
     d_em_f = dalloc<float>(nn * 4);
-    d_em_deriv_f = dalloc<float>(nn * 12);
-    d_rij_f = dalloc<float>(nn * 3);
+    // the force kernel rebuilds em_deriv and rij from the coordinates when
+    // it can (76 -> 16 bytes per slot written by the environment kernel)
+    recompute_slots = force_recompute_supported(m->nnei);
+    if (!recompute_slots) {
+      d_em_deriv_f = dalloc<float>(nn * 12);
+      d_rij_f = dalloc<float>(nn * 3);
+    }
     d_net_deriv_f = dalloc<float>(nn * 4);
     d_xyz_f = dalloc<float>((size_t)ntot * 4 * m->M);
     const int k0 = m->kpad[0];
@@ -879,9 +884,14 @@ void SeaEngine::compute(const double* coord, const double* box_h, double* energy
     fa.rij = d_rij;
   } else {
     fa.net_deriv = d_net_deriv_f;
-    fa.em_deriv = d_em_deriv_f;
+    fa.em_deriv = d_em_deriv_f;  // null: recomputed
     fa.rij = d_rij_f;
     fa.fp32 = true;
+    fa.coord_ext = d_coord_ext;
+    fa.type_ext = d_type_ext;
+    fa.dstd = m.d_dstd;
+    fa.rmin = (double)m.rcut_smth;
+    fa.rmax = (double)m.rcut;
   }
   fa.nlist = d_nlist;
   fa.row_frame = d_frame_of;

@@ -125,6 +125,7 @@ struct SeaEngine {
   double *d_g0 = nullptr, *d_g1 = nullptr;
   // mixed engine
   float *d_em_f = nullptr, *d_em_deriv_f = nullptr, *d_rij_f = nullptr, *d_net_deriv_f = nullptr;
+  bool recompute_slots = false;  // em_deriv / rij rebuilt by the force kernel (not stored)
   float *d_xyz_f = nullptr, *d_gD_f = nullptr, *d_eperm = nullptr, *d_ga = nullptr, *d_gb = nullptr;
   _Float16* d_Dh = nullptr;
   int4* d_tiles = nullptr;  // fused fitting tiles of 64 rows

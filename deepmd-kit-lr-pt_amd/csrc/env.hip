@@ -9,33 +9,19 @@
 // Consecutive threads own consecutive slots, so the 4-, 12- and 3-wide rows
 // are written as contiguous, coalesced spans.
 #include "dpmi_common.h"
+#include "env_core.h"
 #include "kernels.h"
 
 namespace dpmi {
 
 namespace {
 
-template <typename T>
-__device__ __forceinline__ void smooth_switch(T& v, T& dv, T r, T rmin, T rmax) {
-  // switcher.h:61-80
-  if (r < rmin) {
-    v = 1;
-    dv = 0;
-  } else if (r < rmax) {
-    const T u = (r - rmin) / (rmax - rmin);
-    const T du = (T)1 / (rmax - rmin);
-    v = u * u * u * ((T)-6 * u * u + (T)15 * u - (T)10) + (T)1;
-    dv = ((T)3 * u * u * ((T)-6 * u * u + (T)15 * u - (T)10) + u * u * u * ((T)-12 * u + (T)15)) * du;
-  } else {
-    v = 0;
-    dv = 0;
-  }
-}
-
 // TO: storage type of em / em_deriv / rij (double for the FP64 engine,
 // float for the mixed engine: the image displacement, the switch and every
 // intermediate are evaluated in FP64 and only the stored values are rounded).
-template <typename TO>
+// DERIV false: em only (the mixed engine's force kernel recomputes the
+// derivative from the coordinates).
+template <typename TO, bool DERIV>
 __global__ void k_env_mat_a(const double* __restrict__ coord, const int* __restrict__ type,
                             const int* __restrict__ nlist, const int* __restrict__ row_frame,
                             const int* __restrict__ row_ext_off, const int* __restrict__ row_center,
@@ -65,40 +51,7 @@ __global__ void k_env_mat_a(const double* __restrict__ coord, const int* __restr
   double rr[3] = {0., 0., 0.};
   if (ti >= 0 && j >= 0) {
     for (int d = 0; d < 3; ++d) rr[d] = coord[3 * (eo + j) + d] - coord[3 * (eo + i) + d];
-    const double nr2 = rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2];
-    const double inr = 1. / sqrt(nr2);
-    const double nr = nr2 * inr;
-    const double inr2 = inr * inr;
-    const double inr4 = inr2 * inr2;
-    const double inr3 = inr4 * nr;
-    double sw, dsw;
-    smooth_switch(sw, dsw, nr, rmin, rmax);
-    if constexpr (sizeof(TO) == 4) {
-      // stored in FP32: reciprocal products differ from the quotients by
-      // about one FP64 ulp, far below the FP32 rounding that follows
-      val[0] = inr;
-      val[1] = rr[0] * inr2;
-      val[2] = rr[1] * inr2;
-      val[3] = rr[2] * inr2;
-    } else {
-      val[0] = 1. / nr;
-      val[1] = rr[0] / nr2;
-      val[2] = rr[1] / nr2;
-      val[3] = rr[2] / nr2;
-    }
-#pragma unroll
-    for (int d = 0; d < 3; ++d) der[d] = rr[d] * inr3 * sw - val[0] * dsw * rr[d] * inr;
-#pragma unroll
-    for (int a = 1; a < 4; ++a) {
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        double t = 2. * rr[a - 1] * rr[d] * inr4;
-        if (d == a - 1) t -= inr2;
-        der[a * 3 + d] = t * sw - val[a] * dsw * rr[d] * inr;
-      }
-    }
-#pragma unroll
-    for (int a = 0; a < 4; ++a) val[a] *= sw;
+    envc::slot_terms<sizeof(TO) == 4, DERIV>(rr, rmin, rmax, val, der);
   }
   const size_t base = (size_t)g * nnei + s;
   if (ti >= 0) {
@@ -109,22 +62,26 @@ __global__ void k_env_mat_a(const double* __restrict__ coord, const int* __restr
       if constexpr (sizeof(TO) == 4) {
         const double isd = 1. / sd[a];
         em[base * 4 + a] = (TO)((val[a] - av[a]) * isd);
+        if constexpr (DERIV)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) em_deriv[base * 12 + a * 3 + d] = (TO)(der[a * 3 + d] * isd);
+          for (int d = 0; d < 3; ++d) em_deriv[base * 12 + a * 3 + d] = (TO)(der[a * 3 + d] * isd);
       } else {
         em[base * 4 + a] = (TO)((val[a] - av[a]) / sd[a]);
+        if constexpr (DERIV)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) em_deriv[base * 12 + a * 3 + d] = (TO)(der[a * 3 + d] / sd[a]);
+          for (int d = 0; d < 3; ++d) em_deriv[base * 12 + a * 3 + d] = (TO)(der[a * 3 + d] / sd[a]);
       }
     }
   } else {
 #pragma unroll
     for (int a = 0; a < 4; ++a) em[base * 4 + a] = (TO)0;
+    if constexpr (DERIV)
 #pragma unroll
-    for (int k = 0; k < 12; ++k) em_deriv[base * 12 + k] = (TO)0;
+      for (int k = 0; k < 12; ++k) em_deriv[base * 12 + k] = (TO)0;
   }
+  if constexpr (DERIV)
 #pragma unroll
-  for (int d = 0; d < 3; ++d) rij[base * 3 + d] = (TO)rr[d];
+    for (int d = 0; d < 3; ++d) rij[base * 3 + d] = (TO)rr[d];
 }
 
 }  // namespace
@@ -133,7 +90,7 @@ void launch_env_mat_a(const EnvArgs& a, hipStream_t st) {
   const long long n = a.nrows * (long long)a.nnei;
   if (n <= 0) return;
   const int tpb = a.tpb > 0 ? a.tpb : 256;
-  hipLaunchKernelGGL(k_env_mat_a<double>, dim3(ceil_div(n, tpb)), dim3(tpb), 0, st, a.coord, a.type, a.nlist, a.row_frame,
+  hipLaunchKernelGGL((k_env_mat_a<double, true>), dim3(ceil_div(n, tpb)), dim3(tpb), 0, st, a.coord, a.type, a.nlist, a.row_frame,
                      a.row_ext_off, a.row_center, a.strided_nloc, a.nrows, a.nnei, a.nall_stride, a.avg, a.std_, a.rmin, a.rmax,
                      a.em, a.em_deriv, a.rij);
   DPMI_LAUNCH_CHECK();
@@ -143,9 +100,14 @@ void launch_env_mat_a_f32(const EnvArgs& a, float* em, float* em_deriv, float* r
   const long long n = a.nrows * (long long)a.nnei;
   if (n <= 0) return;
   const int tpb = a.tpb > 0 ? a.tpb : 256;
-  hipLaunchKernelGGL(k_env_mat_a<float>, dim3(ceil_div(n, tpb)), dim3(tpb), 0, st, a.coord, a.type, a.nlist,
-                     a.row_frame, a.row_ext_off, a.row_center, a.strided_nloc, a.nrows, a.nnei, a.nall_stride, a.avg,
-                     a.std_, a.rmin, a.rmax, em, em_deriv, rij);
+  if (em_deriv)
+    hipLaunchKernelGGL((k_env_mat_a<float, true>), dim3(ceil_div(n, tpb)), dim3(tpb), 0, st, a.coord, a.type, a.nlist,
+                       a.row_frame, a.row_ext_off, a.row_center, a.strided_nloc, a.nrows, a.nnei, a.nall_stride, a.avg,
+                       a.std_, a.rmin, a.rmax, em, em_deriv, rij);
+  else
+    hipLaunchKernelGGL((k_env_mat_a<float, false>), dim3(ceil_div(n, tpb)), dim3(tpb), 0, st, a.coord, a.type,
+                       a.nlist, a.row_frame, a.row_ext_off, a.row_center, a.strided_nloc, a.nrows, a.nnei,
+                       a.nall_stride, a.avg, a.std_, a.rmin, a.rmax, em, em_deriv, rij);
   DPMI_LAUNCH_CHECK();
 }
 

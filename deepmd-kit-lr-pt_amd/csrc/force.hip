@@ -10,8 +10,10 @@
 // atomics, per-row sums use a wave butterfly, and totals are reduced per row
 // into a partial buffer that a second kernel sums with compensation.
 #include <algorithm>
+#include <stdexcept>
 
 #include "dpmi_common.h"
+#include "env_core.h"
 #include "kernels.h"
 
 namespace dpmi {
@@ -265,12 +267,36 @@ __device__ __forceinline__ int hash_slot(int* keys, int key) {
   }
 }
 
-template <typename TI>
+// Environment-matrix derivative and displacement of slot s of a centre of
+// type ti at ci, rebuilt from the coordinates exactly as the FP32-storing
+// environment kernel rounds them (FP64 evaluation, then FP32).
+__device__ __forceinline__ void recompute_slot(const double (&ci)[3], const double* __restrict__ cj,
+                                               const double* __restrict__ sd, double rmin, double rmax, float* e12,
+                                               float* r3) {
+  double rr[3], val[4], der[12];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) rr[d] = cj[d] - ci[d];
+  envc::slot_terms<true, true>(rr, rmin, rmax, val, der);
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const double isd = 1. / sd[a];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) e12[a * 3 + d] = (float)(der[a * 3 + d] * isd);
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) r3[d] = (float)rr[d];
+}
+
+// RECOMP (mixed engine): ed and rij are not stored; each slot's derivative
+// and displacement are rebuilt from coord_ext and the centre type's dstd.
+template <typename TI, bool RECOMP>
 __global__ void __launch_bounds__(1024) k_force_virial_hash(
     const TI* __restrict__ nd, const TI* __restrict__ ed, const TI* __restrict__ rij, const int* __restrict__ nlist,
     const int* __restrict__ row_frame, const int* __restrict__ row_center, const int* __restrict__ frame_ext_off,
     const FrameGeom* __restrict__ geom, const int* __restrict__ sorted_loc, long long nrows, int nnei, int R,
-    double* __restrict__ force_ext, double* __restrict__ vpart) {
+    const double* __restrict__ coord_ext, const int* __restrict__ type_ext, const double* __restrict__ dstd,
+    double rmin, double rmax, double* __restrict__ force_ext, double* __restrict__ vpart) {
+  static_assert(!RECOMP || sizeof(TI) == 4, "recomputed slots are FP32");
   __shared__ int hkey[kHashSize];
   __shared__ double hval[kHashSize][3];
   for (int h = threadIdx.x; h < kHashSize; h += blockDim.x) {
@@ -285,32 +311,51 @@ __global__ void __launch_bounds__(1024) k_force_virial_hash(
     const long long g = geom[f].loc_off + sorted_loc[p];  // packed row of the p-th atom in cell order
     const int eo = frame_ext_off[f];
     const int i = row_center[g];
+    double ci[3] = {0., 0., 0.};
+    const double* sdrow = dstd;
+    bool live = true;
+    if constexpr (RECOMP) {
+      const int ti = type_ext[eo + i];
+      live = ti >= 0;  // rows of negative type carry no forces
+      sdrow = dstd + (size_t)max(ti, 0) * nnei * 4;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) ci[d] = coord_ext[(size_t)(eo + i) * 3 + d];
+    }
     double fi[3] = {0., 0., 0.};
     double vs[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) vs[k] = 0.;
     // SU slots per lane are loaded before any is used (out-of-range slots
     // read slot 0 and are dropped), so their loads overlap
-    constexpr int SU = sizeof(TI) == 8 ? 1 : 3;
+    constexpr int SU = RECOMP ? 2 : (sizeof(TI) == 8 ? 1 : 3);
     for (int s0 = 0; s0 < nnei; s0 += SU * kWave) {
       TI n4[SU][4], e12[SU][12], r3[SU][3];
-      int jj[SU];
+      double cj[SU][3];
+      int jj[SU], ss[SU];
 #pragma unroll
       for (int u = 0; u < SU; ++u) {
         const int s = s0 + u * kWave + lane;
-        const bool ok = s < nnei;
-        const size_t b = (size_t)g * nnei + (ok ? s : 0);
+        const bool ok = s < nnei && live;
+        ss[u] = ok ? s : 0;
+        const size_t b = (size_t)g * nnei + ss[u];
         jj[u] = ok ? nlist[b] : -1;
 #pragma unroll
         for (int a = 0; a < 4; ++a) n4[u][a] = nd[b * 4 + a];
+        if constexpr (RECOMP) {
+          const size_t e = (size_t)eo + (jj[u] >= 0 ? jj[u] : i);
 #pragma unroll
-        for (int k = 0; k < 12; ++k) e12[u][k] = ed[b * 12 + k];
+          for (int d = 0; d < 3; ++d) cj[u][d] = coord_ext[e * 3 + d];
+        } else {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) r3[u][d] = rij[b * 3 + d];
+          for (int k = 0; k < 12; ++k) e12[u][k] = ed[b * 12 + k];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) r3[u][d] = rij[b * 3 + d];
+        }
       }
 #pragma unroll
       for (int u = 0; u < SU; ++u) {
         if (jj[u] < 0) continue;  // padding slots have zero em_deriv
+        if constexpr (RECOMP) recompute_slot(ci, cj[u], sdrow + ss[u] * 4, rmin, rmax, e12[u], r3[u]);
         TI fs[3], v[9];
         slot_force(n4[u], e12[u], fs);
         slot_virial(n4[u], e12[u], r3[u], v);
@@ -471,19 +516,29 @@ void launch_prod_virial_grad_a(T* grad_net, const T* grad, const T* env_deriv, c
 DPMI_FORCE_INST(double)
 DPMI_FORCE_INST(float)
 
+bool force_recompute_supported(int nnei) { return nnei + 1 <= kHashSize; }
+
 void launch_force_virial_rows(const ForceRowsArgs& a, hipStream_t st) {
   if (a.nrows <= 0) return;
+  if (!a.em_deriv && !(a.fp32 && a.sorted_loc && a.geom && a.nnei + 1 <= kHashSize))
+    throw std::invalid_argument("recomputed slot derivatives need the FP32 LDS-aggregated force kernel");
   if (a.sorted_loc && a.geom && a.nnei + 1 <= kHashSize) {
     const int R = std::max(1, std::min(a.rows > 0 ? a.rows : 16, kHashSize / (a.nnei + 1)));
     const dim3 grid(ceil_div(a.nrows, R)), block(R * kWave);
-    if (a.fp32)
-      hipLaunchKernelGGL(k_force_virial_hash<float>, grid, block, 0, st, (const float*)a.net_deriv,
+    if (a.fp32 && !a.em_deriv)
+      hipLaunchKernelGGL((k_force_virial_hash<float, true>), grid, block, 0, st, (const float*)a.net_deriv, nullptr,
+                         nullptr, a.nlist, a.row_frame, a.row_center, a.frame_ext_off, a.geom, a.sorted_loc, a.nrows,
+                         a.nnei, R, a.coord_ext, a.type_ext, a.dstd, a.rmin, a.rmax, a.force_ext, a.vpart);
+    else if (a.fp32)
+      hipLaunchKernelGGL((k_force_virial_hash<float, false>), grid, block, 0, st, (const float*)a.net_deriv,
                          (const float*)a.em_deriv, (const float*)a.rij, a.nlist, a.row_frame, a.row_center,
-                         a.frame_ext_off, a.geom, a.sorted_loc, a.nrows, a.nnei, R, a.force_ext, a.vpart);
+                         a.frame_ext_off, a.geom, a.sorted_loc, a.nrows, a.nnei, R, nullptr, nullptr, nullptr, 0., 0.,
+                         a.force_ext, a.vpart);
     else
-      hipLaunchKernelGGL(k_force_virial_hash<double>, grid, block, 0, st, (const double*)a.net_deriv,
+      hipLaunchKernelGGL((k_force_virial_hash<double, false>), grid, block, 0, st, (const double*)a.net_deriv,
                          (const double*)a.em_deriv, (const double*)a.rij, a.nlist, a.row_frame, a.row_center,
-                         a.frame_ext_off, a.geom, a.sorted_loc, a.nrows, a.nnei, R, a.force_ext, a.vpart);
+                         a.frame_ext_off, a.geom, a.sorted_loc, a.nrows, a.nnei, R, nullptr, nullptr, nullptr, 0., 0.,
+                         a.force_ext, a.vpart);
     DPMI_LAUNCH_CHECK();
     return;
   }

@@ -88,7 +88,8 @@ struct EnvArgs {
   int tpb = 0;  // threads per block (0: 256)
 };
 void launch_env_mat_a(const EnvArgs& a, hipStream_t st);
-// same, storing em / em_deriv / rij in FP32 (em, em_deriv, rij of a are ignored)
+// same, storing em / em_deriv / rij in FP32 (em, em_deriv, rij of a are
+// ignored); em_deriv null: em only
 void launch_env_mat_a_f32(const EnvArgs& a, float* em, float* em_deriv, float* rij, hipStream_t st);
 
 // ---------------------------------------------------------- tabulate.hip --
@@ -187,9 +188,15 @@ void launch_prod_virial_grad_a(T* grad_net, const T* grad, const T* env_deriv, c
 // engine: force/virial over packed rows into extended force buffers
 struct ForceRowsArgs {
   const void* net_deriv = nullptr;  // [nrows, nnei*4]  (double, or float when fp32)
-  const void* em_deriv = nullptr;   // [nrows, nnei*12]
+  const void* em_deriv = nullptr;   // [nrows, nnei*12]; null (fp32 only): recomputed from coord_ext
   const void* rij = nullptr;        // [nrows, nnei*3]
   bool fp32 = false;
+  // for recomputed slots: extended coordinates and types, dstd [ntypes, nnei, 4], rcut_smth, rcut
+  const double* coord_ext = nullptr;
+  const int* type_ext = nullptr;
+  const double* dstd = nullptr;
+  double rmin = 0, rmax = 0;
+  // (the LDS-aggregated kernel's rows must then be given)
   const int* nlist = nullptr;
   const int* row_frame = nullptr;
   const int* row_center = nullptr;
@@ -204,6 +211,9 @@ struct ForceRowsArgs {
   const FrameGeom* geom = nullptr;
   int rows = 0;  // rows per workgroup of the LDS-aggregated kernel (0: 16)
 };
+// whether launch_force_virial_rows can rebuild the slot derivatives of rows
+// with nnei slots (the LDS-aggregated FP32 kernel applies)
+bool force_recompute_supported(int nnei);
 
 // Launch-shape knobs of the engine stages, picked per size bucket by
 // SeaEngine::autotune.
