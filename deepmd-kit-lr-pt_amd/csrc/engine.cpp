@@ -398,8 +398,13 @@ SeaEngine::SeaEngine(const SeaModel* m, int nframes_, const int* natoms_h, const
   const int D = m->M * m->A;
   if (precision == DPMI_FP64) {
     d_em = dalloc<double>(nn * 4);
-    d_em_deriv = dalloc<double>(nn * 12);
-    d_rij = dalloc<double>(nn * 3);
+    // the force kernel rebuilds em_deriv and rij from the coordinates when
+    // it can (the environment kernel then writes em only)
+    recompute_slots = force_recompute_supported(m->nnei);
+    if (!recompute_slots) {
+      d_em_deriv = dalloc<double>(nn * 12);
+      d_rij = dalloc<double>(nn * 3);
+    }
     d_net_deriv = dalloc<double>(nn * 4);
     d_xyz = dalloc<double>((size_t)ntot * 4 * m->M);
     d_D = dalloc<double>((size_t)ntot * D);
@@ -416,8 +421,7 @@ SeaEngine::SeaEngine(const SeaModel* m, int nframes_, const int* natoms_h, const
     d_g1 = dalloc<double>((size_t)ntot * wmax);
   } else {
     d_em_f = dalloc<float>(nn * 4);
-    // the force kernel rebuilds em_deriv and rij from the coordinates when
-    // it can (76 -> 16 bytes per slot written by the environment kernel)
+    // as above (76 -> 16 bytes per slot written by the environment kernel)
     recompute_slots = force_recompute_supported(m->nnei);
     if (!recompute_slots) {
       d_em_deriv_f = dalloc<float>(nn * 12);
@@ -880,19 +884,19 @@ void SeaEngine::compute(const double* coord, const double* box_h, double* energy
   ForceRowsArgs fa;
   if (precision == DPMI_FP64) {
     fa.net_deriv = d_net_deriv;
-    fa.em_deriv = d_em_deriv;
+    fa.em_deriv = d_em_deriv;  // null: recomputed
     fa.rij = d_rij;
   } else {
     fa.net_deriv = d_net_deriv_f;
     fa.em_deriv = d_em_deriv_f;  // null: recomputed
     fa.rij = d_rij_f;
     fa.fp32 = true;
-    fa.coord_ext = d_coord_ext;
-    fa.type_ext = d_type_ext;
-    fa.dstd = m.d_dstd;
-    fa.rmin = (double)m.rcut_smth;
-    fa.rmax = (double)m.rcut;
   }
+  fa.coord_ext = d_coord_ext;
+  fa.type_ext = d_type_ext;
+  fa.dstd = m.d_dstd;
+  fa.rmin = (double)m.rcut_smth;
+  fa.rmax = (double)m.rcut;
   fa.nlist = d_nlist;
   fa.row_frame = d_frame_of;
   fa.row_center = d_center;

@@ -119,13 +119,13 @@ struct SeaEngine {
   int *d_type_ext = nullptr, *d_map_ext = nullptr, *d_cell_atoms = nullptr, *d_xframe_of_ext = nullptr;
   double *d_coord_ext = nullptr, *d_force_ext = nullptr;
   double *d_em = nullptr, *d_em_deriv = nullptr, *d_rij = nullptr, *d_net_deriv = nullptr;
+  bool recompute_slots = false;  // em_deriv / rij rebuilt by the force kernel (not stored)
   double *d_xyz = nullptr, *d_D = nullptr, *d_gD = nullptr, *d_atom_e = nullptr,
          *d_vpart = nullptr;
   std::vector<double*> d_fz, d_fy, d_fx, d_outw;
   double *d_g0 = nullptr, *d_g1 = nullptr;
   // mixed engine
   float *d_em_f = nullptr, *d_em_deriv_f = nullptr, *d_rij_f = nullptr, *d_net_deriv_f = nullptr;
-  bool recompute_slots = false;  // em_deriv / rij rebuilt by the force kernel (not stored)
   float *d_xyz_f = nullptr, *d_gD_f = nullptr, *d_eperm = nullptr, *d_ga = nullptr, *d_gb = nullptr;
   _Float16* d_Dh = nullptr;
   int4* d_tiles = nullptr;  // fused fitting tiles of 64 rows

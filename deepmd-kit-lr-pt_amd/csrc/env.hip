@@ -90,9 +90,14 @@ void launch_env_mat_a(const EnvArgs& a, hipStream_t st) {
   const long long n = a.nrows * (long long)a.nnei;
   if (n <= 0) return;
   const int tpb = a.tpb > 0 ? a.tpb : 256;
-  hipLaunchKernelGGL((k_env_mat_a<double, true>), dim3(ceil_div(n, tpb)), dim3(tpb), 0, st, a.coord, a.type, a.nlist, a.row_frame,
-                     a.row_ext_off, a.row_center, a.strided_nloc, a.nrows, a.nnei, a.nall_stride, a.avg, a.std_, a.rmin, a.rmax,
-                     a.em, a.em_deriv, a.rij);
+  if (a.em_deriv)
+    hipLaunchKernelGGL((k_env_mat_a<double, true>), dim3(ceil_div(n, tpb)), dim3(tpb), 0, st, a.coord, a.type, a.nlist, a.row_frame,
+                       a.row_ext_off, a.row_center, a.strided_nloc, a.nrows, a.nnei, a.nall_stride, a.avg, a.std_, a.rmin, a.rmax,
+                       a.em, a.em_deriv, a.rij);
+  else
+    hipLaunchKernelGGL((k_env_mat_a<double, false>), dim3(ceil_div(n, tpb)), dim3(tpb), 0, st, a.coord, a.type, a.nlist, a.row_frame,
+                       a.row_ext_off, a.row_center, a.strided_nloc, a.nrows, a.nnei, a.nall_stride, a.avg, a.std_, a.rmin, a.rmax,
+                       a.em, a.em_deriv, a.rij);
   DPMI_LAUNCH_CHECK();
 }
 

@@ -267,28 +267,34 @@ __device__ __forceinline__ int hash_slot(int* keys, int key) {
   }
 }
 
-// Environment-matrix derivative and displacement of slot s of a centre of
-// type ti at ci, rebuilt from the coordinates exactly as the FP32-storing
-// environment kernel rounds them (FP64 evaluation, then FP32).
+// Environment-matrix derivative and displacement of a slot of a centre at
+// ci, rebuilt from the coordinates exactly as the environment kernel stores
+// them (FP64 evaluation; TI float: rounded as the FP32-storing kernel does).
+template <typename TI>
 __device__ __forceinline__ void recompute_slot(const double (&ci)[3], const double* __restrict__ cj,
-                                               const double* __restrict__ sd, double rmin, double rmax, float* e12,
-                                               float* r3) {
+                                               const double* __restrict__ sd, double rmin, double rmax, TI* e12,
+                                               TI* r3) {
   double rr[3], val[4], der[12];
 #pragma unroll
   for (int d = 0; d < 3; ++d) rr[d] = cj[d] - ci[d];
-  envc::slot_terms<true, true>(rr, rmin, rmax, val, der);
+  envc::slot_terms<sizeof(TI) == 4, true>(rr, rmin, rmax, val, der);
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
-    const double isd = 1. / sd[a];
+    if constexpr (sizeof(TI) == 4) {
+      const double isd = 1. / sd[a];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) e12[a * 3 + d] = (float)(der[a * 3 + d] * isd);
+      for (int d = 0; d < 3; ++d) e12[a * 3 + d] = (TI)(der[a * 3 + d] * isd);
+    } else {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) e12[a * 3 + d] = der[a * 3 + d] / sd[a];
+    }
   }
 #pragma unroll
-  for (int d = 0; d < 3; ++d) r3[d] = (float)rr[d];
+  for (int d = 0; d < 3; ++d) r3[d] = (TI)rr[d];
 }
 
-// RECOMP (mixed engine): ed and rij are not stored; each slot's derivative
-// and displacement are rebuilt from coord_ext and the centre type's dstd.
+// RECOMP: ed and rij are not stored; each slot's derivative and displacement
+// are rebuilt from coord_ext and the centre type's dstd.
 template <typename TI, bool RECOMP>
 __global__ void __launch_bounds__(1024) k_force_virial_hash(
     const TI* __restrict__ nd, const TI* __restrict__ ed, const TI* __restrict__ rij, const int* __restrict__ nlist,
@@ -296,7 +302,6 @@ __global__ void __launch_bounds__(1024) k_force_virial_hash(
     const FrameGeom* __restrict__ geom, const int* __restrict__ sorted_loc, long long nrows, int nnei, int R,
     const double* __restrict__ coord_ext, const int* __restrict__ type_ext, const double* __restrict__ dstd,
     double rmin, double rmax, double* __restrict__ force_ext, double* __restrict__ vpart) {
-  static_assert(!RECOMP || sizeof(TI) == 4, "recomputed slots are FP32");
   __shared__ int hkey[kHashSize];
   __shared__ double hval[kHashSize][3];
   for (int h = threadIdx.x; h < kHashSize; h += blockDim.x) {
@@ -327,7 +332,7 @@ __global__ void __launch_bounds__(1024) k_force_virial_hash(
     for (int k = 0; k < 9; ++k) vs[k] = 0.;
     // SU slots per lane are loaded before any is used (out-of-range slots
     // read slot 0 and are dropped), so their loads overlap
-    constexpr int SU = RECOMP ? 2 : (sizeof(TI) == 8 ? 1 : 3);
+    constexpr int SU = sizeof(TI) == 8 ? 1 : (RECOMP ? 2 : 3);
     for (int s0 = 0; s0 < nnei; s0 += SU * kWave) {
       TI n4[SU][4], e12[SU][12], r3[SU][3];
       double cj[SU][3];
@@ -520,8 +525,8 @@ bool force_recompute_supported(int nnei) { return nnei + 1 <= kHashSize; }
 
 void launch_force_virial_rows(const ForceRowsArgs& a, hipStream_t st) {
   if (a.nrows <= 0) return;
-  if (!a.em_deriv && !(a.fp32 && a.sorted_loc && a.geom && a.nnei + 1 <= kHashSize))
-    throw std::invalid_argument("recomputed slot derivatives need the FP32 LDS-aggregated force kernel");
+  if (!a.em_deriv && !(a.sorted_loc && a.geom && a.nnei + 1 <= kHashSize))
+    throw std::invalid_argument("recomputed slot derivatives need the LDS-aggregated force kernel");
   if (a.sorted_loc && a.geom && a.nnei + 1 <= kHashSize) {
     const int R = std::max(1, std::min(a.rows > 0 ? a.rows : 16, kHashSize / (a.nnei + 1)));
     const dim3 grid(ceil_div(a.nrows, R)), block(R * kWave);
@@ -529,6 +534,10 @@ void launch_force_virial_rows(const ForceRowsArgs& a, hipStream_t st) {
       hipLaunchKernelGGL((k_force_virial_hash<float, true>), grid, block, 0, st, (const float*)a.net_deriv, nullptr,
                          nullptr, a.nlist, a.row_frame, a.row_center, a.frame_ext_off, a.geom, a.sorted_loc, a.nrows,
                          a.nnei, R, a.coord_ext, a.type_ext, a.dstd, a.rmin, a.rmax, a.force_ext, a.vpart);
+    else if (!a.em_deriv)
+      hipLaunchKernelGGL((k_force_virial_hash<double, true>), grid, block, 0, st, (const double*)a.net_deriv,
+                         nullptr, nullptr, a.nlist, a.row_frame, a.row_center, a.frame_ext_off, a.geom, a.sorted_loc,
+                         a.nrows, a.nnei, R, a.coord_ext, a.type_ext, a.dstd, a.rmin, a.rmax, a.force_ext, a.vpart);
     else if (a.fp32)
       hipLaunchKernelGGL((k_force_virial_hash<float, false>), grid, block, 0, st, (const float*)a.net_deriv,
                          (const float*)a.em_deriv, (const float*)a.rij, a.nlist, a.row_frame, a.row_center,

@@ -188,7 +188,7 @@ void launch_prod_virial_grad_a(T* grad_net, const T* grad, const T* env_deriv, c
 // engine: force/virial over packed rows into extended force buffers
 struct ForceRowsArgs {
   const void* net_deriv = nullptr;  // [nrows, nnei*4]  (double, or float when fp32)
-  const void* em_deriv = nullptr;   // [nrows, nnei*12]; null (fp32 only): recomputed from coord_ext
+  const void* em_deriv = nullptr;   // [nrows, nnei*12]; null: recomputed from coord_ext
   const void* rij = nullptr;        // [nrows, nnei*3]
   bool fp32 = false;
   // for recomputed slots: extended coordinates and types, dstd [ntypes, nnei, 4], rcut_smth, rcut
