@@ -380,6 +380,163 @@ __global__ void __launch_bounds__(256) k_embed_bwd(T* __restrict__ net_deriv, co
 }
 
 // ---------------------------------------------------------------------------
+// Adjoint of the mixed engine in groups of 16 lanes.
+//
+// The lanes-over-columns layout above covers M columns with 64 KQ lane slots
+// (M = 100: 128 slots, 22% of every operation idle).  Here a wave splits
+// into 4 groups of 16 lanes: group g takes the neighbours q = g (mod 4) of a
+// block and lane l of a group holds the C = ceil(M / 16) columns
+// k = l + 16 c (M = 100: 112 slots).  A neighbour's record moves from the lane
+// that located it with shuffles; a group reads its table row as 16-lane,
+// 256-byte spans of packed records (columns k >= M through out-of-range
+// buffer offsets, which read zeros), and each neighbour's 4 sums are reduced
+// over its 16 lanes with a transposing butterfly (5 exchanges).  The adjoint
+// is bound by arithmetic, so the idle slots it saves are time saved; the
+// forward, bound by the table reads (whose row index here would first
+// travel through a shuffle), keeps the lanes-over-columns layout.
+
+// Packed records of the lane's C columns of any table row (per-lane rows).
+template <int C>
+struct GRows {
+  __amdgpu_buffer_rsrc_t rs;
+  // byte offset of column k in a row; k >= M: kDeadOffset, so that row +
+  // offset (row < 2^31) stays at or past 2^31, beyond the record count
+  int M, col[C];
+  __device__ __forceinline__ GRows(const float* t, int M_, int gl) : M(M_) {
+    rs = pk_rsrc(t);
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int k = gl + 16 * c;
+      col[c] = k < M ? k * 16 : kDeadOffset;
+    }
+  }
+  __device__ __forceinline__ void load(int idx, float (&a)[C][6]) const {
+    const unsigned row = (unsigned)(idx * pk_stride(M) * 16);
+#pragma unroll
+    for (int c = 0; c < C; ++c) load_coef_pk(rs, 0, (int)(row + (unsigned)col[c]), a[c]);
+  }
+};
+
+// Sum over the 16 lanes of a group of the 4 values p[0..3]: lane l of the
+// group ends with the total of value l >> 2 (exchanges across lane bits 3
+// and 2 keep half of the values each, then bits 1 and 0 add).
+__device__ __forceinline__ float group_sum4(const float (&p)[4], int gl) {
+  const bool h3 = gl & 8, h2 = gl & 4;
+  const float k0 = h3 ? p[2] : p[0], k1 = h3 ? p[3] : p[1];
+  const float s0 = h3 ? p[0] : p[2], s1 = h3 ? p[1] : p[3];
+  const float t0 = k0 + __shfl_xor(s0, 8, kWave), t1 = k1 + __shfl_xor(s1, 8, kWave);
+  float u = (h2 ? t1 : t0) + __shfl_xor(h2 ? t0 : t1, 4, kWave);
+  u += dpp_partner<1>(u);
+  u += dpp_partner<0>(u);
+  return u;
+}
+
+template <int C>
+__device__ void bwd_block_g(float* __restrict__ nd, const float (&dy)[C][4], const float* __restrict__ table,
+                            const Info<float>& f, const float* __restrict__ em, int nb, int M, int grp, int gl) {
+  if (nb <= 0) return;
+  const int lane = lane_id();
+  const float last = em[(size_t)(nb - 1) * 4];
+  int jt = nb;  // first neighbour of the sorted zero-padding tail (tabulate.cc:188-201)
+  for (int c0 = 0; c0 < nb; c0 += kWave) {
+    const int j = c0 + lane;
+    bool tail = false;
+    if (j < nb) {
+      const float* r = em + (size_t)j * 4;
+      tail = r[0] == last && r[1] == 0.f && r[2] == 0.f && r[3] == 0.f;
+    }
+    const unsigned long long m = __ballot(tail);
+    if (m) {
+      jt = c0 + __ffsll((long long)m) - 1;
+      break;
+    }
+  }
+  const int jend = jt < nb ? jt + 1 : nb;
+  const GRows<C> rows(table, M, gl);
+  for (int c0 = 0; c0 < jend; c0 += kWave) {
+    Nbr<float, 4> r;
+    load_nbr(r, f, em + (size_t)c0 * 4, 4, em + (size_t)c0 * 4, lane, jend - c0, last, false);
+    const int cnt = min(kWave, jend - c0);
+    for (int q0 = 0; q0 < cnt; q0 += 8) {
+      float a[2][C][6], x[2], delta[2], ll[2][4];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int qc = min(q0 + 4 * u + grp, cnt - 1);
+        x[u] = __shfl(r.x, qc, kWave);
+        delta[u] = __shfl(r.delta, qc, kWave);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) ll[u][m] = __shfl(r.ll[m], qc, kWave);
+        rows.load(__shfl(r.idx, qc, kWave), a[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        // columns k >= M: zero coefficients times dy = 0
+        float p[4];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          float v, d;
+          quintic(a[u][c], x[u], delta[u], v, d);
+          const float proj = ll[u][0] * dy[c][0] + ll[u][1] * dy[c][1] + ll[u][2] * dy[c][2] + ll[u][3] * dy[c][3];
+          if (c == 0) {
+            p[0] = v * dy[c][0] + d * proj;
+#pragma unroll
+            for (int m = 1; m < 4; ++m) p[m] = v * dy[c][m];
+          } else {
+            p[0] += v * dy[c][0] + d * proj;
+#pragma unroll
+            for (int m = 1; m < 4; ++m) p[m] += v * dy[c][m];
+          }
+        }
+        const float tot = group_sum4(p, gl);
+        const int q = q0 + 4 * u + grp, j = c0 + q;
+        if (q < cnt && (gl & 3) == 0) {
+          const float scale = (j == jt) ? (float)(nb - jt) : 1.f;
+          nd[(size_t)j * 4 + (gl >> 2)] = tot * scale;
+        }
+      }
+    }
+  }
+  for (int j = jend + lane; j < nb; j += kWave)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) nd[(size_t)j * 4 + c] = 0.f;
+}
+
+template <int C>
+__global__ void __launch_bounds__(256) k_embed_bwd_g(float* __restrict__ net_deriv, const float* __restrict__ xyz,
+                                                     const float* __restrict__ gD, const int* __restrict__ pos,
+                                                     const float* __restrict__ tables,
+                                                     const long long* __restrict__ tab_off,
+                                                     const Info<float>* __restrict__ infos,
+                                                     const int* __restrict__ net_of, const float* __restrict__ em,
+                                                     const int* __restrict__ row_type, const int* __restrict__ sec,
+                                                     int ntypes, long long nrows, int nnei, int M, int A, int ldD,
+                                                     float inv_nnei) {
+  constexpr int KQO = (C + 3) / 4;  // lanes-over-columns groups of dy
+  const long long g = wave_index();
+  const int lane = lane_id(), grp = lane >> 4, gl = lane & 15;
+  if (g >= nrows) return;
+  const int ti = row_type[g];
+  if (ti < 0) {
+    for (int s = lane; s < nnei * 4; s += kWave) net_deriv[(size_t)g * nnei * 4 + s] = 0.f;
+    return;
+  }
+  float dyo[KQO][4];
+  bwd_dy<float, float, KQO>(dyo, xyz + (size_t)g * 4 * M, gD + (size_t)pos[g] * ldD, M, A, inv_nnei);
+  // column k = gl + 16 c sits on lane k % 64 of group k / 64
+  float dy[C][4];
+#pragma unroll
+  for (int c = 0; c < C; ++c)
+#pragma unroll
+    for (int m = 0; m < 4; ++m) dy[c][m] = __shfl(dyo[c / 4][m], gl + 16 * (c % 4), kWave);
+  for (int tj = 0; tj < ntypes; ++tj) {
+    const int net = net_of[ti * ntypes + tj];
+    const int b = sec[tj], nb = sec[tj + 1] - sec[tj];
+    bwd_block_g<C>(net_deriv + ((size_t)g * nnei + b) * 4, dy, tables + tab_off[net], infos[net],
+                   em + ((size_t)g * nnei + b) * 4, nb, M, grp, gl);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Matrix-core kernels of the mixed engine: the table embedding in table space.
 //
 // With t' = t / s and delta' = delta / s (s the stride of the row's table
@@ -928,6 +1085,23 @@ void free_table_infos(void* p, bool fp32) {
     delete[] static_cast<Info<double>*>(p);
 }
 
+// Packed FP32 tables, A <= 16, M <= 128: the group-of-16 adjoint (C columns
+// per lane).
+bool embed_groups_apply(const EmbedArgs& a) { return a.packed && a.A <= 16 && a.M <= 128 && a.M > 0; }
+#define DPMI_C_DISPATCH(M, ...)                                          \
+  do {                                                                   \
+    switch (((M) + 15) / 16) {                                           \
+      case 1: { constexpr int C = 1; __VA_ARGS__; } break;               \
+      case 2: { constexpr int C = 2; __VA_ARGS__; } break;               \
+      case 3: { constexpr int C = 3; __VA_ARGS__; } break;               \
+      case 4: { constexpr int C = 4; __VA_ARGS__; } break;               \
+      case 5: { constexpr int C = 5; __VA_ARGS__; } break;               \
+      case 6: { constexpr int C = 6; __VA_ARGS__; } break;               \
+      case 7: { constexpr int C = 7; __VA_ARGS__; } break;               \
+      default: { constexpr int C = 8; __VA_ARGS__; } break;              \
+    }                                                                    \
+  } while (0)
+
 template <typename T, typename TD>
 void launch_embed_fwd(const EmbedArgs& a, hipStream_t st) {
   if (a.nrows <= 0) return;
@@ -959,6 +1133,17 @@ void launch_embed_bwd(const EmbedArgs& a, hipStream_t st) {
   const dim3 grid(ceil_div(a.nrows * kWave, tpb));
   const T* tables = (const T*)(a.packed ? a.packed : a.tables);
   const long long* off = a.packed ? a.packed_off : a.tab_off;
+  if constexpr (sizeof(T) == 4 && sizeof(TG) == 4) {
+    if (embed_groups_apply(a)) {
+      DPMI_C_DISPATCH(a.M, hipLaunchKernelGGL((k_embed_bwd_g<C>), grid, dim3(tpb), 0, st, (float*)a.net_deriv,
+                                              (const float*)a.xyz, (const float*)a.gD, a.pos, (const float*)tables,
+                                              off, (const Info<float>*)a.infos, a.net_of, (const float*)a.em,
+                                              a.row_type, a.sec, a.ntypes, a.nrows, a.nnei, a.M, a.A,
+                                              a.ldD > 0 ? a.ldD : a.M * a.A, (float)a.inv_nnei));
+      DPMI_LAUNCH_CHECK();
+      return;
+    }
+  }
 #define DPMI_BWD_LAUNCH(PKV)                                                                                        \
   DPMI_KQ_DISPATCH(a.M, hipLaunchKernelGGL((k_embed_bwd<T, TG, KQ, PKV>), grid, dim3(tpb), 0, st, (T*)a.net_deriv,    \
                                            (const T*)a.xyz, (const TG*)a.gD, a.pos, tables, off,                      \
@@ -981,17 +1166,18 @@ bool embed_mfma_supported(int M) { return M > 0 && M <= 128 && M % 4 == 0; }
 
 void build_packed_tables(const double* table, const long long* tab_off, const int* nrows, int nnets, int M,
                          std::vector<float>& out, std::vector<long long>& off) {
+  const int S = pk_stride(M);
   off.assign(nnets, 0);
   long long tot = 0;
   for (int n = 0; n < nnets; ++n) {
     off[n] = tot;
-    tot += (long long)nrows[n] * M * 4;
+    tot += (long long)nrows[n] * S * 4;
   }
-  out.assign((size_t)tot, 0.f);
+  out.assign((size_t)tot, 0.f);  // padding records stay zero
   for (int n = 0; n < nnets; ++n)
     for (long long q = 0; q < (long long)nrows[n] * M; ++q) {
       const double* a = table + tab_off[n] + q * 6;
-      float* r = out.data() + off[n] + q * 4;
+      float* r = out.data() + off[n] + (q / M * S + q % M) * 4;
       r[0] = (float)a[0];
       r[1] = (float)a[1];
       const tab_half2 h23 = {(_Float16)a[2], (_Float16)a[3]}, h45 = {(_Float16)a[4], (_Float16)a[5]};

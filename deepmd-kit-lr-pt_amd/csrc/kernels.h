@@ -130,8 +130,8 @@ struct EmbedArgs {
   int waves = 0;      // forward: waves (rows) per workgroup (0: 4)
   int waves_bwd = 0;  // backward: waves (rows) per workgroup (0: 1)
   // packed FP32 tables of the per-neighbour FP32 kernels (one 16-byte record
-  // per column: a0, a1 FP32, a2..a5 FP16) and their per-net offsets in
-  // floats; null: the plain tables above
+  // per column: a0, a1 FP32, a2..a5 FP16; rows of pk_stride(M) records) and
+  // their per-net offsets in floats; null: the plain tables above
   const void* packed = nullptr;
   const long long* packed_off = nullptr;
   // matrix-core (table-space) kernels of the mixed engine
@@ -159,7 +159,8 @@ void launch_embed_bwd(const EmbedArgs& a, hipStream_t st);
 // in pscale[8 net + p]) in layout [row][8 p][32 ceil(M/32)].
 constexpr int kEmbedTileAtoms = 4;
 bool embed_mfma_supported(int M);
-// packed FP32 tables (records of 4 floats per column) and per-net offsets
+// packed FP32 tables (records of 4 floats per column, rows padded to
+// pk_stride(M) = a multiple of 8 records) and per-net offsets
 void build_packed_tables(const double* table, const long long* tab_off, const int* nrows, int nnets, int M,
                          std::vector<float>& out, std::vector<long long>& off);
 void build_split_tables(const double* table, const long long* tab_off, const int* nrows, const double* infos,

@@ -60,15 +60,18 @@ __device__ __forceinline__ void load6(const T* p, T* a) {
 
 // Coefficients a0..a5 of column `col` of table row `idx`.  Plain tables hold
 // 6 values per column ([row][M][6]); packed FP32 tables (PK) hold one 16-byte
-// record per column, a0 and a1 in FP32 and a2..a5 in FP16 ([row][M] records;
-// the terms a_p t^p, p >= 2, are at most a_p stride^2 in size, so FP16 keeps
-// them to FP32-level absolute accuracy of the value), read with one load.
+// record per column, a0 and a1 in FP32 and a2..a5 in FP16 ([row][pk_stride(M)]
+// records, rows padded to 8 records so that every row starts on a 128-byte
+// cache line; the terms a_p t^p, p >= 2, are at most a_p stride^2 in size, so
+// FP16 keeps them to FP32-level absolute accuracy of the value), read with
+// one load.
 typedef _Float16 tab_half2 __attribute__((ext_vector_type(2)));
+__host__ __device__ constexpr int pk_stride(int M) { return (M + 7) / 8 * 8; }
 template <typename T, bool PK>
 __device__ __forceinline__ void load_coef(const T* __restrict__ table, int idx, int col, int M, T* a) {
   if constexpr (PK) {
     static_assert(sizeof(T) == 4, "packed tables are FP32");
-    const float4 r = reinterpret_cast<const float4*>(table)[(size_t)idx * M + col];
+    const float4 r = reinterpret_cast<const float4*>(table)[(size_t)idx * pk_stride(M) + col];
     const float w2 = r.z, w3 = r.w;  // scalars for the bit casts (as load_coef_pk)
     const tab_half2 h23 = __builtin_bit_cast(tab_half2, w2), h45 = __builtin_bit_cast(tab_half2, w3);
     a[0] = r.x;
@@ -82,7 +85,7 @@ __device__ __forceinline__ void load_coef(const T* __restrict__ table, int idx, 
   }
 }
 
-// Packed records through a buffer resource: the row offset idx * M * 16 is
+// Packed records through a buffer resource: the row offset idx * stride * 16 is
 // wave-uniform and goes in the scalar offset, the lane's column offset
 // col * 16 in the vector offset, so a record read needs no per-lane 64-bit
 // address arithmetic.  The resource's record count is 2^31 - 1 bytes; a lane
@@ -127,7 +130,7 @@ struct RowReader {
   __device__ __forceinline__ void load(int idx, T (&a)[KQ][6]) const {
 #pragma unroll
     for (int kq = 0; kq < KQ; ++kq) {
-      if constexpr (PK) load_coef_pk(rs, idx * M * 16, col[kq], a[kq]);
+      if constexpr (PK) load_coef_pk(rs, idx * pk_stride(M) * 16, col[kq], a[kq]);
       else load_coef<T, PK>(table, idx, col[kq], M, a[kq]);
     }
   }

@@ -86,9 +86,11 @@ def test_matrix_core_embedding_matches_per_neighbour_kernels(case):
         e_got = {k: _rel(got[k], d[k]) for k in ("e", "f", "v", "nd", "desc")}
         print(case, mask, "vs fp64: per-neighbour", e_ref, "table-space", e_got, flush=True)
         # both errors are dominated by FP16 rounding of the descriptor, so
-        # they are compared as magnitudes, not digit for digit
+        # they are compared as magnitudes, not digit for digit; the floor is
+        # a tenth of that rounding (2^-11 relative), below which either
+        # error can cancel by chance
         for k in e_ref:
-            assert e_got[k] <= 3.0 * e_ref[k] + 1e-6, (mask, k, e_got[k], e_ref[k])
+            assert e_got[k] <= 3.0 * e_ref[k] + 5e-5, (mask, k, e_got[k], e_ref[k])
         assert _rel(got["nd"], ref["nd"]) < 5e-3
         assert _rel(got["desc"], ref["desc"]) < 2e-3  # FP16 descriptor
         assert _rel(got["f"], ref["f"]) < 2e-2 and _rel(got["e"], ref["e"]) < 1e-3
