@@ -395,25 +395,22 @@ __global__ void __launch_bounds__(256) k_embed_bwd(T* __restrict__ net_deriv, co
 // forward, bound by the table reads (whose row index here would first
 // travel through a shuffle), keeps the lanes-over-columns layout.
 
-// Packed records of the lane's C columns of any table row (per-lane rows).
+// Packed records of the lane's C columns of any table row (per-lane rows):
+// rows hold pk_stride(M) >= 16 C records (zero past column M), so the C
+// reads share one vector offset and differ by immediate offsets of 256 c.
 template <int C>
 struct GRows {
   __amdgpu_buffer_rsrc_t rs;
-  // byte offset of column k in a row; k >= M: kDeadOffset, so that row +
-  // offset (row < 2^31) stays at or past 2^31, beyond the record count
-  int M, col[C];
-  __device__ __forceinline__ GRows(const float* t, int M_, int gl) : M(M_) {
+  int stride16, base;
+  __device__ __forceinline__ GRows(const float* t, int M, int gl) {
     rs = pk_rsrc(t);
-#pragma unroll
-    for (int c = 0; c < C; ++c) {
-      const int k = gl + 16 * c;
-      col[c] = k < M ? k * 16 : kDeadOffset;
-    }
+    stride16 = pk_stride(M) * 16;
+    base = gl * 16;
   }
   __device__ __forceinline__ void load(int idx, float (&a)[C][6]) const {
-    const unsigned row = (unsigned)(idx * pk_stride(M) * 16);
+    const int off = idx * stride16 + base;
 #pragma unroll
-    for (int c = 0; c < C; ++c) load_coef_pk(rs, 0, (int)(row + (unsigned)col[c]), a[c]);
+    for (int c = 0; c < C; ++c) load_coef_pk(rs, 0, off + 256 * c, a[c]);
   }
 };
 
@@ -457,6 +454,7 @@ __device__ void bwd_block_g(float* __restrict__ nd, const float (&dy)[C][4], con
     Nbr<float, 4> r;
     load_nbr(r, f, em + (size_t)c0 * 4, 4, em + (size_t)c0 * 4, lane, jend - c0, last, false);
     const int cnt = min(kWave, jend - c0);
+    const bool anyd = __ballot(r.delta != 0.f) != 0ull;  // a neighbour outside the table
     for (int q0 = 0; q0 < cnt; q0 += 8) {
       float a[2][C][6], x[2], delta[2], ll[2][4];
 #pragma unroll
@@ -470,23 +468,22 @@ __device__ void bwd_block_g(float* __restrict__ nd, const float (&dy)[C][4], con
       }
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
-        // columns k >= M: zero coefficients times dy = 0
-        float p[4];
+        // columns k >= M: zero coefficients times dy = 0.  p[m] = sum_k
+        // P_k dy[k, m] and e[m] = sum_k P'_k dy[k, m] over the lane's
+        // columns; the slope term of component 0 is ll . e.
+        float p[4], e[4];
 #pragma unroll
         for (int c = 0; c < C; ++c) {
           float v, d;
-          quintic(a[u][c], x[u], delta[u], v, d);
-          const float proj = ll[u][0] * dy[c][0] + ll[u][1] * dy[c][1] + ll[u][2] * dy[c][2] + ll[u][3] * dy[c][3];
-          if (c == 0) {
-            p[0] = v * dy[c][0] + d * proj;
+          quintic(a[u][c], x[u], 0.f, v, d);
+          if (anyd) v += d * delta[u];  // linear extension outside the table
 #pragma unroll
-            for (int m = 1; m < 4; ++m) p[m] = v * dy[c][m];
-          } else {
-            p[0] += v * dy[c][0] + d * proj;
-#pragma unroll
-            for (int m = 1; m < 4; ++m) p[m] += v * dy[c][m];
+          for (int m = 0; m < 4; ++m) {
+            p[m] = c == 0 ? v * dy[c][m] : p[m] + v * dy[c][m];
+            e[m] = c == 0 ? d * dy[c][m] : e[m] + d * dy[c][m];
           }
         }
+        p[0] += ll[u][0] * e[0] + ll[u][1] * e[1] + ll[u][2] * e[2] + ll[u][3] * e[3];
         const float tot = group_sum4(p, gl);
         const int q = q0 + 4 * u + grp, j = c0 + q;
         if (q < cnt && (gl & 3) == 0) {
@@ -1087,7 +1084,9 @@ void free_table_infos(void* p, bool fp32) {
 
 // Packed FP32 tables, A <= 16, M <= 128: the group-of-16 adjoint (C columns
 // per lane).
-bool embed_groups_apply(const EmbedArgs& a) { return a.packed && a.A <= 16 && a.M <= 128 && a.M > 0; }
+bool embed_groups_apply(const EmbedArgs& a) {
+  return a.packed && a.A <= 16 && a.M <= 128 && a.M > 0 && pk_stride(a.M) >= (a.M + 15) / 16 * 16;
+}
 #define DPMI_C_DISPATCH(M, ...)                                          \
   do {                                                                   \
     switch (((M) + 15) / 16) {                                           \

@@ -159,8 +159,8 @@ void launch_embed_bwd(const EmbedArgs& a, hipStream_t st);
 // in pscale[8 net + p]) in layout [row][8 p][32 ceil(M/32)].
 constexpr int kEmbedTileAtoms = 4;
 bool embed_mfma_supported(int M);
-// packed FP32 tables (records of 4 floats per column, rows padded to
-// pk_stride(M) = a multiple of 8 records) and per-net offsets
+// packed FP32 tables (records of 4 floats per column, rows padded with zero
+// records to pk_stride(M) = a multiple of 16 records) and per-net offsets
 void build_packed_tables(const double* table, const long long* tab_off, const int* nrows, int nnets, int M,
                          std::vector<float>& out, std::vector<long long>& off);
 void build_split_tables(const double* table, const long long* tab_off, const int* nrows, const double* infos,

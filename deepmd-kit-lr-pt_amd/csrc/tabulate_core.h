@@ -61,12 +61,12 @@ __device__ __forceinline__ void load6(const T* p, T* a) {
 // Coefficients a0..a5 of column `col` of table row `idx`.  Plain tables hold
 // 6 values per column ([row][M][6]); packed FP32 tables (PK) hold one 16-byte
 // record per column, a0 and a1 in FP32 and a2..a5 in FP16 ([row][pk_stride(M)]
-// records, rows padded to 8 records so that every row starts on a 128-byte
-// cache line; the terms a_p t^p, p >= 2, are at most a_p stride^2 in size, so
-// FP16 keeps them to FP32-level absolute accuracy of the value), read with
-// one load.
+// records, rows padded with zero records to a multiple of 16, so that every
+// row starts on a 256-byte boundary and 16-lane groups read whole spans; the
+// terms a_p t^p, p >= 2, are at most a_p stride^2 in size, so FP16 keeps them
+// to FP32-level absolute accuracy of the value), read with one load.
 typedef _Float16 tab_half2 __attribute__((ext_vector_type(2)));
-__host__ __device__ constexpr int pk_stride(int M) { return (M + 7) / 8 * 8; }
+__host__ __device__ constexpr int pk_stride(int M) { return (M + 15) / 16 * 16; }
 template <typename T, bool PK>
 __device__ __forceinline__ void load_coef(const T* __restrict__ table, int idx, int col, int M, T* a) {
   if constexpr (PK) {
