@@ -58,14 +58,16 @@ def stage_model(m, nloc, nall, precision):
     mm_peak = FP16_PEAK_TFLOPS if mixed else FP64_PEAK_TFLOPS
     vec_peak = FP32_PEAK_TFLOPS if mixed else FP64_PEAK_TFLOPS
     per_atom = {
-        # reads nlist (4N) and extended coordinates, writes em/em_deriv/rij (19 N s)
-        "env_mat": ("hbm", 4 * N + 19 * N * s + 24 * rho, None, None),
+        # reads nlist (4N) and extended coordinates, writes em (4 N s); the
+        # slot derivative and displacement are rebuilt by the force stage
+        "env_mat": ("hbm", 4 * N + 4 * N * s + 24 * rho, None, None),
         # quintic + contraction (18 N M), descriptor (8 M A); reads em, writes xyz and D
         "embed_fwd": ("flop", 18 * N * M + 8 * M * A, vec_peak, 4 * N * s + 4 * M * s + D * h),
         # dT from dE/dD (16 M A), table adjoint (36 N M); reads em, xyz, dE/dD, writes net_deriv
         "embed_bwd": ("flop", 36 * N * M + 16 * M * A, vec_peak, 8 * N * s + 4 * M * s + D * s),
-        # reads net_deriv, em_deriv, rij, nlist; scatters forces (FP64)
-        "force_virial": ("hbm", 19 * N * s + 4 * N + 24 * rho + 72, None, None),
+        # reads net_deriv and nlist, the extended coordinates once; scatters
+        # forces (FP64) and writes the per-row virial
+        "force_virial": ("hbm", 4 * N * s + 4 * N + 24 * rho + 72, None, None),
         # reads the cell lists and ghost coordinates, writes the sorted rows
         "nlist": ("hbm", 8 * N, None, None),
         "fitting": ("flop", fit_fwd + fit_bwd, mm_peak, None),
@@ -79,10 +81,10 @@ def stage_model(m, nloc, nall, precision):
 # FETCH_SIZE / WRITE_SIZE on this workload, scaled by a 256 MiB copy
 # (profiles/water12k_kernels_and_traffic.txt).
 MEASURED_TRAFFIC_MIB = {
-    "mixed": {"embed_bwd": (118.54, 24.67), "embed_fwd": (27.83, 53.18), "fitting": (77.90, 100.35),
-              "force_virial": (125.16, 25.07), "env_mat": (14.38, 117.25), "nlist": (7.78, 6.77)},
-    "fp64": {"embed_bwd": (250.04, 49.19), "embed_fwd": (66.17, 212.02), "force_virial": (156.98, 24.42),
-             "env_mat": (15.49, 249.86), "nlist": (7.78, 6.77)},
+    "mixed": {"embed_bwd": (123.03, 24.90), "embed_fwd": (30.94, 53.91), "fitting": (70.24, 94.69),
+              "force_virial": (36.29, 23.96), "env_mat": (11.49, 24.93), "nlist": (7.83, 6.81)},
+    "fp64": {"embed_bwd": (254.00, 49.15), "embed_fwd": (67.53, 223.70), "force_virial": (42.76, 24.42),
+             "env_mat": (12.68, 48.92), "nlist": (7.83, 6.81)},
 }
 
 
