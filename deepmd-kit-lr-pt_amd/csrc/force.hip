@@ -254,16 +254,19 @@ __global__ void __launch_bounds__(256) k_force_virial_rows(const TI* __restrict_
 // list, so the rows' neighbour sets overlap), accumulates every force term
 // into an LDS hash keyed by extended atom (FP64 LDS atomics, linear
 // probing), and then issues one global atomic per distinct atom and
-// component.  R (nnei + 1) <= kHashSize distinct keys at most, so every probe
-// sequence ends.
-constexpr int kHashBits = 12, kHashSize = 1 << kHashBits;
+// component.  The table has 2^HB entries, the smallest power of two
+// (>= 2^10) holding the R (nnei + 1) distinct keys a workgroup can insert,
+// so every probe sequence ends, and smaller row groups leave LDS for more
+// workgroups per compute unit.
+constexpr int kHashBitsMax = 12, kHashSize = 1 << kHashBitsMax;
 
+template <int HB>
 __device__ __forceinline__ int hash_slot(int* keys, int key) {
-  unsigned h = ((unsigned)key * 2654435761u) >> (32 - kHashBits);
+  unsigned h = ((unsigned)key * 2654435761u) >> (32 - HB);
   while (true) {
     const int prev = atomicCAS(&keys[h], -1, key);
     if (prev == -1 || prev == key) return (int)h;
-    h = (h + 1) & (kHashSize - 1);
+    h = (h + 1) & ((1u << HB) - 1);
   }
 }
 
@@ -295,16 +298,17 @@ __device__ __forceinline__ void recompute_slot(const double (&ci)[3], const doub
 
 // RECOMP: ed and rij are not stored; each slot's derivative and displacement
 // are rebuilt from coord_ext and the centre type's dstd.
-template <typename TI, bool RECOMP>
+template <typename TI, bool RECOMP, int HB>
 __global__ void __launch_bounds__(1024) k_force_virial_hash(
     const TI* __restrict__ nd, const TI* __restrict__ ed, const TI* __restrict__ rij, const int* __restrict__ nlist,
     const int* __restrict__ row_frame, const int* __restrict__ row_center, const int* __restrict__ frame_ext_off,
     const FrameGeom* __restrict__ geom, const int* __restrict__ sorted_loc, long long nrows, int nnei, int R,
     const double* __restrict__ coord_ext, const int* __restrict__ type_ext, const double* __restrict__ dstd,
     double rmin, double rmax, double* __restrict__ force_ext, double* __restrict__ vpart) {
-  __shared__ int hkey[kHashSize];
-  __shared__ double hval[kHashSize][3];
-  for (int h = threadIdx.x; h < kHashSize; h += blockDim.x) {
+  constexpr int NH = 1 << HB;
+  __shared__ int hkey[NH];
+  __shared__ double hval[NH][3];
+  for (int h = threadIdx.x; h < NH; h += blockDim.x) {
     hkey[h] = -1;
     hval[h][0] = hval[h][1] = hval[h][2] = 0.;
   }
@@ -364,7 +368,7 @@ __global__ void __launch_bounds__(1024) k_force_virial_hash(
         TI fs[3], v[9];
         slot_force(n4[u], e12[u], fs);
         slot_virial(n4[u], e12[u], r3[u], v);
-        const int h = hash_slot(hkey, eo + jj[u]);
+        const int h = hash_slot<HB>(hkey, eo + jj[u]);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
           fi[d] -= fs[d];
@@ -379,7 +383,7 @@ __global__ void __launch_bounds__(1024) k_force_virial_hash(
 #pragma unroll
     for (int k = 0; k < 9; ++k) vs[k] = wave_sum(vs[k]);
     if (lane == 0) {
-      const int h = hash_slot(hkey, eo + i);
+      const int h = hash_slot<HB>(hkey, eo + i);
 #pragma unroll
       for (int d = 0; d < 3; ++d) atomicAdd(&hval[h][d], fi[d]);
     }
@@ -392,7 +396,7 @@ __global__ void __launch_bounds__(1024) k_force_virial_hash(
     }
   }
   __syncthreads();
-  for (int h = threadIdx.x; h < kHashSize; h += blockDim.x) {
+  for (int h = threadIdx.x; h < NH; h += blockDim.x) {
     const int key = hkey[h];
     if (key >= 0) {
 #pragma unroll
@@ -530,24 +534,36 @@ void launch_force_virial_rows(const ForceRowsArgs& a, hipStream_t st) {
   if (a.sorted_loc && a.geom && a.nnei + 1 <= kHashSize) {
     const int R = std::max(1, std::min(a.rows > 0 ? a.rows : 16, kHashSize / (a.nnei + 1)));
     const dim3 grid(ceil_div(a.nrows, R)), block(R * kWave);
-    if (a.fp32 && !a.em_deriv)
-      hipLaunchKernelGGL((k_force_virial_hash<float, true>), grid, block, 0, st, (const float*)a.net_deriv, nullptr,
-                         nullptr, a.nlist, a.row_frame, a.row_center, a.frame_ext_off, a.geom, a.sorted_loc, a.nrows,
-                         a.nnei, R, a.coord_ext, a.type_ext, a.dstd, a.rmin, a.rmax, a.force_ext, a.vpart);
-    else if (!a.em_deriv)
-      hipLaunchKernelGGL((k_force_virial_hash<double, true>), grid, block, 0, st, (const double*)a.net_deriv,
-                         nullptr, nullptr, a.nlist, a.row_frame, a.row_center, a.frame_ext_off, a.geom, a.sorted_loc,
-                         a.nrows, a.nnei, R, a.coord_ext, a.type_ext, a.dstd, a.rmin, a.rmax, a.force_ext, a.vpart);
-    else if (a.fp32)
-      hipLaunchKernelGGL((k_force_virial_hash<float, false>), grid, block, 0, st, (const float*)a.net_deriv,
-                         (const float*)a.em_deriv, (const float*)a.rij, a.nlist, a.row_frame, a.row_center,
-                         a.frame_ext_off, a.geom, a.sorted_loc, a.nrows, a.nnei, R, nullptr, nullptr, nullptr, 0., 0.,
-                         a.force_ext, a.vpart);
+    const int keys = R * (a.nnei + 1);
+    const int hb = keys <= (1 << 10) ? 10 : (keys <= (1 << 11) ? 11 : 12);
+#define DPMI_HASH_LAUNCH(HB)                                                                                          \
+  do {                                                                                                                \
+    if (a.fp32 && !a.em_deriv)                                                                                        \
+      hipLaunchKernelGGL((k_force_virial_hash<float, true, HB>), grid, block, 0, st, (const float*)a.net_deriv,        \
+                         nullptr, nullptr, a.nlist, a.row_frame, a.row_center, a.frame_ext_off, a.geom, a.sorted_loc,  \
+                         a.nrows, a.nnei, R, a.coord_ext, a.type_ext, a.dstd, a.rmin, a.rmax, a.force_ext, a.vpart);  \
+    else if (!a.em_deriv)                                                                                             \
+      hipLaunchKernelGGL((k_force_virial_hash<double, true, HB>), grid, block, 0, st, (const double*)a.net_deriv,      \
+                         nullptr, nullptr, a.nlist, a.row_frame, a.row_center, a.frame_ext_off, a.geom, a.sorted_loc,  \
+                         a.nrows, a.nnei, R, a.coord_ext, a.type_ext, a.dstd, a.rmin, a.rmax, a.force_ext, a.vpart);  \
+    else if (a.fp32)                                                                                                  \
+      hipLaunchKernelGGL((k_force_virial_hash<float, false, HB>), grid, block, 0, st, (const float*)a.net_deriv,       \
+                         (const float*)a.em_deriv, (const float*)a.rij, a.nlist, a.row_frame, a.row_center,            \
+                         a.frame_ext_off, a.geom, a.sorted_loc, a.nrows, a.nnei, R, nullptr, nullptr, nullptr, 0., 0., \
+                         a.force_ext, a.vpart);                                                                       \
+    else                                                                                                              \
+      hipLaunchKernelGGL((k_force_virial_hash<double, false, HB>), grid, block, 0, st, (const double*)a.net_deriv,     \
+                         (const double*)a.em_deriv, (const double*)a.rij, a.nlist, a.row_frame, a.row_center,          \
+                         a.frame_ext_off, a.geom, a.sorted_loc, a.nrows, a.nnei, R, nullptr, nullptr, nullptr, 0., 0., \
+                         a.force_ext, a.vpart);                                                                       \
+  } while (0)
+    if (hb == 10)
+      DPMI_HASH_LAUNCH(10);
+    else if (hb == 11)
+      DPMI_HASH_LAUNCH(11);
     else
-      hipLaunchKernelGGL((k_force_virial_hash<double, false>), grid, block, 0, st, (const double*)a.net_deriv,
-                         (const double*)a.em_deriv, (const double*)a.rij, a.nlist, a.row_frame, a.row_center,
-                         a.frame_ext_off, a.geom, a.sorted_loc, a.nrows, a.nnei, R, nullptr, nullptr, nullptr, 0., 0.,
-                         a.force_ext, a.vpart);
+      DPMI_HASH_LAUNCH(12);
+#undef DPMI_HASH_LAUNCH
     DPMI_LAUNCH_CHECK();
     return;
   }
