@@ -133,10 +133,12 @@ int copy_coord_run(FrameGeom g, double* out_c, int* out_t, int* mapping, int* na
     g.xcell_off = 0;
     const int nc = dpmi::geom_local_cells(g), nx = dpmi::geom_ext_cells(g) + 1;
     auto& sc = scratch();
-    char* base = static_cast<char*>(sc.get("copy_coord", sizeof(int) * ((size_t)4 * nloc + 3 * nc + 4 * nx +
-                                                                         (size_t)mem_nall + 8) +
-                                                              sizeof(dpmi::FrameGeom)));
-    int* ip = reinterpret_cast<int*>(base + sizeof(dpmi::FrameGeom));
+    // [FrameGeom][pad to 16][CellAtom x mem_nall][ints]
+    const size_t rec_off = (sizeof(dpmi::FrameGeom) + 15) / 16 * 16;
+    char* base = static_cast<char*>(sc.get("copy_coord", rec_off + sizeof(dpmi::CellAtom) * (size_t)mem_nall +
+                                                              sizeof(int) * ((size_t)4 * nloc + 3 * nc + 4 * nx + 8)));
+    dpmi::CellAtom* recs = reinterpret_cast<dpmi::CellAtom*>(base + rec_off);
+    int* ip = reinterpret_cast<int*>(recs + mem_nall);
     auto take = [&](size_t n) {
       int* p = ip;
       ip += n;
@@ -160,7 +162,7 @@ int copy_coord_run(FrameGeom g, double* out_c, int* out_t, int* mapping, int* na
     w.coord_ext = out_c;
     w.type_ext = out_t;
     w.map_ext = mapping;
-    w.cell_atoms = take(mem_nall);
+    w.cell_atoms = recs;
     w.cell_of = take(nloc);
     w.cell_cnt = take(nc);
     w.cell_start = take(nc);

@@ -15,7 +15,8 @@
 //   3. k_scatter_cells one thread per local atom: place into its cell
 //   4. k_sort_cells    one wave per cell: ascending index inside the cell
 //   5. k_emit_ghosts   one wave per extended cell: write ghost images and
-//                      the extended cell -> atom index lists
+//                      the extended cells' atom records (coordinates,
+//                      extended index, type) in cell order
 // Floating-point expressions that decide cell membership and image
 // coordinates are evaluated without contraction, in the reference's order.
 #include "dpmi_common.h"
@@ -225,7 +226,8 @@ __global__ void k_emit_ghosts(const int* __restrict__ xcell_frame, const FrameGe
                               const int* __restrict__ cell_start, const int* __restrict__ sorted_loc,
                               const int* __restrict__ xcnt, const int* __restrict__ xstart,
                               const int* __restrict__ xgstart, double* __restrict__ coord_ext,
-                              int* __restrict__ type_ext, int* __restrict__ map_ext, int* __restrict__ cell_atoms) {
+                              int* __restrict__ type_ext, int* __restrict__ map_ext,
+                              CellAtom* __restrict__ cell_atoms) {
 #pragma clang fp contract(off)
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int lane = threadIdx.x & (kWave - 1);
@@ -247,20 +249,29 @@ __global__ void k_emit_ghosts(const int* __restrict__ xcell_frame, const FrameGe
   for (int k = lane; k < cnt; k += kWave) {
     const int p = src[k];
     const int slot = xstart[wave] + k;
+    const size_t a = (size_t)g.ext_off + p;
+    CellAtom rec;
+    rec.x = coord_ext[3 * a + 0];
+    rec.y = coord_ext[3 * a + 1];
+    rec.z = coord_ext[3 * a + 2];
+    rec.type = type_ext[a];
+    rec.j = p;
     if (gh) {
       const int gi = g.nloc + xgstart[wave] + k;
+      rec.x = rec.x - sv[0];
+      rec.y = rec.y - sv[1];
+      rec.z = rec.z - sv[2];
+      rec.j = gi;
       if (gi < g.ext_cap) {
-        const size_t a = (size_t)g.ext_off + p, b = (size_t)g.ext_off + gi;
-        coord_ext[3 * b + 0] = coord_ext[3 * a + 0] - sv[0];
-        coord_ext[3 * b + 1] = coord_ext[3 * a + 1] - sv[1];
-        coord_ext[3 * b + 2] = coord_ext[3 * a + 2] - sv[2];
-        type_ext[b] = type_ext[a];
+        const size_t b = (size_t)g.ext_off + gi;
+        coord_ext[3 * b + 0] = rec.x;
+        coord_ext[3 * b + 1] = rec.y;
+        coord_ext[3 * b + 2] = rec.z;
+        type_ext[b] = rec.type;
         map_ext[b] = p;
       }
-      if (slot < g.ext_cap) cell_atoms[g.ext_off + slot] = gi;
-    } else {
-      if (slot < g.ext_cap) cell_atoms[g.ext_off + slot] = p;
     }
+    if (slot < g.ext_cap) cell_atoms[g.ext_off + slot] = rec;
   }
 }
 

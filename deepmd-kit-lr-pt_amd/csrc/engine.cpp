@@ -491,9 +491,10 @@ SeaEngine::~SeaEngine() {
   if (geom_pinned) (void)hipHostFree(geom_pinned);
   for (auto* p : {&d_frame_of, &d_center, &d_type, &d_pos, &d_perm, &d_flag, &d_nall, &d_frame_ext_off, &d_nlist,
                   &d_ncand, &d_cell_of, &d_sorted_loc, &d_cell_frame, &d_xcell_frame, &d_cell_cnt, &d_cell_start,
-                  &d_cell_fill, &d_xcnt, &d_xstart, &d_xgcnt, &d_xgstart, &d_type_ext, &d_map_ext, &d_cell_atoms,
+                  &d_cell_fill, &d_xcnt, &d_xstart, &d_xgcnt, &d_xgstart, &d_type_ext, &d_map_ext,
                   &d_xframe_of_ext, &d_chunks, &d_chunk_start})
     dfree(*p);
+  dfree(d_cell_atoms);
   for (auto* p : {&d_em, &d_em_deriv, &d_rij, &d_net_deriv, &d_xyz, &d_D, &d_gD, &d_atom_e, &d_vpart,
                   &d_coord_ext, &d_force_ext, &d_g0, &d_g1, &d_part})
     dfree(*p);
@@ -549,8 +550,9 @@ void SeaEngine::prepare_geometry(const double* box_h) {
       ext_cap_h[f] = g.ext_cap;
     }
     for (auto* p : {&d_cell_frame, &d_xcell_frame, &d_cell_cnt, &d_cell_start, &d_cell_fill, &d_xcnt, &d_xstart,
-                    &d_xgcnt, &d_xgstart, &d_type_ext, &d_map_ext, &d_cell_atoms, &d_xframe_of_ext})
+                    &d_xgcnt, &d_xgstart, &d_type_ext, &d_map_ext, &d_xframe_of_ext})
       dfree(*p);
+    dfree(d_cell_atoms);
     dfree(d_coord_ext);
     dfree(d_force_ext);
     d_cell_frame = dupload(cf.data(), cf.size());
@@ -565,7 +567,7 @@ void SeaEngine::prepare_geometry(const double* box_h) {
     d_xgstart = dalloc<int>(nxcells);
     d_type_ext = dalloc<int>(next);
     d_map_ext = dalloc<int>(next);
-    d_cell_atoms = dalloc<int>(next);
+    d_cell_atoms = dalloc<CellAtom>(next);
     d_coord_ext = dalloc<double>((size_t)next * 3);
     d_force_ext = dalloc<double>((size_t)next * 3);
     DPMI_CHECK(hipMemcpy(d_frame_ext_off, eoff.data(), sizeof(int) * nframes, hipMemcpyHostToDevice));

@@ -116,7 +116,8 @@ struct SeaEngine {
   int *d_cell_frame = nullptr, *d_xcell_frame = nullptr, *d_cell_cnt = nullptr, *d_cell_start = nullptr,
       *d_cell_fill = nullptr;
   int *d_xcnt = nullptr, *d_xstart = nullptr, *d_xgcnt = nullptr, *d_xgstart = nullptr;
-  int *d_type_ext = nullptr, *d_map_ext = nullptr, *d_cell_atoms = nullptr, *d_xframe_of_ext = nullptr;
+  int *d_type_ext = nullptr, *d_map_ext = nullptr, *d_xframe_of_ext = nullptr;
+  CellAtom* d_cell_atoms = nullptr;
   double *d_coord_ext = nullptr, *d_force_ext = nullptr;
   double *d_em = nullptr, *d_em_deriv = nullptr, *d_rij = nullptr, *d_net_deriv = nullptr;
   bool recompute_slots = false;  // em_deriv / rij rebuilt by the force kernel (not stored)

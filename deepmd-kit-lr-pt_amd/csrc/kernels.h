@@ -10,6 +10,12 @@
 namespace dpmi {
 
 // ------------------------------------------------------------- coord.hip --
+// Extended atom in the cell-sorted list: its coordinates, extended index and
+// type, so that a neighbour search reads candidates as contiguous records.
+struct __attribute__((aligned(16))) CellAtom {
+  double x, y, z;
+  int j, type;
+};
 struct CellWork {
   int nframes = 0;
   int ntot = 0;          // packed local atoms of all frames
@@ -25,7 +31,7 @@ struct CellWork {
   double* coord_ext = nullptr;       // [sum ext_cap, 3]
   int* type_ext = nullptr;
   int* map_ext = nullptr;
-  int* cell_atoms = nullptr;         // [sum ext_cap]
+  CellAtom* cell_atoms = nullptr;    // [sum ext_cap], extended cells' atoms in cell order
   int* cell_of = nullptr;            // [ntot]
   int* cell_cnt = nullptr;           // [ncells_tot]
   int* cell_start = nullptr;

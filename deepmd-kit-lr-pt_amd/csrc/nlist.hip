@@ -109,7 +109,7 @@ template <int CAP>
 __global__ void __launch_bounds__(64) k_cell_nlist(const int* __restrict__ frame_of, const FrameGeom* __restrict__ geom,
                                                    int ntot, const double* __restrict__ coord_ext,
                                                    const int* __restrict__ type_ext, const int* __restrict__ cell_of,
-                                                   const int* __restrict__ xstart, const int* __restrict__ cell_atoms,
+                                                   const int* __restrict__ xstart, const CellAtom* __restrict__ cell_atoms,
                                                    const int* __restrict__ sec_g, int ntypes, int nnei, double rc2,
                                                    float rc2f, int* __restrict__ nlist, int* __restrict__ ncand,
                                                    int* __restrict__ flag) {
@@ -168,12 +168,15 @@ __global__ void __launch_bounds__(64) k_cell_nlist(const int* __restrict__ frame
           }
           // slots and indices beyond the frame's capacity exist only after a
           // ghost-capacity overflow (already flagged): skip them
+          // candidates of a cell are consecutive records (coalesced reads)
           const int slot = cstart[lo] + (p - cpre[lo]);
-          const int j = slot < g.ext_cap ? cell_atoms[eo + slot] : -1;
-          const int tj = (j >= 0 && j < g.ext_cap) ? type_ext[eo + j] : -1;
+          CellAtom rec;
+          rec.j = -1;
+          if (slot < g.ext_cap) rec = cell_atoms[eo + slot];
+          const int j = rec.j;
+          const int tj = (j >= 0 && j < g.ext_cap) ? rec.type : -1;
           if (j != i && tj >= 0) {
-            const double cj[3] = {coord_ext[3 * (eo + j) + 0], coord_ext[3 * (eo + j) + 1],
-                                  coord_ext[3 * (eo + j) + 2]};
+            const double cj[3] = {rec.x, rec.y, rec.z};
             float fr2;
             if (accept(ci, cj, rc2, rc2f, &fr2)) {
               keep = true;
