@@ -15,6 +15,8 @@
 //   key = type << 56 | float_bits(r^2) << 24 | index      (index < 2^24)
 // which is bitonic-sorted in LDS; per-type first/end positions then map
 // slot s of type t to key[first[t] + s - sec[t]].
+#include <type_traits>
+
 #include "dpmi_common.h"
 #include "geometry.h"
 #include "kernels.h"
@@ -59,7 +61,46 @@ __device__ __forceinline__ bool accept(const double* ci, const double* cj, doubl
   return q <= rc2f;
 }
 
-// Sorts keys[0, n) in LDS and writes the formatted row.
+// Bitonic sort of the wave's P = 64 KR keys in registers: element
+// t = lane + 64 r is kr[r].  Stages with partner distance j < 64 exchange
+// across lanes (64-bit shuffles), the others swap registers within a lane.
+template <int KR>
+__device__ __forceinline__ void wave_bitonic(unsigned long long (&kr)[KR]) {
+  constexpr int P = 64 * KR;
+  const int lane = lane_id();
+#pragma unroll
+  for (int k = 2; k <= P; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j >= kWave) {
+        const int rj = j / kWave;
+#pragma unroll
+        for (int r = 0; r < KR; ++r) {
+          if (r & rj) continue;
+          const int t = lane + kWave * r;
+          const bool up = (t & k) == 0;
+          const unsigned long long a = kr[r], b = kr[r | rj];
+          const bool sw = (a > b) == up;
+          kr[r] = sw ? b : a;
+          kr[r | rj] = sw ? a : b;
+        }
+      } else {
+        const bool lower = (lane & j) == 0;
+#pragma unroll
+        for (int r = 0; r < KR; ++r) {
+          const int t = lane + kWave * r;
+          const bool up = (t & k) == 0;
+          const unsigned long long o = __shfl_xor(kr[r], j, kWave);
+          // the lower element of an ascending pair keeps the minimum
+          const bool keep_min = lower == up;
+          kr[r] = keep_min ? (o < kr[r] ? o : kr[r]) : (o > kr[r] ? o : kr[r]);
+        }
+      }
+    }
+  }
+}
+
+// Sorts keys[0, n) (LDS) and writes the formatted row.
 template <int CAP>
 __device__ void sort_and_slot(unsigned long long* keys, int n, int* tfirst, int* tend, const int* sec, int ntypes,
                               int nnei, int* __restrict__ row) {
@@ -67,28 +108,54 @@ __device__ void sort_and_slot(unsigned long long* keys, int n, int* tfirst, int*
   if (n > CAP) n = CAP;
   int P = 64;
   while (P < n) P <<= 1;
-  for (int t = n + lane; t < P; t += kWave) keys[t] = ~0ull;
+  __syncthreads();  // the appended keys are visible to every lane
   for (int t = lane; t < ntypes; t += kWave) {
     tfirst[t] = 0;
     tend[t] = 0;
   }
-  __syncthreads();
-  for (int k = 2; k <= P; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int t = lane; t < P; t += kWave) {
-        const int u = t ^ j;
-        if (u > t) {
-          const unsigned long long a = keys[t], b = keys[u];
-          const bool up = (t & k) == 0;
-          if ((a > b) == up) {
-            keys[t] = b;
-            keys[u] = a;
+  // registers: P / 64 keys per lane (padding keys are all-ones, last)
+  auto sort_in_registers = [&](auto kr_tag) {
+    constexpr int KR = decltype(kr_tag)::value;
+    unsigned long long kr[KR];
+#pragma unroll
+    for (int r = 0; r < KR; ++r) {
+      const int t = lane + kWave * r;
+      kr[r] = t < n ? keys[t] : ~0ull;
+    }
+    wave_bitonic<KR>(kr);
+#pragma unroll
+    for (int r = 0; r < KR; ++r) keys[lane + kWave * r] = kr[r];
+  };
+  if (P <= 64)
+    sort_in_registers(std::integral_constant<int, 1>{});
+  else if (P <= 128)
+    sort_in_registers(std::integral_constant<int, 2>{});
+  else if (P <= 256)
+    sort_in_registers(std::integral_constant<int, 4>{});
+  else if (P <= 512)
+    sort_in_registers(std::integral_constant<int, 8>{});
+  else {
+    // larger capacities: the same network through LDS
+    for (int t = n + lane; t < P; t += kWave) keys[t] = ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int t = lane; t < P; t += kWave) {
+          const int u = t ^ j;
+          if (u > t) {
+            const unsigned long long a = keys[t], b = keys[u];
+            const bool up = (t & k) == 0;
+            if ((a > b) == up) {
+              keys[t] = b;
+              keys[u] = a;
+            }
           }
         }
+        __syncthreads();
       }
-      __syncthreads();
     }
   }
+  __syncthreads();
   for (int t = lane; t < n; t += kWave) {
     const int ty = (int)(keys[t] >> 56);
     if (t == 0 || (int)(keys[t - 1] >> 56) != ty) tfirst[ty] = t;
