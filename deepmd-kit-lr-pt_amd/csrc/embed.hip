@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(256) k_embed_fwd(T* __restrict__ xyz, TD* __re
       const int net = net_of[ti * ntypes + tj];
       const int b = sec[tj], nb = sec[tj + 1] - sec[tj];
       const T* e = em + ((size_t)g * nnei + b) * 4;
-      fwd_block<T, 4, KQ, PK>(acc, tables + tab_off[net], infos[net], e, 4, e, nb, M, true);
+      fwd_block_runs<T, 4, KQ, PK>(acc, tables + tab_off[net], infos[net], e, 4, e, nb, M, true);
     }
   }
 #pragma unroll

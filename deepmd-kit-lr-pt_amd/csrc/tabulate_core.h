@@ -259,6 +259,49 @@ __device__ __forceinline__ void fwd_block(T (&acc)[KQ][ND], const T* __restrict_
   }
 }
 
+// fwd_block reading each table row once per run of consecutive neighbours
+// that share it (a block is sorted by distance, so its rows come in runs;
+// 2.7 neighbours per row for water with unit statistics, 9 for fcc Cu).
+// Run starts are the neighbours whose row differs from the previous one;
+// each step requests the rows of U runs before using them.  Neighbours are
+// accumulated in the same order and with the same weights as fwd_block.
+template <typename T, int ND, int KQ, bool PK>
+__device__ __forceinline__ void fwd_block_runs(T (&acc)[KQ][ND], const T* __restrict__ table, const Info<T>& f,
+                                               const T* __restrict__ em_x, int xs, const T* __restrict__ em, int nnei,
+                                               int M, bool sorted) {
+  if (nnei <= 0) return;
+  const int lane = lane_id();
+  const T last = em_x[(size_t)(nnei - 1) * xs];
+  const RowReader<T, KQ, PK> rows(table, M, lane);
+  for (int c0 = 0; c0 < nnei; c0 += kWave) {
+    Nbr<T, ND> r;
+    const bool tail = load_nbr(r, f, em_x + (size_t)c0 * xs, xs, em + (size_t)c0 * ND, lane, nnei - c0, last, sorted);
+    const unsigned long long tm = __ballot(tail);
+    int jend = min(kWave, nnei - c0);
+    if (tm) jend = __ffsll((long long)tm);  // up to and including the first tail neighbour
+    const int prev = __shfl_up(r.idx, 1, kWave);
+    const unsigned long long live = jend >= kWave ? ~0ull : ((1ull << jend) - 1ull);
+    unsigned long long starts = __ballot(lane == 0 || r.idx != prev) & live;
+    constexpr int U = KQ <= 2 ? 4 : (KQ <= 4 ? 2 : 1);
+    while (starts) {
+      int s[U + 1];
+#pragma unroll
+      for (int u = 0; u <= U; ++u) {
+        s[u] = starts ? __ffsll((long long)starts) - 1 : jend;
+        if (u < U) starts &= starts - 1ull;
+      }
+      T a[U][KQ][6];
+#pragma unroll
+      for (int u = 0; u < U; ++u) rows.load(bcast(r.idx, min(s[u], jend - 1)), a[u]);
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        for (int q = s[u]; q < s[u + 1]; ++q)
+          fwd_neighbour<T, ND, KQ>(acc, a[u], r, q, (tm && q == jend - 1) ? (T)(nnei - (c0 + q)) : (T)1);
+    }
+    if (tm) break;
+  }
+}
+
 
 template <typename T>
 Info<T> make_info(const T* ti) {
