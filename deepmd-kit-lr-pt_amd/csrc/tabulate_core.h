@@ -195,14 +195,21 @@ __device__ __forceinline__ bool load_nbr(Nbr<T, ND>& r, const Info<T>& f, const 
 }
 
 // acc += w P(x_q) ll_q for neighbour q of the chunk record r (the slope
-// term only outside the table, delta != 0: a uniform branch).
-template <typename T, int ND, int KQ>
+// term only outside the table, delta != 0: a uniform branch).  em_q, when
+// given, is neighbour q's em row (read with uniform, scalar loads instead of
+// broadcasts from r).
+template <typename T, int ND, int KQ, bool UNIT = false>
 __device__ __forceinline__ void fwd_neighbour(T (&acc)[KQ][ND], const T (&a)[KQ][6], const Nbr<T, ND>& r, int q,
-                                              T w) {
+                                              T w, const T* __restrict__ em_q = nullptr) {
   const T x = bcast(r.x, q), delta = bcast(r.delta, q);
   T ll[ND];
+  if (em_q) {
 #pragma unroll
-  for (int m = 0; m < ND; ++m) ll[m] = bcast(r.ll[m], q);
+    for (int m = 0; m < ND; ++m) ll[m] = em_q[m];
+  } else {
+#pragma unroll
+    for (int m = 0; m < ND; ++m) ll[m] = bcast(r.ll[m], q);
+  }
   T v[KQ];
 #pragma unroll
   for (int kq = 0; kq < KQ; ++kq) v[kq] = quintic_value(a[kq], x);
@@ -210,7 +217,7 @@ __device__ __forceinline__ void fwd_neighbour(T (&acc)[KQ][ND], const T (&a)[KQ]
 #pragma unroll
     for (int kq = 0; kq < KQ; ++kq) v[kq] += quintic_slope(a[kq], x) * delta;
   }
-  if (w != (T)1) {
+  if (!UNIT && w != (T)1) {  // UNIT: the caller guarantees w = 1
 #pragma unroll
     for (int kq = 0; kq < KQ; ++kq) v[kq] *= w;
   }
@@ -283,6 +290,9 @@ __device__ __forceinline__ void fwd_block_runs(T (&acc)[KQ][ND], const T* __rest
     const unsigned long long live = jend >= kWave ? ~0ull : ((1ull << jend) - 1ull);
     unsigned long long starts = __ballot(lane == 0 || r.idx != prev) & live;
     constexpr int U = KQ <= 2 ? 4 : (KQ <= 4 ? 2 : 1);
+    // unit-weight neighbours [0, jmain); the first tail neighbour (weight
+    // nnei - jt) follows on its own
+    const int jmain = tm ? jend - 1 : jend;
     while (starts) {
       int s[U + 1];
 #pragma unroll
@@ -294,11 +304,19 @@ __device__ __forceinline__ void fwd_block_runs(T (&acc)[KQ][ND], const T* __rest
 #pragma unroll
       for (int u = 0; u < U; ++u) rows.load(bcast(r.idx, min(s[u], jend - 1)), a[u]);
 #pragma unroll
-      for (int u = 0; u < U; ++u)
-        for (int q = s[u]; q < s[u + 1]; ++q)
-          fwd_neighbour<T, ND, KQ>(acc, a[u], r, q, (tm && q == jend - 1) ? (T)(nnei - (c0 + q)) : (T)1);
+      for (int u = 0; u < U; ++u) {
+        const int qe = min(s[u + 1], jmain);
+        for (int q = s[u]; q < qe; ++q)
+          fwd_neighbour<T, ND, KQ, true>(acc, a[u], r, q, (T)1, em + (size_t)(c0 + q) * ND);
+      }
     }
-    if (tm) break;
+    if (tm) {
+      const int q = jend - 1;
+      T a[KQ][6];
+      rows.load(bcast(r.idx, q), a);
+      fwd_neighbour<T, ND, KQ>(acc, a, r, q, (T)(nnei - (c0 + q)), em + (size_t)(c0 + q) * ND);
+      break;
+    }
   }
 }
 
