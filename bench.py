@@ -81,10 +81,10 @@ def stage_model(m, nloc, nall, precision):
 # FETCH_SIZE / WRITE_SIZE on this workload, scaled by a 256 MiB copy
 # (profiles/water12k_kernels_and_traffic.txt).
 MEASURED_TRAFFIC_MIB = {
-    "mixed": {"embed_bwd": (123.03, 24.90), "embed_fwd": (30.94, 53.91), "fitting": (70.24, 94.69),
-              "force_virial": (36.29, 23.96), "env_mat": (11.49, 24.93), "nlist": (7.83, 6.81)},
-    "fp64": {"embed_bwd": (254.00, 49.15), "embed_fwd": (67.53, 223.70), "force_virial": (42.76, 24.42),
-             "env_mat": (12.68, 48.92), "nlist": (7.83, 6.81)},
+    "mixed": {"embed_bwd": (123.02, 24.99), "embed_fwd": (31.47, 53.92), "fitting": (70.28, 94.94),
+              "force_virial": (35.10, 18.60), "env_mat": (11.42, 24.93), "nlist": (9.24, 6.81)},
+    "fp64": {"embed_bwd": (253.71, 49.15), "embed_fwd": (84.65, 196.23), "force_virial": (41.07, 16.30),
+             "env_mat": (12.60, 48.92), "nlist": (9.24, 6.81)},
 }
 
 
