@@ -210,6 +210,38 @@ def test_mixed_fitting_variants(fit_neuron, resnet_dt, one_side):
     assert err["virial_meV_atom"] <= V_TOL_MEV_ATOM
 
 
+@pytest.mark.parametrize("neuron,axis", [
+    ([16, 32, 48], 8),     # M = 48: one 64-lane column group, 3 columns per lane of the 16-lane adjoint
+    ([32, 64, 64], 16),    # M = 64: column groups exactly full
+    ([25, 50, 100], 4),    # narrow descriptor axis
+    ([40, 80, 160], 16),   # M = 160: beyond the 16-lane adjoint and the matrix-core kernels
+])
+def test_mixed_embedding_widths(neuron, axis):
+    """Embedding widths and descriptor axes other than the benchmark's cover
+    the other column layouts of the table kernels (lanes over 1, 2 or 3
+    column groups; 3, 4, 7 columns per lane of the 16-lane adjoint; the
+    lanes-over-columns adjoint for M > 128)."""
+    print(f"case neuron={neuron} axis={axis}", flush=True)
+    m = dmodel.SeAModel.random(2, [46, 92], 6.0, 0.5, neuron, axis, [64, 64], seed=7)
+    m = _perturbed(m, 8)
+    m.build_tables(0.5)
+    coord, atype, box = systems.water(64, seed=13)
+    n = len(atype)
+    dm = DeviceModel(m)
+    ed, e_d, f_d, v_d = _run(dm, [n], coord, atype, box, "fp64")
+    em, e_m, f_m, v_m = _run(dm, [n], coord, atype, box, "mixed")
+    for what, tol in (("descriptor", 2e-3), ("net_deriv", 5e-3)):
+        a, b = em.fetch(what), ed.fetch(what)
+        rel = float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+        print(what, "relative deviation from fp64", rel, flush=True)
+        assert rel < tol, (what, rel)
+    err = errors(e_m[0], f_m, v_m[0], e_d[0], f_d, v_d[0], n)
+    print("errors vs fp64:", err, flush=True)
+    assert err["energy_meV_atom"] <= E_TOL_MEV_ATOM
+    assert err["force_rmse_meV_A"] <= F_RMSE_TOL_MEV_A
+    assert err["virial_meV_atom"] <= V_TOL_MEV_ATOM
+
+
 def test_mixed_rejects_width_doubling_residual():
     """A residual layer of doubled width (concatenated input) is an FP64-only
     feature: the mixed engine refuses the model instead of mis-evaluating."""
