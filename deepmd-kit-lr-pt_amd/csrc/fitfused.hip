@@ -11,7 +11,8 @@
 //   * every wave covers all rows of the tile and one eighth of the output
 //     columns, so no weight fragment is fetched twice by a workgroup;
 //   * the descriptor tile streams from HBM through a double-buffered LDS
-//     stage; the layer inputs x_l and the backward operands gz_l live in LDS
+//     stage of kKC k-steps (A-fragment order, one barrier per stage); the
+//     layer inputs x_l and the backward operands gz_l live in LDS
 //     (FP16); the activation derivatives idt*(1 - y^2) (FP16 pairs) and the
 //     residual gradients stay in registers (a wave owns the same output
 //     positions in every layer);
@@ -47,8 +48,8 @@ using halfv = _Float16 __attribute__((ext_vector_type(N)));
 constexpr int kWaves = 8;
 constexpr int kMaxW = 256;       // widest hidden layer (padded)
 constexpr int kXld = kMaxW + 8;  // LDS row stride of the activation tiles (halves)
-constexpr int kAld = 32 + 8;     // LDS row stride of the descriptor stage (halves)
 constexpr int kPf = 4;           // prefetch distance in k-steps
+constexpr int kKC = kPf;         // k-steps per descriptor chunk (one barrier each)
 
 __device__ __forceinline__ half8 zero8() {
   half8 v;
@@ -150,10 +151,11 @@ __device__ __forceinline__ void zero_acc(floatx4 (&acc)[RT][2]) {
 
 template <int H, int RT>
 __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
-  constexpr int kRows = 16 * RT;  // rows per workgroup tile
-  constexpr int kLdT = 32 / RT;   // descriptor loaders per row (RT halves each)
+  constexpr int kRows = 16 * RT;              // rows per workgroup tile
+  constexpr int kFr = kKC * RT / kWaves;      // descriptor fragments per wave and chunk
+  static_assert(kFr * kWaves == kKC * RT, "a chunk's fragments are shared evenly by the waves");
   __shared__ _Float16 xs[2][kRows][kXld];
-  __shared__ _Float16 as[2][kRows][kAld];
+  __shared__ half8 afr[2][kKC * RT][kWave];  // descriptor chunks, A-fragment order
   __shared__ float esum[kWaves][kRows];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -182,39 +184,48 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
     floatx4 acc[RT][2];
     zero_acc(acc);
     if (l == 0) {
-      // descriptor rows: 2 RT bytes per thread per k-step through a register
-      // ring (kPf steps ahead) into the double-buffered LDS stage; rows past
-      // the tile are read from its first row and zeroed
-      typedef __attribute__((address_space(1))) const halfv<RT> ghv;
-      const int arow = tid / kLdT, acol = (tid % kLdT) * RT;
-      const bool aval = arow < nrows;
-      const _Float16* dsrc = p.D + (size_t)(row0 + (aval ? arow : 0)) * p.ldD + acol;
+      // descriptor rows through LDS in chunks of kKC k-steps, stored in
+      // A-fragment order (a fragment is 16 rows x 32 k; lane r + 16 g holds
+      // row r, k 8g..8g+7, so a wave reads one with one conflict-free
+      // 16-byte LDS read per lane); each wave loads kFr fragments of chunk
+      // c + 1 into registers while chunk c is multiplied, one barrier per
+      // chunk.  Rows past the tile read the tile's first row and are zeroed.
+      typedef __attribute__((address_space(1))) const half8 gh8;
+      const int NC = KT / kKC;
       const bool use0 = ntw > 0 && ct0 < NT, use1 = ntw > 1 && ct0 + 1 < NT;
       const int c0 = min(ct0, NT - 1), c1 = min(ct0 + 1, NT - 1);
-      halfv<RT> hz;
+      const _Float16* dsrc[kFr];
+      bool dval[kFr];
 #pragma unroll
-      for (int e = 0; e < RT; ++e) hz[e] = (_Float16)0.f;
-      halfv<RT> ar[kPf];
+      for (int m = 0; m < kFr; ++m) {
+        const int f = wave + kWaves * m, r = (f % RT) * 16 + lr;
+        dval[m] = r < nrows;
+        dsrc[m] = p.D + (size_t)(row0 + (dval[m] ? r : 0)) * p.ldD + (f / RT) * 32 + lg * 8;
+      }
+      half8 ar[kFr];
       half8 bq[kPf][2], bl[kPf][2];
 #pragma unroll
+      for (int m = 0; m < kFr; ++m) ar[m] = __builtin_nontemporal_load((gh8*)dsrc[m]);
+#pragma unroll
       for (int s = 0; s < kPf; ++s) {
-        ar[s] = __builtin_nontemporal_load((ghv*)(dsrc + s * 32));
         bq[s][0] = load_bfrag(L.wf, s, NT, c0, lane);
         bq[s][1] = load_bfrag(L.wf, s, NT, c1, lane);
         bl[s][0] = load_bfrag(L.wf_lo, s, NT, c0, lane);
         bl[s][1] = load_bfrag(L.wf_lo, s, NT, c1, lane);
       }
-      *reinterpret_cast<halfv<RT>*>(&as[0][arow][acol]) = aval ? ar[0] : hz;
-      __syncthreads();
-      for (int kt0 = 0; kt0 < KT; kt0 += kPf) {
 #pragma unroll
-        for (int s = 0; s < kPf; ++s) {
-          const int k = kt0 + s;
-          // stage k + 1 (loaded kPf - 1 steps ago) into the other LDS buffer
-          *reinterpret_cast<halfv<RT>*>(&as[(k + 1) & 1][arow][acol]) = aval ? ar[(s + 1) % kPf] : hz;
+      for (int m = 0; m < kFr; ++m) afr[0][wave + kWaves * m][lane] = dval[m] ? ar[m] : zero8();
+      __syncthreads();
+      for (int c = 0; c < NC; ++c) {
+        if (c + 1 < NC)
+#pragma unroll
+          for (int m = 0; m < kFr; ++m) ar[m] = __builtin_nontemporal_load((gh8*)(dsrc[m] + (c + 1) * kKC * 32));
+#pragma unroll
+        for (int s = 0; s < kKC; ++s) {
+          const int k = c * kKC + s;
           half8 a[RT];
 #pragma unroll
-          for (int i = 0; i < RT; ++i) a[i] = *reinterpret_cast<const half8*>(&as[k & 1][i * 16 + lr][lg * 8]);
+          for (int i = 0; i < RT; ++i) a[i] = afr[c & 1][s * RT + i][lane];
           if (use0)
 #pragma unroll
             for (int i = 0; i < RT; ++i) {
@@ -228,13 +239,17 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
               acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bl[s][1], acc[i][1], 0, 0, 0);
             }
           const int kn = min(k + kPf, KT - 1);
-          ar[s] = __builtin_nontemporal_load((ghv*)(dsrc + kn * 32));
           bq[s][0] = load_bfrag(L.wf, kn, NT, c0, lane);
           bq[s][1] = load_bfrag(L.wf, kn, NT, c1, lane);
           bl[s][0] = load_bfrag(L.wf_lo, kn, NT, c0, lane);
           bl[s][1] = load_bfrag(L.wf_lo, kn, NT, c1, lane);
-          __syncthreads();
         }
+        // chunk c + 1 goes to the buffer read in chunk c - 1 (every wave has
+        // passed the barrier after that chunk)
+        if (c + 1 < NC)
+#pragma unroll
+          for (int m = 0; m < kFr; ++m) afr[(c + 1) & 1][wave + kWaves * m][lane] = dval[m] ? ar[m] : zero8();
+        __syncthreads();
       }
     } else {
       gemm_lds_a<RT, true>(acc, &xs[src][0][0], kXld, L.wf, L.wf_lo, KT, NT, ct0, ntw, lane);
