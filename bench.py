@@ -230,7 +230,7 @@ def run_batched(args) -> int:
     dm = DeviceModel(m)
     x = torch.as_tensor(coord, device="cuda")
     eng = Engine(dm, natoms, atype, precision="mixed")
-    knobs = eng.autotune(x, box, reps=3)
+    knobs = eng.autotune(x, box, reps=8)
     s = torch.cuda.Stream()
     with torch.cuda.stream(s):
         eng.capture(x, box)
@@ -313,7 +313,7 @@ def run_copper(args) -> int:
     for prec, steps in (("mixed", args.steps), ("fp64", max(2, args.steps // 4))):
         print(f"copper {nloc} atoms: {prec} engine, {steps} steps", file=sys.stderr, flush=True)
         eng = Engine(dm, [nloc], atype, precision=prec)
-        knobs[prec] = eng.autotune(x, box, reps=2)
+        knobs[prec] = eng.autotune(x, box, reps=3)
         print(f"  tuned {knobs[prec]}", file=sys.stderr, flush=True)
         dt, stages = time_engine(eng, x, box, steps, min(args.warmup, 3))
         e, f, v = eng.compute(x, box)
@@ -436,7 +436,7 @@ def main() -> int:
     knobs = {}
     for prec in (args.precision, other):
         eng = Engine(dm, [nloc], atype, precision=prec)
-        knobs[prec] = eng.autotune(x, box, reps=3)
+        knobs[prec] = eng.autotune(x, box, reps=8)
         dt_eager, stages = time_engine(eng, x, box, args.steps, args.warmup)
         dt = dt_eager
         if prec == "mixed" and not args.eager:

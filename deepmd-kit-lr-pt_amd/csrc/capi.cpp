@@ -420,6 +420,7 @@ void dpmi_sea_engine_destroy(dpmi_sea_engine* engine) {
 int dpmi_sea_engine_compute(dpmi_sea_engine* engine, const double* coord, const double* box_host, double* energy,
                             double* force, double* virial, double* atom_energy, void* stream) {
   return guarded([&] {
+    if (!engine || !coord || !box_host || !energy || !force || !virial) throw std::invalid_argument("null argument");
     engine->impl->compute(coord, box_host, energy, force, virial, atom_energy, S(stream));
     return kOk;
   });

@@ -9,6 +9,7 @@
 // produced with counting sorts and prefix sums instead of the reference's
 // per-cell vectors:
 //   1. k_wrap_bin      one thread per local atom: wrap, bin, count per cell
+//                      (and clear the atom's force row for the force kernel)
 //   2. k_scan_cells    one workgroup per frame: exclusive scan of the counts,
 //                      then the prefix over extended cells (all atoms, and
 //                      ghosts only) -> nall
@@ -43,10 +44,15 @@ __device__ __forceinline__ int image_shift(int idx, int n) {
 __global__ void k_wrap_bin(const double* __restrict__ coord_in, const int* __restrict__ type_in,
                            const int* __restrict__ frame_of, const FrameGeom* __restrict__ geom, int ntot,
                            double* __restrict__ coord_ext, int* __restrict__ type_ext, int* __restrict__ map_ext,
-                           int* __restrict__ cell_of, int* __restrict__ cell_cnt, int do_wrap) {
+                           int* __restrict__ cell_of, int* __restrict__ cell_cnt, int do_wrap,
+                           double* __restrict__ force_clear) {
 #pragma clang fp contract(off)
   const int gidx = blockIdx.x * blockDim.x + threadIdx.x;
   if (gidx >= ntot) return;
+  if (force_clear) {  // the step's force accumulator, summed into from the force kernel on
+#pragma unroll
+    for (int d = 0; d < 3; ++d) force_clear[3 * (size_t)gidx + d] = 0.;
+  }
   const FrameGeom& g = geom[frame_of[gidx]];
   const int i = gidx - g.loc_off;
   double p[3] = {coord_in[3 * gidx + 0], coord_in[3 * gidx + 1], coord_in[3 * gidx + 2]};
@@ -281,7 +287,8 @@ void launch_build_extended(const CellWork& w, hipStream_t st) {
   launch_fill_int(w.cell_cnt, 0, w.ncells_tot, st);  // a kernel, not a memset node, under graph capture
   const int tpb = 256;
   hipLaunchKernelGGL(k_wrap_bin, dim3(ceil_div(w.ntot, tpb)), dim3(tpb), 0, st, w.coord_in, w.type_in, w.frame_of,
-                     w.geom, w.ntot, w.coord_ext, w.type_ext, w.map_ext, w.cell_of, w.cell_cnt, w.do_wrap);
+                     w.geom, w.ntot, w.coord_ext, w.type_ext, w.map_ext, w.cell_of, w.cell_cnt, w.do_wrap,
+                     w.force_clear);
   DPMI_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_scan_cells, dim3(w.nframes), dim3(1024), 0, st, w.geom, w.cell_cnt, w.cell_start, w.cell_fill,
                      w.xcnt, w.xstart, w.xgcnt, w.xgstart, w.nall, w.flag);

@@ -388,6 +388,7 @@ SeaEngine::SeaEngine(const SeaModel* m, int nframes_, const int* natoms_h, const
   DPMI_CHECK(hipHostMalloc((void**)&geom_pinned, sizeof(FrameGeom) * nframes, hipHostMallocDefault));
   d_geom = dalloc<FrameGeom>(nframes);
   d_flag = dalloc<int>(1);
+  DPMI_CHECK(hipMemset(d_flag, 0, sizeof(int)));  // status bits stay set until status() reads them
   d_nall = dalloc<int>(nframes);
   d_frame_ext_off = dalloc<int>(nframes);
   const size_t nn = (size_t)ntot * m->nnei;
@@ -470,10 +471,17 @@ SeaEngine::SeaEngine(const SeaModel* m, int nframes_, const int* natoms_h, const
   {
     std::vector<int> cxy, cst;
     frame_chunks(natoms.data(), nframes, cxy, cst);
-    nchunks = (int)cxy.size() / 2;
+    nchunks = (int)cxy.size() / 4;
     d_chunks = dupload(cxy.data(), cxy.size());
     d_chunk_start = dupload(cst.data(), cst.size());
     d_part = dalloc<double>((size_t)std::max(nchunks, 1) * 10);
+    // every frame holds a chunk: its last chunk combines it (one launch)
+    bool every = true;
+    for (int f = 0; f < nframes; ++f) every = every && cst[f + 1] > cst[f];
+    if (every) {
+      d_ticket = dalloc<int>(nframes);
+      DPMI_CHECK(hipMemset(d_ticket, 0, sizeof(int) * nframes));
+    }
   }
   d_outw.resize(nt, nullptr);
   for (int t = 0; t < nt; ++t) d_outw[t] = dupload(m->out_w_host[t].data(), m->out_w_host[t].size());
@@ -492,7 +500,7 @@ SeaEngine::~SeaEngine() {
   for (auto* p : {&d_frame_of, &d_center, &d_type, &d_pos, &d_perm, &d_flag, &d_nall, &d_frame_ext_off, &d_nlist,
                   &d_ncand, &d_cell_of, &d_sorted_loc, &d_cell_frame, &d_xcell_frame, &d_cell_cnt, &d_cell_start,
                   &d_cell_fill, &d_xcnt, &d_xstart, &d_xgcnt, &d_xgstart, &d_type_ext, &d_map_ext,
-                  &d_xframe_of_ext, &d_chunks, &d_chunk_start})
+                  &d_xframe_of_ext, &d_chunks, &d_chunk_start, &d_ticket})
     dfree(*p);
   dfree(d_cell_atoms);
   for (auto* p : {&d_em, &d_em_deriv, &d_rij, &d_net_deriv, &d_xyz, &d_D, &d_gD, &d_atom_e, &d_vpart,
@@ -569,7 +577,9 @@ void SeaEngine::prepare_geometry(const double* box_h) {
     d_map_ext = dalloc<int>(next);
     d_cell_atoms = dalloc<CellAtom>(next);
     d_coord_ext = dalloc<double>((size_t)next * 3);
-    d_force_ext = dalloc<double>((size_t)next * 3);
+    // the extended force buffer and the ghost fold serve only the row
+    // kernel; the aggregated kernel adds onto the owners directly
+    if (!recompute_slots) d_force_ext = dalloc<double>((size_t)next * 3);
     DPMI_CHECK(hipMemcpy(d_frame_ext_off, eoff.data(), sizeof(int) * nframes, hipMemcpyHostToDevice));
     cell_frame_h = cf;
     ncells_tot = ncells;
@@ -833,7 +843,38 @@ void SeaEngine::compute(const double* coord, const double* box_h, double* energy
     DPMI_CHECK(hipMemcpyAsync(d_geom, geom_pinned, sizeof(FrameGeom) * nframes, hipMemcpyHostToDevice, st));
     geom_uploaded = true;
   }
-  launch_fill_int(d_flag, 0, 1, st);
+  // force/virial arguments (the aggregated force kernel adds onto the owner
+  // rows of `force`, which the wrap kernel clears)
+  ForceRowsArgs fa;
+  if (precision == DPMI_FP64) {
+    fa.net_deriv = d_net_deriv;
+    fa.em_deriv = d_em_deriv;  // null: recomputed
+    fa.rij = d_rij;
+  } else {
+    fa.net_deriv = d_net_deriv_f;
+    fa.em_deriv = d_em_deriv_f;  // null: recomputed
+    fa.rij = d_rij_f;
+    fa.fp32 = true;
+  }
+  fa.coord_ext = d_coord_ext;
+  fa.type_ext = d_type_ext;
+  fa.dstd = m.d_dstd;
+  fa.rmin = (double)m.rcut_smth;
+  fa.rmax = (double)m.rcut;
+  fa.nlist = d_nlist;
+  fa.row_frame = d_frame_of;
+  fa.row_center = d_center;
+  fa.frame_ext_off = d_frame_ext_off;
+  fa.nrows = ntot;
+  fa.nnei = m.nnei;
+  fa.force_ext = d_force_ext;
+  fa.vpart = d_vpart;
+  fa.sorted_loc = d_sorted_loc;
+  fa.geom = d_geom;
+  fa.rows = tune.force_rows;
+  fa.map_ext = d_map_ext;
+  fa.force = force;
+  const bool owner_force = force_rows_aggregated(fa);
 
   CellWork w;
   w.nframes = nframes;
@@ -862,6 +903,7 @@ void SeaEngine::compute(const double* coord, const double* box_h, double* energy
   w.xgstart = d_xgstart;
   w.nall = d_nall;
   w.flag = d_flag;
+  w.force_clear = owner_force ? force : nullptr;
   launch_build_extended(w, st);
   mark(st, stage++);
 
@@ -882,39 +924,17 @@ void SeaEngine::compute(const double* coord, const double* box_h, double* energy
   else
     compute_mixed_descriptor_fitting(st, stage);
 
-  launch_fill_double(d_force_ext, 0.0, next_tot * 3, st);
-  ForceRowsArgs fa;
-  if (precision == DPMI_FP64) {
-    fa.net_deriv = d_net_deriv;
-    fa.em_deriv = d_em_deriv;  // null: recomputed
-    fa.rij = d_rij;
+  if (owner_force) {
+    launch_force_virial_rows(fa, st);
   } else {
-    fa.net_deriv = d_net_deriv_f;
-    fa.em_deriv = d_em_deriv_f;  // null: recomputed
-    fa.rij = d_rij_f;
-    fa.fp32 = true;
+    launch_fill_double(d_force_ext, 0.0, next_tot * 3, st);
+    launch_force_virial_rows(fa, st);
+    launch_fill_double(force, 0.0, (long long)ntot * 3, st);
+    launch_fold_ghost_forces(d_force_ext, d_map_ext, d_xframe_of_ext, d_geom, d_nall, next_tot, force, st);
   }
-  fa.coord_ext = d_coord_ext;
-  fa.type_ext = d_type_ext;
-  fa.dstd = m.d_dstd;
-  fa.rmin = (double)m.rcut_smth;
-  fa.rmax = (double)m.rcut;
-  fa.nlist = d_nlist;
-  fa.row_frame = d_frame_of;
-  fa.row_center = d_center;
-  fa.frame_ext_off = d_frame_ext_off;
-  fa.nrows = ntot;
-  fa.nnei = m.nnei;
-  fa.force_ext = d_force_ext;
-  fa.vpart = d_vpart;
-  fa.sorted_loc = d_sorted_loc;
-  fa.geom = d_geom;
-  fa.rows = tune.force_rows;
-  launch_force_virial_rows(fa, st);
-  launch_fill_double(force, 0.0, (long long)ntot * 3, st);
-  launch_fold_ghost_forces(d_force_ext, d_map_ext, d_xframe_of_ext, d_geom, d_nall, next_tot, force, st);
   mark(st, stage++);
-  launch_frame_reduce(d_atom_e, d_vpart, d_chunks, nchunks, d_chunk_start, nframes, d_part, energy, virial, st);
+  launch_frame_reduce(d_atom_e, d_vpart, d_chunks, nchunks, d_chunk_start, nframes, d_part, d_ticket, energy, virial,
+                      st);
   if (atom_energy)
     launch_copy_double(atom_energy, d_atom_e, ntot, st);
   mark(st, stage++);
@@ -942,9 +962,11 @@ void SeaEngine::load_cached_tune() {
   if (it != tune_cache().end()) tune = it->second;
 }
 
-// Times `reps` steps for each of three candidate values of every knob (the
-// knobs act on different stages, so all are varied together) and keeps,
-// per stage, the fastest value; the result is cached for the size bucket.
+// Times `reps` rounds of one step per candidate (three candidate values of
+// every knob, varied together since the knobs act on different stages; the
+// candidates interleaved so that clock and cache drift spread over all of
+// them) and keeps, per stage, the value with the fastest single step; the
+// result is cached for the size bucket.
 void SeaEngine::autotune(const double* coord, const double* box_h, int reps, hipStream_t st) {
   const int env_c[3] = {128, 256, 512}, fwd_c[3] = {1, 2, 4}, bwd_c[3] = {1, 2, 4}, frc_c[3] = {4, 8, 16},
             fit_c[3] = {32, 64, 64};
@@ -972,24 +994,30 @@ void SeaEngine::autotune(const double* coord, const double* box_h, int reps, hip
   };
   try {
     compute(coord, box_h, e, f, v, nullptr, st);  // geometry and buffers
-    for (int c = 0; c < ncand; ++c) {
-      const int k = c < 3 ? c : 1;
-      tune.env_tpb = env_c[k];
-      tune.fwd_waves = fwd_c[k];
-      tune.bwd_waves = bwd_c[k];
-      tune.force_rows = frc_c[k];
-      tune.fit_rows = fit_c[k];
-      tune.embed_mfma = c < 3 ? 0 : 3;
-      timing = true;
-      reset_timing();
-      for (int r = 0; r < std::max(reps, 1); ++r) compute(coord, box_h, e, f, v, nullptr, st);
-      std::vector<double> ms(names.size(), 0.0);
-      timing_report(ms.data(), (int)ms.size(), nullptr, 0);
-      t_env[c] = ms[idx("env_mat")];
-      t_fwd[c] = ms[idx("embed_fwd")];
-      t_bwd[c] = ms[idx("embed_bwd")];
-      t_frc[c] = ms[idx("force_virial")];
-      t_fit[c] = idx("fitting") >= 0 ? ms[idx("fitting")] : 0.0;
+    std::vector<double> ms(names.size(), 0.0);
+    for (int r = 0; r < std::max(reps, 1); ++r) {
+      for (int c = 0; c < ncand; ++c) {
+        const int k = c < 3 ? c : 1;
+        tune.env_tpb = env_c[k];
+        tune.fwd_waves = fwd_c[k];
+        tune.bwd_waves = bwd_c[k];
+        tune.force_rows = frc_c[k];
+        tune.fit_rows = fit_c[k];
+        tune.embed_mfma = c < 3 ? 0 : 3;
+        timing = true;
+        reset_timing();
+        compute(coord, box_h, e, f, v, nullptr, st);
+        timing_report(ms.data(), (int)ms.size(), nullptr, 0);
+        auto keep = [&](double* t, const char* n) {
+          const double x = idx(n) >= 0 ? ms[idx(n)] : 0.0;
+          t[c] = r == 0 ? x : std::min(t[c], x);
+        };
+        keep(t_env, "env_mat");
+        keep(t_fwd, "embed_fwd");
+        keep(t_bwd, "embed_bwd");
+        keep(t_frc, "force_virial");
+        keep(t_fit, "fitting");
+      }
     }
   } catch (...) {
     timing = was_timing;
@@ -1060,6 +1088,7 @@ int SeaEngine::status() {
   DPMI_CHECK(hipStreamSynchronize(last_stream));
   int flag = 0;
   DPMI_CHECK(hipMemcpy(&flag, d_flag, sizeof(int), hipMemcpyDeviceToHost));
+  if (flag) DPMI_CHECK(hipMemset(d_flag, 0, sizeof(int)));
   if (flag & kFlagNlistCapacity) return DPMI_ERR_NLIST_CAPACITY;
   if (flag & kFlagGhostCapacity) return DPMI_ERR_MEMORY;
   if (flag & (kFlagIndexRange | kFlagTypeRange | kFlagHalfRange)) return DPMI_ERR_RANGE;

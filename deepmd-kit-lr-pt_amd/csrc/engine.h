@@ -143,7 +143,7 @@ struct SeaEngine {
   EmbedArgs embed_args() const;
   void compute_fp64_descriptor_fitting(hipStream_t st, int& stage);
   int nchunks = 0;
-  int *d_chunks = nullptr, *d_chunk_start = nullptr;
+  int *d_chunks = nullptr, *d_chunk_start = nullptr, *d_ticket = nullptr;
   double* d_part = nullptr;
 };
 

@@ -252,12 +252,13 @@ __global__ void __launch_bounds__(256) k_force_virial_rows(const TI* __restrict_
 // their rate, not bandwidth, bounds the direct scatter.  Here one workgroup
 // takes R centre rows in cell order (positions p of the cell-sorted atom
 // list, so the rows' neighbour sets overlap), accumulates every force term
-// into an LDS hash keyed by extended atom (FP64 LDS atomics, linear
-// probing), and then issues one global atomic per distinct atom and
-// component.  The table has 2^HB entries, the smallest power of two
-// (>= 2^10) holding the R (nnei + 1) distinct keys a workgroup can insert,
-// so every probe sequence ends, and smaller row groups leave LDS for more
-// workgroups per compute unit.
+// into an LDS hash keyed by the owner atom (a ghost image counts for the
+// local atom it copies: map_ext), and then issues one global atomic per
+// distinct owner and component straight into the packed force rows, so no
+// extended force buffer and no ghost fold are needed.  The table has 2^HB
+// entries, the smallest power of two (>= 2^10) holding the R (nnei + 1)
+// distinct keys a workgroup can insert, so every probe sequence ends, and
+// smaller row groups leave LDS for more workgroups per compute unit.
 constexpr int kHashBitsMax = 12, kHashSize = 1 << kHashBitsMax;
 
 template <int HB>
@@ -304,7 +305,8 @@ __global__ void __launch_bounds__(1024) k_force_virial_hash(
     const int* __restrict__ row_frame, const int* __restrict__ row_center, const int* __restrict__ frame_ext_off,
     const FrameGeom* __restrict__ geom, const int* __restrict__ sorted_loc, long long nrows, int nnei, int R,
     const double* __restrict__ coord_ext, const int* __restrict__ type_ext, const double* __restrict__ dstd,
-    double rmin, double rmax, double* __restrict__ force_ext, double* __restrict__ vpart) {
+    double rmin, double rmax, const int* __restrict__ map_ext, double* __restrict__ force,
+    double* __restrict__ vpart) {
   constexpr int NH = 1 << HB;
   __shared__ int hkey[NH];
   __shared__ double hval[NH][3];
@@ -319,6 +321,7 @@ __global__ void __launch_bounds__(1024) k_force_virial_hash(
     const int f = row_frame[p];
     const long long g = geom[f].loc_off + sorted_loc[p];  // packed row of the p-th atom in cell order
     const int eo = frame_ext_off[f];
+    const int lo = geom[f].loc_off;
     const int i = row_center[g];
     double ci[3] = {0., 0., 0.};
     const double* sdrow = dstd;
@@ -340,7 +343,7 @@ __global__ void __launch_bounds__(1024) k_force_virial_hash(
     for (int s0 = 0; s0 < nnei; s0 += SU * kWave) {
       TI n4[SU][4], e12[SU][12], r3[SU][3];
       double cj[SU][3];
-      int jj[SU], ss[SU];
+      int jj[SU], ss[SU], own[SU];
 #pragma unroll
       for (int u = 0; u < SU; ++u) {
         const int s = s0 + u * kWave + lane;
@@ -350,8 +353,9 @@ __global__ void __launch_bounds__(1024) k_force_virial_hash(
         jj[u] = ok ? nlist[b] : -1;
 #pragma unroll
         for (int a = 0; a < 4; ++a) n4[u][a] = nd[b * 4 + a];
+        const size_t e = (size_t)eo + (jj[u] >= 0 ? jj[u] : i);
+        own[u] = map_ext[e];
         if constexpr (RECOMP) {
-          const size_t e = (size_t)eo + (jj[u] >= 0 ? jj[u] : i);
 #pragma unroll
           for (int d = 0; d < 3; ++d) cj[u][d] = coord_ext[e * 3 + d];
         } else {
@@ -368,7 +372,7 @@ __global__ void __launch_bounds__(1024) k_force_virial_hash(
         TI fs[3], v[9];
         slot_force(n4[u], e12[u], fs);
         slot_virial(n4[u], e12[u], r3[u], v);
-        const int h = hash_slot<HB>(hkey, eo + jj[u]);
+        const int h = hash_slot<HB>(hkey, lo + own[u]);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
           fi[d] -= fs[d];
@@ -383,7 +387,7 @@ __global__ void __launch_bounds__(1024) k_force_virial_hash(
 #pragma unroll
     for (int k = 0; k < 9; ++k) vs[k] = wave_sum(vs[k]);
     if (lane == 0) {
-      const int h = hash_slot<HB>(hkey, eo + i);
+      const int h = hash_slot<HB>(hkey, lo + map_ext[eo + i]);
 #pragma unroll
       for (int d = 0; d < 3; ++d) atomicAdd(&hval[h][d], fi[d]);
     }
@@ -400,7 +404,7 @@ __global__ void __launch_bounds__(1024) k_force_virial_hash(
     const int key = hkey[h];
     if (key >= 0) {
 #pragma unroll
-      for (int d = 0; d < 3; ++d) atomicAdd(force_ext + (size_t)key * 3 + d, hval[h][d]);
+      for (int d = 0; d < 3; ++d) atomicAdd(force + (size_t)key * 3 + d, hval[h][d]);
     }
   }
 }
@@ -422,16 +426,32 @@ __global__ void k_fold_ghosts(const double* __restrict__ force_ext, const int* _
 // Two-level compensated per-frame sums of the atomic energies and of the
 // per-row virial partials.  Level 1: one workgroup per chunk of at most
 // kChunkRows rows of one frame (chunk list built on the host): each thread
-// sums its rows with compensation, the block combines the thread sums with
-// compensation into 10 values.  Level 2: one wave per frame combines its
-// chunks.
+// sums its rows with compensation, the block combines the thread sums by a
+// pairwise tree into 10 values.  Level 2: the frame's chunk partials are
+// summed with compensation in chunk order, either by the last chunk of the
+// frame to finish (a per-frame ticket counted with a device-scope atomic,
+// fences on both sides, reset to zero for the next call) or, without
+// tickets, by one wave per frame in a second launch.  Both give the same
+// bits.
 constexpr int kChunkRows = 1024;
+
+__device__ __forceinline__ void frame_total(const double* __restrict__ part, int c0, int c1, int q,
+                                            double* __restrict__ energy, double* __restrict__ virial, int f) {
+  KahanD t;
+  for (int c = c0; c < c1; ++c) t.add(part[(size_t)c * 10 + q]);
+  if (q == 0)
+    energy[f] = t.value();
+  else
+    virial[f * 9 + q - 1] = t.value();
+}
 
 __global__ void __launch_bounds__(256) k_chunk_reduce(const double* __restrict__ atom_e,
                                                       const double* __restrict__ vpart,
-                                                      const int2* __restrict__ chunks, double* __restrict__ part) {
+                                                      const int4* __restrict__ chunks, double* __restrict__ part,
+                                                      int* __restrict__ ticket, const int* __restrict__ chunk_start,
+                                                      double* __restrict__ energy, double* __restrict__ virial) {
   __shared__ double sh[10][256];
-  const int2 ch = chunks[blockIdx.x];  // (first row, row count)
+  const int4 ch = chunks[blockIdx.x];  // (first row, row count, frame, chunks of the frame)
   KahanD acc[10];
   for (int r = threadIdx.x; r < ch.y; r += blockDim.x) {
     const size_t row = (size_t)ch.x + r;
@@ -450,7 +470,17 @@ __global__ void __launch_bounds__(256) k_chunk_reduce(const double* __restrict__
       for (int k = 0; k < 10; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + h];
     __syncthreads();
   }
-  if (threadIdx.x < 10) part[(size_t)blockIdx.x * 10 + threadIdx.x] = sh[threadIdx.x][0];
+  if (threadIdx.x >= kWave) return;  // wave 0 writes the partials and takes the ticket
+  const int q = threadIdx.x;
+  if (q < 10) part[(size_t)blockIdx.x * 10 + q] = sh[q][0];
+  if (!ticket) return;
+  __threadfence();  // this chunk's partials before its ticket
+  int last = 0;
+  if (q == 0) last = atomicAdd(&ticket[ch.z], 1) == ch.w - 1;
+  if (!__shfl(last, 0, kWave)) return;
+  __threadfence();  // every ticket of the frame before its partials are read
+  if (q < 10) frame_total(part, chunk_start[ch.z], chunk_start[ch.z + 1], q, energy, virial, ch.z);
+  if (q == 0) ticket[ch.z] = 0;
 }
 
 __global__ void k_frame_combine(const double* __restrict__ part, const int* __restrict__ chunk_start, int nframes,
@@ -458,12 +488,7 @@ __global__ void k_frame_combine(const double* __restrict__ part, const int* __re
   const int f = blockIdx.x;
   const int q = threadIdx.x;
   if (f >= nframes || q >= 10) return;
-  KahanD t;
-  for (int c = chunk_start[f]; c < chunk_start[f + 1]; ++c) t.add(part[(size_t)c * 10 + q]);
-  if (q == 0)
-    energy[f] = t.value();
-  else
-    virial[f * 9 + q - 1] = t.value();
+  frame_total(part, chunk_start[f], chunk_start[f + 1], q, energy, virial, f);
 }
 
 }  // namespace
@@ -515,23 +540,27 @@ void launch_prod_virial_grad_a(T* grad_net, const T* grad, const T* env_deriv, c
   DPMI_LAUNCH_CHECK();
 }
 
-#define DPMI_FORCE_INST(T)                                                                                          \
-  template void launch_prod_force_a<T>(T*, const T*, const T*, const int*, int, int, int, int, hipStream_t);        \
-  template void launch_prod_virial_a<T>(double*, T*, const T*, const T*, const T*, const int*, int, int, int,      \
-                                        hipStream_t);                                                               \
-  template void launch_reduce_rows9<T>(T*, const double*, int, hipStream_t);                                        \
-  template void launch_prod_force_grad_a<T>(T*, const T*, const T*, const int*, int, int, int, hipStream_t);        \
+#define DPMI_FORCE_INST(T)                                                                                            \
+  template void launch_prod_force_a<T>(T*, const T*, const T*, const int*, int, int, int, int, hipStream_t);          \
+  template void launch_prod_virial_a<T>(double*, T*, const T*, const T*, const T*, const int*, int, int, int,         \
+                                        hipStream_t);                                                                 \
+  template void launch_reduce_rows9<T>(T*, const double*, int, hipStream_t);                                          \
+  template void launch_prod_force_grad_a<T>(T*, const T*, const T*, const int*, int, int, int, hipStream_t);          \
   template void launch_prod_virial_grad_a<T>(T*, const T*, const T*, const T*, const int*, int, int, hipStream_t);
 DPMI_FORCE_INST(double)
 DPMI_FORCE_INST(float)
 
 bool force_recompute_supported(int nnei) { return nnei + 1 <= kHashSize; }
 
+bool force_rows_aggregated(const ForceRowsArgs& a) {
+  return a.sorted_loc && a.geom && a.map_ext && a.force && a.nnei + 1 <= kHashSize;
+}
+
 void launch_force_virial_rows(const ForceRowsArgs& a, hipStream_t st) {
   if (a.nrows <= 0) return;
-  if (!a.em_deriv && !(a.sorted_loc && a.geom && a.nnei + 1 <= kHashSize))
+  if (!a.em_deriv && !force_rows_aggregated(a))
     throw std::invalid_argument("recomputed slot derivatives need the LDS-aggregated force kernel");
-  if (a.sorted_loc && a.geom && a.nnei + 1 <= kHashSize) {
+  if (force_rows_aggregated(a)) {
     const int R = std::max(1, std::min(a.rows > 0 ? a.rows : 16, kHashSize / (a.nnei + 1)));
     const dim3 grid(ceil_div(a.nrows, R)), block(R * kWave);
     const int keys = R * (a.nnei + 1);
@@ -539,23 +568,25 @@ void launch_force_virial_rows(const ForceRowsArgs& a, hipStream_t st) {
 #define DPMI_HASH_LAUNCH(HB)                                                                                          \
   do {                                                                                                                \
     if (a.fp32 && !a.em_deriv)                                                                                        \
-      hipLaunchKernelGGL((k_force_virial_hash<float, true, HB>), grid, block, 0, st, (const float*)a.net_deriv,        \
-                         nullptr, nullptr, a.nlist, a.row_frame, a.row_center, a.frame_ext_off, a.geom, a.sorted_loc,  \
-                         a.nrows, a.nnei, R, a.coord_ext, a.type_ext, a.dstd, a.rmin, a.rmax, a.force_ext, a.vpart);  \
+      hipLaunchKernelGGL((k_force_virial_hash<float, true, HB>), grid, block, 0, st, (const float*)a.net_deriv,       \
+                         nullptr, nullptr, a.nlist, a.row_frame, a.row_center, a.frame_ext_off, a.geom, a.sorted_loc, \
+                         a.nrows, a.nnei, R, a.coord_ext, a.type_ext, a.dstd, a.rmin, a.rmax, a.map_ext, a.force,     \
+                         a.vpart);                                                                                    \
     else if (!a.em_deriv)                                                                                             \
-      hipLaunchKernelGGL((k_force_virial_hash<double, true, HB>), grid, block, 0, st, (const double*)a.net_deriv,      \
-                         nullptr, nullptr, a.nlist, a.row_frame, a.row_center, a.frame_ext_off, a.geom, a.sorted_loc,  \
-                         a.nrows, a.nnei, R, a.coord_ext, a.type_ext, a.dstd, a.rmin, a.rmax, a.force_ext, a.vpart);  \
+      hipLaunchKernelGGL((k_force_virial_hash<double, true, HB>), grid, block, 0, st, (const double*)a.net_deriv,     \
+                         nullptr, nullptr, a.nlist, a.row_frame, a.row_center, a.frame_ext_off, a.geom, a.sorted_loc, \
+                         a.nrows, a.nnei, R, a.coord_ext, a.type_ext, a.dstd, a.rmin, a.rmax, a.map_ext, a.force,     \
+                         a.vpart);                                                                                    \
     else if (a.fp32)                                                                                                  \
-      hipLaunchKernelGGL((k_force_virial_hash<float, false, HB>), grid, block, 0, st, (const float*)a.net_deriv,       \
-                         (const float*)a.em_deriv, (const float*)a.rij, a.nlist, a.row_frame, a.row_center,            \
+      hipLaunchKernelGGL((k_force_virial_hash<float, false, HB>), grid, block, 0, st, (const float*)a.net_deriv,      \
+                         (const float*)a.em_deriv, (const float*)a.rij, a.nlist, a.row_frame, a.row_center,           \
                          a.frame_ext_off, a.geom, a.sorted_loc, a.nrows, a.nnei, R, nullptr, nullptr, nullptr, 0., 0., \
-                         a.force_ext, a.vpart);                                                                       \
+                         a.map_ext, a.force, a.vpart);                                                                \
     else                                                                                                              \
-      hipLaunchKernelGGL((k_force_virial_hash<double, false, HB>), grid, block, 0, st, (const double*)a.net_deriv,     \
-                         (const double*)a.em_deriv, (const double*)a.rij, a.nlist, a.row_frame, a.row_center,          \
+      hipLaunchKernelGGL((k_force_virial_hash<double, false, HB>), grid, block, 0, st, (const double*)a.net_deriv,    \
+                         (const double*)a.em_deriv, (const double*)a.rij, a.nlist, a.row_frame, a.row_center,         \
                          a.frame_ext_off, a.geom, a.sorted_loc, a.nrows, a.nnei, R, nullptr, nullptr, nullptr, 0., 0., \
-                         a.force_ext, a.vpart);                                                                       \
+                         a.map_ext, a.force, a.vpart);                                                                \
   } while (0)
     if (hb == 10)
       DPMI_HASH_LAUNCH(10);
@@ -587,29 +618,33 @@ void launch_fold_ghost_forces(const double* force_ext, const int* map_ext, const
   DPMI_LAUNCH_CHECK();
 }
 
-void frame_chunks(const int* natoms, int nframes, std::vector<int>& chunks_xy, std::vector<int>& chunk_start) {
-  chunks_xy.clear();
+void frame_chunks(const int* natoms, int nframes, std::vector<int>& chunks4, std::vector<int>& chunk_start) {
+  chunks4.clear();
   chunk_start.assign(nframes + 1, 0);
   int off = 0;
   for (int f = 0; f < nframes; ++f) {
-    chunk_start[f] = (int)chunks_xy.size() / 2;
+    chunk_start[f] = (int)chunks4.size() / 4;
+    const int nc = (natoms[f] + kChunkRows - 1) / kChunkRows;
     for (int r = 0; r < natoms[f]; r += kChunkRows) {
-      chunks_xy.push_back(off + r);
-      chunks_xy.push_back(std::min(kChunkRows, natoms[f] - r));
+      chunks4.push_back(off + r);
+      chunks4.push_back(std::min(kChunkRows, natoms[f] - r));
+      chunks4.push_back(f);
+      chunks4.push_back(nc);
     }
     off += natoms[f];
   }
-  chunk_start[nframes] = (int)chunks_xy.size() / 2;
+  chunk_start[nframes] = (int)chunks4.size() / 4;
 }
 
-void launch_frame_reduce(const double* atom_e, const double* vpart, const int* chunks_xy, int nchunks,
-                         const int* chunk_start, int nframes, double* part, double* energy, double* virial,
-                         hipStream_t st) {
+void launch_frame_reduce(const double* atom_e, const double* vpart, const int* chunks4, int nchunks,
+                         const int* chunk_start, int nframes, double* part, int* ticket, double* energy,
+                         double* virial, hipStream_t st) {
   if (nchunks > 0) {
     hipLaunchKernelGGL(k_chunk_reduce, dim3(nchunks), dim3(256), 0, st, atom_e, vpart,
-                       reinterpret_cast<const int2*>(chunks_xy), part);
+                       reinterpret_cast<const int4*>(chunks4), part, ticket, chunk_start, energy, virial);
     DPMI_LAUNCH_CHECK();
   }
+  if (ticket) return;
   hipLaunchKernelGGL(k_frame_combine, dim3(nframes), dim3(kWave), 0, st, part, chunk_start, nframes, energy, virial);
   DPMI_LAUNCH_CHECK();
 }

@@ -43,6 +43,7 @@ struct CellWork {
   int* xgstart = nullptr;
   int* nall = nullptr;               // [nframes]
   int* flag = nullptr;               // device status word
+  double* force_clear = nullptr;     // [ntot, 3] zeroed by the wrap kernel when set
 };
 void launch_build_extended(const CellWork& w, hipStream_t st);
 // one frame, caller buffers; returns a dpmi_status (capi.cpp)
@@ -210,14 +211,21 @@ struct ForceRowsArgs {
   const int* frame_ext_off = nullptr;
   long long nrows = 0;
   int nnei = 0;
-  double* force_ext = nullptr;        // [sum ext_cap, 3] (zeroed by caller)
+  double* force_ext = nullptr;        // [sum ext_cap, 3] (zeroed by caller; row kernel only)
   double* vpart = nullptr;            // [nrows, 9]
   // cell order of the rows (per-frame local indices, packed like the rows)
   // and the frame layout; when set, the LDS-aggregated kernel is used
   const int* sorted_loc = nullptr;
   const FrameGeom* geom = nullptr;
   int rows = 0;  // rows per workgroup of the LDS-aggregated kernel (0: 16)
+  // the LDS-aggregated kernel adds every term straight onto the owner atom
+  // (ghost images folded through map_ext) in force [ntot, 3] (zeroed by caller)
+  const int* map_ext = nullptr;
+  double* force = nullptr;
 };
+// whether launch_force_virial_rows takes the LDS-aggregated kernel (which
+// writes owner forces) for these arguments
+bool force_rows_aggregated(const ForceRowsArgs& a);
 // whether launch_force_virial_rows can rebuild the slot derivatives of rows
 // with nnei slots (the LDS-aggregated FP32 kernel applies)
 bool force_recompute_supported(int nnei);
@@ -241,11 +249,15 @@ void launch_fold_ghost_forces(const double* force_ext, const int* map_ext, const
                               const FrameGeom* geom, const int* nall, long long next_tot, double* force,
                               hipStream_t st);
 // per-frame energy and virial (Neumaier-compensated, two levels): chunks of
-// at most 1024 rows of one frame -> partials [nchunks, 10] -> frame totals
-void frame_chunks(const int* natoms, int nframes, std::vector<int>& chunks_xy, std::vector<int>& chunk_start);
-void launch_frame_reduce(const double* atom_e, const double* vpart, const int* chunks_xy, int nchunks,
-                         const int* chunk_start, int nframes, double* part, double* energy, double* virial,
-                         hipStream_t st);
+// at most 1024 rows of one frame -> partials [nchunks, 10] -> frame totals.
+// A chunk record is 4 ints: first row, rows, frame, chunks of that frame.
+// With `ticket` ([nframes] ints, zero before the first call; the kernel
+// leaves them zero) and every frame holding a chunk, the last chunk of a
+// frame to finish combines it in the same launch.
+void frame_chunks(const int* natoms, int nframes, std::vector<int>& chunks4, std::vector<int>& chunk_start);
+void launch_frame_reduce(const double* atom_e, const double* vpart, const int* chunks4, int nchunks,
+                         const int* chunk_start, int nframes, double* part, int* ticket, double* energy,
+                         double* virial, hipStream_t st);
 
 // ----------------------------------------------------------- fitting.hip --
 // y = act(z + b); x_next = y * idt + residual(x); stores y (for backward)

@@ -195,8 +195,6 @@ void dpmi_sea_engine_destroy(dpmi_sea_engine* engine);
 int dpmi_sea_engine_compute(dpmi_sea_engine* engine, const double* coord, const double* box_host,
                             double* energy, double* force, double* virial, double* atom_energy, void* stream);
 
-/* Synchronises the stream of the last compute and returns its device-side
- * status (DPMI_OK, DPMI_ERR_NLIST_CAPACITY, DPMI_ERR_MEMORY, DPMI_ERR_RANGE). */
 /* hipGraph of one step (no counterpart in the reference; the batched-launch
  * pattern): runs one eager step, then captures the same step on `stream`
  * (must not be the NULL stream).  dpmi_sea_engine_graph_launch replays it:
@@ -227,6 +225,11 @@ int dpmi_sea_engine_autotune(dpmi_sea_engine* engine, const double* coord, const
 int dpmi_sea_engine_tuning(dpmi_sea_engine* engine, int* knobs_host);
 int dpmi_sea_engine_set_tuning(dpmi_sea_engine* engine, const int* knobs_host);
 
+/* Synchronises the stream of the last compute and returns the device-side
+ * status of the computes (and graph replays) since the previous status call
+ * (DPMI_OK, DPMI_ERR_NLIST_CAPACITY, DPMI_ERR_MEMORY, DPMI_ERR_RANGE); the
+ * bits are cleared when read, so steps between two calls are not reset one
+ * by one. */
 int dpmi_sea_engine_status(dpmi_sea_engine* engine);
 
 /* Copies an intermediate of the last compute of frame `frame` to host memory
