@@ -133,10 +133,11 @@ int copy_coord_run(FrameGeom g, double* out_c, int* out_t, int* mapping, int* na
     g.xcell_off = 0;
     const int nc = dpmi::geom_local_cells(g), nx = dpmi::geom_ext_cells(g) + 1;
     auto& sc = scratch();
-    // [FrameGeom][pad to 16][CellAtom x mem_nall][ints]
+    // [FrameGeom][pad to 16][CellAtom x mem_nall][ints: 5 of nloc, 3 of nc,
+    // 5 of nx, nall, flag]
     const size_t rec_off = (sizeof(dpmi::FrameGeom) + 15) / 16 * 16;
     char* base = static_cast<char*>(sc.get("copy_coord", rec_off + sizeof(dpmi::CellAtom) * (size_t)mem_nall +
-                                                              sizeof(int) * ((size_t)4 * nloc + 3 * nc + 4 * nx + 8)));
+                                                              sizeof(int) * ((size_t)5 * nloc + 3 * nc + 5 * nx + 8)));
     dpmi::CellAtom* recs = reinterpret_cast<dpmi::CellAtom*>(base + rec_off);
     int* ip = reinterpret_cast<int*>(recs + mem_nall);
     auto take = [&](size_t n) {
@@ -164,9 +165,10 @@ int copy_coord_run(FrameGeom g, double* out_c, int* out_t, int* mapping, int* na
     w.map_ext = mapping;
     w.cell_atoms = recs;
     w.cell_of = take(nloc);
+    w.cell_slot = take(nloc);
     w.cell_cnt = take(nc);
     w.cell_start = take(nc);
-    w.cell_fill = take(nc);
+    w.cell_list = take(nloc);
     w.sorted_loc = take(nloc);
     w.xcnt = take(nx);
     w.xstart = take(nx);

@@ -9,15 +9,19 @@
 // produced with counting sorts and prefix sums instead of the reference's
 // per-cell vectors:
 //   1. k_wrap_bin      one thread per local atom: wrap, bin, count per cell
-//                      (and clear the atom's force row for the force kernel)
+//                      (the count's old value is the atom's slot in its
+//                      cell; the atom's force row is cleared for the force
+//                      kernel when asked)
 //   2. k_scan_cells    one workgroup per frame: exclusive scan of the counts,
 //                      then the prefix over extended cells (all atoms, and
 //                      ghosts only) -> nall
-//   3. k_scatter_cells one thread per local atom: place into its cell
-//   4. k_sort_cells    one wave per cell: ascending index inside the cell
-//   5. k_emit_ghosts   one wave per extended cell: write ghost images and
+//   3. k_scatter_cells one thread per local atom: place into its cell slot
+//   4. k_emit_ghosts   one wave per extended cell: sort the source cell's
+//                      atoms by index in registers, write ghost images and
 //                      the extended cells' atom records (coordinates,
-//                      extended index, type) in cell order
+//                      extended index, type) in cell order; the wave of a
+//                      cell's own (non-ghost) image stores the cell's sorted
+//                      atom list and clears its count for the next build
 // Floating-point expressions that decide cell membership and image
 // coordinates are evaluated without contraction, in the reference's order.
 #include "dpmi_common.h"
@@ -44,8 +48,8 @@ __device__ __forceinline__ int image_shift(int idx, int n) {
 __global__ void k_wrap_bin(const double* __restrict__ coord_in, const int* __restrict__ type_in,
                            const int* __restrict__ frame_of, const FrameGeom* __restrict__ geom, int ntot,
                            double* __restrict__ coord_ext, int* __restrict__ type_ext, int* __restrict__ map_ext,
-                           int* __restrict__ cell_of, int* __restrict__ cell_cnt, int do_wrap,
-                           double* __restrict__ force_clear) {
+                           int* __restrict__ cell_of, int* __restrict__ cell_slot, int* __restrict__ cell_cnt,
+                           int do_wrap, double* __restrict__ force_clear) {
 #pragma clang fp contract(off)
   const int gidx = blockIdx.x * blockDim.x + threadIdx.x;
   if (gidx >= ntot) return;
@@ -85,7 +89,7 @@ __global__ void k_wrap_bin(const double* __restrict__ coord_in, const int* __res
   }
   const int c = (id[0] * g.nc[1] + id[1]) * g.nc[2] + id[2];
   cell_of[gidx] = c;
-  atomicAdd(&cell_cnt[g.cell_off + c], 1);
+  cell_slot[gidx] = atomicAdd(&cell_cnt[g.cell_off + c], 1);
 }
 
 // Block-wide exclusive scan of n ints (one workgroup, blockDim = 1024): each
@@ -130,50 +134,16 @@ __device__ int block_exclusive_scan(const int* __restrict__ in, int* __restrict_
   return total;
 }
 
+// Places every atom at its slot of its cell (slots from the counting
+// atomics, so the order inside a cell is arbitrary until k_emit_ghosts).
 __global__ void k_scatter_cells(const int* __restrict__ frame_of, const FrameGeom* __restrict__ geom, int ntot,
-                                const int* __restrict__ cell_of, const int* __restrict__ cell_start,
-                                int* __restrict__ cell_fill, int* __restrict__ sorted_loc) {
+                                const int* __restrict__ cell_of, const int* __restrict__ cell_slot,
+                                const int* __restrict__ cell_start, int* __restrict__ cell_list) {
   const int gidx = blockIdx.x * blockDim.x + threadIdx.x;
   if (gidx >= ntot) return;
   const FrameGeom& g = geom[frame_of[gidx]];
   const int c = g.cell_off + cell_of[gidx];
-  const int slot = atomicAdd(&cell_fill[c], 1);
-  sorted_loc[g.loc_off + cell_start[c] + slot] = gidx - g.loc_off;
-}
-
-// One wave per cell: ascending atom index inside the cell (bitonic network
-// over the 64 lanes; cells holding more than 64 atoms fall back to an
-// insertion sort by lane 0).
-__global__ void k_sort_cells(const int* __restrict__ cell_frame, const FrameGeom* __restrict__ geom, int ncells_tot,
-                             const int* __restrict__ cell_cnt, const int* __restrict__ cell_start,
-                             int* __restrict__ sorted_loc) {
-  const int c = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  const int lane = threadIdx.x & (kWave - 1);
-  if (c >= ncells_tot) return;
-  const FrameGeom& g = geom[cell_frame[c]];
-  int* a = sorted_loc + g.loc_off + cell_start[c];
-  const int n = cell_cnt[c];
-  if (n <= kWave) {
-    int v = lane < n ? a[lane] : 0x7fffffff;
-    for (int k = 2; k <= kWave; k <<= 1) {
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        const int p = __shfl_xor(v, j, kWave);
-        const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
-        v = keep_min ? min(v, p) : max(v, p);
-      }
-    }
-    if (lane < n) a[lane] = v;
-  } else if (lane == 0) {
-    for (int k = 1; k < n; ++k) {
-      const int v = a[k];
-      int m = k - 1;
-      while (m >= 0 && a[m] > v) {
-        a[m + 1] = a[m];
-        --m;
-      }
-      a[m + 1] = v;
-    }
-  }
+  cell_list[g.loc_off + cell_start[c] + cell_slot[gidx]] = gidx - g.loc_off;
 }
 
 __device__ __forceinline__ int src_cell(const FrameGeom& g, int e, int* cx, bool* ghost) {
@@ -194,12 +164,11 @@ __device__ __forceinline__ int src_cell(const FrameGeom& g, int e, int* cx, bool
   return (s[0] * g.nc[1] + s[1]) * g.nc[2] + s[2];
 }
 
-// One workgroup per frame: exclusive scan of the cell counts (cell_start,
-// cell_fill cleared), then the per-extended-cell counts (all atoms, and
-// ghosts only) and their scans -> nall.
+// One workgroup per frame: exclusive scan of the cell counts (cell_start),
+// then the per-extended-cell counts (all atoms, and ghosts only) and their
+// scans -> nall.
 __global__ void __launch_bounds__(1024) k_scan_cells(const FrameGeom* __restrict__ geom, const int* __restrict__ cell_cnt,
-                                                     int* __restrict__ cell_start, int* __restrict__ cell_fill,
-                                                     int* __restrict__ xcnt, int* __restrict__ xstart,
+                                                     int* __restrict__ cell_start, int* __restrict__ xcnt, int* __restrict__ xstart,
                                                      int* __restrict__ xgcnt, int* __restrict__ xgstart,
                                                      int* __restrict__ nall_out, int* __restrict__ flag) {
   __shared__ int sh[2 * kWave];
@@ -207,7 +176,6 @@ __global__ void __launch_bounds__(1024) k_scan_cells(const FrameGeom* __restrict
   const FrameGeom& g = geom[f];
   const int nc = g.nc[0] * g.nc[1] * g.nc[2];
   block_exclusive_scan(cell_cnt + g.cell_off, cell_start + g.cell_off, nc, sh);
-  for (int k = threadIdx.x; k < nc; k += blockDim.x) cell_fill[g.cell_off + k] = 0;
   const int n = g.nx[0] * g.nx[1] * g.nx[2];
   const int off = g.xcell_off;
   for (int e = threadIdx.x; e < n; e += blockDim.x) {
@@ -228,8 +196,21 @@ __global__ void __launch_bounds__(1024) k_scan_cells(const FrameGeom* __restrict
   }
 }
 
+// Ascending order of one int per lane across the wave (bitonic network).
+__device__ __forceinline__ int wave_sort_asc(int v, int lane) {
+  for (int k = 2; k <= kWave; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const int p = __shfl_xor(v, j, kWave);
+      const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+      v = keep_min ? min(v, p) : max(v, p);
+    }
+  }
+  return v;
+}
+
 __global__ void k_emit_ghosts(const int* __restrict__ xcell_frame, const FrameGeom* __restrict__ geom, int nx_tot,
-                              const int* __restrict__ cell_start, const int* __restrict__ sorted_loc,
+                              const int* __restrict__ cell_start, const int* __restrict__ cell_list,
+                              int* __restrict__ sorted_loc, int* __restrict__ cell_cnt,
                               const int* __restrict__ xcnt, const int* __restrict__ xstart,
                               const int* __restrict__ xgstart, double* __restrict__ coord_ext,
                               int* __restrict__ type_ext, int* __restrict__ map_ext,
@@ -245,15 +226,16 @@ __global__ void k_emit_ghosts(const int* __restrict__ xcell_frame, const FrameGe
   bool gh;
   const int s = src_cell(g, e, c, &gh);
   const int cnt = xcnt[wave];
-  const int* src = sorted_loc + g.loc_off + cell_start[g.cell_off + s];
+  const size_t base = (size_t)g.loc_off + cell_start[g.cell_off + s];
+  const int* src = cell_list + base;
   double sv[3] = {0., 0., 0.};
   if (gh) {
     double sh[3];
     for (int d = 0; d < 3; ++d) sh[d] = (double)image_shift(c[d], g.nc[d]);
     for (int d = 0; d < 3; ++d) sv[d] = sh[0] * g.t[0 * 3 + d] + sh[1] * g.t[1 * 3 + d] + sh[2] * g.t[2 * 3 + d];
   }
-  for (int k = lane; k < cnt; k += kWave) {
-    const int p = src[k];
+  // atom p is the k-th of its cell in ascending index order
+  auto emit = [&](int p, int k) {
     const int slot = xstart[wave] + k;
     const size_t a = (size_t)g.ext_off + p;
     CellAtom rec;
@@ -276,33 +258,45 @@ __global__ void k_emit_ghosts(const int* __restrict__ xcell_frame, const FrameGe
         type_ext[b] = rec.type;
         map_ext[b] = p;
       }
+    } else {
+      sorted_loc[base + k] = p;
     }
     if (slot < g.ext_cap) cell_atoms[g.ext_off + slot] = rec;
+  };
+  if (cnt <= kWave) {
+    const int v = wave_sort_asc(lane < cnt ? src[lane] : 0x7fffffff, lane);
+    if (lane < cnt) emit(v, lane);
+  } else {
+    // rank by counting (indices in a cell are distinct)
+    for (int k = lane; k < cnt; k += kWave) {
+      const int p = src[k];
+      int r = 0;
+      for (int m = 0; m < cnt; ++m) r += src[m] < p;
+      emit(p, r);
+    }
   }
+  if (!gh && lane == 0) cell_cnt[g.cell_off + s] = 0;  // counts start at zero in the next build
 }
 
 }  // namespace
 
 void launch_build_extended(const CellWork& w, hipStream_t st) {
-  launch_fill_int(w.cell_cnt, 0, w.ncells_tot, st);  // a kernel, not a memset node, under graph capture
+  // the emission leaves every count at zero; a first build clears them
+  if (!w.counts_clear) launch_fill_int(w.cell_cnt, 0, w.ncells_tot, st);  // a kernel, not a memset node
   const int tpb = 256;
   hipLaunchKernelGGL(k_wrap_bin, dim3(ceil_div(w.ntot, tpb)), dim3(tpb), 0, st, w.coord_in, w.type_in, w.frame_of,
-                     w.geom, w.ntot, w.coord_ext, w.type_ext, w.map_ext, w.cell_of, w.cell_cnt, w.do_wrap,
-                     w.force_clear);
+                     w.geom, w.ntot, w.coord_ext, w.type_ext, w.map_ext, w.cell_of, w.cell_slot, w.cell_cnt,
+                     w.do_wrap, w.force_clear);
   DPMI_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_scan_cells, dim3(w.nframes), dim3(1024), 0, st, w.geom, w.cell_cnt, w.cell_start, w.cell_fill,
-                     w.xcnt, w.xstart, w.xgcnt, w.xgstart, w.nall, w.flag);
+  hipLaunchKernelGGL(k_scan_cells, dim3(w.nframes), dim3(1024), 0, st, w.geom, w.cell_cnt, w.cell_start, w.xcnt,
+                     w.xstart, w.xgcnt, w.xgstart, w.nall, w.flag);
   DPMI_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_scatter_cells, dim3(ceil_div(w.ntot, tpb)), dim3(tpb), 0, st, w.frame_of, w.geom, w.ntot,
-                     w.cell_of, w.cell_start, w.cell_fill, w.sorted_loc);
-  DPMI_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_sort_cells, dim3(ceil_div((long long)w.ncells_tot * kWave, tpb)), dim3(tpb), 0, st,
-                     w.cell_frame, w.geom,
-                     w.ncells_tot, w.cell_cnt, w.cell_start, w.sorted_loc);
+                     w.cell_of, w.cell_slot, w.cell_start, w.cell_list);
   DPMI_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_emit_ghosts, dim3(ceil_div((long long)w.nx_tot * kWave, tpb)), dim3(tpb), 0, st,
-                     w.xcell_frame, w.geom, w.nx_tot, w.cell_start, w.sorted_loc, w.xcnt, w.xstart, w.xgstart,
-                     w.coord_ext, w.type_ext, w.map_ext, w.cell_atoms);
+                     w.xcell_frame, w.geom, w.nx_tot, w.cell_start, w.cell_list, w.sorted_loc, w.cell_cnt, w.xcnt,
+                     w.xstart, w.xgstart, w.coord_ext, w.type_ext, w.map_ext, w.cell_atoms);
   DPMI_LAUNCH_CHECK();
 }
 

@@ -396,6 +396,8 @@ SeaEngine::SeaEngine(const SeaModel* m, int nframes_, const int* natoms_h, const
   d_ncand = dalloc<int>(ntot);
   d_cell_of = dalloc<int>(ntot);
   d_sorted_loc = dalloc<int>(ntot);
+  d_cell_slot = dalloc<int>(ntot);
+  d_cell_list = dalloc<int>(ntot);
   const int D = m->M * m->A;
   if (precision == DPMI_FP64) {
     d_em = dalloc<double>(nn * 4);
@@ -499,7 +501,7 @@ SeaEngine::~SeaEngine() {
   if (geom_pinned) (void)hipHostFree(geom_pinned);
   for (auto* p : {&d_frame_of, &d_center, &d_type, &d_pos, &d_perm, &d_flag, &d_nall, &d_frame_ext_off, &d_nlist,
                   &d_ncand, &d_cell_of, &d_sorted_loc, &d_cell_frame, &d_xcell_frame, &d_cell_cnt, &d_cell_start,
-                  &d_cell_fill, &d_xcnt, &d_xstart, &d_xgcnt, &d_xgstart, &d_type_ext, &d_map_ext,
+                  &d_cell_slot, &d_cell_list, &d_xcnt, &d_xstart, &d_xgcnt, &d_xgstart, &d_type_ext, &d_map_ext,
                   &d_xframe_of_ext, &d_chunks, &d_chunk_start, &d_ticket})
     dfree(*p);
   dfree(d_cell_atoms);
@@ -557,7 +559,7 @@ void SeaEngine::prepare_geometry(const double* box_h) {
       eoff[f] = g.ext_off;
       ext_cap_h[f] = g.ext_cap;
     }
-    for (auto* p : {&d_cell_frame, &d_xcell_frame, &d_cell_cnt, &d_cell_start, &d_cell_fill, &d_xcnt, &d_xstart,
+    for (auto* p : {&d_cell_frame, &d_xcell_frame, &d_cell_cnt, &d_cell_start, &d_xcnt, &d_xstart,
                     &d_xgcnt, &d_xgstart, &d_type_ext, &d_map_ext, &d_xframe_of_ext})
       dfree(*p);
     dfree(d_cell_atoms);
@@ -567,8 +569,8 @@ void SeaEngine::prepare_geometry(const double* box_h) {
     d_xcell_frame = dupload(xf.data(), xf.size());
     d_xframe_of_ext = dupload(ef.data(), ef.size());
     d_cell_cnt = dalloc<int>(ncells);
+    counts_clean = false;
     d_cell_start = dalloc<int>(ncells);
-    d_cell_fill = dalloc<int>(ncells);
     d_xcnt = dalloc<int>(nxcells);
     d_xstart = dalloc<int>(nxcells);
     d_xgcnt = dalloc<int>(nxcells);
@@ -895,7 +897,9 @@ void SeaEngine::compute(const double* coord, const double* box_h, double* energy
   w.cell_of = d_cell_of;
   w.cell_cnt = d_cell_cnt;
   w.cell_start = d_cell_start;
-  w.cell_fill = d_cell_fill;
+  w.cell_slot = d_cell_slot;
+  w.cell_list = d_cell_list;
+  w.counts_clear = counts_clean;
   w.sorted_loc = d_sorted_loc;
   w.xcnt = d_xcnt;
   w.xstart = d_xstart;
@@ -904,7 +908,9 @@ void SeaEngine::compute(const double* coord, const double* box_h, double* energy
   w.nall = d_nall;
   w.flag = d_flag;
   w.force_clear = owner_force ? force : nullptr;
+  counts_clean = false;
   launch_build_extended(w, st);
+  counts_clean = true;
   mark(st, stage++);
 
   NlistArgs na;

@@ -113,8 +113,9 @@ struct SeaEngine {
   int *d_frame_of = nullptr, *d_center = nullptr, *d_type = nullptr, *d_pos = nullptr, *d_perm = nullptr;
   int *d_flag = nullptr, *d_nall = nullptr, *d_frame_ext_off = nullptr;
   int *d_nlist = nullptr, *d_ncand = nullptr, *d_cell_of = nullptr, *d_sorted_loc = nullptr;
-  int *d_cell_frame = nullptr, *d_xcell_frame = nullptr, *d_cell_cnt = nullptr, *d_cell_start = nullptr,
-      *d_cell_fill = nullptr;
+  int *d_cell_frame = nullptr, *d_xcell_frame = nullptr, *d_cell_cnt = nullptr, *d_cell_start = nullptr;
+  int *d_cell_slot = nullptr, *d_cell_list = nullptr;
+  bool counts_clean = false;  // d_cell_cnt is zero (left so by the last complete build)
   int *d_xcnt = nullptr, *d_xstart = nullptr, *d_xgcnt = nullptr, *d_xgstart = nullptr;
   int *d_type_ext = nullptr, *d_map_ext = nullptr, *d_xframe_of_ext = nullptr;
   CellAtom* d_cell_atoms = nullptr;

@@ -33,10 +33,12 @@ struct CellWork {
   int* map_ext = nullptr;
   CellAtom* cell_atoms = nullptr;    // [sum ext_cap], extended cells' atoms in cell order
   int* cell_of = nullptr;            // [ntot]
-  int* cell_cnt = nullptr;           // [ncells_tot]
+  int* cell_slot = nullptr;          // [ntot] slot of the atom in its cell (counting order)
+  int* cell_cnt = nullptr;           // [ncells_tot]; left zeroed by the build
+  bool counts_clear = false;         // cell_cnt is zero on entry (a previous build left it so)
   int* cell_start = nullptr;
-  int* cell_fill = nullptr;
-  int* sorted_loc = nullptr;         // [ntot]
+  int* cell_list = nullptr;          // [ntot] atoms by cell, counting order inside a cell
+  int* sorted_loc = nullptr;         // [ntot] atoms by cell, ascending index inside a cell
   int* xcnt = nullptr;               // [nx_tot]
   int* xstart = nullptr;
   int* xgcnt = nullptr;
