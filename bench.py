@@ -70,6 +70,8 @@ def stage_model(m, nloc, nall, precision):
         "force_virial": ("hbm", 4 * N * s + 4 * N + 24 * rho + 72, None, None),
         # reads the cell lists and ghost coordinates, writes the sorted rows
         "nlist": ("hbm", 8 * N, None, None),
+        # the same, storing the environment matrix of the rows as well
+        "nlist_env": ("hbm", 8 * N + 4 * N * s + 24 * rho, None, None),
         "fitting": ("flop", fit_fwd + fit_bwd, mm_peak, None),
         "fitting_fwd": ("flop", fit_fwd, mm_peak, None),
         "fitting_bwd": ("flop", fit_bwd, mm_peak, None),

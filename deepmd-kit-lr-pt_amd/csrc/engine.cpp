@@ -465,9 +465,11 @@ SeaEngine::SeaEngine(const SeaModel* m, int nframes_, const int* natoms_h, const
     netiles = (int)et.size();
     if (netiles) d_etiles = dupload(et.data(), et.size());
   }
-  stage_names = precision == DPMI_MIXED && m->fused
-                    ? "geometry;nlist;env_mat;embed_fwd;fitting;embed_bwd;force_virial;reduce"
-                    : "geometry;nlist;env_mat;embed_fwd;fitting_fwd;fitting_bwd;embed_bwd;force_virial;reduce";
+  // with recomputed slot derivatives the neighbour-list kernel also stores
+  // the environment matrix: one stage "nlist_env"
+  stage_names = std::string(recompute_slots ? "geometry;nlist_env;" : "geometry;nlist;env_mat;") +
+                (precision == DPMI_MIXED && m->fused ? "embed_fwd;fitting;embed_bwd;force_virial;reduce"
+                                                     : "embed_fwd;fitting_fwd;fitting_bwd;embed_bwd;force_virial;reduce");
   d_atom_e = dalloc<double>(ntot);
   d_vpart = dalloc<double>((size_t)ntot * 9);
   {
@@ -638,12 +640,14 @@ EmbedArgs SeaEngine::embed_args() const {
 // FP64 stages env_mat .. embed_bwd: rocBLAS DGEMM fitting per type block.
 void SeaEngine::compute_fp64_descriptor_fitting(hipStream_t st, int& stage) {
   const SeaModel& m = *model;
-  EnvArgs ea = env_args();
-  ea.em = d_em;
-  ea.em_deriv = d_em_deriv;
-  ea.rij = d_rij;
-  launch_env_mat_a(ea, st);
-  mark(st, stage++);
+  if (!recompute_slots) {  // else the neighbour-list kernel stored em
+    EnvArgs ea = env_args();
+    ea.em = d_em;
+    ea.em_deriv = d_em_deriv;
+    ea.rij = d_rij;
+    launch_env_mat_a(ea, st);
+    mark(st, stage++);
+  }
 
   EmbedArgs ta = embed_args();
   ta.xyz = d_xyz;
@@ -718,8 +722,10 @@ void SeaEngine::compute_fp64_descriptor_fitting(hipStream_t st, int& stage) {
 // with FP32 accumulation and FP32 activations/gradients.
 void SeaEngine::compute_mixed_descriptor_fitting(hipStream_t st, int& stage) {
   const SeaModel& m = *model;
-  launch_env_mat_a_f32(env_args(), d_em_f, d_em_deriv_f, d_rij_f, st);
-  mark(st, stage++);
+  if (!recompute_slots) {  // else the neighbour-list kernel stored em
+    launch_env_mat_a_f32(env_args(), d_em_f, d_em_deriv_f, d_rij_f, st);
+    mark(st, stage++);
+  }
 
   const int k0 = m.kpad[0];
   EmbedArgs ta = embed_args();
@@ -922,6 +928,17 @@ void SeaEngine::compute(const double* coord, const double* box_h, double* energy
   na.rc2f = m.rcut * m.rcut;
   na.nlist = d_nlist;
   na.ncand = d_ncand;
+  if (recompute_slots) {
+    // the environment matrix without derivatives comes with the rows
+    na.avg = m.d_davg;
+    na.stdv = m.d_dstd;
+    na.rmin = (double)m.rcut_smth;
+    na.rmax = (double)m.rcut;
+    if (precision == DPMI_FP64)
+      na.em_d = d_em;
+    else
+      na.em_f = d_em_f;
+  }
   launch_cell_nlist(w, na, st);
   mark(st, stage++);
 

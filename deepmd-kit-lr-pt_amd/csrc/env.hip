@@ -57,19 +57,18 @@ __global__ void k_env_mat_a(const double* __restrict__ coord, const int* __restr
   if (ti >= 0) {
     const double* av = avg + (size_t)ti * nnei * 4 + s * 4;
     const double* sd = stdv + (size_t)ti * nnei * 4 + s * 4;
+    envc::em_row<TO>(val, av, sd, em + base * 4);
+    if constexpr (DERIV) {
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      if constexpr (sizeof(TO) == 4) {
-        const double isd = 1. / sd[a];
-        em[base * 4 + a] = (TO)((val[a] - av[a]) * isd);
-        if constexpr (DERIV)
+      for (int a = 0; a < 4; ++a) {
+        if constexpr (sizeof(TO) == 4) {
+          const double isd = 1. / sd[a];
 #pragma unroll
           for (int d = 0; d < 3; ++d) em_deriv[base * 12 + a * 3 + d] = (TO)(der[a * 3 + d] * isd);
-      } else {
-        em[base * 4 + a] = (TO)((val[a] - av[a]) / sd[a]);
-        if constexpr (DERIV)
+        } else {
 #pragma unroll
           for (int d = 0; d < 3; ++d) em_deriv[base * 12 + a * 3 + d] = (TO)(der[a * 3 + d] / sd[a]);
+        }
       }
     }
   } else {

@@ -67,5 +67,43 @@ __device__ __forceinline__ void slot_terms(const double (&rr)[3], double rmin, d
   for (int a = 0; a < 4; ++a) val[a] *= sw;
 }
 
+// Stored environment-matrix row of a slot: (val - avg) / std (FP32
+// storage: times the FP64 reciprocal of std, rounded once).
+template <typename TO>
+__device__ __forceinline__ void em_row(const double (&val)[4], const double* __restrict__ av,
+                                       const double* __restrict__ sd, TO* __restrict__ out) {
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    if constexpr (sizeof(TO) == 4)
+      out[a] = (TO)((val[a] - av[a]) * (1. / sd[a]));
+    else
+      out[a] = (TO)((val[a] - av[a]) / sd[a]);
+  }
+}
+
+// Environment-matrix row of slot s of centre i (type ti, coordinates ci)
+// with neighbour j (< 0: padding), as the environment kernel stores it
+// without derivatives (prod_env_mat.cc:107-123): padding slots give
+// -avg/std, a centre of negative type zeros.
+template <typename TO>
+__device__ __forceinline__ void slot_em(const double (&ci)[3], const double* __restrict__ cj, int j, int ti, int s,
+                                        int nnei, const double* __restrict__ avg, const double* __restrict__ stdv,
+                                        double rmin, double rmax, TO* __restrict__ out) {
+  if (ti < 0) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a) out[a] = (TO)0;
+    return;
+  }
+  double val[4] = {0., 0., 0., 0.}, der[12];
+  if (j >= 0) {
+    double rr[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) rr[d] = cj[d] - ci[d];
+    slot_terms<sizeof(TO) == 4, false>(rr, rmin, rmax, val, der);
+  }
+  const size_t o = (size_t)ti * nnei * 4 + (size_t)s * 4;
+  em_row<TO>(val, avg + o, stdv + o, out);
+}
+
 }  // namespace envc
 }  // namespace dpmi

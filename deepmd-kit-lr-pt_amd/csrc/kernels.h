@@ -63,6 +63,14 @@ struct NlistArgs {
   float rc2f = 0;
   int* nlist = nullptr;
   int* ncand = nullptr;
+  // cell search only: when em_f or em_d is set, the kernel also stores the
+  // environment-matrix rows of the formatted row (no derivatives), as the
+  // environment kernel would, from avg/stdv [ntypes, nnei, 4], rmin, rmax
+  const double* avg = nullptr;
+  const double* stdv = nullptr;
+  double rmin = 0, rmax = 0;
+  float* em_f = nullptr;
+  double* em_d = nullptr;
 };
 int nlist_row_capacity(int want);
 void launch_cell_nlist(const CellWork& w, const NlistArgs& a, hipStream_t st);

@@ -18,6 +18,7 @@
 #include <type_traits>
 
 #include "dpmi_common.h"
+#include "env_core.h"
 #include "geometry.h"
 #include "kernels.h"
 
@@ -179,7 +180,9 @@ __global__ void __launch_bounds__(64) k_cell_nlist(const int* __restrict__ frame
                                                    const int* __restrict__ xstart, const CellAtom* __restrict__ cell_atoms,
                                                    const int* __restrict__ sec_g, int ntypes, int nnei, double rc2,
                                                    float rc2f, int* __restrict__ nlist, int* __restrict__ ncand,
-                                                   int* __restrict__ flag) {
+                                                   int* __restrict__ flag, const double* __restrict__ avg,
+                                                   const double* __restrict__ stdv, double rmin, double rmax,
+                                                   float* __restrict__ em_f, double* __restrict__ em_d) {
   __shared__ unsigned long long keys[CAP];
   __shared__ int tfirst[kMaxTypes], tend[kMaxTypes], sec[kMaxTypes + 1];
   __shared__ int cstart[kWave], cpre[kWave + 1];
@@ -259,7 +262,22 @@ __global__ void __launch_bounds__(64) k_cell_nlist(const int* __restrict__ frame
     }
   }
   if (ncand && lane == 0) ncand[gi] = n;
-  sort_and_slot<CAP>(keys, n, tfirst, tend, sec, ntypes, nnei, nlist + (size_t)gi * nnei);
+  int* row = nlist + (size_t)gi * nnei;
+  sort_and_slot<CAP>(keys, n, tfirst, tend, sec, ntypes, nnei, row);
+  // environment-matrix rows of the slots this lane just formatted
+  if (em_f) {
+    for (int s = lane; s < nnei; s += kWave) {
+      const int j = row[s];
+      envc::slot_em<float>(ci, coord_ext + 3 * (eo + (j >= 0 ? j : i)), j, ti, s, nnei, avg, stdv, rmin, rmax,
+                           em_f + ((size_t)gi * nnei + s) * 4);
+    }
+  } else if (em_d) {
+    for (int s = lane; s < nnei; s += kWave) {
+      const int j = row[s];
+      envc::slot_em<double>(ci, coord_ext + 3 * (eo + (j >= 0 ? j : i)), j, ti, s, nnei, avg, stdv, rmin, rmax,
+                            em_d + ((size_t)gi * nnei + s) * 4);
+    }
+  }
 }
 
 // Drop-in path: candidates from caller rows (InputNlist), one frame stride.
@@ -372,7 +390,7 @@ void launch_cell_nlist(const CellWork& w, const NlistArgs& a, hipStream_t st) {
   DPMI_CAP_DISPATCH(a.cap, hipLaunchKernelGGL(k_cell_nlist<CAP>, dim3(w.ntot), dim3(kWave), 0, st, w.frame_of,
                                               w.geom, w.ntot, w.coord_ext, w.type_ext, w.cell_of, w.xstart,
                                               w.cell_atoms, a.sec, a.ntypes, a.nnei, a.rc2, a.rc2f, a.nlist,
-                                              a.ncand, w.flag));
+                                              a.ncand, w.flag, a.avg, a.stdv, a.rmin, a.rmax, a.em_f, a.em_d));
   DPMI_LAUNCH_CHECK();
 }
 
