@@ -83,10 +83,10 @@ def stage_model(m, nloc, nall, precision):
 # FETCH_SIZE / WRITE_SIZE on this workload, scaled by a 256 MiB copy
 # (profiles/water12k_kernels_and_traffic.txt).
 MEASURED_TRAFFIC_MIB = {
-    "mixed": {"embed_bwd": (123.02, 24.99), "embed_fwd": (31.47, 53.92), "fitting": (70.28, 94.94),
-              "force_virial": (35.10, 18.60), "env_mat": (11.42, 24.93), "nlist": (9.24, 6.81)},
-    "fp64": {"embed_bwd": (253.71, 49.15), "embed_fwd": (84.65, 196.23), "force_virial": (41.07, 16.30),
-             "env_mat": (12.60, 48.92), "nlist": (9.24, 6.81)},
+    "mixed": {"embed_bwd": (124.76, 25.44), "embed_fwd": (31.05, 54.89), "fitting": (61.84, 97.52),
+              "force_virial": (36.28, 18.02), "nlist_env": (19.60, 33.00)},
+    "fp64": {"embed_bwd": (259.61, 50.48), "embed_fwd": (85.64, 201.01), "force_virial": (41.69, 18.60),
+             "nlist_env": (22.37, 58.88)},
 }
 
 
