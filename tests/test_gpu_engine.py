@@ -114,3 +114,27 @@ def test_engine_copper_small():
     assert abs(e[0] - ref.energy) / abs(ref.energy) < 1e-10
     assert rel(f, ref.force) < 1e-10
     assert rel(v[0], ref.virial) < 1e-10
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_status_reports_ghost_capacity_until_read(water, precision):
+    """A ghost shell larger than the engine's capacity (192 atoms in a 12.4 A
+    box need 1,536 extended atoms; capacity nloc + 64) is flagged, the
+    kernels stay inside the buffers, the status stays set over later steps
+    until it is read, and reading clears it."""
+    from dpmi.lib import DpmiError
+
+    print(f"case ghost capacity precision={precision}", flush=True)
+    coord, atype, box = systems.water(64, seed=1)
+    dm = DeviceModel(water)
+    eng = Engine(dm, [len(atype)], atype, precision=precision, nall_per_atom=1.0)
+    x = torch.as_tensor(coord, device="cuda")
+    eng.compute(x, box)
+    eng.compute(x, box)  # a second step does not reset the status
+    with pytest.raises(DpmiError, match="MEMORY"):
+        eng.status()
+    eng.status()  # read once: cleared
+    # an engine with room evaluates the same system normally afterwards
+    ok = Engine(dm, [len(atype)], atype, precision=precision)
+    ok.compute(x, box)
+    ok.status()
