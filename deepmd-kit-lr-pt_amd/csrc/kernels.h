@@ -203,7 +203,8 @@ template <typename T>
 void launch_prod_virial_grad_a(T* grad_net, const T* grad, const T* env_deriv, const T* rij, const int* nlist,
                                int nloc, int nnei, hipStream_t st);
 
-// engine: force/virial over packed rows into extended force buffers
+// engine: force/virial over packed rows (the LDS-aggregated kernel adds onto
+// the owner rows of `force`; the row kernel into extended force buffers)
 struct ForceRowsArgs {
   const void* net_deriv = nullptr;  // [nrows, nnei*4]  (double, or float when fp32)
   const void* em_deriv = nullptr;   // [nrows, nnei*12]; null: recomputed from coord_ext
