@@ -105,11 +105,19 @@ def test_engine_batched_uneven_frames(water):
         off += len(a)
 
 
-def test_engine_copper_small():
+@pytest.mark.parametrize("ncell", [5, 6])
+def test_engine_copper_small(ncell):
+    """fcc Cu, rcut 8: 5^3 cells hold about 62 atoms per search cell, 6^3
+    about 108 (more than a wave: the ghost emission ranks by counting
+    instead of the in-register sort)."""
+    print(f"case copper ncell={ncell}", flush=True)
     m = dmodel.copper_model(seed=2)
-    coord, atype, box = systems.fcc_copper(5, seed=4)
+    coord, atype, box = systems.fcc_copper(ncell, seed=4)
     eng, e, f, v = _run(m, coord, atype, box)
     ref = evaluate(m, coord, atype, box, oracle_lib(), keep=True)
+    x = ref.extra
+    assert np.array_equal(eng.fetch("coord_ext").reshape(-1, 3), x["coord_ext"])
+    assert np.array_equal(eng.fetch("mapping"), x["mapping"])
     assert np.array_equal(eng.fetch("nlist").reshape(ref.nlist.shape), ref.nlist)
     assert abs(e[0] - ref.energy) / abs(ref.energy) < 1e-10
     assert rel(f, ref.force) < 1e-10
