@@ -245,6 +245,7 @@ int dpmi_prod_env_mat_a_f64(double* em, double* em_deriv, double* rij, int* nlis
     ea.nrows = (long long)nframes * nloc;
     ea.nnei = nnei;
     ea.nall_stride = nall;
+    ea.ntypes = ntypes;
     ea.avg = avg;
     ea.std_ = std_;
     ea.rmin = (double)rcut_smth;
@@ -420,10 +421,18 @@ void dpmi_sea_engine_destroy(dpmi_sea_engine* engine) {
 }
 
 int dpmi_sea_engine_compute(dpmi_sea_engine* engine, const double* coord, const double* box_host, double* energy,
-                            double* force, double* virial, double* atom_energy, void* stream) {
+                            double* force, double* virial, double* atom_energy, double* atom_virial, void* stream) {
   return guarded([&] {
     if (!engine || !coord || !box_host || !energy || !force || !virial) throw std::invalid_argument("null argument");
-    engine->impl->compute(coord, box_host, energy, force, virial, atom_energy, S(stream));
+    engine->impl->compute(coord, box_host, energy, force, virial, atom_energy, atom_virial, S(stream));
+    return kOk;
+  });
+}
+
+int dpmi_sea_engine_set_growth(dpmi_sea_engine* engine, int enable) {
+  return guarded([&] {
+    if (!engine) throw std::invalid_argument("null engine");
+    engine->impl->grow = enable != 0;
     return kOk;
   });
 }

@@ -199,6 +199,7 @@ void prod_env_mat_a_gpu<double>(double* em, double* em_deriv, double* rij, int* 
     ea.nrows = (long long)nframes * nloc;
     ea.nnei = nnei;
     ea.nall_stride = nall;
+    ea.ntypes = ntypes;
     ea.avg = avg;
     ea.std_ = std_;
     ea.rmin = (double)rcut_smth;

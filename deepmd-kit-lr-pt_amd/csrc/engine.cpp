@@ -385,7 +385,9 @@ SeaEngine::SeaEngine(const SeaModel* m, int nframes_, const int* natoms_h, const
   d_pos = dupload(pos.data(), ntot);
   d_perm = dupload(perm.data(), ntot);
   geom_h.resize(nframes);
+  ext_need_h.assign(nframes, 0);
   DPMI_CHECK(hipHostMalloc((void**)&geom_pinned, sizeof(FrameGeom) * nframes, hipHostMallocDefault));
+  DPMI_CHECK(hipHostMalloc((void**)&stat_pinned, sizeof(int) * (nframes + 1), hipHostMallocDefault));
   d_geom = dalloc<FrameGeom>(nframes);
   d_flag = dalloc<int>(1);
   DPMI_CHECK(hipMemset(d_flag, 0, sizeof(int)));  // status bits stay set until status() reads them
@@ -501,6 +503,7 @@ SeaEngine::~SeaEngine() {
   if (graph) (void)hipGraphDestroy(graph);
   if (blas) rocblas_destroy_handle((rocblas_handle)blas);
   if (geom_pinned) (void)hipHostFree(geom_pinned);
+  if (stat_pinned) (void)hipHostFree(stat_pinned);
   for (auto* p : {&d_frame_of, &d_center, &d_type, &d_pos, &d_perm, &d_flag, &d_nall, &d_frame_ext_off, &d_nlist,
                   &d_ncand, &d_cell_of, &d_sorted_loc, &d_cell_frame, &d_xcell_frame, &d_cell_cnt, &d_cell_start,
                   &d_cell_slot, &d_cell_list, &d_xcnt, &d_xstart, &d_xgcnt, &d_xgstart, &d_type_ext, &d_map_ext,
@@ -544,13 +547,14 @@ void SeaEngine::prepare_geometry(const double* box_h) {
     nxcells += geom_ext_cells(g) + 1;
     const int cap =
         nall_per_atom > 0 ? (int)std::ceil(natoms[f] * nall_per_atom) + 64 : geom_nall_estimate(g, natoms[f], 1.25);
-    g.ext_cap = std::max(cap, natoms[f]);
+    g.ext_cap = std::max(std::max(cap, natoms[f]), ext_need_h[f]);
     if (!grid_changed && f < (int)ext_cap_h.size()) g.ext_cap = std::max(g.ext_cap, ext_cap_h[f]);
     g.ext_off = (int)next;
     next += g.ext_cap;
   }
   if (grid_changed || next > next_tot || ncells > ncells_tot || nxcells > nx_tot) {
     DPMI_CHECK(hipDeviceSynchronize());
+    drop_graph();
     std::vector<int> cf(ncells), xf(nxcells), ef(next), eoff(nframes);
     ext_cap_h.resize(nframes);
     for (int f = 0; f < nframes; ++f) {
@@ -603,6 +607,7 @@ EnvArgs SeaEngine::env_args() const {
   ea.row_center = d_center;
   ea.nrows = ntot;
   ea.nnei = m.nnei;
+  ea.ntypes = m.ntypes;
   ea.avg = m.d_davg;
   ea.std_ = m.d_dstd;
   ea.rmin = (double)m.rcut_smth;
@@ -836,17 +841,64 @@ void SeaEngine::mark(hipStream_t st, int stage) {
 }
 
 
+void SeaEngine::drop_graph() {
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  graph_exec = nullptr;
+  graph = nullptr;
+}
+
+// The reference grows its ghost and neighbour buffers and retries the step
+// (prod_env_mat_multi_device.cc:2015-2028, 2069-2083).  Here an eager step
+// with growth on reads back its status word and per-frame nall; a ghost
+// overflow raises that frame's extended capacity to the nall it reported
+// (plus a margin), a neighbour-row overflow doubles the row capacity (up to
+// 4096 candidates), and the step is run again.  Recovered bits are not
+// reported by status(); anything else stays set for it.
 void SeaEngine::compute(const double* coord, const double* box_h, double* energy, double* force, double* virial,
-                        double* atom_energy, hipStream_t st) {
+                        double* atom_energy, double* atom_virial, hipStream_t st) {
+  constexpr int kMaxRegrow = 8;
+  for (int attempt = 0;; ++attempt) {
+    enqueue_step(coord, box_h, energy, force, virial, atom_energy, atom_virial, st);
+    if (!grow || capturing) return;
+    DPMI_CHECK(hipMemcpyAsync(stat_pinned, d_flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    DPMI_CHECK(hipMemcpyAsync(stat_pinned + 1, d_nall, sizeof(int) * nframes, hipMemcpyDeviceToHost, st));
+    DPMI_CHECK(hipStreamSynchronize(st));
+    const int flag = stat_pinned[0];
+    if (flag) DPMI_CHECK(hipMemset(d_flag, 0, sizeof(int)));
+    int left = flag;
+    bool again = false;
+    if ((flag & kFlagGhostCapacity) && attempt < kMaxRegrow) {
+      for (int f = 0; f < nframes; ++f)
+        if (stat_pinned[1 + f] > geom_h[f].ext_cap)
+          ext_need_h[f] = std::max(ext_need_h[f], stat_pinned[1 + f] + stat_pinned[1 + f] / 16 + 64);
+      left &= ~kFlagGhostCapacity;
+      again = true;
+    }
+    if ((flag & kFlagNlistCapacity) && attempt < kMaxRegrow && nlist_cap < 4096) {
+      nlist_cap = nlist_row_capacity(2 * nlist_cap);
+      left &= ~kFlagNlistCapacity;
+      again = true;
+    }
+    sticky |= left;
+    if (!again) return;
+    ++regrows;
+  }
+}
+
+void SeaEngine::enqueue_step(const double* coord, const double* box_h, double* energy, double* force,
+                             double* virial, double* atom_energy, double* atom_virial, hipStream_t st) {
   const SeaModel& m = *model;
   prepare_geometry(box_h);
   last_stream = st;
   int stage = 0;
   mark(st, stage++);
   // upload the per-frame layout only when it changed; the pinned staging
-  // buffer is rewritten only after the previous upload has completed
+  // buffer is rewritten only after the previous upload has completed.  A new
+  // layout (box) invalidates a captured graph, which holds the old one.
   if (!geom_uploaded || std::memcmp(geom_pinned, geom_h.data(), sizeof(FrameGeom) * nframes) != 0) {
     DPMI_CHECK(hipStreamSynchronize(st));
+    if (!capturing) drop_graph();
     std::memcpy(geom_pinned, geom_h.data(), sizeof(FrameGeom) * nframes);
     DPMI_CHECK(hipMemcpyAsync(d_geom, geom_pinned, sizeof(FrameGeom) * nframes, hipMemcpyHostToDevice, st));
     geom_uploaded = true;
@@ -882,7 +934,9 @@ void SeaEngine::compute(const double* coord, const double* box_h, double* energy
   fa.rows = tune.force_rows;
   fa.map_ext = d_map_ext;
   fa.force = force;
+  fa.atom_virial = atom_virial;
   const bool owner_force = force_rows_aggregated(fa);
+  if (atom_virial && !owner_force) throw Unsupported("per-atom virials need the LDS-aggregated force kernel");
 
   CellWork w;
   w.nframes = nframes;
@@ -948,6 +1002,7 @@ void SeaEngine::compute(const double* coord, const double* box_h, double* energy
     compute_mixed_descriptor_fitting(st, stage);
 
   if (owner_force) {
+    if (atom_virial) launch_fill_double(atom_virial, 0.0, (long long)ntot * 9, st);
     launch_force_virial_rows(fa, st);
   } else {
     launch_fill_double(d_force_ext, 0.0, next_tot * 3, st);
@@ -1016,7 +1071,7 @@ void SeaEngine::autotune(const double* coord, const double* box_h, int reps, hip
     return -1;
   };
   try {
-    compute(coord, box_h, e, f, v, nullptr, st);  // geometry and buffers
+    compute(coord, box_h, e, f, v, nullptr, nullptr, st);  // geometry and buffers
     std::vector<double> ms(names.size(), 0.0);
     for (int r = 0; r < std::max(reps, 1); ++r) {
       for (int c = 0; c < ncand; ++c) {
@@ -1029,7 +1084,7 @@ void SeaEngine::autotune(const double* coord, const double* box_h, int reps, hip
         tune.embed_mfma = c < 3 ? 0 : 3;
         timing = true;
         reset_timing();
-        compute(coord, box_h, e, f, v, nullptr, st);
+        compute(coord, box_h, e, f, v, nullptr, nullptr, st);
         timing_report(ms.data(), (int)ms.size(), nullptr, 0);
         auto keep = [&](double* t, const char* n) {
           const double x = idx(n) >= 0 ? ms[idx(n)] : 0.0;
@@ -1078,31 +1133,35 @@ void SeaEngine::capture(const double* coord, const double* box_h, double* energy
   if (precision != DPMI_MIXED || !model->fused)
     throw Unsupported("graph capture needs the mixed engine with the fused fitting kernel");
   if (!st) throw std::invalid_argument("graph capture needs a non-default stream");
-  compute(coord, box_h, energy, force, virial, atom_energy, st);  // geometry, buffers, uploads for these boxes
+  // geometry, buffers, uploads (and any capacity growth) for these boxes
+  compute(coord, box_h, energy, force, virial, atom_energy, nullptr, st);
   DPMI_CHECK(hipStreamSynchronize(st));
-  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-  if (graph) (void)hipGraphDestroy(graph);
-  graph_exec = nullptr;
-  graph = nullptr;
+  drop_graph();
   const bool was_timing = timing;
   timing = false;
+  capturing = true;
   DPMI_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   try {
-    compute(coord, box_h, energy, force, virial, atom_energy, st);
+    compute(coord, box_h, energy, force, virial, atom_energy, nullptr, st);
   } catch (...) {
     hipGraph_t g = nullptr;
     (void)hipStreamEndCapture(st, &g);
     if (g) (void)hipGraphDestroy(g);
     timing = was_timing;
+    capturing = false;
     throw;
   }
+  capturing = false;
   timing = was_timing;
   DPMI_CHECK(hipStreamEndCapture(st, &graph));
   DPMI_CHECK(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
 }
 
 void SeaEngine::launch_graph(hipStream_t st) {
-  if (!graph_exec) throw std::invalid_argument("no captured step: capture one first");
+  if (!graph_exec)
+    throw std::invalid_argument(
+        "no captured step for the current buffers and boxes (none captured, or a later step changed the box or "
+        "reallocated): capture again");
   DPMI_CHECK(hipGraphLaunch(graph_exec, st));
   last_stream = st;
 }
@@ -1112,6 +1171,8 @@ int SeaEngine::status() {
   int flag = 0;
   DPMI_CHECK(hipMemcpy(&flag, d_flag, sizeof(int), hipMemcpyDeviceToHost));
   if (flag) DPMI_CHECK(hipMemset(d_flag, 0, sizeof(int)));
+  flag |= sticky;
+  sticky = 0;
   if (flag & kFlagNlistCapacity) return DPMI_ERR_NLIST_CAPACITY;
   if (flag & kFlagGhostCapacity) return DPMI_ERR_MEMORY;
   if (flag & (kFlagIndexRange | kFlagTypeRange | kFlagHalfRange)) return DPMI_ERR_RANGE;
@@ -1178,6 +1239,15 @@ long long SeaEngine::fetch(int frame, int what, void* dst) {
       return n;
     }
     case 8: src = d_atom_e + r0; n = nloc; es = 8; break;
+    case 9:  // extended capacity of the frame
+      if (dst) std::memcpy(dst, &g.ext_cap, sizeof(int));
+      return 1;
+    case 10:  // capacity increases made so far
+      if (dst) std::memcpy(dst, &regrows, sizeof(int));
+      return 1;
+    case 11:  // neighbour-row candidate capacity
+      if (dst) std::memcpy(dst, &nlist_cap, sizeof(int));
+      return 1;
     default: throw std::invalid_argument("unknown fetch target");
   }
   if (dst && n) DPMI_CHECK(hipMemcpy(dst, src, es * n, hipMemcpyDeviceToHost));

@@ -61,9 +61,14 @@ struct SeaEngine {
   SeaEngine(const SeaModel* m, int nframes, const int* natoms, const int* atype, int precision,
             double nall_per_atom);
   ~SeaEngine();
+  // One E+F+V step.  With growth on (the default) an eager step reads its
+  // status word back and, when the ghost or neighbour-row capacity
+  // overflowed, grows that capacity and recomputes the step.
   void compute(const double* coord, const double* box_h, double* energy, double* force, double* virial,
-               double* atom_energy, hipStream_t st);
+               double* atom_energy, double* atom_virial, hipStream_t st);
   int status();
+  bool grow = true;
+  int regrows = 0;  // capacity increases made by compute()
   long long fetch(int frame, int what, void* dst);
   // one step (eager) then the same step captured into a hipGraph; replays
   // read the coordinates at `coord` and use the boxes given here
@@ -88,6 +93,15 @@ struct SeaEngine {
   void prepare_geometry(const double* box_h);
   void load_cached_tune();
   void mark(hipStream_t st, int stage);
+  void enqueue_step(const double* coord, const double* box_h, double* energy, double* force, double* virial,
+                    double* atom_energy, double* atom_virial, hipStream_t st);
+  // a captured graph refers to the buffers and boxes of its capture: any
+  // reallocation or new box drops it (launch_graph then asks for a capture)
+  void drop_graph();
+  bool capturing = false;
+  int sticky = 0;               // status bits read back by growing steps, reported by status()
+  int* stat_pinned = nullptr;   // [1 + nframes]: status word and per-frame nall of the last growing step
+  std::vector<int> ext_need_h;  // per-frame extended capacity learned from overflows
 
   std::vector<FrameGeom> geom_h;
   FrameGeom* geom_pinned = nullptr;

@@ -25,7 +25,8 @@ template <typename TO, bool DERIV>
 __global__ void k_env_mat_a(const double* __restrict__ coord, const int* __restrict__ type,
                             const int* __restrict__ nlist, const int* __restrict__ row_frame,
                             const int* __restrict__ row_ext_off, const int* __restrict__ row_center,
-                            int strided_nloc, long long nrows, int nnei, int nall_stride, const double* __restrict__ avg,
+                            int strided_nloc, long long nrows, int nnei, int nall_stride, int ntypes,
+                            const double* __restrict__ avg,
                             const double* __restrict__ stdv, double rmin, double rmax, TO* __restrict__ em,
                             TO* __restrict__ em_deriv, TO* __restrict__ rij) {
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -42,7 +43,10 @@ __global__ void k_env_mat_a(const double* __restrict__ coord, const int* __restr
     eo = (size_t)(g / strided_nloc) * nall_stride;
     i = (int)(g % strided_nloc);
   }
-  const int ti = type[eo + i];
+  // a centre type outside [0, ntypes) has no avg/std row: its row is zero
+  // (the neighbour-list stage reports such types)
+  int ti = type[eo + i];
+  if (ti >= ntypes) ti = -1;
   const int j = nlist[g * nnei + s];
   double val[4] = {0., 0., 0., 0.};
   double der[12];
@@ -91,11 +95,11 @@ void launch_env_mat_a(const EnvArgs& a, hipStream_t st) {
   const int tpb = a.tpb > 0 ? a.tpb : 256;
   if (a.em_deriv)
     hipLaunchKernelGGL((k_env_mat_a<double, true>), dim3(ceil_div(n, tpb)), dim3(tpb), 0, st, a.coord, a.type, a.nlist, a.row_frame,
-                       a.row_ext_off, a.row_center, a.strided_nloc, a.nrows, a.nnei, a.nall_stride, a.avg, a.std_, a.rmin, a.rmax,
+                       a.row_ext_off, a.row_center, a.strided_nloc, a.nrows, a.nnei, a.nall_stride, a.ntypes, a.avg, a.std_, a.rmin, a.rmax,
                        a.em, a.em_deriv, a.rij);
   else
     hipLaunchKernelGGL((k_env_mat_a<double, false>), dim3(ceil_div(n, tpb)), dim3(tpb), 0, st, a.coord, a.type, a.nlist, a.row_frame,
-                       a.row_ext_off, a.row_center, a.strided_nloc, a.nrows, a.nnei, a.nall_stride, a.avg, a.std_, a.rmin, a.rmax,
+                       a.row_ext_off, a.row_center, a.strided_nloc, a.nrows, a.nnei, a.nall_stride, a.ntypes, a.avg, a.std_, a.rmin, a.rmax,
                        a.em, a.em_deriv, a.rij);
   DPMI_LAUNCH_CHECK();
 }
@@ -106,12 +110,12 @@ void launch_env_mat_a_f32(const EnvArgs& a, float* em, float* em_deriv, float* r
   const int tpb = a.tpb > 0 ? a.tpb : 256;
   if (em_deriv)
     hipLaunchKernelGGL((k_env_mat_a<float, true>), dim3(ceil_div(n, tpb)), dim3(tpb), 0, st, a.coord, a.type, a.nlist,
-                       a.row_frame, a.row_ext_off, a.row_center, a.strided_nloc, a.nrows, a.nnei, a.nall_stride, a.avg,
+                       a.row_frame, a.row_ext_off, a.row_center, a.strided_nloc, a.nrows, a.nnei, a.nall_stride, a.ntypes, a.avg,
                        a.std_, a.rmin, a.rmax, em, em_deriv, rij);
   else
     hipLaunchKernelGGL((k_env_mat_a<float, false>), dim3(ceil_div(n, tpb)), dim3(tpb), 0, st, a.coord, a.type,
                        a.nlist, a.row_frame, a.row_ext_off, a.row_center, a.strided_nloc, a.nrows, a.nnei,
-                       a.nall_stride, a.avg, a.std_, a.rmin, a.rmax, em, em_deriv, rij);
+                       a.nall_stride, a.ntypes, a.avg, a.std_, a.rmin, a.rmax, em, em_deriv, rij);
   DPMI_LAUNCH_CHECK();
 }
 

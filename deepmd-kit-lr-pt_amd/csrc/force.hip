@@ -299,20 +299,29 @@ __device__ __forceinline__ void recompute_slot(const double (&ci)[3], const doub
 
 // RECOMP: ed and rij are not stored; each slot's derivative and displacement
 // are rebuilt from coord_ext and the centre type's dstd.
-template <typename TI, bool RECOMP, int HB>
+// AV: per-atom virials as well (prod_virial.cc:55-63: slot s of centre i adds
+// its virial term to atom_virial[j(s)]), pre-summed per owner key in LDS like
+// the forces and added into atom_virial [ntot, 9] (zeroed by the caller).
+template <typename TI, bool RECOMP, int HB, bool AV = false>
 __global__ void __launch_bounds__(1024) k_force_virial_hash(
     const TI* __restrict__ nd, const TI* __restrict__ ed, const TI* __restrict__ rij, const int* __restrict__ nlist,
     const int* __restrict__ row_frame, const int* __restrict__ row_center, const int* __restrict__ frame_ext_off,
     const FrameGeom* __restrict__ geom, const int* __restrict__ sorted_loc, long long nrows, int nnei, int R,
     const double* __restrict__ coord_ext, const int* __restrict__ type_ext, const double* __restrict__ dstd,
     double rmin, double rmax, const int* __restrict__ map_ext, double* __restrict__ force,
-    double* __restrict__ vpart) {
+    double* __restrict__ vpart, double* __restrict__ atom_virial) {
   constexpr int NH = 1 << HB;
+  constexpr int NV = AV ? NH : 1;
   __shared__ int hkey[NH];
   __shared__ double hval[NH][3];
+  __shared__ double hvir[NV][9];
   for (int h = threadIdx.x; h < NH; h += blockDim.x) {
     hkey[h] = -1;
     hval[h][0] = hval[h][1] = hval[h][2] = 0.;
+    if constexpr (AV) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) hvir[h][k] = 0.;
+    }
   }
   __syncthreads();
   const int lane = lane_id(), wave = threadIdx.x / kWave;
@@ -380,6 +389,10 @@ __global__ void __launch_bounds__(1024) k_force_virial_hash(
         }
 #pragma unroll
         for (int k = 0; k < 9; ++k) vs[k] += v[k];
+        if constexpr (AV) {
+#pragma unroll
+          for (int k = 0; k < 9; ++k) atomicAdd(&hvir[h][k], (double)v[k]);
+        }
       }
     }
 #pragma unroll
@@ -405,6 +418,10 @@ __global__ void __launch_bounds__(1024) k_force_virial_hash(
     if (key >= 0) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) atomicAdd(force + (size_t)key * 3 + d, hval[h][d]);
+      if constexpr (AV) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) atomicAdd(atom_virial + (size_t)key * 9 + k, hvir[h][k]);
+      }
     }
   }
 }
@@ -561,6 +578,35 @@ void launch_force_virial_rows(const ForceRowsArgs& a, hipStream_t st) {
   if (!a.em_deriv && !force_rows_aggregated(a))
     throw std::invalid_argument("recomputed slot derivatives need the LDS-aggregated force kernel");
   if (force_rows_aggregated(a)) {
+    if (a.atom_virial) {
+      // per-atom virials: 9 more LDS sums per key, so the table stays at
+      // 2^10 keys (about 100 KB of LDS) and the row group shrinks to fit
+      if (a.nnei + 1 > (1 << 10)) throw Unsupported("per-atom virials need nnei < 1024");
+      const int R = std::max(1, std::min(a.rows > 0 ? a.rows : 16, (1 << 10) / (a.nnei + 1)));
+      const dim3 grid(ceil_div(a.nrows, R)), block(R * kWave);
+      if (a.fp32 && !a.em_deriv)
+        hipLaunchKernelGGL((k_force_virial_hash<float, true, 10, true>), grid, block, 0, st,
+                           (const float*)a.net_deriv, nullptr, nullptr, a.nlist, a.row_frame, a.row_center,
+                           a.frame_ext_off, a.geom, a.sorted_loc, a.nrows, a.nnei, R, a.coord_ext, a.type_ext, a.dstd,
+                           a.rmin, a.rmax, a.map_ext, a.force, a.vpart, a.atom_virial);
+      else if (!a.em_deriv)
+        hipLaunchKernelGGL((k_force_virial_hash<double, true, 10, true>), grid, block, 0, st,
+                           (const double*)a.net_deriv, nullptr, nullptr, a.nlist, a.row_frame, a.row_center,
+                           a.frame_ext_off, a.geom, a.sorted_loc, a.nrows, a.nnei, R, a.coord_ext, a.type_ext, a.dstd,
+                           a.rmin, a.rmax, a.map_ext, a.force, a.vpart, a.atom_virial);
+      else if (a.fp32)
+        hipLaunchKernelGGL((k_force_virial_hash<float, false, 10, true>), grid, block, 0, st,
+                           (const float*)a.net_deriv, (const float*)a.em_deriv, (const float*)a.rij, a.nlist,
+                           a.row_frame, a.row_center, a.frame_ext_off, a.geom, a.sorted_loc, a.nrows, a.nnei, R,
+                           nullptr, nullptr, nullptr, 0., 0., a.map_ext, a.force, a.vpart, a.atom_virial);
+      else
+        hipLaunchKernelGGL((k_force_virial_hash<double, false, 10, true>), grid, block, 0, st,
+                           (const double*)a.net_deriv, (const double*)a.em_deriv, (const double*)a.rij, a.nlist,
+                           a.row_frame, a.row_center, a.frame_ext_off, a.geom, a.sorted_loc, a.nrows, a.nnei, R,
+                           nullptr, nullptr, nullptr, 0., 0., a.map_ext, a.force, a.vpart, a.atom_virial);
+      DPMI_LAUNCH_CHECK();
+      return;
+    }
     const int R = std::max(1, std::min(a.rows > 0 ? a.rows : 16, kHashSize / (a.nnei + 1)));
     const dim3 grid(ceil_div(a.nrows, R)), block(R * kWave);
     const int keys = R * (a.nnei + 1);
@@ -571,22 +617,22 @@ void launch_force_virial_rows(const ForceRowsArgs& a, hipStream_t st) {
       hipLaunchKernelGGL((k_force_virial_hash<float, true, HB>), grid, block, 0, st, (const float*)a.net_deriv,       \
                          nullptr, nullptr, a.nlist, a.row_frame, a.row_center, a.frame_ext_off, a.geom, a.sorted_loc, \
                          a.nrows, a.nnei, R, a.coord_ext, a.type_ext, a.dstd, a.rmin, a.rmax, a.map_ext, a.force,     \
-                         a.vpart);                                                                                    \
+                         a.vpart, nullptr);                                                                           \
     else if (!a.em_deriv)                                                                                             \
       hipLaunchKernelGGL((k_force_virial_hash<double, true, HB>), grid, block, 0, st, (const double*)a.net_deriv,     \
                          nullptr, nullptr, a.nlist, a.row_frame, a.row_center, a.frame_ext_off, a.geom, a.sorted_loc, \
                          a.nrows, a.nnei, R, a.coord_ext, a.type_ext, a.dstd, a.rmin, a.rmax, a.map_ext, a.force,     \
-                         a.vpart);                                                                                    \
+                         a.vpart, nullptr);                                                                           \
     else if (a.fp32)                                                                                                  \
       hipLaunchKernelGGL((k_force_virial_hash<float, false, HB>), grid, block, 0, st, (const float*)a.net_deriv,      \
                          (const float*)a.em_deriv, (const float*)a.rij, a.nlist, a.row_frame, a.row_center,           \
                          a.frame_ext_off, a.geom, a.sorted_loc, a.nrows, a.nnei, R, nullptr, nullptr, nullptr, 0., 0., \
-                         a.map_ext, a.force, a.vpart);                                                                \
+                         a.map_ext, a.force, a.vpart, nullptr);                                                       \
     else                                                                                                              \
       hipLaunchKernelGGL((k_force_virial_hash<double, false, HB>), grid, block, 0, st, (const double*)a.net_deriv,    \
                          (const double*)a.em_deriv, (const double*)a.rij, a.nlist, a.row_frame, a.row_center,         \
                          a.frame_ext_off, a.geom, a.sorted_loc, a.nrows, a.nnei, R, nullptr, nullptr, nullptr, 0., 0., \
-                         a.map_ext, a.force, a.vpart);                                                                \
+                         a.map_ext, a.force, a.vpart, nullptr);                                                       \
   } while (0)
     if (hb == 10)
       DPMI_HASH_LAUNCH(10);

@@ -96,6 +96,7 @@ struct EnvArgs {
   long long nrows = 0;
   int nnei = 0;
   int nall_stride = 0;
+  int ntypes = 1 << 30;  // rows of centres of type >= ntypes are zero
   const double* avg = nullptr;
   const double* std_ = nullptr;
   double rmin = 0, rmax = 0;
@@ -233,6 +234,8 @@ struct ForceRowsArgs {
   // (ghost images folded through map_ext) in force [ntot, 3] (zeroed by caller)
   const int* map_ext = nullptr;
   double* force = nullptr;
+  // per-atom virials [ntot, 9] keyed like the forces (zeroed by caller), or null
+  double* atom_virial = nullptr;
 };
 // whether launch_force_virial_rows takes the LDS-aggregated kernel (which
 // writes owner forces) for these arguments

@@ -249,9 +249,11 @@ __global__ void __launch_bounds__(64) k_cell_nlist(const int* __restrict__ frame
             const double cj[3] = {rec.x, rec.y, rec.z};
             float fr2;
             if (accept(ci, cj, rc2, rc2f, &fr2)) {
-              keep = true;
+              // out-of-range keys are flagged and dropped, so that the
+              // per-type tables below are indexed by valid types only
               if (j >= (1 << 24)) atomicOr(flag, kFlagIndexRange);
               if (tj >= ntypes) atomicOr(flag, kFlagTypeRange);
+              keep = j < (1 << 24) && tj < ntypes;
               key = make_key(tj, fr2, j);
             }
           }
@@ -302,7 +304,11 @@ __global__ void __launch_bounds__(64) k_row_format(int inum, const int* __restri
   const double* cf = coord + (size_t)f * nall * 3;
   const int* tf = type + (size_t)f * nall;
   const double ci[3] = {cf[3 * i], cf[3 * i + 1], cf[3 * i + 2]};
-  const int cnt = numneigh[r];
+  int cnt = numneigh[r];
+  if (!firstneigh && cnt > jstride) {  // a row longer than its stride: flagged, read up to the stride
+    if (lane == 0) atomicOr(flag, kFlagNlistCapacity);
+    cnt = jstride;
+  }
   const int* cand = firstneigh ? firstneigh[r] : jlist + (size_t)r * jstride;
   __syncthreads();
   int n = 0;
@@ -312,7 +318,9 @@ __global__ void __launch_bounds__(64) k_row_format(int inum, const int* __restri
     unsigned long long key = 0;
     if (p < cnt) {
       const int j = cand[p];
-      const int tj = tf[j];
+      // an index outside the frame is a caller error: flagged, never read
+      if (j < 0 || j >= nall) atomicOr(flag, kFlagIndexRange);
+      const int tj = (j >= 0 && j < nall) ? tf[j] : -1;
       if (tj >= 0) {
 #pragma clang fp contract(off)
         const float f0 = (float)cf[3 * j + 0] - (float)ci[0];
@@ -320,9 +328,9 @@ __global__ void __launch_bounds__(64) k_row_format(int inum, const int* __restri
         const float f2 = (float)cf[3 * j + 2] - (float)ci[2];
         const float q = f0 * f0 + f1 * f1 + f2 * f2;
         if (q <= rc2f) {
-          keep = true;
           if (j >= (1 << 24)) atomicOr(flag, kFlagIndexRange);
           if (tj >= ntypes) atomicOr(flag, kFlagTypeRange);
+          keep = j < (1 << 24) && tj < ntypes;
           key = make_key(tj, q, j);
         }
       }

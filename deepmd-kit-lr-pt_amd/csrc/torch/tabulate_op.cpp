@@ -9,8 +9,11 @@
 //   forward  -> TabulateFusionSeAGradOp (backward) -> grad-grad (double backward),
 //   no gradient to table / table_info, table_info read on the host.
 // The kernels behind it are the dpmi_tabulate_* entry points of
-// include/dpmi.h, launched on torch's current HIP stream.  Only HIP tensors
-// are accepted: this library carries no CPU implementation.
+// include/dpmi.h, launched on torch's current HIP stream.  The library has
+// no CPU arithmetic: host tensors (the reference's CPU kernels,
+// tabulate_multi_device.cc:147-152) are staged to the current HIP device,
+// computed there and the results copied back, so host and device callers
+// get the same HIP results; without a visible GPU the call raises.
 #include <ATen/hip/HIPContext.h>
 #include <torch/torch.h>
 
@@ -55,9 +58,21 @@ void check_last_layer(int64_t M) {
   TORCH_CHECK(M > 0 && M <= 1024, "last_layer_size must be between 1 and 1024 for GPU tabulation, but got ", M);
 }
 
+// The reference reads every tensor through .view({-1}), which raises for a
+// non-contiguous tensor; raw pointers of such a tensor would be read with the
+// wrong strides, so they are rejected the same way.
 template <typename T>
 const T* cptr(const torch::Tensor& t) {
+  TORCH_CHECK(t.is_contiguous(), "tabulate_fusion_se_a: input tensors must be contiguous");
   return t.data_ptr<T>();
+}
+
+// Host tensors run on the current HIP device (see the file comment).
+torch::Tensor to_dev(const torch::Tensor& t) {
+  if (!t.defined() || t.device().is_cuda()) return t;
+  TORCH_CHECK(torch::cuda::is_available(),
+              "dpmi computes tabulate_fusion_se_a on the GPU; host tensors need a visible HIP device");
+  return t.to(torch::Device(torch::kCUDA, at::hip::current_device()));
 }
 
 template <typename T>
@@ -133,26 +148,33 @@ class GradOp : public torch::autograd::Function<GradOp> {
                                                 const torch::Tensor& info, const torch::Tensor& em_x,
                                                 const torch::Tensor& em, const torch::Tensor& dy,
                                                 const torch::Tensor& descriptor) {
-    auto dy_dem_x = torch::zeros_like(em_x);
-    auto dy_dem = torch::zeros_like(em);
-    if (table.dtype() == torch::kDouble)
-      grad<double>(table, info, em_x, em, dy, descriptor, dy_dem_x, dy_dem);
+    const bool host = table.device().is_cpu();
+    const torch::Tensor tb = to_dev(table), ex = to_dev(em_x), e = to_dev(em), d = to_dev(dy),
+                        de = to_dev(descriptor);
+    auto dy_dem_x = torch::zeros_like(ex);
+    auto dy_dem = torch::zeros_like(e);
+    if (tb.dtype() == torch::kDouble)
+      grad<double>(tb, info, ex, e, d, de, dy_dem_x, dy_dem);
     else
-      grad<float>(table, info, em_x, em, dy, descriptor, dy_dem_x, dy_dem);
+      grad<float>(tb, info, ex, e, d, de, dy_dem_x, dy_dem);
     ctx->save_for_backward({table, info, em_x, em, descriptor});
+    if (host) return {dy_dem_x.cpu(), dy_dem.cpu()};
     return {dy_dem_x, dy_dem};
   }
   static torch::autograd::variable_list backward(torch::autograd::AutogradContext* ctx,
                                                  torch::autograd::variable_list g) {
     auto s = ctx->get_saved_variables();
-    auto dz_dx = g[0].defined() ? g[0].contiguous() : torch::zeros_like(s[2]);
-    auto dz_dem = g[1].defined() ? g[1].contiguous() : torch::zeros_like(s[3]);
+    const bool host = s[0].device().is_cpu();
+    for (size_t k = 0; k < s.size(); ++k)
+      if (k != 1) s[k] = to_dev(s[k]);
+    auto dz_dx = g[0].defined() ? to_dev(g[0].contiguous()) : torch::zeros_like(s[2]);
+    auto dz_dem = g[1].defined() ? to_dev(g[1].contiguous()) : torch::zeros_like(s[3]);
     auto dz_dy = torch::empty_like(s[4]);
     if (s[0].dtype() == torch::kDouble)
       grad_grad<double>(s[0], s[1], s[2], s[3], dz_dx, dz_dem, s[4], true, dz_dy);
     else
       grad_grad<float>(s[0], s[1], s[2], s[3], dz_dx, dz_dem, s[4], true, dz_dy);
-    return {at::Tensor(), at::Tensor(), at::Tensor(), at::Tensor(), dz_dy, at::Tensor()};
+    return {at::Tensor(), at::Tensor(), at::Tensor(), at::Tensor(), host ? dz_dy.cpu() : dz_dy, at::Tensor()};
   }
 };
 
@@ -161,15 +183,18 @@ class FwdOp : public torch::autograd::Function<FwdOp> {
   static torch::autograd::variable_list forward(torch::autograd::AutogradContext* ctx, const torch::Tensor& table,
                                                 const torch::Tensor& info, const torch::Tensor& em_x,
                                                 const torch::Tensor& em, int64_t M) {
-    if (table.device().is_cuda()) check_last_layer(M);
-    auto opts = torch::TensorOptions().dtype(table.dtype()).device(table.device());
-    auto out = torch::empty({em.size(0), em.size(2), M}, opts);
-    if (table.dtype() == torch::kDouble)
-      fwd<double>(table, info, em_x, em, M, out);
+    check_last_layer(M);
+    const bool host = table.device().is_cpu() && em_x.device().is_cpu() && em.device().is_cpu();
+    const torch::Tensor tb = host ? to_dev(table) : table, ex = host ? to_dev(em_x) : em_x,
+                        e = host ? to_dev(em) : em;
+    auto opts = torch::TensorOptions().dtype(tb.dtype()).device(tb.device());
+    auto out = torch::empty({e.size(0), e.size(2), M}, opts);
+    if (tb.dtype() == torch::kDouble)
+      fwd<double>(tb, info, ex, e, M, out);
     else
-      fwd<float>(table, info, em_x, em, M, out);
+      fwd<float>(tb, info, ex, e, M, out);
     ctx->save_for_backward({table, info, em_x, em, out});
-    return {out};
+    return {host ? out.cpu() : out};
   }
   static torch::autograd::variable_list backward(torch::autograd::AutogradContext* ctx,
                                                  torch::autograd::variable_list g) {

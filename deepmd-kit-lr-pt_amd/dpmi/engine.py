@@ -10,7 +10,8 @@ from .lib import DPMI_FP64, DPMI_MIXED, ModelDesc, check, load
 
 FETCH = {"nall": (0, np.int32), "coord_ext": (1, np.float64), "atype_ext": (2, np.int32),
          "mapping": (3, np.int32), "nlist": (4, np.int32), "em": (5, np.float64), "descriptor": (6, np.float64),
-         "net_deriv": (7, np.float64), "atom_energy": (8, np.float64)}
+         "net_deriv": (7, np.float64), "atom_energy": (8, np.float64), "ext_capacity": (9, np.int32),
+         "regrows": (10, np.int32), "row_capacity": (11, np.int32)}
 
 
 def _arr(a, dtype):
@@ -104,6 +105,7 @@ class Engine:
         self.force = torch.empty((self.ntot, 3), dtype=torch.float64, device=dev)
         self.virial = torch.empty((self.nframes, 9), dtype=torch.float64, device=dev)
         self.atom_energy = torch.empty(self.ntot, dtype=torch.float64, device=dev)
+        self.atom_virial = None  # allocated on the first request
 
     def __del__(self):
         h = getattr(self, "handle", None)
@@ -111,24 +113,40 @@ class Engine:
             load().dpmi_sea_engine_destroy(h)
             self.handle = None
 
-    def compute(self, coord: torch.Tensor, box, atom_energy: bool = False):
+    def _check_coord(self, coord: torch.Tensor) -> None:
+        if not (coord.dtype == torch.float64 and coord.is_cuda and coord.is_contiguous()):
+            raise ValueError("coord must be a contiguous float64 HIP tensor")
+        if coord.numel() != 3 * self.ntot:
+            raise ValueError(f"coord holds {coord.numel()} values; the engine's frames need {3 * self.ntot}")
+
+    def compute(self, coord: torch.Tensor, box, atom_energy: bool = False, atom_virial: bool = False):
         """coord: [ntot, 3] float64 device tensor; box: [nframes, 9] host
         array.  Returns (energy [nf], force [ntot, 3], virial [nf, 9]) views
-        of engine-owned buffers (overwritten by the next call)."""
-        assert coord.dtype == torch.float64 and coord.is_cuda and coord.is_contiguous()
+        of engine-owned buffers (overwritten by the next call).  With
+        atom_virial, the per-atom virials [ntot, 9] are left in
+        self.atom_virial."""
+        self._check_coord(coord)
         b = _arr(box, np.float64).reshape(self.nframes, 9)
         st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        if atom_virial and self.atom_virial is None:
+            self.atom_virial = torch.empty((self.ntot, 9), dtype=torch.float64, device=self.force.device)
         check(load().dpmi_sea_engine_compute(self.handle, C.c_void_p(coord.data_ptr()), b.ctypes.data_as(C.c_void_p),
                                              C.c_void_p(self.energy.data_ptr()), C.c_void_p(self.force.data_ptr()),
                                              C.c_void_p(self.virial.data_ptr()),
-                                             C.c_void_p(self.atom_energy.data_ptr()) if atom_energy else None, st))
+                                             C.c_void_p(self.atom_energy.data_ptr()) if atom_energy else None,
+                                             C.c_void_p(self.atom_virial.data_ptr()) if atom_virial else None, st))
         return self.energy, self.force, self.virial
+
+    def set_growth(self, on: bool = True) -> None:
+        """Capacity growth of eager steps (on by default): an overflowing
+        ghost or neighbour-row buffer is grown and the step recomputed."""
+        check(load().dpmi_sea_engine_set_growth(self.handle, int(on)))
 
     def capture(self, coord: torch.Tensor, box, atom_energy: bool = False):
         """One eager step, then the same step captured into a hipGraph on the
         current (non-default) stream.  replay() re-runs it on the values
         stored in `coord` at that time (same tensor, same boxes)."""
-        assert coord.dtype == torch.float64 and coord.is_cuda and coord.is_contiguous()
+        self._check_coord(coord)
         b = _arr(box, np.float64).reshape(self.nframes, 9)
         st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
         check(load().dpmi_sea_engine_graph_capture(
@@ -146,6 +164,7 @@ class Engine:
     def autotune(self, coord: torch.Tensor, box, reps: int = 5) -> dict:
         """Picks the stage launch shapes for this size bucket (cached for
         later engines of the same bucket); returns the chosen knobs."""
+        self._check_coord(coord)
         b = _arr(box, np.float64).reshape(self.nframes, 9)
         st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
         check(load().dpmi_sea_engine_autotune(self.handle, C.c_void_p(coord.data_ptr()),

@@ -50,7 +50,8 @@ SIGNATURES = {
     "dpmi_sea_model_destroy": ([_P], None),
     "dpmi_sea_engine_create": ([_P, _P, _I, _P, _P, _I, _D], _I),
     "dpmi_sea_engine_destroy": ([_P], None),
-    "dpmi_sea_engine_compute": ([_P, _P, _P, _P, _P, _P, _P, _P], _I),
+    "dpmi_sea_engine_compute": ([_P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
+    "dpmi_sea_engine_set_growth": ([_P, _I], _I),
     "dpmi_sea_engine_graph_capture": ([_P, _P, _P, _P, _P, _P, _P, _P], _I),
     "dpmi_sea_engine_graph_launch": ([_P, _P], _I),
     "dpmi_sea_engine_autotune": ([_P, _P, _P, _I, _P], _I),

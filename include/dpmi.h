@@ -189,11 +189,23 @@ void dpmi_sea_engine_destroy(dpmi_sea_engine* engine);
 
 /* One E+F+V evaluation of every frame.  coord [sum natoms, 3] device,
  * box_host [nframes, 9] host.  Outputs (device): energy [nframes],
- * force [sum natoms, 3], virial [nframes, 9]; atom_energy
- * [sum natoms] may be NULL.  Asynchronous: capacity errors detected on the
- * device are reported by dpmi_sea_engine_status(). */
+ * force [sum natoms, 3], virial [nframes, 9]; atom_energy [sum natoms] and
+ * atom_virial [sum natoms, 9] may be NULL.  The per-atom virial is that of
+ * DeepPot's atomic output (deepmd/pt/model/model/transform_output.py:70-101
+ * with the atomic correction, equal to source/lib/src/prod_virial.cc:55-63
+ * on the extended atoms, ghost images summed onto their owners).
+ * With capacity growth on (the default, dpmi_sea_engine_set_growth) the
+ * call reads the step's status back: a ghost or neighbour-row overflow
+ * grows that buffer and recomputes the step, as the reference retries
+ * (source/op/tf/prod_env_mat_multi_device.cc:2015-2028), and the call
+ * returns when the results are final.  With growth off the call is
+ * asynchronous and device-side errors are reported by
+ * dpmi_sea_engine_status(). */
 int dpmi_sea_engine_compute(dpmi_sea_engine* engine, const double* coord, const double* box_host,
-                            double* energy, double* force, double* virial, double* atom_energy, void* stream);
+                            double* energy, double* force, double* virial, double* atom_energy, double* atom_virial,
+                            void* stream);
+/* Capacity growth on (1, the default) or off (0): see dpmi_sea_engine_compute. */
+int dpmi_sea_engine_set_growth(dpmi_sea_engine* engine, int enable);
 
 /* hipGraph of one step (no counterpart in the reference; the batched-launch
  * pattern): runs one eager step, then captures the same step on `stream`
@@ -237,8 +249,10 @@ int dpmi_sea_engine_status(dpmi_sea_engine* engine);
  * (double, nall*3), 2 extended type (int, nall), 3 mapping (int, nall),
  * 4 nlist (int, nloc*nnei), 5 em (double, nloc*nnei*4), 6 descriptor
  * (double, nloc*M*A), 7 net_deriv (double, nloc*nnei*4), 8 atom energy
- * (double, nloc).  *count_host receives the element count; dst_host may be
- * NULL to query it. */
+ * (double, nloc), 9 extended capacity of the frame (int, 1), 10 capacity
+ * increases made by growth so far (int, 1), 11 neighbour-row candidate
+ * capacity (int, 1).  *count_host receives the element count; dst_host may
+ * be NULL to query it. */
 int dpmi_sea_engine_fetch(dpmi_sea_engine* engine, int frame, int what, void* dst_host, long long* count_host);
 
 /* Device-time breakdown of the last compute in milliseconds per stage,

@@ -136,6 +136,7 @@ def test_status_reports_ghost_capacity_until_read(water, precision):
     coord, atype, box = systems.water(64, seed=1)
     dm = DeviceModel(water)
     eng = Engine(dm, [len(atype)], atype, precision=precision, nall_per_atom=1.0)
+    eng.set_growth(False)  # report only (growth is tested in test_gpu_edge_cases.py)
     x = torch.as_tensor(coord, device="cuda")
     eng.compute(x, box)
     eng.compute(x, box)  # a second step does not reset the status

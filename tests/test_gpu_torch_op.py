@@ -78,3 +78,29 @@ def test_device_contract_errors():
         torch.ops.deepmd.tabulate_fusion_se_a(t, info, ex.cpu(), em, p["M"])
     with pytest.raises(RuntimeError, match="last_layer_size must be between 1 and 1024"):
         torch.ops.deepmd.tabulate_fusion_se_a(t, info, ex, em, 0)
+
+
+@pytest.mark.parametrize("dtype,prec", [(torch.float64, 1e-10), (torch.float32, 1e-5)])
+def test_host_tensors_staged_to_the_gpu(dtype, prec):
+    """Host tensors (the reference's CPU kernels) give the golden values and
+    gradients on the host; the arithmetic runs on the HIP device."""
+    p = pt_tabulate()
+    t = torch.tensor(p["table_tensor"], dtype=dtype)
+    info = torch.tensor(p["table_info_tensor"], dtype=dtype)
+    ex = torch.tensor(p["em_x_tensor"], dtype=dtype, requires_grad=True)
+    em = torch.tensor(p["em_tensor"], dtype=dtype, requires_grad=True)
+    out = torch.ops.deepmd.tabulate_fusion_se_a(t, info, ex, em, p["M"])[0]
+    assert out.device.type == "cpu"
+    torch.testing.assert_close(out, torch.tensor(p["expected_descriptor_tensor"], dtype=dtype), atol=prec, rtol=prec)
+    out.sum().backward()
+    assert ex.grad.device.type == "cpu"
+    torch.testing.assert_close(ex.grad, torch.tensor(p["expected_dy_dem_x"], dtype=dtype), atol=prec, rtol=prec)
+    torch.testing.assert_close(em.grad, torch.tensor(p["expected_dy_dem"], dtype=dtype), atol=prec, rtol=prec)
+
+
+def test_non_contiguous_input_raises():
+    p, t, info, ex, em = _inputs(torch.float64)
+    wide = torch.cat([em, em], dim=2)[:, :, :4]  # a strided view of the same shape
+    assert not wide.is_contiguous()
+    with pytest.raises(RuntimeError, match="contiguous"):
+        torch.ops.deepmd.tabulate_fusion_se_a(t, info, ex, wide, p["M"])

@@ -313,3 +313,36 @@ def test_oracle_matches_reference_lib(nmol, seed):
     g9 = rng.normal(size=9)
     np.testing.assert_allclose(O.prod_virial_grad_a(g9, pa[1], pa[2], pa[3], nloc),
                                R.prod_virial_grad_a(g9, pb[1], pb[2], pb[3], nloc), rtol=1e-12, atol=1e-12)
+
+
+@needs_ref
+@pytest.mark.parametrize("box", [np.diag([12.42, 12.42, 12.42]).reshape(9),
+                                 np.array([12.0, 0.0, 0.0, 2.5, 11.0, 0.0, 1.5, -2.0, 13.0]),
+                                 np.diag([49.68, 30.0, 9.0]).reshape(9)])
+@pytest.mark.parametrize("rc", [6.0, 8.0, 2.5])
+def test_cell_info_matches_reference_lib(box, rc):
+    """compute_cell_info (coord.cc:69-108): the 23 ints of the cell grid."""
+    O, R = oracle_lib(), ref_lib()
+    assert np.array_equal(O.compute_cell_info(rc, box), R.compute_cell_info(rc, box))
+
+
+@needs_ref
+@pytest.mark.parametrize("sel", [[46, 92], [10, 20], [3, 5]])
+def test_format_nlist_matches_reference_lib(sel):
+    """format_nlist_cpu (fmt_nlist.cc:99-177) on shuffled rc + skin candidate
+    rows (as an MD code hands them over), with per-type truncation."""
+    O, R = oracle_lib(), ref_lib()
+    coord, atype, box = systems.water(64, seed=4)
+    cn = O.normalize_coord(coord, box)
+    cext, text, _ = O.copy_coord(cn, atype, box, 6.0)
+    nloc = len(atype)
+    il, nn, jl = O.build_nlist(cext, nloc, 6.5, text)  # rcut + skin
+    rng = np.random.default_rng(1)
+    jl = jl.copy()
+    for i in range(nloc):
+        jl[i, : nn[i]] = rng.permutation(jl[i, : nn[i]])
+    a = O.format_nlist(il, nn, jl, cext, text, nloc, 6.0, sel)
+    b = R.format_nlist(il, nn, jl, cext, text, nloc, 6.0, sel)
+    assert np.array_equal(a, b)
+    if sel == [3, 5]:
+        assert np.all(a >= 0)  # every row truncated at sel
