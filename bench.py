@@ -80,8 +80,8 @@ def stage_model(m, nloc, nall, precision):
 
 
 # HBM bytes per launch (read MiB, write MiB) measured with rocprofv3 --pmc
-# FETCH_SIZE / WRITE_SIZE on this workload, scaled by a 256 MiB copy
-# (profiles/water12k_kernels_and_traffic.txt).
+# FETCH_SIZE / WRITE_SIZE on this workload, scaled by a 256 MiB copy; the
+# readings and the commands are in TRAFFIC_PROFILE.
 MEASURED_TRAFFIC_MIB = {
     "mixed": {"embed_bwd": (124.76, 25.44), "embed_fwd": (31.05, 54.89), "fitting": (61.84, 97.52),
               "force_virial": (36.28, 18.02), "nlist_env": (19.60, 33.00)},
@@ -90,26 +90,79 @@ MEASURED_TRAFFIC_MIB = {
 }
 
 
-def cpu_baseline(m, coord, atype, box, max_seconds=15.0, what="the same 12,288-atom water frame"):
-    """Reference CPU path (the compiled reference lib where present, else the
-    oracle restatement) plus numpy glue for the fitting, on the host cores."""
+def host_cpu():
+    """lscpu model name, sockets and physical cores of the host."""
+    import subprocess
+
+    info = {"model": None, "sockets": None, "physical_cores": None}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+    except Exception:  # noqa: BLE001
+        return info
+    kv = {}
+    for line in out.splitlines():
+        if ":" in line:
+            k, v = line.split(":", 1)
+            kv[k.strip()] = v.strip()
+    info["model"] = kv.get("Model name")
+    try:
+        sockets = int(kv.get("Socket(s)", "0"))
+        cps = int(kv.get("Core(s) per socket", "0"))
+        info["sockets"] = sockets
+        info["physical_cores"] = sockets * cps
+    except ValueError:
+        pass
+    return info
+
+
+TRAFFIC_PROFILE = "profiles/water12k_kernels_and_traffic.txt"
+
+
+def cpu_baseline(m, coord, atype, box, evals=10, warmup=3, what="the same 12,288-atom water frame"):
+    """Reference CPU path, as its benchmark plan states it: the compiled
+    reference lib (the oracle restatement where the reference is absent) for
+    normalize/copy_coord, build_nlist, prod_env_mat_a, tabulate and
+    prod_force/prod_virial, the fitting GEMMs in libtorch FP64 on the CPU;
+    OpenMP threads bound to cores (OMP_PROC_BIND=close, OMP_PLACES=cores,
+    set before the library's OpenMP runtime starts), `warmup` untimed
+    evaluations, then the median of `evals` timed ones with the per-stage
+    breakdown."""
+    import statistics
+
+    import torch
+
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    cpu = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS", 0) or cpu["physical_cores"] or os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    torch.set_num_threads(threads)
     from oracle import oracle_lib, ref_available, ref_lib
+    from oracle import sea
     from oracle.sea import evaluate
 
+    sea.USE_TORCH = True
     lib = ref_lib() if ref_available() else oracle_lib()
     kind = "reference" if ref_available() else "port"
-    t0 = time.perf_counter()
-    r = evaluate(m, coord, atype, box, lib)  # warm-up, also the CPU result
-    first = time.perf_counter() - t0
-    n = max(1, min(8, int(max_seconds / max(first, 1e-3))))
-    t0 = time.perf_counter()
-    for _ in range(n):
-        evaluate(m, coord, atype, box, lib)
-    dt = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return r, {"value": len(atype) * n / dt, "unit": "atom-steps/s", "cores": cores, "kind": kind,
-               "sample": f"{n} E+F+V evaluations of {what} "
-                         f"(reference lib + numpy glue, OMP threads {cores})"}
+    r = None
+    for _ in range(warmup):
+        r = evaluate(m, coord, atype, box, lib)  # also the CPU result
+    times, stages = [], []
+    for _ in range(evals):
+        st = {}
+        t0 = time.perf_counter()
+        r = evaluate(m, coord, atype, box, lib, stage_seconds=st)
+        times.append(time.perf_counter() - t0)
+        stages.append(st)
+    med = statistics.median(times)
+    k_med = times.index(sorted(times)[len(times) // 2])
+    return r, {"value": len(atype) / med, "unit": "atom-steps/s", "cores": threads, "kind": kind,
+               "sample": f"median of {evals} E+F+V evaluations (after {warmup} warm-up) of {what}: reference "
+                         f"CPU lib + libtorch FP64 fitting, OMP_NUM_THREADS={threads}, OMP_PROC_BIND=close, "
+                         f"OMP_PLACES=cores",
+               "median_s": med, "min_s": min(times), "max_s": max(times),
+               "stages_s_of_median_eval": {k: round(v, 5) for k, v in stages[k_med].items()},
+               "host_cpu": cpu}
 
 
 def time_engine(eng, x, box, steps, warmup):
@@ -118,8 +171,11 @@ def time_engine(eng, x, box, steps, warmup):
     import torch
 
     for _ in range(warmup):
-        eng.compute(x, box)
+        eng.compute(x, box)  # capacity growth, if any, happens here
     eng.status()
+    # the capacities now fit this configuration: the timed steps run without
+    # the per-step status read-back (status() is checked after them)
+    eng.set_growth(False)
     eng.set_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -130,6 +186,7 @@ def time_engine(eng, x, box, steps, warmup):
     eng.status()
     stages = eng.timing()
     eng.set_timing(False)
+    eng.set_growth(True)
     return dt, stages
 
 
@@ -164,11 +221,16 @@ def roofline(m, nloc, nall, precision, stages):
         roof = {"kernel": dom, "bound": "hbm", "achieved": amount / (t_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s"}
     else:
-        roof = {"kernel": dom, "bound": "mfma", "achieved": amount / (t_ms * 1e-3) / 1e12, "peak": peak,
+        # the table stages are vector-ALU kernels (FP32 / FP64 FMA on the
+        # SIMDs), the fitting runs on the matrix cores
+        bound = "mfma" if dom.startswith("fitting") else "valu"
+        roof = {"kernel": dom, "bound": bound, "achieved": amount / (t_ms * 1e-3) / 1e12, "peak": peak,
                 "unit": "TFLOP/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
     tr = MEASURED_TRAFFIC_MIB.get(precision, {}).get(dom)
     roof["traffic"] = None if tr is None else (tr[0] + tr[1]) * 1048576.0
+    if tr is not None:
+        roof["traffic_source"] = TRAFFIC_PROFILE
     roof["avg_ms"] = t_ms
     if nbytes is not None:
         roof["algorithmic_hbm_GBps"] = nbytes / (t_ms * 1e-3) / 1e9
@@ -210,7 +272,17 @@ def data_statistics_errors(nmol: int = 512):
             "virial_meV_atom": float(np.max(np.abs(v_m[0] - v_d[0])) / n * 1e3)}
 
 
-def run_batched(args) -> int:
+def batched_leg(frames: int, steps: int, warmup: int, eager: bool = True) -> dict:
+    """Times the batched workload (see run_batched); returns its JSON object."""
+    class A:
+        pass
+
+    a = A()
+    a.frames, a.steps, a.warmup, a.eager = frames, steps, warmup, eager
+    return run_batched(a, emit=False)
+
+
+def run_batched(args, emit: bool = True):
     """256 water frames with atom counts uniform over {96, 99, ..., 3000}, each
     in its own 1.0 g/cm^3 box, in one mixed-precision engine: autotuned launch
     shapes, one hipGraph launch per step, per-frame errors against the FP64
@@ -248,12 +320,15 @@ def run_batched(args) -> int:
     dt = time.perf_counter() - t0
     eng.status()
     e_m, f_m, v_m = [t.cpu().numpy().copy() for t in (eng.energy, eng.force, eng.virial)]
-    # eager (no graph) beside it
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
-        eng.compute(x, box)
-    torch.cuda.synchronize()
-    dt_eager = time.perf_counter() - t1
+    dt_eager = None
+    if getattr(args, "eager", True):  # eager (no graph) beside it
+        eng.set_growth(False)
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            eng.compute(x, box)
+        torch.cuda.synchronize()
+        dt_eager = time.perf_counter() - t1
+        eng.status()
     del eng
     ref = Engine(dm, natoms, atype, precision="fp64")
     e_d, f_d, v_d = [t.cpu().numpy().copy() for t in ref.compute(x, box)]
@@ -281,13 +356,15 @@ def run_batched(args) -> int:
         "data": "synthetic",
         "config": {"workload": f"{args.frames} water frames, {ntot} atoms, 1.0 g/cm3 boxes", "frames": args.frames,
                    "atoms": ntot, "parallelism": "single-gpu"},
-        "eager_value": ntot * args.steps / dt_eager,
+        "eager_value": None if dt_eager is None else ntot * args.steps / dt_eager,
         "tuned_knobs": knobs,
         "per_frame_error_vs_fp64": {
             "energy_meV_atom_max": float(errs[:, 0].max()), "energy_meV_atom_mean": float(errs[:, 0].mean()),
             "force_rmse_meV_A_max": float(errs[:, 1].max()), "force_rmse_meV_A_mean": float(errs[:, 1].mean()),
             "virial_meV_atom_max": float(errs[:, 2].max())},
     }
+    if not emit:
+        return out
     print(json.dumps(out), flush=True)
     return 0
 
@@ -376,7 +453,7 @@ def run_copper(args) -> int:
     out["roofline"].pop("measured_hbm_GBps", None)
     if not args.no_cpu_baseline:
         cc, ca, cbx = systems.fcc_copper(6, seed=7)
-        _, cb = cpu_baseline(m, cc, ca, cbx, max_seconds=10.0, what=f"a {len(ca)}-atom fcc Cu frame (6^3 cells)")
+        _, cb = cpu_baseline(m, cc, ca, cbx, what=f"a {len(ca)}-atom fcc Cu frame (6^3 cells)")
         out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
     return 0
@@ -391,6 +468,7 @@ def main() -> int:
                     help="path reported as value (the other one is timed beside it)")
     ap.add_argument("--nmol", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 256-frame batched leg of the default run")
     ap.add_argument("--eager", action="store_true", help="time the mixed engine with eager launches only")
     ap.add_argument("--workload", default="water12k", choices=["water12k", "batched", "copper1m"],
                     help="water12k: the 12,288-atom water frame (default); batched: uneven water frames; "
@@ -492,6 +570,10 @@ def main() -> int:
         "tuned_knobs": knobs,
     }
     out["mixed_error_vs_fp64_data_statistics"] = data_statistics_errors()
+    if not args.no_batched:
+        b = batched_leg(256, steps=20, warmup=3, eager=False)
+        out["batched_256_frames"] = {k: b[k] for k in ("metric", "value", "unit", "ms_per_step", "steps", "config",
+                                                       "tuned_knobs", "per_frame_error_vs_fp64")}
     if not args.no_cpu_baseline:
         ref, cb = cpu_baseline(m, coord, atype, box)
         out["cpu_baseline"] = cb

@@ -36,12 +36,26 @@ class EFV:
     extra: dict | None = None
 
 
+def _matmul(a, b):
+    """FP64 product of the fitting layers: numpy, or libtorch on the CPU when
+    the caller sets USE_TORCH (the reference PT CPU path runs these GEMMs in
+    libtorch)."""
+    if USE_TORCH:
+        import torch
+
+        return (torch.from_numpy(np.ascontiguousarray(a)) @ torch.from_numpy(np.ascontiguousarray(b))).numpy()
+    return a @ b
+
+
+USE_TORCH = False
+
+
 def _fitting_fwd(layers, x):
     """Returns output and the per-layer tape for the backward pass."""
     tape = []
     h = x
     for l in layers:
-        z = h @ l.w + l.b
+        z = _matmul(h, l.w) + l.b
         y = np.tanh(z) if l.act else z
         yo = y * l.idt if l.idt is not None else y
         res = 0
@@ -65,7 +79,7 @@ def _fitting_bwd(layers, tape, gout):
         gy = g * l.idt if l.idt is not None else g
         if l.act:
             gy = gy * (1.0 - y * y)
-        gh = gy @ l.w.T
+        gh = _matmul(gy, l.w.T)
         if res == 1:
             gh = gh + g
         elif res == 2:
@@ -75,7 +89,26 @@ def _fitting_bwd(layers, tape, gout):
     return g
 
 
-def evaluate(model, coord, atype, box, lib, embedding: str = "table", keep: bool = False) -> EFV:
+class _Clock:
+    """Wall time per stage, accumulated into `out` (seconds) when given."""
+
+    def __init__(self, out):
+        import time
+
+        self.out, self.time = out, time.perf_counter
+        self.t = self.time()
+
+    def __call__(self, name):
+        if self.out is None:
+            return
+        t = self.time()
+        self.out[name] = self.out.get(name, 0.0) + (t - self.t)
+        self.t = t
+
+
+def evaluate(model, coord, atype, box, lib, embedding: str = "table", keep: bool = False,
+             stage_seconds: dict | None = None) -> EFV:
+    clock = _Clock(stage_seconds)
     nloc = len(atype)
     nnei = model.nnei
     sec = model.sec
@@ -84,10 +117,13 @@ def evaluate(model, coord, atype, box, lib, embedding: str = "table", keep: bool
     cn = lib.normalize_coord(coord, box)
     cext, text, mapping = lib.copy_coord(cn, atype, box, rc)
     nall = len(text)
+    clock("normalize_copy_coord")
     il, nn, jl = lib.build_nlist(cext, nloc, rc, text)
+    clock("build_nlist")
     avg = model.davg.reshape(model.ntypes, -1)
     std = model.dstd.reshape(model.ntypes, -1)
     em, ed, rij, nlist = lib.prod_env_mat_a(cext, text, il, nn, jl, avg, std, nloc, rc, rs, model.sel)
+    clock("prod_env_mat_a")
     em3 = em.reshape(nloc, nnei, 4)
     xyz = np.zeros((nloc, 4, M))
     blocks = []
@@ -106,6 +142,7 @@ def evaluate(model, coord, atype, box, lib, embedding: str = "table", keep: bool
                 gr = np.einsum("njc,njm->ncm", rr, gg)
             xyz[rows] += gr
             blocks.append((ti, tj, rows, rr, ss, net))
+    clock("tabulate_fusion_se_a")
     T = xyz / nnei
     D = np.einsum("icm,ica->ima", T, T[:, :, :A]).reshape(nloc, M * A)
     e = np.zeros(nloc)
@@ -122,6 +159,7 @@ def evaluate(model, coord, atype, box, lib, embedding: str = "table", keep: bool
     gD = np.zeros_like(D)
     for ti, (rows, tape) in tapes.items():
         gD[rows] = _fitting_bwd(model.fitting[ti], tape, np.ones((len(rows), 1)))
+    clock("descriptor_fitting_fwd_bwd")
     G = gD.reshape(nloc, M, A)
     gT = np.einsum("ima,ica->icm", G, T[:, :, :A])
     gT[:, :, :A] += np.einsum("ima,icm->ica", G, T)
@@ -141,10 +179,12 @@ def evaluate(model, coord, atype, box, lib, embedding: str = "table", keep: bool
         nd[:, :, 0] += dx
         net_deriv[rows, sec[tj]:sec[tj + 1], :] = nd
     nd2 = net_deriv.reshape(nloc, nnei * 4)
+    clock("tabulate_fusion_se_a_grad")
     fext = lib.prod_force_a(nd2, ed, nlist, nloc, nall)[0]
     vir, avir = lib.prod_virial_a(nd2, ed, rij, nlist, nloc, nall)
     force = np.zeros((nloc, 3))
     np.add.at(force, mapping, fext)
+    clock("prod_force_virial")
     extra = None
     if keep:
         extra = dict(em=em, em_deriv=ed, rij=rij, coord_ext=cext, atype_ext=text, mapping=mapping,
