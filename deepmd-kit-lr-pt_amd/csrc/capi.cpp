@@ -358,6 +358,25 @@ int dpmi_prod_virial_a_f32(float* virial, float* atom_virial, const float* net_d
                            const float* rij, const int* nlist, int nloc, int nall, int nnei, void* stream) {
   return prod_virial_impl(virial, atom_virial, net_deriv, env_deriv, rij, nlist, nloc, nall, nnei, stream);
 }
+int dpmi_prod_virial_a_f64_noalloc(double* virial, double* atom_virial, const double* net_deriv,
+                                   const double* env_deriv, const double* rij, const int* nlist, int nloc, int nall,
+                                   int nnei, void* stream) {
+  return guarded([&] {
+    if (nloc < 0 || nloc > dpmi::kNoAllocVirialRows)
+      throw std::invalid_argument("dpmi_prod_virial_a_f64_noalloc: nloc " + std::to_string(nloc) +
+                                  " exceeds the fixed scratch of " + std::to_string(dpmi::kNoAllocVirialRows) +
+                                  " rows (use dpmi_prod_virial_a_f64)");
+    hipStream_t st = S(stream);
+    double* vpart = dpmi::noalloc_virial_scratch();
+    // every row r < nloc of the scratch is written before it is summed
+    dpmi::launch_prod_virial_a<double>(vpart, atom_virial, net_deriv, env_deriv, rij, nlist, nloc, nall, nnei, st);
+    if (nloc > 0)
+      dpmi::launch_reduce_rows9<double>(virial, vpart, nloc, st);
+    else
+      DPMI_CHECK(hipMemsetAsync(virial, 0, sizeof(double) * 9, st));
+    return kOk;
+  });
+}
 int dpmi_prod_force_grad_a_f64(double* grad_net, const double* grad, const double* env_deriv, const int* nlist,
                                int nloc, int nnei, int nframes, void* stream) {
   return prod_force_grad_impl(grad_net, grad, env_deriv, nlist, nloc, nnei, nframes, stream);

@@ -197,6 +197,10 @@ void launch_prod_virial_a(double* vpart, T* atom_virial, const T* net_deriv, con
                           const int* nlist, int nloc, int nall, int nnei, hipStream_t st);
 template <typename T>
 void launch_reduce_rows9(T* out9, const double* part, int nrows, hipStream_t st);
+// Rows of the fixed virial scratch (noalloc.hip) and its device address on
+// the current device (looked up once per device, never allocated).
+constexpr int kNoAllocVirialRows = 1 << 17;
+double* noalloc_virial_scratch();
 template <typename T>
 void launch_prod_force_grad_a(T* grad_net, const T* grad, const T* env_deriv, const int* nlist, int nloc, int nnei,
                               int nframes, hipStream_t st);

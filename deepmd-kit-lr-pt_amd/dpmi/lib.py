@@ -40,6 +40,7 @@ SIGNATURES = {
     "dpmi_tabulate_fusion_se_a_grad_grad_f32": ([_P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P], _I),
     "dpmi_prod_force_a_f64": ([_P, _P, _P, _P, _I, _I, _I, _I, _P], _I),
     "dpmi_prod_virial_a_f64": ([_P, _P, _P, _P, _P, _P, _I, _I, _I, _P], _I),
+    "dpmi_prod_virial_a_f64_noalloc": ([_P, _P, _P, _P, _P, _P, _I, _I, _I, _P], _I),
     "dpmi_prod_force_grad_a_f64": ([_P, _P, _P, _P, _I, _I, _I, _P], _I),
     "dpmi_prod_virial_grad_a_f64": ([_P, _P, _P, _P, _P, _I, _I, _P], _I),
     "dpmi_prod_force_a_f32": ([_P, _P, _P, _P, _I, _I, _I, _I, _P], _I),

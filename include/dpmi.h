@@ -117,6 +117,18 @@ int dpmi_prod_force_a_f64(double* force, const double* net_deriv, const double* 
 int dpmi_prod_virial_a_f64(double* virial, double* atom_virial, const double* net_deriv,
                            const double* env_deriv, const double* rij, const int* nlist, int nloc, int nall,
                            int nnei, void* stream);
+/* No counterpart in the reference: prod_virial_a_gpu<double> as
+ * dpmi_prod_virial_a_f64, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The per-row partials go to a fixed device array of the
+ * library (9 * 131072 doubles = 9 MiB per GPU, held until the process ends):
+ * nloc above 131072 returns DPMI_ERR_ARGUMENT.  No hipMalloc/hipFree is
+ * called.  Asynchronous on `stream`; every call on a GPU shares that array,
+ * so calls that overlap on one GPU (different streams, or threads) race and
+ * must be serialised by the caller. */
+int dpmi_prod_virial_a_f64_noalloc(double* virial, double* atom_virial, const double* net_deriv,
+                                   const double* env_deriv, const double* rij, const int* nlist, int nloc, int nall,
+                                   int nnei, void* stream);
 /* prod_force_grad.h:27-34 prod_force_grad_a_gpu<double>. */
 int dpmi_prod_force_grad_a_f64(double* grad_net, const double* grad, const double* env_deriv,
                                const int* nlist, int nloc, int nnei, int nframes, void* stream);
