@@ -75,6 +75,10 @@ int flag_to_status(int flag) {
   if (flag & dpmi::kFlagGhostCapacity) return fail(DPMI_ERR_MEMORY, "nall exceeds mem_nall");
   if (flag & dpmi::kFlagIndexRange) return fail(DPMI_ERR_RANGE, "extended index >= 2^24 in the neighbour sort key");
   if (flag & dpmi::kFlagTypeRange) return fail(DPMI_ERR_RANGE, "atom type >= ntypes in the neighbour sort key");
+  if (flag & dpmi::kFlagHalfRange)
+    return fail(DPMI_ERR_RANGE, "a mixed-precision matrix-core operand exceeds the FP16 range");
+  if (flag & dpmi::kFlagTypeMismatch)
+    return fail(DPMI_ERR_ARGUMENT, "the local atom types differ from the types the engine was created for");
   return kOk;
 }
 
@@ -394,6 +398,34 @@ int dpmi_prod_virial_grad_a_f32(float* grad_net, const float* grad, const float*
   return prod_virial_grad_impl(grad_net, grad, env_deriv, rij, nlist, nloc, nnei, stream);
 }
 
+int dpmi_tabulate_build_se_a(double* table_host, int* nspline_host, int nlayers, const int* width_host,
+                             const double* const* w_host, const double* const* b_host,
+                             const double* const* idt_host, double lower, double upper, double extrapolate,
+                             double stride0, double stride1, void* stream) {
+  return guarded([&] {
+    if (!nspline_host || nlayers < 1 || !width_host || !w_host || !b_host)
+      throw std::invalid_argument("null argument");
+    const int ns = dpmi::tab_nspline(lower, upper, extrapolate, stride0, stride1);
+    if (!table_host) {
+      *nspline_host = ns;
+      return kOk;
+    }
+    dpmi::TabNet net;
+    net.width.assign(width_host, width_host + nlayers + 1);
+    for (int l = 0; l < nlayers; ++l) {
+      net.w.push_back(w_host[l]);
+      net.b.push_back(b_host[l]);
+      net.idt.push_back(idt_host ? idt_host[l] : nullptr);
+    }
+    std::vector<double> tab;
+    int n = 0;
+    dpmi::build_table_device(net, lower, upper, extrapolate, stride0, stride1, tab, n, S(stream));
+    std::memcpy(table_host, tab.data(), sizeof(double) * tab.size());
+    *nspline_host = n;
+    return kOk;
+  });
+}
+
 // ---------------------------------------------------------------- engine --
 int dpmi_sea_model_create(dpmi_sea_model** model, const dpmi_sea_model_desc* desc) {
   return guarded([&] {
@@ -444,6 +476,20 @@ int dpmi_sea_engine_compute(dpmi_sea_engine* engine, const double* coord, const 
   return guarded([&] {
     if (!engine || !coord || !box_host || !energy || !force || !virial) throw std::invalid_argument("null argument");
     engine->impl->compute(coord, box_host, energy, force, virial, atom_energy, atom_virial, S(stream));
+    return kOk;
+  });
+}
+
+int dpmi_sea_engine_compute_lower(dpmi_sea_engine* engine, const double* coord_ext, const int* atype_ext, int nall,
+                                  int inum, const int* ilist, const int* numneigh, const int* jlist, int jstride,
+                                  double* energy, double* force_ext, double* virial, double* atom_energy,
+                                  double* atom_virial_ext, void* stream) {
+  return guarded([&] {
+    if (!engine || !coord_ext || !atype_ext || !ilist || !numneigh || (!jlist && jstride > 0) || !energy ||
+        !force_ext || !virial)
+      throw std::invalid_argument("null argument");
+    engine->impl->compute_lower(coord_ext, atype_ext, nall, inum, ilist, numneigh, jlist, jstride, energy, force_ext,
+                                virial, atom_energy, atom_virial_ext, S(stream));
     return kOk;
   });
 }
@@ -531,11 +577,8 @@ int dpmi_sea_engine_set_tuning(dpmi_sea_engine* engine, const int* knobs_host) {
 
 int dpmi_sea_engine_status(dpmi_sea_engine* engine) {
   return guarded([&] {
-    const int s = engine->impl->status();
-    if (s != DPMI_OK) return flag_to_status(s == DPMI_ERR_NLIST_CAPACITY ? dpmi::kFlagNlistCapacity
-                                            : s == DPMI_ERR_MEMORY     ? dpmi::kFlagGhostCapacity
-                                                                       : dpmi::kFlagIndexRange);
-    return kOk;
+    if (!engine) throw std::invalid_argument("null engine");
+    return flag_to_status(engine->impl->status_flags());
   });
 }
 

@@ -23,6 +23,7 @@ enum DeviceFlag : int {
   kFlagTypeRange = 8,         // atom type >= 255 (sort key)
   kFlagCellCapacity = 16,     // a cell holds more atoms than the binning assumes
   kFlagHalfRange = 32,        // a mixed-precision matrix-core operand exceeds the FP16 range
+  kFlagTypeMismatch = 64,     // a caller's local atom type differs from the engine's
 };
 
 struct HipError : std::runtime_error {

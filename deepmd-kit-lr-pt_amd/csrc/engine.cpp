@@ -1020,6 +1020,90 @@ void SeaEngine::enqueue_step(const double* coord, const double* box_h, double* e
   if (timing && timed_calls < kMaxTimedCalls) ++timed_calls;
 }
 
+// forward_lower (make_model.py:372-453) on the lib's external-list
+// conventions (prod_env_mat_multi_device.cc:2313-2396): the caller owns the
+// ghosts and the candidate rows; the rows are formatted as format_nlist does
+// (fmt_nlist.cc:99-177: FP32 distance test at rcut, so an rcut + skin list is
+// cut, sorted by (type, r^2, index), truncated per type at sel), the
+// environment rows are computed from the caller's coordinates, and every
+// force and per-atom virial term stays on the extended atom it belongs to,
+// as extended_force / extended_virial (transform_output.py:70-101).
+void SeaEngine::compute_lower(const double* coord_ext, const int* atype_ext, int nall, int inum, const int* ilist,
+                              const int* numneigh, const int* jlist, int jstride, double* energy, double* force_ext,
+                              double* virial, double* atom_energy, double* atom_virial_ext, hipStream_t st) {
+  const SeaModel& m = *model;
+  if (nframes != 1) throw std::invalid_argument("the lower interface takes engines of one frame");
+  if (nall < ntot) throw std::invalid_argument("nall must be at least the engine's local atom count");
+  if (inum != ntot) throw std::invalid_argument("inum must equal the number of local atoms");
+  if (jstride < 0 || jstride > 4096) throw std::invalid_argument("jstride (the longest candidate row) must be <= 4096");
+  if (!recompute_slots) throw Unsupported("the lower interface needs nnei < 4096");
+  last_stream = st;
+  if (!geom_uploaded) {  // one frame at offset 0 (the force kernel reads the frame layout)
+    std::memset(geom_pinned, 0, sizeof(FrameGeom));
+    DPMI_CHECK(hipMemcpyAsync(d_geom, geom_pinned, sizeof(FrameGeom), hipMemcpyHostToDevice, st));
+    DPMI_CHECK(hipMemsetAsync(d_frame_ext_off, 0, sizeof(int), st));
+    geom_uploaded = true;
+  }
+  launch_lower_prep(force_ext, atom_virial_ext, nall, atype_ext, d_type, ntot, m.ntypes, d_flag, st);
+  launch_fill_int(d_nlist, -1, (long long)ntot * m.nnei, st);
+  NlistArgs na;
+  na.cap = nlist_row_capacity(std::max(jstride, 64));
+  na.sec = m.d_sec;
+  na.ntypes = m.ntypes;
+  na.nnei = m.nnei;
+  na.rc2f = m.rcut * m.rcut;
+  na.nlist = d_nlist;
+  launch_row_format(inum, ilist, numneigh, jlist, jstride, coord_ext, atype_ext, ntot, nall, na, d_flag, st);
+  EnvArgs ea;
+  ea.coord = coord_ext;
+  ea.type = atype_ext;
+  ea.nlist = d_nlist;
+  ea.strided_nloc = ntot;
+  ea.nrows = ntot;
+  ea.nnei = m.nnei;
+  ea.nall_stride = nall;
+  ea.ntypes = m.ntypes;
+  ea.avg = m.d_davg;
+  ea.std_ = m.d_dstd;
+  ea.rmin = (double)m.rcut_smth;
+  ea.rmax = (double)m.rcut;
+  ea.tpb = tune.env_tpb;
+  int stage = 2;
+  if (precision == DPMI_FP64) {
+    ea.em = d_em;
+    launch_env_mat_a(ea, st);
+    compute_fp64_descriptor_fitting(st, stage);
+  } else {
+    launch_env_mat_a_f32(ea, d_em_f, nullptr, nullptr, st);
+    compute_mixed_descriptor_fitting(st, stage);
+  }
+  ForceRowsArgs fa;
+  fa.net_deriv = precision == DPMI_FP64 ? (const void*)d_net_deriv : (const void*)d_net_deriv_f;
+  fa.fp32 = precision != DPMI_FP64;
+  fa.coord_ext = coord_ext;
+  fa.type_ext = d_type;  // centre types: the engine's (validated) local types
+  fa.dstd = m.d_dstd;
+  fa.rmin = (double)m.rcut_smth;
+  fa.rmax = (double)m.rcut;
+  fa.nlist = d_nlist;
+  fa.row_frame = d_frame_of;
+  fa.row_center = d_center;
+  fa.frame_ext_off = d_frame_ext_off;
+  fa.nrows = ntot;
+  fa.nnei = m.nnei;
+  fa.vpart = d_vpart;
+  fa.sorted_loc = d_center;  // rows in the caller's order
+  fa.geom = d_geom;
+  fa.rows = tune.force_rows;
+  fa.unfolded = true;
+  fa.force = force_ext;
+  fa.atom_virial = atom_virial_ext;
+  launch_force_virial_rows(fa, st);
+  launch_frame_reduce(d_atom_e, d_vpart, d_chunks, nchunks, d_chunk_start, nframes, d_part, d_ticket, energy, virial,
+                      st);
+  if (atom_energy) launch_copy_double(atom_energy, d_atom_e, ntot, st);
+}
+
 // Tuned launch shapes per (precision, log2 of the packed atom count).
 namespace {
 std::mutex g_tune_mu;
@@ -1166,17 +1250,14 @@ void SeaEngine::launch_graph(hipStream_t st) {
   last_stream = st;
 }
 
-int SeaEngine::status() {
+int SeaEngine::status_flags() {
   DPMI_CHECK(hipStreamSynchronize(last_stream));
   int flag = 0;
   DPMI_CHECK(hipMemcpy(&flag, d_flag, sizeof(int), hipMemcpyDeviceToHost));
   if (flag) DPMI_CHECK(hipMemset(d_flag, 0, sizeof(int)));
   flag |= sticky;
   sticky = 0;
-  if (flag & kFlagNlistCapacity) return DPMI_ERR_NLIST_CAPACITY;
-  if (flag & kFlagGhostCapacity) return DPMI_ERR_MEMORY;
-  if (flag & (kFlagIndexRange | kFlagTypeRange | kFlagHalfRange)) return DPMI_ERR_RANGE;
-  return DPMI_OK;
+  return flag;
 }
 
 long long SeaEngine::fetch(int frame, int what, void* dst) {

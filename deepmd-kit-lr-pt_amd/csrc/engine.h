@@ -66,7 +66,15 @@ struct SeaEngine {
   // overflowed, grows that capacity and recomputes the step.
   void compute(const double* coord, const double* box_h, double* energy, double* force, double* virial,
                double* atom_energy, double* atom_virial, hipStream_t st);
-  int status();
+  // device status bits (DeviceFlag) of the steps since the last call; cleared
+  int status_flags();
+  // forward_lower: one frame whose nloc locals (the engine's atoms, same
+  // types) come first among nall caller-owned extended atoms; candidate rows
+  // (any order, any radius >= rcut) at jlist + r * jstride; outputs on the
+  // extended atoms (no ghost fold)
+  void compute_lower(const double* coord_ext, const int* atype_ext, int nall, int inum, const int* ilist,
+                     const int* numneigh, const int* jlist, int jstride, double* energy, double* force_ext,
+                     double* virial, double* atom_energy, double* atom_virial_ext, hipStream_t st);
   bool grow = true;
   int regrows = 0;  // capacity increases made by compute()
   long long fetch(int frame, int what, void* dst);

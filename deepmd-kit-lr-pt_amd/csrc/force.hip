@@ -363,7 +363,7 @@ __global__ void __launch_bounds__(1024) k_force_virial_hash(
 #pragma unroll
         for (int a = 0; a < 4; ++a) n4[u][a] = nd[b * 4 + a];
         const size_t e = (size_t)eo + (jj[u] >= 0 ? jj[u] : i);
-        own[u] = map_ext[e];
+        own[u] = map_ext ? map_ext[e] : (int)e;  // no map: keyed by the extended atom itself
         if constexpr (RECOMP) {
 #pragma unroll
           for (int d = 0; d < 3; ++d) cj[u][d] = coord_ext[e * 3 + d];
@@ -400,7 +400,7 @@ __global__ void __launch_bounds__(1024) k_force_virial_hash(
 #pragma unroll
     for (int k = 0; k < 9; ++k) vs[k] = wave_sum(vs[k]);
     if (lane == 0) {
-      const int h = hash_slot<HB>(hkey, lo + map_ext[eo + i]);
+      const int h = hash_slot<HB>(hkey, lo + (map_ext ? map_ext[eo + i] : eo + i));
 #pragma unroll
       for (int d = 0; d < 3; ++d) atomicAdd(&hval[h][d], fi[d]);
     }
@@ -570,7 +570,33 @@ DPMI_FORCE_INST(float)
 bool force_recompute_supported(int nnei) { return nnei + 1 <= kHashSize; }
 
 bool force_rows_aggregated(const ForceRowsArgs& a) {
-  return a.sorted_loc && a.geom && a.map_ext && a.force && a.nnei + 1 <= kHashSize;
+  return a.sorted_loc && a.geom && (a.map_ext || a.unfolded) && a.force && a.nnei + 1 <= kHashSize;
+}
+
+// Lower path set-up: clears the extended force [nall, 3] and per-atom virial
+// [nall, 9] outputs and checks the caller's types: locals must equal the
+// engine's types, every type must be below ntypes.
+__global__ void k_lower_prep(double* __restrict__ force_ext, double* __restrict__ atom_virial_ext, int nall,
+                             const int* __restrict__ atype_ext, const int* __restrict__ type_loc, int nloc,
+                             int ntypes, int* __restrict__ flag) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nall) return;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) force_ext[(size_t)e * 3 + d] = 0.;
+  if (atom_virial_ext)
+#pragma unroll
+    for (int k = 0; k < 9; ++k) atom_virial_ext[(size_t)e * 9 + k] = 0.;
+  const int t = atype_ext[e];
+  if (e < nloc && t != type_loc[e]) atomicOr(flag, kFlagTypeMismatch);
+  if (t >= ntypes) atomicOr(flag, kFlagTypeRange);
+}
+
+void launch_lower_prep(double* force_ext, double* atom_virial_ext, int nall, const int* atype_ext,
+                       const int* type_loc, int nloc, int ntypes, int* flag, hipStream_t st) {
+  if (nall <= 0) return;
+  hipLaunchKernelGGL(k_lower_prep, dim3(ceil_div(nall, 256)), dim3(256), 0, st, force_ext, atom_virial_ext, nall,
+                     atype_ext, type_loc, nloc, ntypes, flag);
+  DPMI_LAUNCH_CHECK();
 }
 
 void launch_force_virial_rows(const ForceRowsArgs& a, hipStream_t st) {

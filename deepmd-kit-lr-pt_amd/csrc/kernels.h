@@ -121,6 +121,20 @@ template <typename T>
 int launch_tab_grad_grad(T* dz_dy, const T* table, const T* info_host, const T* em_x, const T* em, const T* dz_dx,
                          const T* dz_dem, int nloc, int nnei, int M, int sorted, int nd, hipStream_t st);
 
+// ---------------------------------------------------------- tablegen.hip --
+// One embedding net (host pointers): width = {1, n0, ..., M}; per layer
+// w [n_in, n_out] row-major, b [n_out], idt [n_out] or null.
+struct TabNet {
+  std::vector<int> width;
+  std::vector<const double*> w, b, idt;
+};
+int tab_nspline(double lower, double upper, double extrapolate, double stride0, double stride1);
+void tab_knots(std::vector<double>& xx, double lower, double upper, double extrapolate, double stride0,
+               double stride1);
+// table [nspline, 6 M] (host) of the net over [lower, extrapolate * upper]
+void build_table_device(const TabNet& net, double lower, double upper, double extrapolate, double stride0,
+                        double stride1, std::vector<double>& table, int& nspline, hipStream_t st);
+
 // ------------------------------------------------------------- embed.hip --
 // Engine descriptor: table embedding + contraction + descriptor (fwd) and the
 // fused adjoint from dE/dD to net_deriv (bwd).  T: table/arithmetic type,
@@ -240,7 +254,14 @@ struct ForceRowsArgs {
   double* force = nullptr;
   // per-atom virials [ntot, 9] keyed like the forces (zeroed by caller), or null
   double* atom_virial = nullptr;
+  // no ghost fold: terms go to the extended atom itself (force and
+  // atom_virial are [nall, 3] / [nall, 9], map_ext unused)
+  bool unfolded = false;
 };
+// clears force_ext / atom_virial_ext [nall] and flags local types that differ
+// from type_loc (kFlagTypeMismatch) and types >= ntypes (kFlagTypeRange)
+void launch_lower_prep(double* force_ext, double* atom_virial_ext, int nall, const int* atype_ext,
+                       const int* type_loc, int nloc, int ntypes, int* flag, hipStream_t st);
 // whether launch_force_virial_rows takes the LDS-aggregated kernel (which
 // writes owner forces) for these arguments
 bool force_rows_aggregated(const ForceRowsArgs& a);

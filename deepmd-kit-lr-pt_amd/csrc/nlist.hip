@@ -301,6 +301,10 @@ __global__ void __launch_bounds__(64) k_row_format(int inum, const int* __restri
   for (int t = lane; t <= ntypes; t += kWave) sec[t] = sec_g[t];
   const int f = r / nloc;
   const int i = ilist[r];
+  if (i < 0 || i >= nloc) {  // a centre outside the frame: flagged, its row is not written
+    if (lane == 0) atomicOr(flag, kFlagIndexRange);
+    return;
+  }
   const double* cf = coord + (size_t)f * nall * 3;
   const int* tf = type + (size_t)f * nall;
   const double ci[3] = {cf[3 * i], cf[3 * i + 1], cf[3 * i + 2]};

@@ -137,6 +137,37 @@ class Engine:
                                              C.c_void_p(self.atom_virial.data_ptr()) if atom_virial else None, st))
         return self.energy, self.force, self.virial
 
+    def compute_lower(self, coord_ext: torch.Tensor, atype_ext: torch.Tensor, ilist: torch.Tensor,
+                      numneigh: torch.Tensor, jlist: torch.Tensor, atom_virial: bool = False):
+        """forward_lower with the caller's ghosts and candidate rows (one
+        frame): coord_ext [nall, 3] float64, atype_ext [nall] int32 (the
+        engine's local types first), ilist / numneigh [nloc] int32, jlist
+        [nloc, jstride] int32 rows (any order, rcut + skin allowed), all HIP
+        tensors.  Returns (energy [1], force_ext [nall, 3], virial [9],
+        atom_energy [nloc], atom_virial_ext [nall, 9] or None) as new
+        tensors."""
+        nall = coord_ext.shape[0]
+        if not (coord_ext.dtype == torch.float64 and coord_ext.is_cuda and coord_ext.is_contiguous()
+                and coord_ext.numel() == 3 * nall):
+            raise ValueError("coord_ext must be a contiguous [nall, 3] float64 HIP tensor")
+        for name, t in (("atype_ext", atype_ext), ("ilist", ilist), ("numneigh", numneigh), ("jlist", jlist)):
+            if not (t.dtype == torch.int32 and t.is_cuda and t.is_contiguous()):
+                raise ValueError(f"{name} must be a contiguous int32 HIP tensor")
+        if atype_ext.numel() != nall or jlist.dim() != 2 or jlist.shape[0] != ilist.numel():
+            raise ValueError("atype_ext must hold nall types and jlist one row per ilist entry")
+        dev = coord_ext.device
+        e = torch.empty(1, dtype=torch.float64, device=dev)
+        f = torch.empty((nall, 3), dtype=torch.float64, device=dev)
+        v = torch.empty(9, dtype=torch.float64, device=dev)
+        ae = torch.empty(self.ntot, dtype=torch.float64, device=dev)
+        av = torch.empty((nall, 9), dtype=torch.float64, device=dev) if atom_virial else None
+        st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        p = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+        check(load().dpmi_sea_engine_compute_lower(
+            self.handle, p(coord_ext), p(atype_ext), nall, ilist.numel(), p(ilist), p(numneigh), p(jlist),
+            int(jlist.shape[1]), p(e), p(f), p(v), p(ae), p(av) if av is not None else None, st))
+        return e, f, v, ae, av
+
     def set_growth(self, on: bool = True) -> None:
         """Capacity growth of eager steps (on by default): an overflowing
         ghost or neighbour-row buffer is grown and the step recomputed."""

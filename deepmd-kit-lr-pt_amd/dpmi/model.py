@@ -245,8 +245,11 @@ class SeAModel:
         stride0: float = 0.01,
         stride1: float = 0.1,
         check_frequency: float = -1.0,
+        native: bool = False,
     ) -> None:
-        """Quintic Hermite table per embedding net (tabulate.py:67-370).
+        """Quintic Hermite table per embedding net (tabulate.py:67-370);
+        with native, built by the library's HIP builder
+        (dpmi_tabulate_build_se_a) instead of numpy.
 
         Knots: arange(lower, upper, stride0) ++ arange(upper, ext*upper,
         stride1) ++ [ext*upper], with lower = floor(-davg/dstd) and upper =
@@ -254,20 +257,14 @@ class SeAModel:
         On each interval of width t the coefficients a0..a5 are the unique
         quintic matching value, slope and curvature at both knots.
         """
-        sw = _switch(min_nbor_dist, self.rcut_smth, self.rcut)
-        davg0 = self.davg[:, 0, 0]
-        dstd0 = self.dstd[:, 0, 0]
-        lower_t = np.floor(-davg0 / dstd0)
-        upper_t = np.ceil(((1.0 / min_nbor_dist) * sw - davg0) / dstd0)
+        if native:
+            from .tables import build_tables_native
+
+            self.tables, self.table_info = build_tables_native(self, min_nbor_dist, extrapolate, stride0, stride1,
+                                                               check_frequency)
+            return
         tables, infos = [], []
-        for net in range(self.nnets):
-            if self.type_one_side:
-                uu = float(np.max(upper_t))
-                ll = float(np.min(lower_t))
-            else:
-                ti = net % self.ntypes
-                uu = float(upper_t[ti])
-                ll = float(lower_t[ti])
+        for net, (ll, uu) in enumerate(self.table_ranges(min_nbor_dist)):
             xx = np.arange(ll, uu, stride0, dtype=np.float64)
             xx = np.append(xx, np.arange(uu, extrapolate * uu, stride1, dtype=np.float64))
             xx = np.append(xx, np.array([extrapolate * uu], dtype=np.float64))
@@ -284,13 +281,36 @@ class SeAModel:
             coef[:, :, 0] = y0
             coef[:, :, 1] = g0
             coef[:, :, 2] = 0.5 * c0
-            coef[:, :, 3] = (20 * hh - (8 * g1 + 12 * g0) * tt - (3 * c0 - c1) * tt * tt) / (2 * tt**3)
-            coef[:, :, 4] = (-30 * hh + (14 * g1 + 16 * g0) * tt + (3 * c0 - 2 * c1) * tt * tt) / (2 * tt**4)
-            coef[:, :, 5] = (12 * hh - 6 * (g1 + g0) * tt + (c1 - c0) * tt * tt) / (2 * tt**5)
+            # the reference's expression order (tabulate.py:305-363)
+            coef[:, :, 3] = (1 / (2 * tt * tt * tt)) * (20 * hh - (8 * g1 + 12 * g0) * tt - (3 * c0 - c1) * tt * tt)
+            coef[:, :, 4] = (1 / (2 * tt * tt * tt * tt)) * (
+                -30 * hh + (14 * g1 + 16 * g0) * tt + (3 * c0 - 2 * c1) * tt * tt)
+            coef[:, :, 5] = (1 / (2 * tt * tt * tt * tt * tt)) * (
+                12 * hh - 6 * (g1 + g0) * tt + (c1 - c0) * tt * tt)
             tables.append(np.ascontiguousarray(coef.reshape(nspline, 6 * M)))
             infos.append([ll, uu, uu * extrapolate, stride0, stride1, check_frequency])
         self.tables = tables
         self.table_info = np.asarray(infos, dtype=np.float64)
+
+    def table_ranges(self, min_nbor_dist: float) -> list:
+        """(lower, upper) of every net's table (tabulate.py:67-140, 468-486):
+        per centre type floor(-davg/dstd) and ceil((s(d_min)/d_min - davg)/dstd)
+        of the first column; one-sided nets take the extremes over the types
+        with sel > 0."""
+        sw = _switch(min_nbor_dist, self.rcut_smth, self.rcut)
+        davg0 = self.davg[:, 0, 0]
+        dstd0 = self.dstd[:, 0, 0]
+        lower_t = np.floor(-davg0 / dstd0)
+        upper_t = np.ceil(((1.0 / min_nbor_dist) * sw - davg0) / dstd0)
+        out = []
+        for net in range(self.nnets):
+            if self.type_one_side:
+                idx = np.asarray(self.sel) > 0
+                out.append((float(np.min(lower_t[idx])), float(np.max(upper_t[idx]))))
+            else:
+                ti = net % self.ntypes
+                out.append((float(lower_t[ti]), float(upper_t[ti])))
+        return out
 
     def table_stack(self) -> tuple[np.ndarray, np.ndarray]:
         """All tables padded to a common row count -> [nnets, nspline, 6M]."""

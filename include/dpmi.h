@@ -146,6 +146,22 @@ int dpmi_prod_force_grad_a_f32(float* grad_net, const float* grad, const float* 
 int dpmi_prod_virial_grad_a_f32(float* grad_net, const float* grad, const float* env_deriv, const float* rij,
                                 const int* nlist, int nloc, int nnei, void* stream);
 
+/* deepmd/utils/tabulate.py:67-370 + tabulate_math.py:353-470 (the compressed
+ * table of one embedding net, DPTabulate.build/_build_lower/_make_data),
+ * built on the device: the net (nlayers layers, widths width_host[0..nlayers]
+ * with width_host[0] == 1; w_host[l] [n_in, n_out] row-major, b_host[l],
+ * idt_host[l] or NULL; idt_host itself may be NULL; all host arrays) is
+ * evaluated with its first and second derivative at the knots
+ * arange(lower, upper, stride0) ++ arange(upper, extrapolate*upper, stride1)
+ * ++ [extrapolate*upper], and table_host [nspline, 6*M] (host) receives the
+ * quintic Hermite coefficients per interval and column.  *nspline_host
+ * receives nspline = int((upper-lower)/stride0 + (extrapolate*upper-upper)/stride1);
+ * table_host NULL: only that count.  Synchronous. */
+int dpmi_tabulate_build_se_a(double* table_host, int* nspline_host, int nlayers, const int* width_host,
+                             const double* const* w_host, const double* const* b_host,
+                             const double* const* idt_host, double lower, double upper, double extrapolate,
+                             double stride0, double stride1, void* stream);
+
 /* ===================================================================== */
 /* 2. whole-path engine                                                  */
 /* ===================================================================== */
@@ -216,6 +232,26 @@ void dpmi_sea_engine_destroy(dpmi_sea_engine* engine);
 int dpmi_sea_engine_compute(dpmi_sea_engine* engine, const double* coord, const double* box_host,
                             double* energy, double* force, double* virial, double* atom_energy, double* atom_virial,
                             void* stream);
+/* forward_lower (deepmd/pt/model/model/make_model.py:372-453) with the lib's
+ * external neighbour list (source/op/tf/prod_env_mat_multi_device.cc:2313-2396,
+ * nei_mode 3/4 on the device, in the row layout of convert_nlist_gpu_device):
+ * one frame of an engine created with nframes == 1, whose nloc atoms (types
+ * fixed at creation) come first in coord_ext [nall, 3] / atype_ext [nall],
+ * followed by nall - nloc ghosts the caller owns (no periodic images are
+ * made).  Candidate row r of centre ilist[r] holds numneigh[r] extended
+ * indices at jlist + r*jstride (jstride <= 4096), in any order and at any
+ * radius >= rcut (an rcut + skin list is cut at rcut, sorted by
+ * (type, r, index) and truncated at sel per type, as format_nlist does);
+ * inum == nloc.  Outputs (device): energy [1], force_ext [nall, 3]
+ * (extended_force: ghost terms are not folded), virial [9], atom_energy
+ * [nloc] or NULL, atom_virial_ext [nall, 9] or NULL (extended_virial with the
+ * atomic correction).  Asynchronous; dpmi_sea_engine_status reports local
+ * types that differ from the engine's (DPMI_ERR_ARGUMENT), out-of-range
+ * types or indices (DPMI_ERR_RANGE) and rows longer than jstride. */
+int dpmi_sea_engine_compute_lower(dpmi_sea_engine* engine, const double* coord_ext, const int* atype_ext, int nall,
+                                  int inum, const int* ilist, const int* numneigh, const int* jlist, int jstride,
+                                  double* energy, double* force_ext, double* virial, double* atom_energy,
+                                  double* atom_virial_ext, void* stream);
 /* Capacity growth on (1, the default) or off (0): see dpmi_sea_engine_compute. */
 int dpmi_sea_engine_set_growth(dpmi_sea_engine* engine, int enable);
 
