@@ -17,6 +17,7 @@
 #include "dpmi_common.h"
 #include "engine.h"
 #include "kernels.h"
+#include "model_file.h"
 
 namespace {
 
@@ -438,6 +439,40 @@ int dpmi_sea_model_create(dpmi_sea_model** model, const dpmi_sea_model_desc* des
       throw;
     }
     *model = m;
+    return kOk;
+  });
+}
+
+int dpmi_sea_model_load(dpmi_sea_model** model, const char* path, void* stream) {
+  return guarded([&] {
+    if (!model || !path) throw std::invalid_argument("null argument");
+    const dpmi::ModelFile mf = dpmi::read_model_file(path);
+    auto* m = new dpmi_sea_model{nullptr};
+    try {
+      m->impl = dpmi::build_model(mf, S(stream));
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    *model = m;
+    return kOk;
+  });
+}
+
+int dpmi_sea_model_info(const dpmi_sea_model* model, int* ntypes_host, double* rcut_host, int* nnei_host,
+                        char* type_map_host, int type_map_len) {
+  return guarded([&] {
+    if (!model) throw std::invalid_argument("null model");
+    const dpmi::SeaModel& m = *model->impl;
+    if (ntypes_host) *ntypes_host = m.ntypes;
+    if (rcut_host) *rcut_host = (double)m.rcut;
+    if (nnei_host) *nnei_host = m.nnei;
+    if (type_map_host && type_map_len > 0) {
+      std::string s;
+      for (size_t k = 0; k < m.type_map.size(); ++k) s += (k ? " " : "") + m.type_map[k];
+      std::strncpy(type_map_host, s.c_str(), type_map_len - 1);
+      type_map_host[type_map_len - 1] = 0;
+    }
     return kOk;
   });
 }

@@ -318,7 +318,35 @@ __global__ void k_normalize(double* __restrict__ coord, int natom, FrameGeom g) 
   }
   for (int d = 0; d < 3; ++d) coord[3 * i + d] = f[0] * g.t[0 * 3 + d] + f[1] * g.t[1 * 3 + d] + f[2] * g.t[2 * 3 + d];
 }
+struct Box9f {
+  float t[9], rec[9];
+};
+__global__ void k_normalize_f32(float* __restrict__ coord, int natom, Box9f b) {
+#pragma clang fp contract(off)
+  // coord.cc:14-28 in FP32, as the float instantiation computes it
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= natom) return;
+  const float p[3] = {coord[3 * i], coord[3 * i + 1], coord[3 * i + 2]};
+  float f[3];
+  for (int d = 0; d < 3; ++d) f[d] = p[0] * b.rec[d * 3 + 0] + p[1] * b.rec[d * 3 + 1] + p[2] * b.rec[d * 3 + 2];
+  for (int d = 0; d < 3; ++d) {
+    f[d] = fmodf(f[d], 1.f);
+    if (f[d] < 0.f) f[d] += 1.f;
+  }
+  for (int d = 0; d < 3; ++d) coord[3 * i + d] = f[0] * b.t[0 * 3 + d] + f[1] * b.t[1 * 3 + d] + f[2] * b.t[2 * 3 + d];
+}
 }  // namespace
+
+void launch_normalize_f32(float* coord, int natom, const float* boxt9, const float* rec9, hipStream_t st) {
+  if (natom <= 0) return;
+  Box9f b;
+  for (int k = 0; k < 9; ++k) {
+    b.t[k] = boxt9[k];
+    b.rec[k] = rec9[k];
+  }
+  hipLaunchKernelGGL(k_normalize_f32, dim3(ceil_div(natom, 256)), dim3(256), 0, st, coord, natom, b);
+  DPMI_LAUNCH_CHECK();
+}
 
 void launch_normalize(double* coord, int natom, const FrameGeom& g, hipStream_t st) {
   if (natom <= 0) return;

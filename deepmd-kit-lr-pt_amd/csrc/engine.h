@@ -55,6 +55,7 @@ struct SeaModel {
   _Float16* d_tab16 = nullptr;
   long long* d_tab16_off = nullptr;
   float* d_tab16_pscale = nullptr;
+  std::vector<std::string> type_map;  // names of the types (model files)
 };
 
 struct SeaEngine {

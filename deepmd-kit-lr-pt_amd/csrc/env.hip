@@ -21,8 +21,8 @@ namespace {
 // intermediate are evaluated in FP64 and only the stored values are rounded).
 // DERIV false: em only (the mixed engine's force kernel recomputes the
 // derivative from the coordinates).
-template <typename TO, bool DERIV>
-__global__ void k_env_mat_a(const double* __restrict__ coord, const int* __restrict__ type,
+template <typename TO, bool DERIV, typename TC = double>
+__global__ void k_env_mat_a(const TC* __restrict__ coord, const int* __restrict__ type,
                             const int* __restrict__ nlist, const int* __restrict__ row_frame,
                             const int* __restrict__ row_ext_off, const int* __restrict__ row_center,
                             int strided_nloc, long long nrows, int nnei, int nall_stride, int ntypes,
@@ -54,7 +54,7 @@ __global__ void k_env_mat_a(const double* __restrict__ coord, const int* __restr
   for (int k = 0; k < 12; ++k) der[k] = 0.;
   double rr[3] = {0., 0., 0.};
   if (ti >= 0 && j >= 0) {
-    for (int d = 0; d < 3; ++d) rr[d] = coord[3 * (eo + j) + d] - coord[3 * (eo + i) + d];
+    for (int d = 0; d < 3; ++d) rr[d] = (double)coord[3 * (eo + j) + d] - (double)coord[3 * (eo + i) + d];
     envc::slot_terms<sizeof(TO) == 4, DERIV>(rr, rmin, rmax, val, der);
   }
   const size_t base = (size_t)g * nnei + s;
@@ -108,6 +108,18 @@ void launch_env_mat_a_f32(const EnvArgs& a, float* em, float* em_deriv, float* r
   const long long n = a.nrows * (long long)a.nnei;
   if (n <= 0) return;
   const int tpb = a.tpb > 0 ? a.tpb : 256;
+  if (a.coord_f) {  // FP32 coordinates (the float instantiation of prod_env_mat_a)
+    if (em_deriv)
+      hipLaunchKernelGGL((k_env_mat_a<float, true, float>), dim3(ceil_div(n, tpb)), dim3(tpb), 0, st, a.coord_f,
+                         a.type, a.nlist, a.row_frame, a.row_ext_off, a.row_center, a.strided_nloc, a.nrows, a.nnei,
+                         a.nall_stride, a.ntypes, a.avg, a.std_, a.rmin, a.rmax, em, em_deriv, rij);
+    else
+      hipLaunchKernelGGL((k_env_mat_a<float, false, float>), dim3(ceil_div(n, tpb)), dim3(tpb), 0, st, a.coord_f,
+                         a.type, a.nlist, a.row_frame, a.row_ext_off, a.row_center, a.strided_nloc, a.nrows, a.nnei,
+                         a.nall_stride, a.ntypes, a.avg, a.std_, a.rmin, a.rmax, em, em_deriv, rij);
+    DPMI_LAUNCH_CHECK();
+    return;
+  }
   if (em_deriv)
     hipLaunchKernelGGL((k_env_mat_a<float, true>), dim3(ceil_div(n, tpb)), dim3(tpb), 0, st, a.coord, a.type, a.nlist,
                        a.row_frame, a.row_ext_off, a.row_center, a.strided_nloc, a.nrows, a.nnei, a.nall_stride, a.ntypes, a.avg,

@@ -25,6 +25,24 @@ struct FrameGeom {
   int pad[2];
 };
 
+// Distance between opposite faces of the box along each cell vector
+// (SimulationRegion_Impl.h:364-372 toFaceDistance: volume / |a x b|).
+inline void geom_face_distances(const double* m, double* tf) {
+#pragma clang fp contract(off)
+  double vol = m[0] * (m[4] * m[8] - m[7] * m[5]) - m[1] * (m[3] * m[8] - m[6] * m[5]) +
+               m[2] * (m[3] * m[7] - m[6] * m[4]);
+  vol = std::fabs(vol);
+  const double* a[3] = {m, m + 3, m + 6};
+  for (int d = 0; d < 3; ++d) {
+    const double* u = a[(d + 1) % 3];
+    const double* v = a[(d + 2) % 3];
+    const double c0 = u[1] * v[2] - u[2] * v[1];
+    const double c1 = u[2] * v[0] - u[0] * v[2];
+    const double c2 = u[0] * v[1] - u[1] * v[0];
+    tf[d] = vol * (1. / std::sqrt(c0 * c0 + c1 * c1 + c2 * c2));
+  }
+}
+
 // Box inverse and cell grid with the same operation order as the reference
 // (region.cc:40-85, SimulationRegion_Impl.h:364-372, coord.cc:69-108,
 // neighbor_list.cc:758-766) so that fractional coordinates, cell indices and
@@ -47,16 +65,8 @@ inline void geom_from_box(FrameGeom& g, const double* boxt, float rcut) {
   r[5] = (-m[0] * m[7] + m[6] * m[1]) * vi;
   r[6] = (m[1] * m[5] - m[4] * m[2]) * vi;
   r[7] = (-m[0] * m[5] + m[3] * m[2]) * vi;
-  const double* a[3] = {m, m + 3, m + 6};
   double tf[3];
-  for (int d = 0; d < 3; ++d) {
-    const double* u = a[(d + 1) % 3];
-    const double* v = a[(d + 2) % 3];
-    const double c0 = u[1] * v[2] - u[2] * v[1];
-    const double c1 = u[2] * v[0] - u[0] * v[2];
-    const double c2 = u[0] * v[1] - u[1] * v[0];
-    tf[d] = vol * (1. / std::sqrt(c0 * c0 + c1 * c1 + c2 * c2));
-  }
+  geom_face_distances(m, tf);
   const double rc = rcut;
   for (int d = 0; d < 3; ++d) {
     int n = (int)(tf[d] / rc);

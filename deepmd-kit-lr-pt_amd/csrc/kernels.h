@@ -52,6 +52,9 @@ void launch_build_extended(const CellWork& w, hipStream_t st);
 int copy_coord_run(FrameGeom g, double* out_c, int* out_t, int* mapping, int* nall_host, const double* in_c,
                    const int* in_t, int nloc, int mem_nall, void* stream);
 void launch_normalize(double* coord, int natom, const FrameGeom& g, hipStream_t st);
+// normalize_coord_gpu<float>: FP32 arithmetic with the caller's box and
+// reciprocal box (host copies of the Region's rows)
+void launch_normalize_f32(float* coord, int natom, const float* boxt9, const float* rec9, hipStream_t st);
 
 // ------------------------------------------------------------- nlist.hip --
 struct NlistArgs {
@@ -74,16 +77,21 @@ struct NlistArgs {
 };
 int nlist_row_capacity(int want);
 void launch_cell_nlist(const CellWork& w, const NlistArgs& a, hipStream_t st);
-// rows: jlist + r * jstride, or firstneigh[r] (device pointer array) if given
+// rows: jlist + r * jstride, or firstneigh[r] (device pointer array) if given;
+// TC: coordinate type (double or float)
+template <typename TC>
 void launch_row_format(int inum, const int* ilist, const int* numneigh, const int* jlist, int jstride,
-                       const double* coord, const int* type, int nloc, int nall, const NlistArgs& a, int* flag,
+                       const TC* coord, const int* type, int nloc, int nall, const NlistArgs& a, int* flag,
                        hipStream_t st, const int* const* firstneigh = nullptr);
-void launch_brute_nlist(const double* c, int nloc, int nall, int mem_size, double rc2, int nframes, const int* type,
+// T: coordinate / distance type of the pair test |x_i - x_j|^2 < rc2
+template <typename T>
+void launch_brute_nlist(const T* c, int nloc, int nall, int mem_size, T rc2, int nframes, const int* type,
                         int* ilist, int* numneigh, int* jlist, hipStream_t st, int* const* firstneigh = nullptr);
 
 // --------------------------------------------------------------- env.hip --
 struct EnvArgs {
   const double* coord = nullptr;
+  const float* coord_f = nullptr;  // FP32 coordinates instead (geometry still evaluated in FP64)
   const int* type = nullptr;
   const int* nlist = nullptr;
   // packed rows: row g -> frame row_frame[g], extended offset

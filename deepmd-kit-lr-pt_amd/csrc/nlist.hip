@@ -285,11 +285,11 @@ __global__ void __launch_bounds__(64) k_cell_nlist(const int* __restrict__ frame
 // Drop-in path: candidates from caller rows (InputNlist), one frame stride.
 // Rows are jlist + r * jstride, or firstneigh[r] when firstneigh is given
 // (InputNlist with device row pointers).
-template <int CAP>
+template <int CAP, typename TC>
 __global__ void __launch_bounds__(64) k_row_format(int inum, const int* __restrict__ ilist,
                                                    const int* __restrict__ numneigh, const int* __restrict__ jlist,
                                                    const int* const* __restrict__ firstneigh, int jstride,
-                                                   const double* __restrict__ coord, const int* __restrict__ type,
+                                                   const TC* __restrict__ coord, const int* __restrict__ type,
                                                    int nloc, int nall, const int* __restrict__ sec_g, int ntypes,
                                                    int nnei, float rc2f, int* __restrict__ nlist,
                                                    int* __restrict__ flag) {
@@ -305,9 +305,9 @@ __global__ void __launch_bounds__(64) k_row_format(int inum, const int* __restri
     if (lane == 0) atomicOr(flag, kFlagIndexRange);
     return;
   }
-  const double* cf = coord + (size_t)f * nall * 3;
+  const TC* cf = coord + (size_t)f * nall * 3;
   const int* tf = type + (size_t)f * nall;
-  const double ci[3] = {cf[3 * i], cf[3 * i + 1], cf[3 * i + 2]};
+  const double ci[3] = {(double)cf[3 * i], (double)cf[3 * i + 1], (double)cf[3 * i + 2]};
   int cnt = numneigh[r];
   if (!firstneigh && cnt > jstride) {  // a row longer than its stride: flagged, read up to the stride
     if (lane == 0) atomicOr(flag, kFlagNlistCapacity);
@@ -347,8 +347,10 @@ __global__ void __launch_bounds__(64) k_row_format(int inum, const int* __restri
 
 // Brute-force candidate rows in ascending j (the build_nlist contract):
 // one wave per row, lanes over j, ballot compaction keeps the order.
-__global__ void __launch_bounds__(256) k_brute_nlist(const double* __restrict__ c, int nloc, int nall, int mem_size,
-                                                     double rc2, int nframes, const int* __restrict__ type,
+// T: coordinate and distance type (the reference's FPTYPE).
+template <typename T>
+__global__ void __launch_bounds__(256) k_brute_nlist(const T* __restrict__ c, int nloc, int nall, int mem_size,
+                                                     T rc2, int nframes, const int* __restrict__ type,
                                                      int* __restrict__ ilist, int* __restrict__ numneigh,
                                                      int* __restrict__ jlist, int* const* __restrict__ firstneigh) {
 #pragma clang fp contract(off)
@@ -356,9 +358,9 @@ __global__ void __launch_bounds__(256) k_brute_nlist(const double* __restrict__ 
   const int lane = lane_id();
   if (r >= nframes * nloc) return;
   const int i = r % nloc, f = r / nloc;
-  const double* cf = c + (size_t)f * nall * 3;
+  const T* cf = c + (size_t)f * nall * 3;
   const int* tf = type ? type + (size_t)f * nall : nullptr;
-  const double ci[3] = {cf[3 * i], cf[3 * i + 1], cf[3 * i + 2]};
+  const T ci[3] = {cf[3 * i], cf[3 * i + 1], cf[3 * i + 2]};
   const bool ci_ok = !tf || tf[i] >= 0;
   int n = 0;
   int* out = firstneigh ? firstneigh[r] : jlist + (size_t)r * mem_size;
@@ -366,7 +368,7 @@ __global__ void __launch_bounds__(256) k_brute_nlist(const double* __restrict__ 
     const int j = j0 + lane;
     bool keep = false;
     if (j < nall && j != i && ci_ok && (!tf || tf[j] >= 0)) {
-      const double d0 = ci[0] - cf[3 * j], d1 = ci[1] - cf[3 * j + 1], d2 = ci[2] - cf[3 * j + 2];
+      const T d0 = ci[0] - cf[3 * j], d1 = ci[1] - cf[3 * j + 1], d2 = ci[2] - cf[3 * j + 2];
       keep = d0 * d0 + d1 * d1 + d2 * d2 < rc2;
     }
     const unsigned long long m = __ballot(keep);
@@ -406,23 +408,33 @@ void launch_cell_nlist(const CellWork& w, const NlistArgs& a, hipStream_t st) {
   DPMI_LAUNCH_CHECK();
 }
 
+template <typename TC>
 void launch_row_format(int inum, const int* ilist, const int* numneigh, const int* jlist, int jstride,
-                       const double* coord, const int* type, int nloc, int nall, const NlistArgs& a, int* flag,
+                       const TC* coord, const int* type, int nloc, int nall, const NlistArgs& a, int* flag,
                        hipStream_t st, const int* const* firstneigh) {
   if (inum <= 0) return;
-  DPMI_CAP_DISPATCH(a.cap, hipLaunchKernelGGL(k_row_format<CAP>, dim3(inum), dim3(kWave), 0, st, inum, ilist,
+  DPMI_CAP_DISPATCH(a.cap, hipLaunchKernelGGL((k_row_format<CAP, TC>), dim3(inum), dim3(kWave), 0, st, inum, ilist,
                                               numneigh, jlist, firstneigh, jstride, coord, type, nloc, nall, a.sec,
                                               a.ntypes, a.nnei, a.rc2f, a.nlist, flag));
   DPMI_LAUNCH_CHECK();
 }
+template void launch_row_format<double>(int, const int*, const int*, const int*, int, const double*, const int*, int,
+                                        int, const NlistArgs&, int*, hipStream_t, const int* const*);
+template void launch_row_format<float>(int, const int*, const int*, const int*, int, const float*, const int*, int,
+                                       int, const NlistArgs&, int*, hipStream_t, const int* const*);
 
-void launch_brute_nlist(const double* c, int nloc, int nall, int mem_size, double rc2, int nframes, const int* type,
+template <typename T>
+void launch_brute_nlist(const T* c, int nloc, int nall, int mem_size, T rc2, int nframes, const int* type,
                         int* ilist, int* numneigh, int* jlist, hipStream_t st, int* const* firstneigh) {
   const long long rows = (long long)nframes * nloc;
   if (rows <= 0) return;
-  hipLaunchKernelGGL(k_brute_nlist, dim3(ceil_div(rows * kWave, 256)), dim3(256), 0, st, c, nloc, nall, mem_size,
+  hipLaunchKernelGGL(k_brute_nlist<T>, dim3(ceil_div(rows * kWave, 256)), dim3(256), 0, st, c, nloc, nall, mem_size,
                      rc2, nframes, type, ilist, numneigh, jlist, firstneigh);
   DPMI_LAUNCH_CHECK();
 }
+template void launch_brute_nlist<double>(const double*, int, int, int, double, int, const int*, int*, int*, int*,
+                                         hipStream_t, int* const*);
+template void launch_brute_nlist<float>(const float*, int, int, int, float, int, const int*, int*, int*, int*,
+                                        hipStream_t, int* const*);
 
 }  // namespace dpmi

@@ -65,6 +65,26 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     return LIB
 
 
+HARNESS_SRC = os.path.join(os.path.dirname(ROOT), "tests", "cpp", "harness.cpp")
+HARNESS = os.path.join(os.path.dirname(ROOT), "tests", "cpp", "_bin", "dpmi_harness")
+
+
+def build_harness(verbose: bool = False) -> str:
+    """The C++ test caller of the drop-in library (tests/cpp/harness.cpp),
+    linked against libdpmi.so."""
+    os.makedirs(os.path.dirname(HARNESS), exist_ok=True)
+    inc = os.path.join(os.path.dirname(ROOT), "include")
+    cmd = [HIPCC, "-std=c++17", "-O2", "-Wall", f"-I{inc}", HARNESS_SRC, "-o", HARNESS + ".tmp", f"-L{PKG}", "-ldpmi",
+           f"-Wl,-rpath,{PKG}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"harness build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    shutil.move(HARNESS + ".tmp", HARNESS)
+    if verbose:
+        print("built", HARNESS)
+    return HARNESS
+
+
 if __name__ == "__main__":
     build(verbose=True)
     sys.exit(0)

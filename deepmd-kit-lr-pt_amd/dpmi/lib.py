@@ -49,6 +49,8 @@ SIGNATURES = {
     "dpmi_prod_virial_grad_a_f32": ([_P, _P, _P, _P, _P, _I, _I, _P], _I),
     "dpmi_tabulate_build_se_a": ([_P, _P, _I, _P, _P, _P, _P, _D, _D, _D, _D, _D, _P], _I),
     "dpmi_sea_model_create": ([_P, _P], _I),
+    "dpmi_sea_model_load": ([_P, C.c_char_p, _P], _I),
+    "dpmi_sea_model_info": ([_P, _P, _P, _P, _P, _I], _I),
     "dpmi_sea_model_destroy": ([_P], None),
     "dpmi_sea_engine_create": ([_P, _P, _I, _P, _P, _I, _D], _I),
     "dpmi_sea_engine_destroy": ([_P], None),

@@ -336,6 +336,49 @@ class SeAModel:
         return out
 
 
+MODEL_MAGIC = b"DPMISEA1"
+
+
+def save_model_file(model: SeAModel, path: str, min_nbor_dist: float, extrapolate: float = 5.0,
+                    stride0: float = 0.01, stride1: float = 0.1, check_frequency: float = -1.0,
+                    type_map=None) -> None:
+    """Writes the model as a dpmi model file, read natively by
+    dpmi_sea_model_load (which builds the compressed tables on the GPU with
+    the given compression settings).  Little-endian; see include/dpmi.h."""
+    import struct
+
+    nt = model.ntypes
+    emb = model.embed
+    fit = model.fitting
+    n_e, n_f = len(emb[0]), len(fit[0])
+    out = bytearray(MODEL_MAGIC)
+    out += struct.pack("<5i", nt, int(model.type_one_side), int(model.axis_neuron), n_e, n_f)
+    out += struct.pack("<2d", float(model.rcut), float(model.rcut_smth))
+    out += struct.pack(f"<{nt}i", *[int(v) for v in model.sel])
+    out += struct.pack(f"<{n_e + 1}i", 1, *[int(l.w.shape[1]) for l in emb[0]])
+    out += struct.pack(f"<{n_f + 1}i", int(fit[0][0].w.shape[0]), *[int(l.w.shape[1]) for l in fit[0]])
+    out += struct.pack(f"<{n_e}i", *[int(l.idt is not None) for l in emb[0]])
+    out += struct.pack(f"<{n_f}i", *[int(l.act) | (int(l.resnet) << 1) | (int(l.idt is not None) << 2)
+                                     for l in fit[0]])
+    out += np.ascontiguousarray(model.davg, np.float64).tobytes()
+    out += np.ascontiguousarray(model.dstd, np.float64).tobytes()
+    for net in list(emb) + list(fit):
+        for l in net:
+            out += np.ascontiguousarray(l.w, np.float64).tobytes() + np.ascontiguousarray(l.b, np.float64).tobytes()
+            if l.idt is not None:
+                out += np.ascontiguousarray(l.idt, np.float64).tobytes()
+    out += np.ascontiguousarray(model.bias_atom_e, np.float64).tobytes()
+    out += struct.pack("<5d", float(min_nbor_dist), float(extrapolate), float(stride0), float(stride1),
+                       float(check_frequency))
+    names = list(type_map) if type_map is not None else [f"T{t}" for t in range(nt)]
+    out += struct.pack("<i", len(names))
+    for nm in names:
+        b = nm.encode()
+        out += struct.pack("<i", len(b)) + b
+    with open(path, "wb") as f:
+        f.write(bytes(out))
+
+
 def _switch(x: float, rmin: float, rmax: float) -> float:
     if x < rmin:
         return 1.0

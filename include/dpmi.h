@@ -197,6 +197,18 @@ typedef struct dpmi_sea_model_desc {
 } dpmi_sea_model_desc;
 
 int dpmi_sea_model_create(dpmi_sea_model** model, const dpmi_sea_model_desc* desc);
+/* Loads a dpmi model file (an se_e2_a energy model as plain arrays: descriptor
+ * settings, davg/dstd, embedding and fitting weights, bias_atom_e, compression
+ * settings and type names; written by dpmi.model.save_model_file from a
+ * DeePMD-kit serialized model) and builds its compressed tables on the
+ * device (dpmi_tabulate_build_se_a).  No counterpart file format in the
+ * reference: it stands in for the frozen model that DeepPot loads
+ * (source/api_cc/src/DeepPotPT.cc:40-120). */
+int dpmi_sea_model_load(dpmi_sea_model** model, const char* path, void* stream);
+/* ntypes, rcut, nnei = sum(sel), and the type names separated by spaces
+ * (any output may be NULL). */
+int dpmi_sea_model_info(const dpmi_sea_model* model, int* ntypes_host, double* rcut_host, int* nnei_host,
+                        char* type_map_host, int type_map_len);
 void dpmi_sea_model_destroy(dpmi_sea_model* model);
 
 /* precision */

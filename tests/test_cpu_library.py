@@ -46,7 +46,19 @@ def test_deepmd_cpp_templates_exported():
                  "deepmd::prod_force_a_gpu<float>", "deepmd::prod_virial_a_gpu<float>",
                  "deepmd::prod_force_grad_a_gpu<float>", "deepmd::prod_virial_grad_a_gpu<float>",
                  "deepmd::build_nlist_gpu<double>", "deepmd::copy_coord_gpu<double>",
-                 "deepmd::normalize_coord_gpu<double>", "deepmd::tabulate_fusion_se_a_gpu<float>"):
+                 "deepmd::normalize_coord_gpu<double>", "deepmd::tabulate_fusion_se_a_gpu<float>",
+                 "deepmd::normalize_coord_gpu<float>", "deepmd::copy_coord_gpu<float>", "deepmd::build_nlist_gpu<float>",
+                 "deepmd::prod_env_mat_a_gpu<float>", "deepmd::format_nbor_list_gpu<double>",
+                 "deepmd::format_nbor_list_gpu<float>", "deepmd::compute_cell_info<double>",
+                 "deepmd::convert_nlist_gpu_device", "deepmd::free_nlist_gpu_device",
+                 "deepmd::validate_nlist_gpu_capacity", "deepmd::use_nlist_map", "deepmd::env_mat_nbor_update",
+                 "deepmd::normalize_coord_cpu<double>", "deepmd::copy_coord_cpu<double>",
+                 "deepmd::build_nlist_cpu<double>", "deepmd::format_nlist_cpu<double>",
+                 "deepmd::prod_env_mat_a_cpu<double>", "deepmd::tabulate_fusion_se_a_cpu<double>",
+                 "deepmd::tabulate_fusion_se_a_grad_cpu<float>", "deepmd::tabulate_fusion_se_a_grad_grad_cpu<double>",
+                 "deepmd::prod_force_a_cpu<double>", "deepmd::prod_virial_a_cpu<float>",
+                 "deepmd::prod_force_grad_a_cpu<double>", "deepmd::prod_virial_grad_a_cpu<double>",
+                 "dpmi::DeepPot::init", "dpmi::DeepPot::cutoff"):
         assert name in out, name
 
 
