@@ -151,7 +151,9 @@ struct DeepPot::Impl {
     if (mx > 4096) throw deepmd::deepmd_exception_nlist_capacity("a neighbour row holds more than 4096 atoms");
     jstride = std::max(64, (mx + 63) / 64 * 64);
     std::vector<int> packed((size_t)l.inum * jstride, -1);
-    for (int i = 0; i < l.inum; ++i) std::copy(l.firstneigh[i], l.firstneigh[i] + l.numneigh[i], packed.begin() + (size_t)i * jstride);
+    // common.cc:297-316: indices carry the caller's mask (LAMMPS special bits)
+    for (int i = 0; i < l.inum; ++i)
+      for (int k = 0; k < l.numneigh[i]; ++k) packed[(size_t)i * jstride + k] = l.firstneigh[i][k] & l.mask;
     hip_ok(hipMemcpyAsync(ilist.get(l.inum), l.ilist, sizeof(int) * l.inum, hipMemcpyHostToDevice, st), "ilist");
     hip_ok(hipMemcpyAsync(numneigh.get(l.inum), l.numneigh, sizeof(int) * l.inum, hipMemcpyHostToDevice, st),
            "numneigh");

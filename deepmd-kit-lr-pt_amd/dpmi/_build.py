@@ -65,23 +65,32 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     return LIB
 
 
-HARNESS_SRC = os.path.join(os.path.dirname(ROOT), "tests", "cpp", "harness.cpp")
-HARNESS = os.path.join(os.path.dirname(ROOT), "tests", "cpp", "_bin", "dpmi_harness")
+TESTS_CPP = os.path.join(os.path.dirname(ROOT), "tests", "cpp")
+HARNESS = os.path.join(TESTS_CPP, "_bin", "dpmi_harness")
+MD_DRIVER = os.path.join(TESTS_CPP, "_bin", "dpmi_md")
 
 
-def build_harness(verbose: bool = False) -> str:
-    """The C++ test caller of the drop-in library (tests/cpp/harness.cpp),
-    linked against libdpmi.so."""
-    os.makedirs(os.path.dirname(HARNESS), exist_ok=True)
+def _build_caller(src: str, out: str) -> None:
+    os.makedirs(os.path.dirname(out), exist_ok=True)
     inc = os.path.join(os.path.dirname(ROOT), "include")
-    cmd = [HIPCC, "-std=c++17", "-O2", "-Wall", f"-I{inc}", HARNESS_SRC, "-o", HARNESS + ".tmp", f"-L{PKG}", "-ldpmi",
+    cmd = [HIPCC, "-std=c++17", "-O2", "-Wall", f"-I{inc}", src, "-o", out + ".tmp", f"-L{PKG}", "-ldpmi",
            f"-Wl,-rpath,{PKG}"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"harness build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    shutil.move(HARNESS + ".tmp", HARNESS)
-    if verbose:
-        print("built", HARNESS)
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    shutil.move(out + ".tmp", out)
+
+
+def build_harness(verbose: bool = False) -> str:
+    """The C++ test callers of the drop-in library, linked against
+    libdpmi.so: tests/cpp/harness.cpp (the deepmd:: templates and DeepPot)
+    and tests/cpp/md_driver.cpp (NVE dynamics through the LAMMPS pair
+    style)."""
+    for src, out in ((os.path.join(TESTS_CPP, "harness.cpp"), HARNESS),
+                     (os.path.join(TESTS_CPP, "md_driver.cpp"), MD_DRIVER)):
+        _build_caller(src, out)
+        if verbose:
+            print("built", out)
     return HARNESS
 
 
