@@ -9,7 +9,6 @@
 // ghost fold and per-frame reductions.  Everything after the per-step box
 // upload runs asynchronously on the caller's stream.
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <map>
@@ -47,9 +46,6 @@ T* dupload(const T* h, size_t n) {
   return d;
 }
 
-void rb_check(rocblas_status s, const char* what) {
-  if (s != rocblas_status_success) throw HipError(std::string("rocBLAS error in ") + what + ": " + std::to_string(s));
-}
 
 // (XCD count, compute units) of the current device
 std::pair<int, int> chip_layout() {
@@ -184,6 +180,7 @@ SeaModel::SeaModel(const dpmi_sea_model_desc& d) {
   d_davg = dupload(davg.data(), davg.size());
   d_dstd = dupload(dstd.data(), dstd.size());
   fit_w.resize((size_t)ntypes * nlayers);
+  fit_wt.resize((size_t)ntypes * nlayers);
   fit_b.resize((size_t)ntypes * nlayers);
   fit_idt.resize((size_t)ntypes * nlayers, nullptr);
   out_w_host.resize(ntypes);
@@ -192,7 +189,16 @@ SeaModel::SeaModel(const dpmi_sea_model_desc& d) {
     for (int l = 0; l < nlayers; ++l) {
       const size_t k = (size_t)t * nlayers + l;
       const int ni = width[l], no = width[l + 1];
-      fit_w[k] = dupload(d.fit_w[k], (size_t)ni * no);
+      {
+        // FP64 MFMA operands: W and W^T ([no][ni], the backward's) as B fragments
+        const std::vector<double> ws = fit64_swizzle(d.fit_w[k], ni, no);
+        fit_w[k] = dupload(ws.data(), ws.size());
+        std::vector<double> wt((size_t)ni * no);
+        for (int i = 0; i < ni; ++i)
+          for (int o = 0; o < no; ++o) wt[(size_t)o * ni + i] = d.fit_w[k][(size_t)i * no + o];
+        const std::vector<double> wts = fit64_swizzle(wt.data(), no, ni);
+        fit_wt[k] = dupload(wts.data(), wts.size());
+      }
       fit_b[k] = dupload(d.fit_b[k], (size_t)no);
       if (d.fit_idt[k]) fit_idt[k] = dupload(d.fit_idt[k], (size_t)no);
       if (l == nlayers - 1) {
@@ -332,6 +338,7 @@ SeaModel::~SeaModel() {
   dfree(d_davg);
   dfree(d_dstd);
   for (auto& p : fit_w) dfree(p);
+  for (auto& p : fit_wt) dfree(p);
   for (auto& p : fit_b) dfree(p);
   for (auto& p : fit_idt) dfree(p);
   dfree(d_table_f);
@@ -416,7 +423,6 @@ SeaEngine::SeaEngine(const SeaModel* m, int nframes_, const int* natoms_h, const
     d_gD = dalloc<double>((size_t)ntot * D);
     DPMI_CHECK(hipMemset(d_gD, 0, sizeof(double) * (size_t)ntot * D));
     for (int l = 0; l < m->nlayers; ++l) {
-      d_fz.push_back(dalloc<double>((size_t)ntot * m->width[l + 1]));
       d_fy.push_back(dalloc<double>((size_t)ntot * m->width[l + 1]));
       d_fx.push_back(dalloc<double>((size_t)ntot * m->width[l + 1]));
     }
@@ -424,6 +430,8 @@ SeaEngine::SeaEngine(const SeaModel* m, int nframes_, const int* natoms_h, const
     for (int w : m->width) wmax = std::max(wmax, w);
     d_g0 = dalloc<double>((size_t)ntot * wmax);
     d_g1 = dalloc<double>((size_t)ntot * wmax);
+    d_gz[0] = dalloc<double>((size_t)ntot * wmax);
+    d_gz[1] = dalloc<double>((size_t)ntot * wmax);
   } else {
     d_em_f = dalloc<float>(nn * 4);
     // as above (76 -> 16 bytes per slot written by the environment kernel)
@@ -491,9 +499,6 @@ SeaEngine::SeaEngine(const SeaModel* m, int nframes_, const int* natoms_h, const
   }
   d_outw.resize(nt, nullptr);
   for (int t = 0; t < nt; ++t) d_outw[t] = dupload(m->out_w_host[t].data(), m->out_w_host[t].size());
-  rocblas_handle h;
-  rb_check(rocblas_create_handle(&h), "create_handle");
-  blas = h;
   nlist_cap = nlist_row_capacity(std::max(128, 2 * m->nnei));
   load_cached_tune();
 }
@@ -501,7 +506,6 @@ SeaEngine::SeaEngine(const SeaModel* m, int nframes_, const int* natoms_h, const
 SeaEngine::~SeaEngine() {
   if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
   if (graph) (void)hipGraphDestroy(graph);
-  if (blas) rocblas_destroy_handle((rocblas_handle)blas);
   if (geom_pinned) (void)hipHostFree(geom_pinned);
   if (stat_pinned) (void)hipHostFree(stat_pinned);
   for (auto* p : {&d_frame_of, &d_center, &d_type, &d_pos, &d_perm, &d_flag, &d_nall, &d_frame_ext_off, &d_nlist,
@@ -511,7 +515,7 @@ SeaEngine::~SeaEngine() {
     dfree(*p);
   dfree(d_cell_atoms);
   for (auto* p : {&d_em, &d_em_deriv, &d_rij, &d_net_deriv, &d_xyz, &d_D, &d_gD, &d_atom_e, &d_vpart,
-                  &d_coord_ext, &d_force_ext, &d_g0, &d_g1, &d_part})
+                  &d_coord_ext, &d_force_ext, &d_g0, &d_g1, &d_gz[0], &d_gz[1], &d_part})
     dfree(*p);
   for (auto* p : {&d_em_f, &d_em_deriv_f, &d_rij_f, &d_net_deriv_f, &d_xyz_f, &d_gD_f, &d_eperm, &d_ga, &d_gb})
     dfree(*p);
@@ -522,7 +526,6 @@ SeaEngine::~SeaEngine() {
   for (auto& p : d_y16) dfree(p);
   for (auto& p : d_x16) dfree(p);
   dfree(d_geom);
-  for (auto& p : d_fz) dfree(p);
   for (auto& p : d_fy) dfree(p);
   for (auto& p : d_fx) dfree(p);
   for (auto& p : d_outw) dfree(p);
@@ -642,7 +645,7 @@ EmbedArgs SeaEngine::embed_args() const {
   return ta;
 }
 
-// FP64 stages env_mat .. embed_bwd: rocBLAS DGEMM fitting per type block.
+// FP64 stages env_mat .. embed_bwd: fitting layers on FP64 MFMA (fit64.hip).
 void SeaEngine::compute_fp64_descriptor_fitting(hipStream_t st, int& stage) {
   const SeaModel& m = *model;
   if (!recompute_slots) {  // else the neighbour-list kernel stored em
@@ -665,54 +668,103 @@ void SeaEngine::compute_fp64_descriptor_fitting(hipStream_t st, int& stage) {
   launch_embed_fwd<double, double>(ta, st);
   mark(st, stage++);
 
-  // fitting forward, one row block per type
-  rocblas_handle h = (rocblas_handle)blas;
-  rb_check(rocblas_set_stream(h, st), "set_stream");
-  const double one = 1.0, zero = 0.0;
+  // fitting forward: one launch per layer over every type (FP64 MFMA)
   launch_fill_double(d_atom_e, 0.0, ntot, st);
   const int L = m.nlayers;
-  for (int t = 0; t < m.ntypes; ++t) {
-    const long long p0 = tstart[t], n = tstart[t + 1] - tstart[t];
-    if (n == 0) continue;
-    const double* x = d_D + (size_t)p0 * m.width[0];
-    for (int l = 0; l < L; ++l) {
-      const int ni = m.width[l], no = m.width[l + 1];
-      const size_t k = (size_t)t * L + l;
-      double* z = d_fz[l] + (size_t)p0 * no;
-      rb_check(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, no, (int)n, ni, &one, m.fit_w[k], no,
-                             x, ni, &zero, z, no),
-               "dgemm fwd");
-      if (l < L - 1) {
-        launch_fit_act(z, m.fit_b[k], m.fit_idt[k], x, m.act[l], m.res_mode[l], n, ni, no, d_fy[l] + (size_t)p0 * no,
-                       d_fx[l] + (size_t)p0 * no, st);
-        x = d_fx[l] + (size_t)p0 * no;
-      } else {
-        launch_fit_out(z, m.out_b_host[t], m.bias_atom_e[t], d_perm, p0, n, d_atom_e, st);
-      }
+  for (int tg = 0; tg < m.ntypes; tg += kFit64MaxTypes) {
+    const int ng = std::min(kFit64MaxTypes, m.ntypes - tg);
+    Fit64Layer f;
+    f.ntypes = ng;
+    for (int i = 0; i < ng; ++i) {
+      f.p0[i] = tstart[tg + i];
+      f.rows[i] = tstart[tg + i + 1] - tstart[tg + i];
     }
+    for (int l = 0; l + 1 < L; ++l) {
+      const bool top = l + 2 == L;
+      for (int i = 0; i < ng; ++i) {
+        const size_t k = (size_t)(tg + i) * L + l;
+        f.w[i] = m.fit_w[k];
+        f.bias[i] = m.fit_b[k];
+        f.idt[i] = m.fit_idt[k];
+        f.wout[i] = top ? d_outw[tg + i] : nullptr;
+      }
+      f.K = m.width[l];
+      f.N = m.width[l + 1];
+      f.act = m.act[l];
+      f.res_mode = m.res_mode[l];
+      f.a = l == 0 ? d_D : d_fx[l - 1];
+      f.lda = m.width[l];
+      f.out0 = top ? nullptr : d_fy[l];  // the top layer's dE/dz is formed here
+      f.out1 = d_fx[l];
+      f.out2 = top ? d_gz[0] : nullptr;
+      f.ldo = m.width[l + 1];
+      launch_fit64(f, false, st);
+    }
+    Fit64Out o;
+    o.ntypes = ng;
+    for (int i = 0; i < ng; ++i) {
+      o.p0[i] = f.p0[i];
+      o.rows[i] = f.rows[i];
+      o.w[i] = d_outw[tg + i];
+      o.b_out[i] = m.out_b_host[tg + i];
+      o.bias[i] = m.bias_atom_e[tg + i];
+    }
+    o.x = L >= 2 ? d_fx[L - 2] : d_D;
+    o.K = m.width[L - 1];
+    o.perm = d_perm;
+    o.atom_e = d_atom_e;
+    launch_fit64_out(o, st);
   }
   mark(st, stage++);
 
-  // fitting backward: dE/dD
-  for (int t = 0; t < m.ntypes; ++t) {
-    const long long p0 = tstart[t], n = tstart[t + 1] - tstart[t];
-    if (n == 0) continue;
-    // g = dE/d(x_{L-1}) = w_out broadcast over rows
-    const int wl = m.width[L - 1];
-    double* g = d_g0;
-    double* gn = d_g1;
-    launch_fill_rows(g, d_outw[t], n, wl, st);
+  // fitting backward: dE/dD.  Each layer's product reads dE/dz of the layer
+  // (formed by the previous launch) and writes dE/dz of the layer below, plus
+  // the raw dE/dx the layer below needs for its residual.
+  for (int tg = 0; tg < m.ntypes; tg += kFit64MaxTypes) {
+    const int ng = std::min(kFit64MaxTypes, m.ntypes - tg);
+    Fit64Layer f;
+    f.ntypes = ng;
+    for (int i = 0; i < ng; ++i) {
+      f.p0[i] = tstart[tg + i];
+      f.rows[i] = tstart[tg + i + 1] - tstart[tg + i];
+    }
+    if (L == 1) {  // no hidden layer: dE/dD = w_out on every row
+      for (int i = 0; i < ng; ++i)
+        if (f.rows[i] > 0) launch_fill_rows(d_gD + f.p0[i] * m.width[0], d_outw[tg + i], f.rows[i], m.width[0], st);
+      continue;
+    }
+    int zb = 0, gb = 0;
+    const double* graw = nullptr;  // dE/dx of this layer's output (the top one: w_out)
     for (int l = L - 2; l >= 0; --l) {
-      const int ni = m.width[l], no = m.width[l + 1];
-      const size_t k = (size_t)t * L + l;
-      double* gz = d_fz[l] + (size_t)p0 * no;  // reuse z storage for g_z
-      launch_fit_act_bwd(g, d_fy[l] + (size_t)p0 * no, m.fit_idt[k], m.act[l], n, no, gz, st);
-      double* gin = (l == 0) ? d_gD + (size_t)p0 * ni : gn;
-      rb_check(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, ni, (int)n, no, &one,
-                             m.fit_w[k], no, gz, no, &zero, gin, ni),
-               "dgemm bwd");
-      launch_fit_res_bwd(g, m.res_mode[l], n, ni, no, gin, st);
-      if (l > 0) std::swap(g, gn);
+      const bool top = l + 2 == L;
+      for (int i = 0; i < ng; ++i) {
+        const size_t k = (size_t)(tg + i) * L + l;
+        f.w[i] = m.fit_wt[k];
+        f.wout[i] = top ? d_outw[tg + i] : nullptr;
+        f.idt_next[i] = l > 0 ? m.fit_idt[k - 1] : nullptr;
+      }
+      f.K = m.width[l + 1];
+      f.N = m.width[l];
+      f.res_mode = m.res_mode[l];
+      f.a = d_gz[zb];
+      f.lda = m.width[l + 1];
+      f.g = graw;
+      f.ldg = m.width[l + 1];
+      f.ldo = m.width[l];
+      if (l == 0) {
+        f.out0 = d_gD;
+        f.out1 = nullptr;
+      } else {
+        f.out0 = m.res_mode[l - 1] ? (gb ? d_g1 : d_g0) : nullptr;
+        f.out1 = d_gz[zb ^ 1];
+        f.act_next = m.act[l - 1];
+        f.y = d_fy[l - 1];
+        f.ldy = m.width[l];
+      }
+      launch_fit64(f, true, st);
+      graw = f.out0;
+      gb ^= 1;
+      zb ^= 1;
     }
   }
   mark(st, stage++);
@@ -1209,13 +1261,11 @@ void SeaEngine::autotune(const double* coord, const double* box_h, int reps, hip
   tune_cache()[tune_key(precision, ntot)] = tune;
 }
 
-// The captured step holds only kernel nodes (every clear is a fill kernel,
-// no library call is made by the mixed engine with the fused fitting), so
-// replays are exact repeats of an eager step on the current coordinates.
+// The captured step holds only kernel nodes (every clear is a fill kernel
+// and no library is called, in either precision), so replays are exact
+// repeats of an eager step on the current coordinates.
 void SeaEngine::capture(const double* coord, const double* box_h, double* energy, double* force, double* virial,
                         double* atom_energy, hipStream_t st) {
-  if (precision != DPMI_MIXED || !model->fused)
-    throw Unsupported("graph capture needs the mixed engine with the fused fitting kernel");
   if (!st) throw std::invalid_argument("graph capture needs a non-default stream");
   // geometry, buffers, uploads (and any capacity growth) for these boxes
   compute(coord, box_h, energy, force, virial, atom_energy, nullptr, st);

@@ -35,7 +35,8 @@ struct SeaModel {
   int* d_sec = nullptr;
   double* d_davg = nullptr;
   double* d_dstd = nullptr;
-  std::vector<double*> fit_w, fit_b, fit_idt;  // [ntypes * nlayers]
+  std::vector<double*> fit_w, fit_b, fit_idt;  // [ntypes * nlayers]; fit_w as FP64 MFMA B fragments
+  std::vector<double*> fit_wt;                 // W^T ([n_out][n_in]) as B fragments, for the backward
   // mixed-precision engine: FP32 tables, FP16 fitting weights padded to
   // multiples of 64, FP64 energy shift per type (output bias + bias_atom_e)
   std::string mixed_unsupported;  // empty when DPMI_MIXED can run this model
@@ -118,7 +119,6 @@ struct SeaEngine {
   int ncells_tot = 0, nx_tot = 0;
   long long next_tot = 0;
   int nlist_cap = 256;
-  void* blas = nullptr;
   hipStream_t last_stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -147,8 +147,9 @@ struct SeaEngine {
   bool recompute_slots = false;  // em_deriv / rij rebuilt by the force kernel (not stored)
   double *d_xyz = nullptr, *d_D = nullptr, *d_gD = nullptr, *d_atom_e = nullptr,
          *d_vpart = nullptr;
-  std::vector<double*> d_fz, d_fy, d_fx, d_outw;
+  std::vector<double*> d_fy, d_fx, d_outw;
   double *d_g0 = nullptr, *d_g1 = nullptr;
+  double* d_gz[2] = {nullptr, nullptr};  // FP64 fitting backward: dE/dz ping-pong
   // mixed engine
   float *d_em_f = nullptr, *d_em_deriv_f = nullptr, *d_rij_f = nullptr, *d_net_deriv_f = nullptr;
   float *d_xyz_f = nullptr, *d_gD_f = nullptr, *d_eperm = nullptr, *d_ga = nullptr, *d_gb = nullptr;

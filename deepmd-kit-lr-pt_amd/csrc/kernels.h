@@ -307,19 +307,50 @@ void launch_frame_reduce(const double* atom_e, const double* vpart, const int* c
                          double* virial, hipStream_t st);
 
 // ----------------------------------------------------------- fitting.hip --
-// y = act(z + b); x_next = y * idt + residual(x); stores y (for backward)
-void launch_fit_act(const double* z, const double* b, const double* idt, const double* x, int act, int res_mode,
-                    long long n, int n_in, int n_out, double* y, double* x_next, hipStream_t st);
-// g_z = g_out * idt * (1 - y^2)  (act) ; residual contribution added later
-void launch_fit_act_bwd(const double* g_out, const double* y, const double* idt, int act, long long n, int n_out,
-                        double* g_z, hipStream_t st);
-// g_in += residual(g_out)
-void launch_fit_res_bwd(const double* g_out, int res_mode, long long n, int n_in, int n_out, double* g_in,
-                        hipStream_t st);
-// atom_e[perm[p]] = z[p] + b + bias ; fills g_out rows with w_out
-void launch_fit_out(const double* z, double b_out, double bias, const int* perm, long long p0, long long n,
-                    double* atom_e, hipStream_t st);
 void launch_fill_rows(double* g, const double* w, long long n, int width, hipStream_t st);
+
+// ------------------------------------------------------------- fit64.hip --
+// One FP64 fitting layer for every atom type (rows of type t are
+// [p0[t], p0[t] + rows[t]) of the row-major activation buffers).
+constexpr int kFit64MaxTypes = 8;
+struct Fit64Layer {
+  int ntypes = 0;
+  long long p0[kFit64MaxTypes] = {}, rows[kFit64MaxTypes] = {};
+  const double* w[kFit64MaxTypes] = {};         // fit64_swizzle of [K][N]: forward W, backward W^T
+  const double* bias[kFit64MaxTypes] = {};      // forward [N]
+  const double* idt[kFit64MaxTypes] = {};       // forward: this layer's timestep [N] (or null)
+  const double* idt_next[kFit64MaxTypes] = {};  // backward: timestep [N] of the layer below (or null)
+  const double* wout[kFit64MaxTypes] = {};      // top hidden layer: output weights [N] (forward) / [K] (backward)
+  int K = 0, N = 0, act = 0, res_mode = 0;      // res_mode 1: + x, 2: + [x, x]
+  int act_next = 0;                             // backward: activation of the layer below
+  const double* a = nullptr;  // product operand rows: forward x, backward dE/dz of this layer
+  int lda = 0;
+  const double* g = nullptr;  // backward: dE/d(layer output) rows for the residual (unless wout)
+  int ldg = 0;
+  const double* y = nullptr;  // backward: act output of the layer below
+  int ldy = 0;
+  // forward: out0 = act(z + b) (or null), out1 = x_next, out2 = dE/dz when
+  // wout is set.  backward: out1 null -> out0 = dE/d(input) (the descriptor
+  // gradient); else out0 = dE/d(input) (or null), out1 = dE/dz of the layer below
+  double* out0 = nullptr;
+  double* out1 = nullptr;
+  double* out2 = nullptr;
+  int ldo = 0;
+};
+void launch_fit64(const Fit64Layer& p, bool backward, hipStream_t st);
+// row-major [K][N] -> MFMA B fragments [ceil(K/16)][ceil(N/16)][4][64], zero padded
+std::vector<double> fit64_swizzle(const double* w, int K, int N);
+struct Fit64Out {
+  int ntypes = 0;
+  long long p0[kFit64MaxTypes] = {}, rows[kFit64MaxTypes] = {};
+  const double* w[kFit64MaxTypes] = {};  // output weights [K]
+  double b_out[kFit64MaxTypes] = {}, bias[kFit64MaxTypes] = {};
+  const double* x = nullptr;  // last hidden activations [.., K]
+  int K = 0;
+  const int* perm = nullptr;  // row position -> atom
+  double* atom_e = nullptr;
+};
+void launch_fit64_out(const Fit64Out& p, hipStream_t st);
 
 // ------------------------------------------------------------- fit16.hip --
 // One fitting layer of the mixed engine, widths padded to multiples of 64.

@@ -273,8 +273,7 @@ int dpmi_sea_engine_set_growth(dpmi_sea_engine* engine, int enable);
  * the whole path (wrap, ghosts, cell list, neighbour list, ... , force,
  * virial, energies) on the coordinates currently stored at `coord`, with the
  * boxes given at capture, into the same output buffers.  A new box needs a
- * new capture.  DPMI_MIXED engines with the fused fitting only
- * (DPMI_ERR_UNSUPPORTED otherwise). */
+ * new capture.  Both precisions. */
 int dpmi_sea_engine_graph_capture(dpmi_sea_engine* engine, const double* coord, const double* box_host,
                                   double* energy, double* force, double* virial, double* atom_energy,
                                   void* stream);
