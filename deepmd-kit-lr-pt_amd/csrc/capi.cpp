@@ -537,6 +537,14 @@ int dpmi_sea_engine_set_growth(dpmi_sea_engine* engine, int enable) {
   });
 }
 
+int dpmi_sea_set_net_chunk_bytes(long long bytes) {
+  return guarded([&] {
+    if (bytes <= 0) throw std::invalid_argument("chunk bytes must be positive");
+    dpmi::SeaEngine::net_chunk_bytes = bytes;
+    return kOk;
+  });
+}
+
 int dpmi_sea_engine_graph_capture(dpmi_sea_engine* engine, const double* coord, const double* box_host,
                                   double* energy, double* force, double* virial, double* atom_energy, void* stream) {
   return guarded([&] {

@@ -454,7 +454,6 @@ __device__ void bwd_block_g(float* __restrict__ nd, const float (&dy)[C][4], con
     Nbr<float, 4> r;
     load_nbr(r, f, em + (size_t)c0 * 4, 4, em + (size_t)c0 * 4, lane, jend - c0, last, false);
     const int cnt = min(kWave, jend - c0);
-    const bool anyd = __ballot(r.delta != 0.f) != 0ull;  // a neighbour outside the table
     for (int q0 = 0; q0 < cnt; q0 += 8) {
       float a[2][C][6], x[2], delta[2], ll[2][4];
 #pragma unroll
@@ -471,18 +470,21 @@ __device__ void bwd_block_g(float* __restrict__ nd, const float (&dy)[C][4], con
         // columns k >= M: zero coefficients times dy = 0.  p[m] = sum_k
         // P_k dy[k, m] and e[m] = sum_k P'_k dy[k, m] over the lane's
         // columns; the slope term of component 0 is ll . e.
+        // outside the table P_k + P'_k delta (linear extension): its share
+        // of p is delta * e, added once per neighbour instead of per column
         float p[4], e[4];
 #pragma unroll
         for (int c = 0; c < C; ++c) {
           float v, d;
           quintic(a[u][c], x[u], 0.f, v, d);
-          if (anyd) v += d * delta[u];  // linear extension outside the table
 #pragma unroll
           for (int m = 0; m < 4; ++m) {
             p[m] = c == 0 ? v * dy[c][m] : p[m] + v * dy[c][m];
             e[m] = c == 0 ? d * dy[c][m] : e[m] + d * dy[c][m];
           }
         }
+#pragma unroll
+        for (int m = 0; m < 4; ++m) p[m] += delta[u] * e[m];
         p[0] += ll[u][0] * e[0] + ll[u][1] * e[1] + ll[u][2] * e[2] + ll[u][3] * e[3];
         const float tot = group_sum4(p, gl);
         const int q = q0 + 4 * u + grp, j = c0 + q;
@@ -1058,6 +1060,159 @@ __global__ void __launch_bounds__(256) k_embed_bwd_mfma(const EmbedArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Uncompressed embedding of the FP64 engine (se_a.py:785-839 without
+// compression).  The embedding net runs as row-wise layer products
+// (fit64.hip) over rows laid out net by net within a chunk of descriptor
+// positions: the slots of block tj of atom g are the rows
+// rowbase[g * ntypes + tj] + s.  These kernels move values between the
+// neighbour slots and those rows, with one wavefront per atom (lanes own
+// columns k = lane + 64 kq):
+//   gather:    x0[row] = em[g, j, 0]
+//   contract:  xyz[g, c, k] = sum_j em[g, j, c] G[row(j), k], then T and D
+//              as the table kernels form them
+//   adjoint:   dy = dE/dxyz from dE/dD;  dE/dG[row, k] = sum_c em[g, j, c] dy[c, k]
+//              (kept for the residual of the last layer) and its dE/dz
+//              (times idt and tanh'), and net_deriv[g, j, c] = sum_k G[row, k] dy[c, k]
+//   slope:     net_deriv[g, j, 0] += dE/dx0[row]   (the net's input is em[g, j, 0])
+// Padding slots are evaluated like the others, as the reference does.
+
+__global__ void __launch_bounds__(256) k_net_gather(const double* __restrict__ em, const int* __restrict__ perm,
+                                                    const int* __restrict__ type, const int* __restrict__ rowbase,
+                                                    const int* __restrict__ sec, int ntypes, int nnei, int q0, int q1,
+                                                    double* __restrict__ x0) {
+  const long long q = q0 + wave_index();
+  if (q >= q1) return;
+  const int lane = lane_id(), g = perm[q];
+  if (type[g] < 0) return;
+  for (int tj = 0; tj < ntypes; ++tj) {
+    const int rb = rowbase[(size_t)g * ntypes + tj], b = sec[tj], nb = sec[tj + 1] - b;
+    for (int s = lane; s < nb; s += kWave) x0[(size_t)rb + s] = em[((size_t)g * nnei + b + s) * 4];
+  }
+}
+
+template <int KQ>
+__global__ void __launch_bounds__(256) k_net_contract(const double* __restrict__ G, const double* __restrict__ em,
+                                                      const int* __restrict__ perm, const int* __restrict__ pos,
+                                                      const int* __restrict__ type, const int* __restrict__ rowbase,
+                                                      const int* __restrict__ sec, int ntypes, int nnei, int M, int A,
+                                                      int ldD, double inv_nnei, int q0, int q1,
+                                                      double* __restrict__ xyz, double* __restrict__ D) {
+  const long long q = q0 + wave_index();
+  if (q >= q1) return;
+  const int lane = lane_id(), g = perm[q], ti = type[g];
+  double acc[KQ][4];
+#pragma unroll
+  for (int kq = 0; kq < KQ; ++kq)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[kq][c] = 0;
+  if (ti >= 0) {
+    for (int tj = 0; tj < ntypes; ++tj) {
+      const int rb = rowbase[(size_t)g * ntypes + tj], b = sec[tj], nb = sec[tj + 1] - b;
+      const double* e = em + ((size_t)g * nnei + b) * 4;
+      for (int s = 0; s < nb; ++s) {
+        const double e0 = e[4 * s], e1 = e[4 * s + 1], e2 = e[4 * s + 2], e3 = e[4 * s + 3];
+        const double* gr = G + ((size_t)rb + s) * M;
+#pragma unroll
+        for (int kq = 0; kq < KQ; ++kq) {
+          const int k = lane + kq * kWave;
+          const double v = k < M ? gr[k] : 0.0;
+          acc[kq][0] += e0 * v;
+          acc[kq][1] += e1 * v;
+          acc[kq][2] += e2 * v;
+          acc[kq][3] += e3 * v;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int kq = 0; kq < KQ; ++kq) {
+    const int k = lane + kq * kWave;
+    if (k < M)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) xyz[((size_t)g * 4 + c) * M + k] = acc[kq][c];
+  }
+  double* out = D + (size_t)pos[g] * ldD;
+  for (int a = 0; a < A; ++a) {
+    double ta[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) ta[c] = bcast(acc[0][c], a) * inv_nnei;
+#pragma unroll
+    for (int kq = 0; kq < KQ; ++kq) {
+      const int k = lane + kq * kWave;
+      if (k < M) {
+        double v = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v += (acc[kq][c] * inv_nnei) * ta[c];
+        out[(size_t)k * A + a] = v;
+      }
+    }
+  }
+}
+
+template <int KQ>
+__global__ void __launch_bounds__(256) k_net_adjoint(
+    const double* __restrict__ G, const double* __restrict__ ytop, const double* const* __restrict__ idt_top,
+    int act_top, const double* __restrict__ xyz, const double* __restrict__ gD, const double* __restrict__ em,
+    const int* __restrict__ perm, const int* __restrict__ pos, const int* __restrict__ type,
+    const int* __restrict__ rowbase, const int* __restrict__ sec, const int* __restrict__ net_of, int ntypes,
+    int nnei, int M, int A, int ldD, double inv_nnei, int q0, int q1, double* __restrict__ gG,
+    double* __restrict__ dz, double* __restrict__ net_deriv) {
+  const long long q = q0 + wave_index();
+  if (q >= q1) return;
+  const int lane = lane_id(), g = perm[q], ti = type[g];
+  if (ti < 0) {
+    for (int s = lane; s < nnei * 4; s += kWave) net_deriv[(size_t)g * nnei * 4 + s] = 0;
+    return;
+  }
+  double dy[KQ][4];
+  bwd_dy<double, double, KQ>(dy, xyz + (size_t)g * 4 * M, gD + (size_t)pos[g] * ldD, M, A, inv_nnei);
+  for (int tj = 0; tj < ntypes; ++tj) {
+    const int rb = rowbase[(size_t)g * ntypes + tj], b = sec[tj], nb = sec[tj + 1] - b;
+    const double* __restrict__ idt = idt_top[net_of[ti * ntypes + tj]];
+    const double* e = em + ((size_t)g * nnei + b) * 4;
+    double* nd = net_deriv + ((size_t)g * nnei + b) * 4;
+    for (int s = 0; s < nb; ++s) {
+      const double e0 = e[4 * s], e1 = e[4 * s + 1], e2 = e[4 * s + 2], e3 = e[4 * s + 3];
+      const size_t row = (size_t)rb + s;
+      double p[4] = {0, 0, 0, 0};
+#pragma unroll
+      for (int kq = 0; kq < KQ; ++kq) {
+        const int k = lane + kq * kWave;
+        if (k < M) {
+          const double gv = G[row * M + k];
+          const double gg = e0 * dy[kq][0] + e1 * dy[kq][1] + e2 * dy[kq][2] + e3 * dy[kq][3];
+          gG[row * M + k] = gg;
+          double z = idt ? gg * idt[k] : gg;
+          if (act_top) {
+            const double yv = ytop[row * M + k];
+            z *= (1. - yv * yv);
+          }
+          dz[row * M + k] = z;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) p[c] += gv * dy[kq][c];
+        }
+      }
+      const double tot = TSum<double, 4>::sum(p);
+      if (lane < 4) nd[4 * s + lane] = tot;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_net_slope(const double* __restrict__ ds, const int* __restrict__ perm,
+                                                   const int* __restrict__ type, const int* __restrict__ rowbase,
+                                                   const int* __restrict__ sec, int ntypes, int nnei, int q0, int q1,
+                                                   double* __restrict__ net_deriv) {
+  const long long q = q0 + wave_index();
+  if (q >= q1) return;
+  const int lane = lane_id(), g = perm[q];
+  if (type[g] < 0) return;
+  for (int tj = 0; tj < ntypes; ++tj) {
+    const int rb = rowbase[(size_t)g * ntypes + tj], b = sec[tj], nb = sec[tj + 1] - b;
+    for (int s = lane; s < nb; s += kWave) net_deriv[((size_t)g * nnei + b + s) * 4] += ds[(size_t)rb + s];
+  }
+}
+
 }  // namespace
 
 void* make_table_infos(const double* infos_host, int nnets, bool fp32) {
@@ -1275,5 +1430,43 @@ template void launch_embed_bwd<double, double>(const EmbedArgs&, hipStream_t);
 template void launch_embed_fwd<float, _Float16>(const EmbedArgs&, hipStream_t);
 template void launch_embed_bwd<float, _Float16>(const EmbedArgs&, hipStream_t);
 template void launch_embed_bwd<float, float>(const EmbedArgs&, hipStream_t);
+
+
+// ---------------------------------------------------------- uncompressed --
+void launch_net_gather(const NetArgs& a, int q0, int q1, double* x0, hipStream_t st) {
+  if (q1 <= q0) return;
+  hipLaunchKernelGGL(k_net_gather, dim3(ceil_div(q1 - q0, 4)), dim3(256), 0, st, a.em, a.perm, a.type, a.rowbase,
+                     a.sec, a.ntypes, a.nnei, q0, q1, x0);
+  DPMI_LAUNCH_CHECK();
+}
+
+void launch_net_contract(const NetArgs& a, const double* G, int q0, int q1, hipStream_t st) {
+  if (q1 <= q0) return;
+  const dim3 grid(ceil_div(q1 - q0, 4)), block(256);
+  const int ldD = a.ldD ? a.ldD : a.M * a.A;
+  DPMI_KQ_DISPATCH(a.M, hipLaunchKernelGGL((k_net_contract<KQ>), grid, block, 0, st, G, a.em, a.perm, a.pos, a.type,
+                                           a.rowbase, a.sec, a.ntypes, a.nnei, a.M, a.A, ldD, a.inv_nnei, q0, q1,
+                                           a.xyz, a.D));
+  DPMI_LAUNCH_CHECK();
+}
+
+void launch_net_adjoint(const NetArgs& a, const double* G, const double* ytop, const double* const* idt_top,
+                        int act_top, int q0, int q1, double* gG, double* dz, hipStream_t st) {
+  if (q1 <= q0) return;
+  const dim3 grid(ceil_div(q1 - q0, 4)), block(256);
+  const int ldD = a.ldD ? a.ldD : a.M * a.A;
+  DPMI_KQ_DISPATCH(a.M, hipLaunchKernelGGL((k_net_adjoint<KQ>), grid, block, 0, st, G, ytop, idt_top, act_top,
+                                           a.xyz, a.gD, a.em, a.perm, a.pos, a.type, a.rowbase, a.sec, a.net_of,
+                                           a.ntypes, a.nnei, a.M, a.A, ldD, a.inv_nnei, q0, q1, gG, dz,
+                                           a.net_deriv));
+  DPMI_LAUNCH_CHECK();
+}
+
+void launch_net_slope(const NetArgs& a, const double* ds, int q0, int q1, hipStream_t st) {
+  if (q1 <= q0) return;
+  hipLaunchKernelGGL(k_net_slope, dim3(ceil_div(q1 - q0, 4)), dim3(256), 0, st, ds, a.perm, a.type, a.rowbase, a.sec,
+                     a.ntypes, a.nnei, q0, q1, a.net_deriv);
+  DPMI_LAUNCH_CHECK();
+}
 
 }  // namespace dpmi

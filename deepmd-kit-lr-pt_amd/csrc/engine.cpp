@@ -137,17 +137,62 @@ SeaModel::SeaModel(const dpmi_sea_model_desc& d) {
   if (nnets != (type_one_side ? ntypes : ntypes * ntypes)) throw std::invalid_argument("nnets mismatch");
   davg.assign(d.davg, d.davg + (size_t)ntypes * nnei * 4);
   dstd.assign(d.dstd, d.dstd + (size_t)ntypes * nnei * 4);
+  compressed = d.tables != nullptr;
   tab_off.assign(nnets, 0);
-  nspline.assign(d.nspline, d.nspline + nnets);
-  long long tot = 0;
-  for (int n = 0; n < nnets; ++n) {
-    tab_off[n] = tot;
-    tot += (long long)d.nspline[n] * 6 * M;
+  nspline.assign(nnets, 0);
+  if (compressed) {
+    if (!d.nspline || !d.table_info) throw std::invalid_argument("tables need nspline and table_info");
+    nspline.assign(d.nspline, d.nspline + nnets);
+    long long tot = 0;
+    for (int n = 0; n < nnets; ++n) {
+      tab_off[n] = tot;
+      tot += (long long)d.nspline[n] * 6 * M;
+    }
+    table_host.resize(tot);
+    for (int n = 0; n < nnets; ++n)
+      std::memcpy(table_host.data() + tab_off[n], d.tables[n], sizeof(double) * (size_t)d.nspline[n] * 6 * M);
+    info_host.assign(d.table_info, d.table_info + nnets * 6);
+  } else {
+    // uncompressed: the embedding nets themselves (FP64 MFMA operands)
+    enl = d.embed_nlayers;
+    if (enl <= 0 || !d.embed_width || !d.embed_w || !d.embed_b || !d.embed_act || !d.embed_resnet)
+      throw std::invalid_argument("a model without tables needs its embedding nets");
+    ewidth.assign(d.embed_width, d.embed_width + enl + 1);
+    if (ewidth[0] != 1 || ewidth[enl] != M) throw std::invalid_argument("embedding widths must run from 1 to M");
+    for (int w : ewidth)
+      if (w <= 0) throw std::invalid_argument("embedding widths must be positive");
+    eact.assign(d.embed_act, d.embed_act + enl);
+    eres.assign(enl, 0);
+    for (int l = 0; l < enl; ++l)
+      if (d.embed_resnet[l]) {
+        if (ewidth[l] == ewidth[l + 1])
+          eres[l] = 1;
+        else if (2 * ewidth[l] == ewidth[l + 1])
+          eres[l] = 2;
+      }
+    const size_t ne = (size_t)nnets * enl;
+    ew.assign(ne, nullptr);
+    ewt.assign(ne, nullptr);
+    eb.assign(ne, nullptr);
+    eidt.assign(ne, nullptr);
+    std::vector<double*> top(nnets, nullptr);
+    for (int n = 0; n < nnets; ++n)
+      for (int l = 0; l < enl; ++l) {
+        const size_t k = (size_t)n * enl + l;
+        const int ni = ewidth[l], no = ewidth[l + 1];
+        const std::vector<double> ws = fit64_swizzle(d.embed_w[k], ni, no);
+        ew[k] = dupload(ws.data(), ws.size());
+        std::vector<double> wt((size_t)ni * no);
+        for (int i = 0; i < ni; ++i)
+          for (int o = 0; o < no; ++o) wt[(size_t)o * ni + i] = d.embed_w[k][(size_t)i * no + o];
+        const std::vector<double> wts = fit64_swizzle(wt.data(), no, ni);
+        ewt[k] = dupload(wts.data(), wts.size());
+        eb[k] = dupload(d.embed_b[k], (size_t)no);
+        if (d.embed_idt && d.embed_idt[k]) eidt[k] = dupload(d.embed_idt[k], (size_t)no);
+        if (l == enl - 1) top[n] = eidt[k];
+      }
+    d_eidt_top = dupload(top.data(), top.size());
   }
-  table_host.resize(tot);
-  for (int n = 0; n < nnets; ++n)
-    std::memcpy(table_host.data() + tab_off[n], d.tables[n], sizeof(double) * (size_t)d.nspline[n] * 6 * M);
-  info_host.assign(d.table_info, d.table_info + nnets * 6);
   net_of.assign(ntypes * ntypes, 0);
   for (int ti = 0; ti < ntypes; ++ti)
     for (int tj = 0; tj < ntypes; ++tj) net_of[ti * ntypes + tj] = type_one_side ? tj : tj * ntypes + ti;
@@ -170,11 +215,13 @@ SeaModel::SeaModel(const dpmi_sea_model_desc& d) {
   for (int l = 0; l < nlayers; ++l) has_idt[l] = d.fit_idt[l] != nullptr;
 
   // device copies
-  d_table = dupload(table_host.data(), table_host.size());
-  d_tab_off = dupload(tab_off.data(), tab_off.size());
-  void* infos = make_table_infos(info_host.data(), nnets, false);
-  d_infos = dupload(static_cast<const char*>(infos), table_info_bytes(false) * nnets);
-  free_table_infos(infos, false);
+  if (compressed) {
+    d_table = dupload(table_host.data(), table_host.size());
+    d_tab_off = dupload(tab_off.data(), tab_off.size());
+    void* infos = make_table_infos(info_host.data(), nnets, false);
+    d_infos = dupload(static_cast<const char*>(infos), table_info_bytes(false) * nnets);
+    free_table_infos(infos, false);
+  }
   d_net_of = dupload(net_of.data(), net_of.size());
   d_sec = dupload(sec.data(), sec.size());
   d_davg = dupload(davg.data(), davg.size());
@@ -213,6 +260,10 @@ SeaModel::SeaModel(const dpmi_sea_model_desc& d) {
 // type t maps x [rows, kpad[l]] -> [rows, kpad[l+1]]; padded weight rows and
 // columns, biases and timesteps are zero, so padded activations stay zero.
 void SeaModel::build_mixed(const dpmi_sea_model_desc& d) {
+  if (!compressed) {
+    mixed_unsupported = "a model without compressed tables runs in the FP64 engine only";
+    return;
+  }
   if (nlayers < 2)
     mixed_unsupported = "the mixed engine needs at least one hidden fitting layer";
   for (int l = 0; l < nlayers; ++l)
@@ -351,6 +402,9 @@ SeaModel::~SeaModel() {
   dfree(d_tab16_off);
   dfree(d_tab16_pscale);
   for (auto& p : f16_mem) dfree(p);
+  for (auto* v : {&ew, &ewt, &eb, &eidt})
+    for (auto& p : *v) dfree(p);
+  dfree(d_eidt_top);
 }
 
 // ---------------------------------------------------------------- engine --
@@ -432,6 +486,7 @@ SeaEngine::SeaEngine(const SeaModel* m, int nframes_, const int* natoms_h, const
     d_g1 = dalloc<double>((size_t)ntot * wmax);
     d_gz[0] = dalloc<double>((size_t)ntot * wmax);
     d_gz[1] = dalloc<double>((size_t)ntot * wmax);
+    if (!m->compressed) net_setup();
   } else {
     d_em_f = dalloc<float>(nn * 4);
     // as above (76 -> 16 bytes per slot written by the environment kernel)
@@ -529,6 +584,10 @@ SeaEngine::~SeaEngine() {
   for (auto& p : d_fy) dfree(p);
   for (auto& p : d_fx) dfree(p);
   for (auto& p : d_outw) dfree(p);
+  for (auto* p : {&d_nx0, &d_nds, &d_ngg, &d_ndz[0], &d_ndz[1], &d_ngx[0], &d_ngx[1]}) dfree(*p);
+  for (auto& p : d_ny) dfree(p);
+  for (auto& p : d_nxl) dfree(p);
+  dfree(d_rowbase);
   for (auto& e : events) (void)hipEventDestroy(e);
 }
 
@@ -665,7 +724,10 @@ void SeaEngine::compute_fp64_descriptor_fitting(hipStream_t st, int& stage) {
   ta.tables = m.d_table;
   ta.infos = m.d_infos;
   ta.em = d_em;
-  launch_embed_fwd<double, double>(ta, st);
+  if (m.compressed)
+    launch_embed_fwd<double, double>(ta, st);
+  else
+    net_descriptor(st);
   mark(st, stage++);
 
   // fitting forward: one launch per layer over every type (FP64 MFMA)
@@ -769,9 +831,214 @@ void SeaEngine::compute_fp64_descriptor_fitting(hipStream_t st, int& stage) {
   }
   mark(st, stage++);
 
-  launch_embed_bwd<double, double>(ta, st);
+  if (m.compressed)
+    launch_embed_bwd<double, double>(ta, st);
+  else
+    net_adjoint(st);
   mark(st, stage++);
 
+}
+
+// ------------------------------------------------- uncompressed embedding --
+long long SeaEngine::net_chunk_bytes = 16LL << 30;
+
+// Chunks of descriptor positions whose neighbour slots fit the row buffers
+// (net_chunk_bytes), and the row of every (atom, neighbour block) inside its
+// chunk: rows run net by net, the atoms of a net in position order.
+void SeaEngine::net_setup() {
+  const SeaModel& m = *model;
+  const int nt = m.ntypes, L = m.enl;
+  int wmax = 0, wsum = 0;
+  for (int l = 1; l <= L; ++l) {
+    wmax = std::max(wmax, m.ewidth[l]);
+    wsum += m.ewidth[l];
+  }
+  // x0, y and x per layer, dE/dG, two dE/dz and two dE/dx buffers, dE/dx0
+  const long long per_row = 8LL * (2 + 2 * wsum + m.M + 4 * wmax);
+  const long long cap = std::max<long long>(m.nnei, net_chunk_bytes / per_row);
+  std::vector<int> perm_h(ntot);
+  {
+    std::vector<int> fill(tstart.begin(), tstart.end() - 1);
+    for (int g = 0; g < ntot; ++g) perm_h[fill[atype[g] < 0 ? nt : atype[g]]++] = g;
+  }
+  std::vector<int> rowbase((size_t)ntot * nt, -1);
+  net_chunks.clear();
+  net_rows_max = 0;
+  int q = 0;
+  while (q < ntot) {
+    NetChunk c;
+    c.q0 = q;
+    long long rows = 0;
+    while (q < ntot && (rows + m.nnei <= cap || rows == 0)) {
+      if (atype[perm_h[q]] >= 0) rows += m.nnei;
+      ++q;
+    }
+    c.q1 = q;
+    c.p0.assign(m.nnets, 0);
+    c.rows.assign(m.nnets, 0);
+    long long off = 0;
+    for (int n = 0; n < m.nnets; ++n) {
+      const int tj = m.type_one_side ? n : n / nt, ti = m.type_one_side ? -1 : n % nt;
+      c.p0[n] = off;
+      for (int p = c.q0; p < c.q1; ++p) {
+        const int g = perm_h[p], t = atype[g];
+        if (t < 0 || (ti >= 0 && t != ti)) continue;
+        rowbase[(size_t)g * nt + tj] = (int)off;
+        off += m.sel[tj];
+      }
+      c.rows[n] = off - c.p0[n];
+    }
+    if (off >= (1LL << 31)) throw std::invalid_argument("uncompressed embedding: chunk rows exceed 2^31");
+    net_rows_max = std::max(net_rows_max, off);
+    net_chunks.push_back(std::move(c));
+  }
+  d_rowbase = dupload(rowbase.data(), rowbase.size());
+  const size_t R = (size_t)std::max<long long>(net_rows_max, 1);
+  d_nx0 = dalloc<double>(R);
+  d_nds = dalloc<double>(R);
+  d_ngg = dalloc<double>(R * m.M);
+  for (int k = 0; k < 2; ++k) {
+    d_ndz[k] = dalloc<double>(R * wmax);
+    d_ngx[k] = dalloc<double>(R * wmax);
+  }
+  for (int l = 0; l < L; ++l) {
+    d_ny.push_back(dalloc<double>(R * m.ewidth[l + 1]));
+    d_nxl.push_back(dalloc<double>(R * m.ewidth[l + 1]));
+  }
+}
+
+// The embedding net on the chunk's rows: x0 -> ... -> G = d_nxl[L - 1].
+void SeaEngine::net_forward(const NetChunk& c, hipStream_t st) {
+  const SeaModel& m = *model;
+  const int L = m.enl;
+  NetArgs na;
+  na.em = d_em;
+  na.perm = d_perm;
+  na.type = d_type;
+  na.rowbase = d_rowbase;
+  na.sec = m.d_sec;
+  na.ntypes = m.ntypes;
+  na.nnei = m.nnei;
+  launch_net_gather(na, c.q0, c.q1, d_nx0, st);
+  for (int n0 = 0; n0 < m.nnets; n0 += kFit64MaxTypes) {
+    const int ng = std::min(kFit64MaxTypes, m.nnets - n0);
+    Fit64Layer f;
+    f.ntypes = ng;
+    for (int i = 0; i < ng; ++i) {
+      f.p0[i] = c.p0[n0 + i];
+      f.rows[i] = c.rows[n0 + i];
+    }
+    for (int l = 0; l < L; ++l) {
+      for (int i = 0; i < ng; ++i) {
+        const size_t k = (size_t)(n0 + i) * L + l;
+        f.w[i] = m.ew[k];
+        f.bias[i] = m.eb[k];
+        f.idt[i] = m.eidt[k];
+      }
+      f.K = m.ewidth[l];
+      f.N = m.ewidth[l + 1];
+      f.act = m.eact[l];
+      f.res_mode = m.eres[l];
+      f.a = l == 0 ? d_nx0 : d_nxl[l - 1];
+      f.lda = m.ewidth[l];
+      f.out0 = d_ny[l];
+      f.out1 = d_nxl[l];
+      f.ldo = m.ewidth[l + 1];
+      launch_fit64(f, false, st);
+    }
+  }
+}
+
+static NetArgs net_args_of(const SeaModel& m) {
+  NetArgs na;
+  na.sec = m.d_sec;
+  na.net_of = m.d_net_of;
+  na.ntypes = m.ntypes;
+  na.nnei = m.nnei;
+  na.M = m.M;
+  na.A = m.A;
+  na.inv_nnei = 1.0 / m.nnei;
+  return na;
+}
+
+// xyz and D of every atom (se_a.py:826-851 with the net evaluated for every
+// slot, padding included).
+void SeaEngine::net_descriptor(hipStream_t st) {
+  NetArgs na = net_args_of(*model);
+  na.em = d_em;
+  na.perm = d_perm;
+  na.pos = d_pos;
+  na.type = d_type;
+  na.rowbase = d_rowbase;
+  na.xyz = d_xyz;
+  na.D = d_D;
+  for (const NetChunk& c : net_chunks) {
+    net_forward(c, st);
+    launch_net_contract(na, d_nxl[model->enl - 1], c.q0, c.q1, st);
+  }
+}
+
+// net_deriv from dE/dD: dE/dG and the em part per slot, then dE/dG back
+// through the net to its input em[g, j, 0] (layer products on W^T).  With
+// more than one chunk the chunk's forward activations are recomputed.
+void SeaEngine::net_adjoint(hipStream_t st) {
+  const SeaModel& m = *model;
+  const int L = m.enl;
+  NetArgs na = net_args_of(m);
+  na.em = d_em;
+  na.perm = d_perm;
+  na.pos = d_pos;
+  na.type = d_type;
+  na.rowbase = d_rowbase;
+  na.xyz = d_xyz;
+  na.gD = d_gD;
+  na.net_deriv = d_net_deriv;
+  for (const NetChunk& c : net_chunks) {
+    if (net_chunks.size() > 1) net_forward(c, st);
+    launch_net_adjoint(na, d_nxl[L - 1], d_ny[L - 1], m.d_eidt_top, m.eact[L - 1], c.q0, c.q1, d_ngg, d_ndz[0],
+                       st);
+    for (int n0 = 0; n0 < m.nnets; n0 += kFit64MaxTypes) {
+      const int ng = std::min(kFit64MaxTypes, m.nnets - n0);
+      Fit64Layer f;
+      f.ntypes = ng;
+      for (int i = 0; i < ng; ++i) {
+        f.p0[i] = c.p0[n0 + i];
+        f.rows[i] = c.rows[n0 + i];
+      }
+      int zb = 0, gb = 0;
+      const double* graw = d_ngg;  // dE/d(output) of the layer
+      for (int l = L - 1; l >= 0; --l) {
+        for (int i = 0; i < ng; ++i) {
+          const size_t k = (size_t)(n0 + i) * L + l;
+          f.w[i] = m.ewt[k];
+          f.idt_next[i] = l > 0 ? m.eidt[k - 1] : nullptr;
+        }
+        f.K = m.ewidth[l + 1];
+        f.N = m.ewidth[l];
+        f.res_mode = m.eres[l];
+        f.a = d_ndz[zb];
+        f.lda = m.ewidth[l + 1];
+        f.g = graw;
+        f.ldg = m.ewidth[l + 1];
+        f.ldo = m.ewidth[l];
+        if (l == 0) {
+          f.out0 = d_nds;
+          f.out1 = nullptr;
+        } else {
+          f.out0 = m.eres[l - 1] ? d_ngx[gb] : nullptr;
+          f.out1 = d_ndz[zb ^ 1];
+          f.act_next = m.eact[l - 1];
+          f.y = d_ny[l - 1];
+          f.ldy = m.ewidth[l];
+        }
+        launch_fit64(f, true, st);
+        graw = f.out0;
+        gb ^= 1;
+        zb ^= 1;
+      }
+    }
+    launch_net_slope(na, d_nds, c.q0, c.q1, st);
+  }
 }
 
 // Mixed stages env_mat .. embed_bwd: FP32 env matrix (FP64 geometry), FP32

@@ -57,6 +57,13 @@ struct SeaModel {
   long long* d_tab16_off = nullptr;
   float* d_tab16_pscale = nullptr;
   std::vector<std::string> type_map;  // names of the types (model files)
+  // uncompressed embedding (FP64 engines): per (net, layer) the weights as
+  // FP64 MFMA B fragments (W and W^T), bias and timestep
+  bool compressed = true;
+  int enl = 0;
+  std::vector<int> ewidth, eact, eres;
+  std::vector<double*> ew, ewt, eb, eidt;  // [nnets * enl]
+  double** d_eidt_top = nullptr;           // [nnets] timestep of each net's last layer (or null)
 };
 
 struct SeaEngine {
@@ -170,6 +177,28 @@ struct SeaEngine {
   int nchunks = 0;
   int *d_chunks = nullptr, *d_chunk_start = nullptr, *d_ticket = nullptr;
   double* d_part = nullptr;
+  // uncompressed embedding: atoms in chunks of descriptor positions whose
+  // neighbour slots fit the row buffers; rows laid out net by net
+  struct NetChunk {
+    int q0 = 0, q1 = 0;                 // descriptor positions [q0, q1)
+    std::vector<long long> p0, rows;    // per net
+  };
+  std::vector<NetChunk> net_chunks;
+  long long net_rows_max = 0;
+  int* d_rowbase = nullptr;  // [ntot * ntypes] row of (atom, block) in its chunk (-1: none)
+  double *d_nx0 = nullptr, *d_nds = nullptr, *d_ngg = nullptr;
+  double* d_ndz[2] = {nullptr, nullptr};
+  double* d_ngx[2] = {nullptr, nullptr};
+  std::vector<double*> d_ny, d_nxl;  // per embedding layer: act output, layer output
+  void net_setup();
+  void net_forward(const NetChunk& c, hipStream_t st);
+  void net_descriptor(hipStream_t st);
+  void net_adjoint(hipStream_t st);
+
+ public:
+  // upper bound of the uncompressed embedding's row buffers (bytes); takes
+  // effect for engines created afterwards in this process
+  static long long net_chunk_bytes;
 };
 
 }  // namespace dpmi

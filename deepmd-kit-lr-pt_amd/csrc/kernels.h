@@ -183,6 +183,36 @@ struct EmbedArgs {
   int netiles = 0;
   int* flag = nullptr;                   // device status word (kFlagHalfRange)
 };
+// Uncompressed embedding (FP64 engine): slots <-> net rows, contraction and
+// adjoint, one wavefront per descriptor position q in [q0, q1) (atom perm[q]);
+// the slots of block tj of atom g are rows rowbase[g * ntypes + tj] + s.
+struct NetArgs {
+  const double* em = nullptr;   // [nrows, nnei, 4]
+  const int* perm = nullptr;    // descriptor position -> row
+  const int* pos = nullptr;     // row -> descriptor position
+  const int* type = nullptr;    // [nrows]
+  const int* rowbase = nullptr;
+  const int* sec = nullptr;     // [ntypes + 1]
+  const int* net_of = nullptr;  // [ntypes * ntypes]
+  int ntypes = 0, nnei = 0, M = 0, A = 0, ldD = 0;
+  double inv_nnei = 0;
+  double* xyz = nullptr;        // [nrows, 4, M]
+  double* D = nullptr;          // [nrows, M*A] by position
+  const double* gD = nullptr;   // [nrows, M*A] by position
+  double* net_deriv = nullptr;  // [nrows, nnei, 4]
+};
+// x0[row] = em[g, j, 0]
+void launch_net_gather(const NetArgs& a, int q0, int q1, double* x0, hipStream_t st);
+// xyz and D from the net outputs G [rows, M]
+void launch_net_contract(const NetArgs& a, const double* G, int q0, int q1, hipStream_t st);
+// dE/dG [rows, M] and dE/dz of the last layer (idt_top [nnets] device array of
+// the nets' last-layer timesteps or nulls; ytop its act output), and
+// net_deriv[g, j, c] = sum_k G dy (component 0 without the slope term)
+void launch_net_adjoint(const NetArgs& a, const double* G, const double* ytop, const double* const* idt_top,
+                        int act_top, int q0, int q1, double* gG, double* dz, hipStream_t st);
+// net_deriv[g, j, 0] += ds[row]
+void launch_net_slope(const NetArgs& a, const double* ds, int q0, int q1, hipStream_t st);
+
 void* make_table_infos(const double* infos_host, int nnets, bool fp32);
 size_t table_info_bytes(bool fp32);
 void free_table_infos(void* p, bool fp32);

@@ -203,7 +203,7 @@ SeaModel* build_model(const ModelFile& m, hipStream_t st) {
       fb.push_back(m.fit[t].b[l].data());
       fi.push_back(m.fit[t].idt[l].empty() ? nullptr : m.fit[t].idt[l].data());
     }
-  dpmi_sea_model_desc d;
+  dpmi_sea_model_desc d{};
   d.ntypes = nt;
   d.sel = m.sel.data();
   d.rcut = (float)m.rcut;

@@ -18,11 +18,19 @@ def _arr(a, dtype):
     return np.ascontiguousarray(np.asarray(a, dtype=dtype))
 
 
+def set_net_chunk_bytes(nbytes: int) -> None:
+    """Row-buffer budget of the uncompressed embedding for engines created
+    afterwards (dpmi_sea_set_net_chunk_bytes)."""
+    check(load().dpmi_sea_set_net_chunk_bytes(int(nbytes)))
+
+
 class DeviceModel:
     """An SeAModel (dpmi.model) uploaded to the GPU."""
 
-    def __init__(self, model):
-        if model.tables is None:
+    def __init__(self, model, compressed: bool = True):
+        """compressed=False uploads the embedding nets instead of the tables
+        (the uncompressed path; FP64 engines only)."""
+        if compressed and model.tables is None:
             raise ValueError("the model has no compressed table: call build_tables() first")
         keep = []  # numpy buffers referenced by the descriptor until creation returns
 
@@ -50,9 +58,19 @@ class DeviceModel:
         d.davg = ptr(model.davg.reshape(nt, -1), C.c_double)
         d.dstd = ptr(model.dstd.reshape(nt, -1), C.c_double)
         d.nnets = model.nnets
-        d.nspline = ptr([t.shape[0] for t in model.tables], C.c_int)
-        d.tables = ptr_array(model.tables)
-        d.table_info = ptr(model.table_info, C.c_double)
+        if compressed:
+            d.nspline = ptr([t.shape[0] for t in model.tables], C.c_int)
+            d.tables = ptr_array(model.tables)
+            d.table_info = ptr(model.table_info, C.c_double)
+        else:
+            layers0 = model.embed[0]
+            d.embed_nlayers = len(layers0)
+            d.embed_width = ptr([1] + [ly.w.shape[1] for ly in layers0], C.c_int)
+            d.embed_w = ptr_array([ly.w for net in model.embed for ly in net])
+            d.embed_b = ptr_array([ly.b for net in model.embed for ly in net])
+            d.embed_idt = ptr_array([ly.idt for net in model.embed for ly in net])
+            d.embed_act = ptr([int(ly.act) for ly in layers0], C.c_int)
+            d.embed_resnet = ptr([int(ly.resnet) for ly in layers0], C.c_int)
         fa = model.fitting_arrays()
         L = len(fa)
         d.fit_nlayers = L

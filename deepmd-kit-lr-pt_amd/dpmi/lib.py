@@ -56,6 +56,7 @@ SIGNATURES = {
     "dpmi_sea_engine_destroy": ([_P], None),
     "dpmi_sea_engine_compute": ([_P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
     "dpmi_sea_engine_set_growth": ([_P, _I], _I),
+    "dpmi_sea_set_net_chunk_bytes": ([C.c_longlong], _I),
     "dpmi_sea_engine_compute_lower": ([_P, _P, _P, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P], _I),
     "dpmi_sea_engine_graph_capture": ([_P, _P, _P, _P, _P, _P, _P, _P], _I),
     "dpmi_sea_engine_graph_launch": ([_P, _P], _I),
@@ -94,6 +95,13 @@ class ModelDesc(C.Structure):
         ("fit_act", C.POINTER(_I)),
         ("fit_resnet", C.POINTER(_I)),
         ("bias_atom_e", C.POINTER(_D)),
+        ("embed_nlayers", _I),
+        ("embed_width", C.POINTER(_I)),
+        ("embed_w", C.POINTER(C.POINTER(_D))),
+        ("embed_b", C.POINTER(C.POINTER(_D))),
+        ("embed_idt", C.POINTER(C.POINTER(_D))),
+        ("embed_act", C.POINTER(_I)),
+        ("embed_resnet", C.POINTER(_I)),
     ]
 
 

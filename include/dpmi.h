@@ -194,6 +194,20 @@ typedef struct dpmi_sea_model_desc {
   const int* fit_act;        /* [fit_nlayers] 1 = tanh, 0 = linear */
   const int* fit_resnet;     /* [fit_nlayers] residual allowed (x or [x, x]) */
   const double* bias_atom_e; /* [ntypes] */
+  /* Uncompressed embedding (deepmd/pt/model/descriptor/se_a.py:785-839
+   * without compression), used when tables == NULL: embed_nlayers layers per
+   * net, widths embed_width [embed_nlayers + 1] = 1, n0, ..., M; per (net,
+   * layer) w [n_in, n_out] row-major, b [n_out], idt [n_out] or NULL; per
+   * layer act (1 = tanh) and resnet (x or [x, x] residual allowed).  Such a
+   * model runs in DPMI_FP64 engines only.  Zero embed_nlayers with tables
+   * given: compressed only. */
+  int embed_nlayers;
+  const int* embed_width;
+  const double* const* embed_w;   /* [nnets * embed_nlayers] */
+  const double* const* embed_b;   /* [nnets * embed_nlayers] */
+  const double* const* embed_idt; /* [nnets * embed_nlayers], entries may be NULL */
+  const int* embed_act;           /* [embed_nlayers] */
+  const int* embed_resnet;        /* [embed_nlayers] */
 } dpmi_sea_model_desc;
 
 int dpmi_sea_model_create(dpmi_sea_model** model, const dpmi_sea_model_desc* desc);
@@ -266,6 +280,12 @@ int dpmi_sea_engine_compute_lower(dpmi_sea_engine* engine, const double* coord_e
                                   double* atom_virial_ext, void* stream);
 /* Capacity growth on (1, the default) or off (0): see dpmi_sea_engine_compute. */
 int dpmi_sea_engine_set_growth(dpmi_sea_engine* engine, int enable);
+/* Engines of uncompressed models evaluate the embedding net for every
+ * neighbour slot, in chunks of atoms whose row buffers stay under this many
+ * bytes (default 16 GiB; more than one chunk recomputes the net in the
+ * backward).  Process-wide, for engines created afterwards.  No counterpart
+ * in the reference (which holds every slot's net output at once). */
+int dpmi_sea_set_net_chunk_bytes(long long bytes);
 
 /* hipGraph of one step (no counterpart in the reference; the batched-launch
  * pattern): runs one eager step, then captures the same step on `stream`
