@@ -201,7 +201,54 @@ __global__ void __launch_bounds__(64) k_cell_nlist(const int* __restrict__ frame
   const int w1 = 2 * r1 + 1, w2 = 2 * r2 + 1;
   const int nsearch = (2 * r0 + 1) * w1 * w2;
   int n = 0;
-  if (ti >= 0) {
+  // candidate test shared by both walks: record `slot` of the frame's
+  // cell-ordered extended atoms
+  auto consider = [&](bool valid, int slot) {
+    bool keep = false;
+    unsigned long long key = 0;
+    if (valid) {
+      // slots and indices beyond the frame's capacity exist only after a
+      // ghost-capacity overflow (already flagged): skip them
+      CellAtom rec;
+      rec.j = -1;
+      if (slot < g.ext_cap) rec = cell_atoms[eo + slot];
+      const int j = rec.j;
+      const int tj = (j >= 0 && j < g.ext_cap) ? rec.type : -1;
+      if (j != i && tj >= 0) {
+        const double cj[3] = {rec.x, rec.y, rec.z};
+        float fr2;
+        if (accept(ci, cj, rc2, rc2f, &fr2)) {
+          // out-of-range keys are flagged and dropped, so that the
+          // per-type tables below are indexed by valid types only
+          if (j >= (1 << 24)) atomicOr(flag, kFlagIndexRange);
+          if (tj >= ntypes) atomicOr(flag, kFlagTypeRange);
+          keep = j < (1 << 24) && tj < ntypes;
+          key = make_key(tj, fr2, j);
+        }
+      }
+    }
+    n = wave_append<CAP>(keys, n, keep, key, flag);
+  };
+  const int nruns = (2 * r0 + 1) * w1;  // runs of w2 consecutive cells (z fastest)
+  if (ti >= 0 && nruns <= 32) {
+    // each run's records are contiguous: 64 / nruns lanes walk each run
+    // (9 runs of 3 cells: 7 lanes per run), no search for a candidate's cell
+    const int lpr = kWave / nruns, run = lane / lpr, sub = lane - run * lpr;
+    int st = 0, cnt = 0;
+    if (run < nruns) {
+      const int dx = run / w1 - r0, dy = run % w1 - r1;
+      const int e = ((cx + g.ng[0] + dx) * g.nx[1] + (cy + g.ng[1] + dy)) * g.nx[2] + (cz + g.ng[2] - r2);
+      st = xstart[g.xcell_off + e];
+      cnt = xstart[g.xcell_off + e + w2] - st;
+    }
+    int iters = (cnt + lpr - 1) / lpr;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) iters = max(iters, __shfl_xor(iters, off, kWave));
+    for (int it = 0; it < iters; ++it) {
+      const int p = it * lpr + sub;
+      consider(run < nruns && p < cnt, st + p);
+    }
+  } else if (ti >= 0) {
     for (int cb = 0; cb < nsearch; cb += kWave) {
       const int k = cb + lane;
       int cnt = 0, st = 0;
@@ -225,10 +272,9 @@ __global__ void __launch_bounds__(64) k_cell_nlist(const int* __restrict__ frame
       __syncthreads();
       for (int p0 = 0; p0 < total; p0 += kWave) {
         const int p = p0 + lane;
-        bool keep = false;
-        unsigned long long key = 0;
+        int lo = 0;
         if (p < total) {
-          int lo = 0, hi = kWave;  // find cell q with cpre[q] <= p < cpre[q+1]
+          int hi = kWave;  // find cell q with cpre[q] <= p < cpre[q+1]
           while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
             if (cpre[mid] <= p)
@@ -236,29 +282,9 @@ __global__ void __launch_bounds__(64) k_cell_nlist(const int* __restrict__ frame
             else
               hi = mid;
           }
-          // slots and indices beyond the frame's capacity exist only after a
-          // ghost-capacity overflow (already flagged): skip them
-          // candidates of a cell are consecutive records (coalesced reads)
-          const int slot = cstart[lo] + (p - cpre[lo]);
-          CellAtom rec;
-          rec.j = -1;
-          if (slot < g.ext_cap) rec = cell_atoms[eo + slot];
-          const int j = rec.j;
-          const int tj = (j >= 0 && j < g.ext_cap) ? rec.type : -1;
-          if (j != i && tj >= 0) {
-            const double cj[3] = {rec.x, rec.y, rec.z};
-            float fr2;
-            if (accept(ci, cj, rc2, rc2f, &fr2)) {
-              // out-of-range keys are flagged and dropped, so that the
-              // per-type tables below are indexed by valid types only
-              if (j >= (1 << 24)) atomicOr(flag, kFlagIndexRange);
-              if (tj >= ntypes) atomicOr(flag, kFlagTypeRange);
-              keep = j < (1 << 24) && tj < ntypes;
-              key = make_key(tj, fr2, j);
-            }
-          }
         }
-        n = wave_append<CAP>(keys, n, keep, key, flag);
+        // candidates of a cell are consecutive records (coalesced reads)
+        consider(p < total, cstart[lo] + (p - cpre[lo]));
       }
       __syncthreads();
     }
