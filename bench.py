@@ -83,10 +83,10 @@ def stage_model(m, nloc, nall, precision):
 # FETCH_SIZE / WRITE_SIZE on this workload, scaled by a 256 MiB copy; the
 # readings and the commands are in TRAFFIC_PROFILE.
 MEASURED_TRAFFIC_MIB = {
-    "mixed": {"embed_bwd": (124.76, 25.44), "embed_fwd": (31.05, 54.89), "fitting": (61.84, 97.52),
-              "force_virial": (36.28, 18.02), "nlist_env": (19.60, 33.00)},
-    "fp64": {"embed_bwd": (259.61, 50.48), "embed_fwd": (85.64, 201.01), "force_virial": (41.69, 18.60),
-             "nlist_env": (22.37, 58.88)},
+    "mixed": {"embed_bwd": (124.95, 25.47), "embed_fwd": (31.57, 54.89), "fitting": (63.55, 93.02),
+              "force_virial": (38.24, 29.14), "nlist_env": (13.17, 33.59)},
+    "fp64": {"embed_bwd": (261.42, 50.73), "embed_fwd": (84.66, 200.97), "force_virial": (64.07, 26.76),
+             "nlist_env": (15.83, 58.91)},
 }
 
 

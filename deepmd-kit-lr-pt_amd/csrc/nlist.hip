@@ -101,10 +101,15 @@ __device__ __forceinline__ void wave_bitonic(unsigned long long (&kr)[KR]) {
   }
 }
 
-// Sorts keys[0, n) (LDS) and writes the formatted row.
-template <int CAP>
+struct NoSlotWork {
+  __device__ void operator()(int, int) const {}
+};
+
+// Sorts keys[0, n) (LDS) and writes the formatted row; slot_work(s, j) runs
+// for every slot s as its entry j is written (same lane).
+template <int CAP, typename F = NoSlotWork>
 __device__ void sort_and_slot(unsigned long long* keys, int n, int* tfirst, int* tend, const int* sec, int ntypes,
-                              int nnei, int* __restrict__ row) {
+                              int nnei, int* __restrict__ row, F slot_work = F()) {
   const int lane = lane_id();
   if (n > CAP) n = CAP;
   int P = 64;
@@ -168,7 +173,9 @@ __device__ void sort_and_slot(unsigned long long* keys, int n, int* tfirst, int*
     while (ty + 1 < ntypes && s >= sec[ty + 1]) ++ty;
     const int r = s - sec[ty];
     const int cnt = tend[ty] - tfirst[ty];
-    row[s] = r < cnt ? (int)(keys[tfirst[ty] + r] & 0xFFFFFFull) : -1;
+    const int j = r < cnt ? (int)(keys[tfirst[ty] + r] & 0xFFFFFFull) : -1;
+    row[s] = j;
+    slot_work(s, j);
   }
 }
 
@@ -291,20 +298,19 @@ __global__ void __launch_bounds__(64) k_cell_nlist(const int* __restrict__ frame
   }
   if (ncand && lane == 0) ncand[gi] = n;
   int* row = nlist + (size_t)gi * nnei;
-  sort_and_slot<CAP>(keys, n, tfirst, tend, sec, ntypes, nnei, row);
-  // environment-matrix rows of the slots this lane just formatted
+  // environment-matrix rows of the slots as the lane formats them
   if (em_f) {
-    for (int s = lane; s < nnei; s += kWave) {
-      const int j = row[s];
+    sort_and_slot<CAP>(keys, n, tfirst, tend, sec, ntypes, nnei, row, [&](int s, int j) {
       envc::slot_em<float>(ci, coord_ext + 3 * (eo + (j >= 0 ? j : i)), j, ti, s, nnei, avg, stdv, rmin, rmax,
                            em_f + ((size_t)gi * nnei + s) * 4);
-    }
+    });
   } else if (em_d) {
-    for (int s = lane; s < nnei; s += kWave) {
-      const int j = row[s];
+    sort_and_slot<CAP>(keys, n, tfirst, tend, sec, ntypes, nnei, row, [&](int s, int j) {
       envc::slot_em<double>(ci, coord_ext + 3 * (eo + (j >= 0 ? j : i)), j, ti, s, nnei, avg, stdv, rmin, rmax,
                             em_d + ((size_t)gi * nnei + s) * 4);
-    }
+    });
+  } else {
+    sort_and_slot<CAP>(keys, n, tfirst, tend, sec, ntypes, nnei, row);
   }
 }
 
