@@ -1,4 +1,4 @@
-"""Training step of a compressed se_e2_a model through hand-written adjoints.
+"""Training step of an se_e2_a model through hand-written adjoints.
 
 The energy is differentiated twice: forces need dE/d(em), and a force loss
 needs the gradient of those forces with respect to the fitting parameters.
@@ -14,7 +14,12 @@ Every non-library derivative on that path is a hand-written GPU kernel:
 
 The GRRG contraction and the fitting net are plain tensor algebra (GEMMs and
 tanh) on the device, as in deepmd/pt/model/descriptor/se_a.py:844-851 and
-deepmd/pt/model/task/fitting.py:1079-1104.  The environment matrix comes from
+deepmd/pt/model/task/fitting.py:1079-1104.  With ``embedding="net"`` the
+table is replaced by the embedding nets themselves (se_a.py:785-839 without
+compression: one net per type pair or per neighbour type, tanh layers with
+timestep and x / [x, x] residuals, network/mlp.py:188-219), whose parameters
+train with the fitting's; their first and second derivatives are torch
+autograd of those tensor ops (GEMMs on the ROCm BLAS libraries).  The environment matrix comes from
 the kernel-level ``prod_env_mat_a``; neighbour indices are mapped to their
 owning local atoms, so the force kernels fold ghost contributions directly
 and the adjoint's local-index convention (prod_force_grad.cc:57) is exact.
@@ -118,17 +123,38 @@ def env_batch(model, coords, atype, boxes, dtype=torch.float32) -> EnvBatch:
 
 # ------------------------------------------------------------------ model --
 class SeaTrainer(torch.nn.Module):
-    """Compressed se_e2_a energy with trainable fitting parameters (tables
-    fixed), forces and virials through the hand-written adjoints."""
+    """se_e2_a energy with trainable fitting parameters, forces and virials
+    through the hand-written adjoints.  embedding="table": the compressed
+    table (fixed) through the tabulate op; embedding="net": the embedding
+    nets, trainable."""
 
-    def __init__(self, model, dtype=torch.float32):
+    def __init__(self, model, dtype=torch.float32, embedding: str = "table"):
         super().__init__()
-        _load_op()
+        if embedding not in ("table", "net"):
+            raise ValueError("embedding is 'table' or 'net'")
         dev = torch.device("cuda")
         self.m = model
         self.dtype = dtype
-        self.tables = [torch.as_tensor(t, dtype=dtype, device=dev) for t in model.tables]
-        self.infos = [torch.as_tensor(model.table_info[n], dtype=dtype) for n in range(len(model.tables))]
+        self.embedding = embedding
+        if embedding == "table":
+            _load_op()
+            self.tables = [torch.as_tensor(t, dtype=dtype, device=dev) for t in model.tables]
+            self.infos = [torch.as_tensor(model.table_info[n], dtype=dtype) for n in range(len(model.tables))]
+        else:
+            # per net and layer: W [n_in, n_out], b, idt (absent: not trained)
+            self.eW = torch.nn.ParameterList()
+            self.eb = torch.nn.ParameterList()
+            self.eidt = torch.nn.ParameterList()
+            self.e_shape = []
+            for net in model.embed:
+                for ly in net:
+                    self.eW.append(torch.nn.Parameter(torch.as_tensor(ly.w, dtype=dtype, device=dev)))
+                    self.eb.append(torch.nn.Parameter(torch.as_tensor(ly.b, dtype=dtype, device=dev)))
+                    self.eidt.append(torch.nn.Parameter(
+                        torch.as_tensor(ly.idt if ly.idt is not None else np.zeros_like(ly.b), dtype=dtype,
+                                        device=dev), requires_grad=ly.idt is not None))
+                    self.e_shape.append((ly.act, ly.resnet, ly.idt is not None))
+            self.e_nl = len(model.embed[0])
         fa = model.fitting_arrays()
         self.nlayers = len(fa)
         self.act = [bool(a) for (_w, _b, _i, a, _r) in fa]
@@ -151,6 +177,34 @@ class SeaTrainer(torch.nn.Module):
     def net_index(self, ti: int, tj: int) -> int:
         return tj if self.m.type_one_side else tj * self.m.ntypes + ti
 
+    def embed_net(self, net: int, x: torch.Tensor) -> torch.Tensor:
+        """Embedding net `net` on inputs x [n] -> [n, M] (mlp.py:188-219)."""
+        h = x.reshape(-1, 1)
+        for li in range(self.e_nl):
+            k = net * self.e_nl + li
+            act, resnet, has_idt = self.e_shape[k]
+            y = h @ self.eW[k] + self.eb[k]
+            if act:
+                y = torch.tanh(y)
+            if has_idt:
+                y = y * self.eidt[k]
+            if resnet and y.shape[1] == h.shape[1]:
+                y = y + h
+            elif resnet and y.shape[1] == 2 * h.shape[1]:
+                y = y + torch.cat([h, h], dim=1)
+            h = y
+        return h
+
+    def contract(self, net: int, blk: torch.Tensor) -> torch.Tensor:
+        """xyz contribution [r, 4, M] of one neighbour block blk [r, nb, 4]."""
+        M = self.m.M
+        if self.embedding == "table":
+            return torch.ops.deepmd.tabulate_fusion_se_a(self.tables[net], self.infos[net],
+                                                         blk[:, :, 0].reshape(-1, 1).contiguous(), blk, M)[0]
+        r, nb = blk.shape[0], blk.shape[1]
+        G = self.embed_net(net, blk[:, :, 0].reshape(-1)).reshape(r, nb, M)
+        return torch.bmm(blk.transpose(1, 2).contiguous(), G)
+
     def atom_energy(self, em: torch.Tensor, atype: torch.Tensor) -> torch.Tensor:
         """em [n, nnei, 4] -> per-atom energy [n]."""
         m = self.m
@@ -166,9 +220,7 @@ class SeaTrainer(torch.nn.Module):
             xyz = None
             for tj in range(m.ntypes):
                 blk = e[:, sec[tj]:sec[tj + 1], :].contiguous()
-                net = self.net_index(ti, tj)
-                o = torch.ops.deepmd.tabulate_fusion_se_a(self.tables[net], self.infos[net],
-                                                          blk[:, :, 0].reshape(-1, 1).contiguous(), blk, M)[0]
+                o = self.contract(self.net_index(ti, tj), blk)
                 xyz = o if xyz is None else xyz + o
             T = xyz / nnei                                   # [r, 4, M]
             D = torch.matmul(T.transpose(1, 2).contiguous(), T[:, :, :A].contiguous())  # [r, M, A]
@@ -187,7 +239,7 @@ class SeaTrainer(torch.nn.Module):
 
     def forward(self, batch: EnvBatch, need_virial: bool = True):
         """Returns (energy [nf], force [nf, nloc, 3], virial [nf, 9] or None);
-        all differentiable with respect to the fitting parameters."""
+        all differentiable with respect to the trainable parameters."""
         em = batch.em.detach().requires_grad_(True)
         ae = self.atom_energy(em, batch.atype)
         energy = ae.reshape(batch.nframes, batch.nloc).sum(1)

@@ -1,5 +1,5 @@
-"""Test-only differentiable restatement of the compressed se_e2_a energy in
-plain torch: E(local coordinates, fitting parameters), so that forces and
+"""Test-only differentiable restatement of the se_e2_a energy (compressed
+table, or the embedding nets themselves) in plain torch: E(local coordinates, fitting parameters), so that forces and
 force-loss parameter gradients come from torch autograd (the role the
 reference's PT autograd plays, deepmd/pt/model/model/transform_output.py
 :70-101).  Ghosts and neighbour lists are taken from the kernel-level path
@@ -64,10 +64,29 @@ def _table_eval(table, info, x):
     return v + d * delta[:, None]
 
 
-def energy(model, topo, x, params):
+def _net_eval(layers, x):
+    """Embedding net on inputs x [n] -> [n, M]; layers: list of (W, b, idt or
+    None, act, resnet) torch tensors / flags (network/mlp.py:188-219)."""
+    h = x.reshape(-1, 1)
+    for (W, b, idt, act, resnet) in layers:
+        y = h @ W + b
+        if act:
+            y = torch.tanh(y)
+        if idt is not None:
+            y = y * idt
+        if resnet and y.shape[1] == h.shape[1]:
+            y = y + h
+        elif resnet and y.shape[1] == 2 * h.shape[1]:
+            y = y + torch.cat([h, h], dim=1)
+        h = y
+    return h
+
+
+def energy(model, topo, x, params, embed=None):
     """Total energy of one frame.  x [nloc, 3] FP64 (may require grad);
     params: list over fitting layers of (W [nt, in, out], b [nt, out],
-    idt [nt, out] or None) torch tensors."""
+    idt [nt, out] or None) torch tensors; embed: None (the compressed table)
+    or, per net, the layers of the embedding net as for _net_eval."""
     nloc = x.shape[0]
     nl = topo["nlist"]
     nnei = nl.shape[1]
@@ -96,8 +115,11 @@ def energy(model, topo, x, params):
         for tj in range(model.ntypes):
             net = tj if model.type_one_side else tj * model.ntypes + ti
             blk = em[rows, sec[tj]:sec[tj + 1], :]
-            G = _table_eval(torch.as_tensor(model.tables[net]), model.table_info[net],
-                            blk[..., 0].reshape(-1)).reshape(blk.shape[0], blk.shape[1], M)
+            if embed is None:
+                G = _table_eval(torch.as_tensor(model.tables[net]), model.table_info[net],
+                                blk[..., 0].reshape(-1)).reshape(blk.shape[0], blk.shape[1], M)
+            else:
+                G = _net_eval(embed[net], blk[..., 0].reshape(-1)).reshape(blk.shape[0], blk.shape[1], M)
             xyz = xyz + torch.einsum("rjc,rjk->rck", blk, G)
         T = xyz / nnei
         D = torch.einsum("rck,rca->rka", T, T[:, :, :A]).reshape(rows.numel(), M * A)
