@@ -80,6 +80,10 @@ int flag_to_status(int flag) {
     return fail(DPMI_ERR_RANGE, "a mixed-precision matrix-core operand exceeds the FP16 range");
   if (flag & dpmi::kFlagTypeMismatch)
     return fail(DPMI_ERR_ARGUMENT, "the local atom types differ from the types the engine was created for");
+  if (flag & dpmi::kFlagGridChanged)
+    return fail(DPMI_ERR_ARGUMENT,
+                "a replayed box needs another cell grid than the captured one (the step used the old box): capture "
+                "again");
   return kOk;
 }
 
@@ -550,6 +554,16 @@ int dpmi_sea_engine_graph_capture(dpmi_sea_engine* engine, const double* coord, 
   return guarded([&] {
     if (!engine || !coord || !box_host || !energy || !force || !virial) throw std::invalid_argument("null argument");
     engine->impl->capture(coord, box_host, energy, force, virial, atom_energy, S(stream));
+    return kOk;
+  });
+}
+
+int dpmi_sea_engine_graph_capture_device_box(dpmi_sea_engine* engine, const double* coord, const double* box_dev,
+                                             double* energy, double* force, double* virial, double* atom_energy,
+                                             void* stream) {
+  return guarded([&] {
+    if (!engine || !coord || !box_dev || !energy || !force || !virial) throw std::invalid_argument("null argument");
+    engine->impl->capture_device_box(coord, box_dev, energy, force, virial, atom_energy, S(stream));
     return kOk;
   });
 }

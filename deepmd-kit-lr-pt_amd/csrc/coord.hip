@@ -348,6 +348,37 @@ void launch_normalize_f32(float* coord, int natom, const float* boxt9, const flo
   DPMI_LAUNCH_CHECK();
 }
 
+namespace {
+// Box of frame f from a device buffer (a captured step replayed for a new
+// box): the frame's box and reciprocal box are replaced when the cell grid
+// (cells and search layers per dimension) stays the same; otherwise the
+// frame keeps its geometry and the grid change is flagged.
+__global__ void k_geom_refresh(const double* __restrict__ box, FrameGeom* __restrict__ geom, int nframes, float rcut,
+                               int* __restrict__ flag) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nframes) return;
+  FrameGeom g = geom[f];
+  geom_from_box(g, box + 9 * (size_t)f, rcut);
+  const FrameGeom& c = geom[f];
+  bool same = true;
+  for (int d = 0; d < 3; ++d) same = same && g.nc[d] == c.nc[d] && g.ng[d] == c.ng[d] && g.nx[d] == c.nx[d];
+  if (!same) {
+    atomicOr(flag, kFlagGridChanged);
+    return;
+  }
+  for (int k = 0; k < 9; ++k) {
+    geom[f].t[k] = g.t[k];
+    geom[f].rec[k] = g.rec[k];
+  }
+}
+}  // namespace
+
+void launch_geom_refresh(const double* box, FrameGeom* geom, int nframes, float rcut, int* flag, hipStream_t st) {
+  if (nframes <= 0) return;
+  hipLaunchKernelGGL(k_geom_refresh, dim3(ceil_div(nframes, 64)), dim3(64), 0, st, box, geom, nframes, rcut, flag);
+  DPMI_LAUNCH_CHECK();
+}
+
 void launch_normalize(double* coord, int natom, const FrameGeom& g, hipStream_t st) {
   if (natom <= 0) return;
   hipLaunchKernelGGL(k_normalize, dim3(ceil_div(natom, 256)), dim3(256), 0, st, coord, natom, g);

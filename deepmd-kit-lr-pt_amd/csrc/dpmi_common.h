@@ -24,6 +24,7 @@ enum DeviceFlag : int {
   kFlagCellCapacity = 16,     // a cell holds more atoms than the binning assumes
   kFlagHalfRange = 32,        // a mixed-precision matrix-core operand exceeds the FP16 range
   kFlagTypeMismatch = 64,     // a caller's local atom type differs from the engine's
+  kFlagGridChanged = 128,     // a box refreshed on the device needs another cell grid
 };
 
 struct HipError : std::runtime_error {

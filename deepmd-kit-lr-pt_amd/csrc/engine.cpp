@@ -1217,11 +1217,13 @@ void SeaEngine::enqueue_step(const double* coord, const double* box_h, double* e
   // layout (box) invalidates a captured graph, which holds the old one.
   if (!geom_uploaded || std::memcmp(geom_pinned, geom_h.data(), sizeof(FrameGeom) * nframes) != 0) {
     DPMI_CHECK(hipStreamSynchronize(st));
-    if (!capturing) drop_graph();
+    if (!capturing && !graph_dbox) drop_graph();  // a device-box graph re-reads its box
     std::memcpy(geom_pinned, geom_h.data(), sizeof(FrameGeom) * nframes);
     DPMI_CHECK(hipMemcpyAsync(d_geom, geom_pinned, sizeof(FrameGeom) * nframes, hipMemcpyHostToDevice, st));
     geom_uploaded = true;
   }
+  // a step captured for a box in device memory re-reads it on every replay
+  if (capturing && graph_box) launch_geom_refresh(graph_box, d_geom, nframes, m.rcut, d_flag, st);
   // force/virial arguments (the aggregated force kernel adds onto the owner
   // rows of `force`, which the wrap kernel clears)
   ForceRowsArgs fa;
@@ -1531,10 +1533,28 @@ void SeaEngine::autotune(const double* coord, const double* box_h, int reps, hip
 // The captured step holds only kernel nodes (every clear is a fill kernel
 // and no library is called, in either precision), so replays are exact
 // repeats of an eager step on the current coordinates.
+void SeaEngine::capture_device_box(const double* coord, const double* box_dev, double* energy, double* force,
+                                   double* virial, double* atom_energy, hipStream_t st) {
+  if (!st) throw std::invalid_argument("graph capture needs a non-default stream");
+  std::vector<double> box_h((size_t)nframes * 9);
+  DPMI_CHECK(hipMemcpyAsync(box_h.data(), box_dev, sizeof(double) * box_h.size(), hipMemcpyDeviceToHost, st));
+  DPMI_CHECK(hipStreamSynchronize(st));
+  graph_box = box_dev;
+  try {
+    capture(coord, box_h.data(), energy, force, virial, atom_energy, st);
+  } catch (...) {
+    graph_box = nullptr;
+    throw;
+  }
+  graph_box = nullptr;
+  graph_dbox = true;
+}
+
 void SeaEngine::capture(const double* coord, const double* box_h, double* energy, double* force, double* virial,
                         double* atom_energy, hipStream_t st) {
   if (!st) throw std::invalid_argument("graph capture needs a non-default stream");
   // geometry, buffers, uploads (and any capacity growth) for these boxes
+  graph_dbox = false;
   compute(coord, box_h, energy, force, virial, atom_energy, nullptr, st);
   DPMI_CHECK(hipStreamSynchronize(st));
   drop_graph();
@@ -1565,6 +1585,7 @@ void SeaEngine::launch_graph(hipStream_t st) {
         "reallocated): capture again");
   DPMI_CHECK(hipGraphLaunch(graph_exec, st));
   last_stream = st;
+  if (graph_dbox) geom_uploaded = false;  // the device geometry now holds the replayed box
 }
 
 int SeaEngine::status_flags() {

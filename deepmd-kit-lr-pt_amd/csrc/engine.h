@@ -92,6 +92,10 @@ struct SeaEngine {
   void capture(const double* coord, const double* box_h, double* energy, double* force, double* virial,
                double* atom_energy, hipStream_t st);
   void launch_graph(hipStream_t st);
+  // the same with the boxes in device memory [nframes, 9]: every replay
+  // re-reads them (NPT), provided the cell grid stays the one of the capture
+  void capture_device_box(const double* coord, const double* box_dev, double* energy, double* force, double* virial,
+                          double* atom_energy, hipStream_t st);
   // picks the launch-shape knobs for this engine's size bucket (and caches
   // them for later engines of the same bucket)
   void autotune(const double* coord, const double* box_h, int reps, hipStream_t st);
@@ -116,6 +120,8 @@ struct SeaEngine {
   // reallocation or new box drops it (launch_graph then asks for a capture)
   void drop_graph();
   bool capturing = false;
+  const double* graph_box = nullptr;  // set while capturing for device boxes
+  bool graph_dbox = false;            // the captured graph re-reads its boxes from device memory
   int sticky = 0;               // status bits read back by growing steps, reported by status()
   int* stat_pinned = nullptr;   // [1 + nframes]: status word and per-frame nall of the last growing step
   std::vector<int> ext_need_h;  // per-frame extended capacity learned from overflows

@@ -27,11 +27,13 @@ struct FrameGeom {
 
 // Distance between opposite faces of the box along each cell vector
 // (SimulationRegion_Impl.h:364-372 toFaceDistance: volume / |a x b|).
-inline void geom_face_distances(const double* m, double* tf) {
+// Host and device share these functions, so a box refreshed on the device
+// gives the same bits as on the host.
+__host__ __device__ inline void geom_face_distances(const double* m, double* tf) {
 #pragma clang fp contract(off)
   double vol = m[0] * (m[4] * m[8] - m[7] * m[5]) - m[1] * (m[3] * m[8] - m[6] * m[5]) +
                m[2] * (m[3] * m[7] - m[6] * m[4]);
-  vol = std::fabs(vol);
+  vol = ::fabs(vol);
   const double* a[3] = {m, m + 3, m + 6};
   for (int d = 0; d < 3; ++d) {
     const double* u = a[(d + 1) % 3];
@@ -39,7 +41,7 @@ inline void geom_face_distances(const double* m, double* tf) {
     const double c0 = u[1] * v[2] - u[2] * v[1];
     const double c1 = u[2] * v[0] - u[0] * v[2];
     const double c2 = u[0] * v[1] - u[1] * v[0];
-    tf[d] = vol * (1. / std::sqrt(c0 * c0 + c1 * c1 + c2 * c2));
+    tf[d] = vol * (1. / ::sqrt(c0 * c0 + c1 * c1 + c2 * c2));
   }
 }
 
@@ -47,13 +49,13 @@ inline void geom_face_distances(const double* m, double* tf) {
 // (region.cc:40-85, SimulationRegion_Impl.h:364-372, coord.cc:69-108,
 // neighbor_list.cc:758-766) so that fractional coordinates, cell indices and
 // image shifts agree bit for bit.
-inline void geom_from_box(FrameGeom& g, const double* boxt, float rcut) {
+__host__ __device__ inline void geom_from_box(FrameGeom& g, const double* boxt, float rcut) {
 #pragma clang fp contract(off)
-  std::memcpy(g.t, boxt, sizeof(double) * 9);
+  for (int k = 0; k < 9; ++k) g.t[k] = boxt[k];
   const double* m = g.t;
   double vol = m[0] * (m[4] * m[8] - m[7] * m[5]) - m[1] * (m[3] * m[8] - m[6] * m[5]) +
                m[2] * (m[3] * m[7] - m[6] * m[4]);
-  vol = std::fabs(vol);
+  vol = ::fabs(vol);
   const double vi = 1.0 / vol;
   double* r = g.rec;
   r[0] = (m[4] * m[8] - m[7] * m[5]) * vi;

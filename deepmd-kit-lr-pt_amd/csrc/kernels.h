@@ -52,6 +52,9 @@ void launch_build_extended(const CellWork& w, hipStream_t st);
 int copy_coord_run(FrameGeom g, double* out_c, int* out_t, int* mapping, int* nall_host, const double* in_c,
                    const int* in_t, int nloc, int mem_nall, void* stream);
 void launch_normalize(double* coord, int natom, const FrameGeom& g, hipStream_t st);
+// geom[f].t / rec from box [nframes, 9] (device) when the cell grid stays the
+// same; kFlagGridChanged otherwise
+void launch_geom_refresh(const double* box, FrameGeom* geom, int nframes, float rcut, int* flag, hipStream_t st);
 // normalize_coord_gpu<float>: FP32 arithmetic with the caller's box and
 // reciprocal box (host copies of the Region's rows)
 void launch_normalize_f32(float* coord, int natom, const float* boxt9, const float* rec9, hipStream_t st);

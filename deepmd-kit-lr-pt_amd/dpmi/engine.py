@@ -205,6 +205,24 @@ class Engine:
         self._graph_coord = coord  # the captured graph reads this storage
         return self.energy, self.force, self.virial
 
+    def capture_device_box(self, coord: torch.Tensor, box: torch.Tensor, atom_energy: bool = False):
+        """As capture(), with the boxes in a device tensor [nframes, 9]
+        (float64) that every replay re-reads (rescale coordinates and boxes
+        in place between replays; a box that needs another cell grid is
+        reported by status())."""
+        self._check_coord(coord)
+        if box.device.type != "cuda" or box.dtype != torch.float64 or not box.is_contiguous() \
+                or box.numel() != 9 * self.nframes:
+            raise ValueError("box: contiguous float64 device tensor of nframes * 9 values")
+        st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        check(load().dpmi_sea_engine_graph_capture_device_box(
+            self.handle, C.c_void_p(coord.data_ptr()), C.c_void_p(box.data_ptr()),
+            C.c_void_p(self.energy.data_ptr()), C.c_void_p(self.force.data_ptr()),
+            C.c_void_p(self.virial.data_ptr()), C.c_void_p(self.atom_energy.data_ptr()) if atom_energy else None, st))
+        self._graph_coord = coord
+        self._graph_box = box  # the captured graph reads this storage
+        return self.energy, self.force, self.virial
+
     def replay(self):
         st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
         check(load().dpmi_sea_engine_graph_launch(self.handle, st))

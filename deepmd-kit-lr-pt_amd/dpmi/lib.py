@@ -59,6 +59,7 @@ SIGNATURES = {
     "dpmi_sea_set_net_chunk_bytes": ([C.c_longlong], _I),
     "dpmi_sea_engine_compute_lower": ([_P, _P, _P, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P], _I),
     "dpmi_sea_engine_graph_capture": ([_P, _P, _P, _P, _P, _P, _P, _P], _I),
+    "dpmi_sea_engine_graph_capture_device_box": ([_P, _P, _P, _P, _P, _P, _P, _P], _I),
     "dpmi_sea_engine_graph_launch": ([_P, _P], _I),
     "dpmi_sea_engine_autotune": ([_P, _P, _P, _I, _P], _I),
     "dpmi_sea_engine_tuning": ([_P, _P], _I),

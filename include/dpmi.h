@@ -298,6 +298,15 @@ int dpmi_sea_engine_graph_capture(dpmi_sea_engine* engine, const double* coord, 
                                   double* energy, double* force, double* virial, double* atom_energy,
                                   void* stream);
 int dpmi_sea_engine_graph_launch(dpmi_sea_engine* engine, void* stream);
+/* The same capture with the boxes in device memory, box_dev [nframes, 9]:
+ * every replay re-reads them (NPT: the caller rescales coordinates and boxes
+ * in place between replays).  A box whose cell grid (cells and search
+ * layers per dimension) differs from the captured one is not applied: the
+ * step runs with the previous box and dpmi_sea_engine_status reports
+ * DPMI_ERR_ARGUMENT (capture again).  No counterpart in the reference. */
+int dpmi_sea_engine_graph_capture_device_box(dpmi_sea_engine* engine, const double* coord, const double* box_dev,
+                                             double* energy, double* force, double* virial, double* atom_energy,
+                                             void* stream);
 
 /* Autotuner (no counterpart in the reference): times `reps` steps for each
  * candidate launch shape of the env-mat, embedding and force stages and

@@ -88,3 +88,56 @@ def test_autotune_keeps_results_and_caches(water, batch, precision):
     assert _close(e, e0, tol) and _close(f, f0, tol) and _close(v, v0, tol)
     later = Engine(dm, natoms, atype, precision=precision)
     assert later.tuning() == knobs
+
+
+def _grid(box, rc=6.0):
+    """Cells per dimension of cubic boxes [nf, 9] (face distance / rcut)."""
+    return np.floor(box[:, [0, 4, 8]] / rc).astype(int)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_graph_device_box_follows_rescaled_boxes(water, batch, precision):
+    """NPT-style replays: coordinates and boxes rescaled in device memory
+    between replays; each replay equals an eager step on the new box.  A box
+    that needs another cell grid is reported (the step keeps the old box)."""
+    natoms, coord, atype, box = batch
+    dm = DeviceModel(water)
+    eg = Engine(dm, natoms, atype, precision=precision)
+    ee = Engine(dm, natoms, atype, precision=precision)
+    x = torch.as_tensor(coord, device="cuda")
+    b = torch.as_tensor(np.asarray(box, np.float64).reshape(-1, 9), device="cuda").contiguous()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        eg.capture_device_box(x, b)
+    s.synchronize()
+    tol_e, tol_fv = (1e-12, 1e-11) if precision == "fp64" else (1e-10, 1e-9)
+    g0 = _grid(b.cpu().numpy())
+    done = 0
+    for scale in (1.0015, 0.9985, 1.003, 0.997, 1.001):
+        nb = b.cpu().numpy() * scale
+        if not np.array_equal(_grid(nb), g0):
+            continue  # this scale would cross a cell-count boundary of some frame
+        print(f"device-box replay {precision} scale {scale}", flush=True)
+        with torch.cuda.stream(s):
+            x.mul_(scale)
+            b.mul_(scale)
+            e, f, v = [t.clone() for t in eg.replay()]
+        s.synchronize()
+        eg.status()
+        e1, f1, v1 = ee.compute(x.clone(), b.cpu().numpy())
+        torch.cuda.synchronize()
+        ee.status()
+        assert _close(e.cpu().numpy(), e1.cpu().numpy(), tol_e)
+        assert _close(f.cpu().numpy(), f1.cpu().numpy(), tol_fv)
+        assert _close(v.cpu().numpy(), v1.cpu().numpy(), tol_fv)
+        g0 = _grid(b.cpu().numpy())
+        done += 1
+    assert done >= 2
+    # a box with more cells per dimension: reported, nothing out of bounds
+    with torch.cuda.stream(s):
+        x.mul_(1.6)
+        b.mul_(1.6)
+        eg.replay()
+    s.synchronize()
+    with pytest.raises(DpmiError, match="ARGUMENT"):
+        eg.status()
