@@ -65,7 +65,9 @@ __global__ void k_wrap_bin(const double* __restrict__ coord_in, const int* __res
     for (int d = 0; d < 3; ++d)
       f[d] = p[0] * g.rec[d * 3 + 0] + p[1] * g.rec[d * 3 + 1] + p[2] * g.rec[d * 3 + 2];
     for (int d = 0; d < 3; ++d) {
-      f[d] = fmod(f[d], 1.0);
+      // fmod(f, 1.0) exactly: f - trunc(f) is representable, and fmod's
+      // zero carries the sign of f (the library routine loops)
+      f[d] = copysign(f[d] - trunc(f[d]), f[d]);
       if (f[d] < 0.) f[d] += 1.;
     }
     for (int d = 0; d < 3; ++d) p[d] = f[0] * g.t[0 * 3 + d] + f[1] * g.t[1 * 3 + d] + f[2] * g.t[2 * 3 + d];
@@ -283,7 +285,8 @@ __global__ void k_emit_ghosts(const int* __restrict__ xcell_frame, const FrameGe
 void launch_build_extended(const CellWork& w, hipStream_t st) {
   // the emission leaves every count at zero; a first build clears them
   if (!w.counts_clear) launch_fill_int(w.cell_cnt, 0, w.ncells_tot, st);  // a kernel, not a memset node
-  const int tpb = 256;
+  // one wave per workgroup: the per-atom kernels spread over more compute units
+  const int tpb = 64;
   hipLaunchKernelGGL(k_wrap_bin, dim3(ceil_div(w.ntot, tpb)), dim3(tpb), 0, st, w.coord_in, w.type_in, w.frame_of,
                      w.geom, w.ntot, w.coord_ext, w.type_ext, w.map_ext, w.cell_of, w.cell_slot, w.cell_cnt,
                      w.do_wrap, w.force_clear);
