@@ -324,17 +324,20 @@ __device__ __forceinline__ void bwd_dy(T (&dy)[KQ][4], const T* __restrict__ xyz
 #pragma unroll
     for (int c = 0; c < 4; ++c) dT[kq][c] = 0;
     const int k = lane + kq * kWave;
-    if (k < M) {
+    // every load issued before the first use (indices clamped into the
+    // row, the values masked), so the 16 reads are in flight together
+    const bool kin = k < M;
+    const size_t kr = (size_t)(kin ? k : 0) * A;
+    TG gv[16];
 #pragma unroll
-      for (int a = 0; a < 16; ++a) {
-        if (a < A) {
-          const T gk = (T)G[(size_t)k * A + a];
+    for (int a = 0; a < 16; ++a) gv[a] = G[kr + (a < A ? a : 0)];
 #pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            dT[kq][c] += gk * ta[a][c];
-            sp[c * 16 + a] += gk * Tk[kq][c];
-          }
-        }
+    for (int a = 0; a < 16; ++a) {
+      const T gk = kin && a < A ? (T)gv[a] : (T)0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        dT[kq][c] += gk * ta[a][c];
+        sp[c * 16 + a] += gk * Tk[kq][c];
       }
     }
   }
@@ -1061,6 +1064,303 @@ __global__ void __launch_bounds__(256) k_embed_bwd_mfma(const EmbedArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Adjoint of the mixed engine from the table rows of each atom's neighbour
+// runs, on the matrix cores (one wavefront per centre row).
+//
+// With T_j[p] = t^p + p delta t^(p-1) (t the neighbour's offset in its table
+// row r_j, delta its linear extension beyond the table) the table value is
+// P_k(x_j) = sum_p a_p[r_j, k] T_j[p] (tabulate.cc:234-341), so
+//   nd[j, c]  = sum_k P_k(x_j) dy[c, k]     = sum_p T_j[p] H[r_j, p, c],
+//   nd[j, 0] += sum_k P'_k(x_j) sum_c em[j, c] dy[c, k]
+//             = sum_p p t^(p-1) sum_c em[j, c] H[r_j, p, c],
+//   H[r, p, c] = sum_k a_p[r, k] dy[c, k].
+// A block's neighbours are sorted by distance, so equal table rows come in
+// runs (2-3 neighbours in water, about 9 in fcc copper with davg 0 / dstd 1):
+// H is formed once per run, 24 products per (row, column) instead of 17 per
+// (neighbour, column), from one read of the row's coefficients.
+// H of two runs is one chain of v_mfma_f32_16x16x16_f16 over the columns.
+// Its A operand holds 8 rows per table row -- sigma_0 a_0 and sigma_1 a_1 as
+// FP16 hi + lo parts, sigma_p a_p (p >= 2) in FP16, as the packed FP32
+// records hold them -- and its B operand the columns dy_hi (0..3) and dy_lo
+// (4..7) of dy scaled by a power of two per atom, so each product
+// accumulates (a_hi + a_lo)(dy_hi + dy_lo) in FP32.  Runs are taken per
+// chunk of 64 neighbours (a run that crosses a chunk boundary is formed in
+// both); H reaches the neighbour's lane through LDS.
+// Run tables (build_run_tables), per net: [row][kc][kq][a][4 halves] holds
+// column k = 16 kc + 4 kq + i of A-row a, 256 bytes per (row, kc), so the
+// 32 lanes of one table row read 256 contiguous bytes per product.
+typedef _Float16 mhalf4 __attribute__((ext_vector_type(4)));
+
+constexpr int kRunWaves = 4;  // wavefronts per workgroup
+constexpr int kRunH = 24;     // floats of H per run: [p][c]
+
+// Four halves of a run table through a buffer resource (kDeadOffset: zeros).
+__device__ __forceinline__ mhalf4 load_h4(__amdgpu_buffer_rsrc_t rs, int off) {
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0);
+  return __builtin_bit_cast(mhalf4, v);
+}
+
+// Orders the LDS accesses of one wavefront's lanes (wave-private buffers).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int KC>
+__global__ void __launch_bounds__(256) k_embed_bwd_runs(const EmbedArgs a) {
+  constexpr int KP = KC * 16;                   // padded columns
+  constexpr int KQ = (KP + kWave - 1) / kWave;  // columns per lane in the dy layout
+  constexpr int GS = 17;                        // LDS row stride of dE/dD (A <= 16; odd, so lanes k miss no bank)
+  constexpr int TOFF = KP * GS;                 // T = xyz / nnei staged after dE/dD, [c][KP]
+  constexpr int BUF = TOFF + 4 * KP > kWave * kRunH ? TOFF + 4 * KP : kWave * kRunH;
+  // per wavefront: dE/dD and T, then dy, then H of the runs of a chunk
+  __shared__ __attribute__((aligned(16))) float s_buf[kRunWaves][BUF];
+  __shared__ int s_row[kRunWaves][kWave];
+  const int lane = lane_id();
+  const int wave = uni((int)(threadIdx.x / kWave));
+  const long long g = (long long)blockIdx.x * kRunWaves + wave;
+  if (g >= a.nrows) return;
+  const int nnei = a.nnei, M = a.M;
+  float* ndg = (float*)a.net_deriv + (size_t)g * nnei * 4;
+  const int ti = uni(a.row_type[g]);
+  if (ti < 0) {
+    for (int s = lane; s < nnei * 4; s += kWave) ndg[s] = 0.f;
+    return;
+  }
+  const int A = a.A, ldD = a.ldD > 0 ? a.ldD : M * A;
+  const float inv = (float)a.inv_nnei;
+  float* buf = s_buf[wave];
+  // dy = dE/dxyz from G = dE/dD (se_a.py:844-851 backward), T = xyz / nnei:
+  //   dT[c, k] = sum_a G[k, a] T[c, a] + [k < A] S[c, k],  S[c, a] = sum_m T[c, m] G[m, a],
+  //   dy = dT / nnei.
+  // G ([M][A] at row pos[g]) and T are read coalesced into LDS; S is a
+  // product on the matrix cores (v_mfma_f32_16x16x4f32, four columns m per
+  // step), dT a per-lane sum (lanes over columns k = lane + 64 kq).
+  {
+    const float* G = (const float*)a.gD + (size_t)uni(a.pos[g]) * ldD;
+    const float* xyz_g = (const float*)a.xyz + (size_t)g * 4 * M;
+    const int MA = M * A;
+    for (int e0 = 0; e0 < MA; e0 += 8 * kWave) {
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = G[min(e0 + u * kWave + lane, MA - 1)];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * kWave + lane;
+        if (e < MA) {
+          const int k = A == 16 ? e >> 4 : e / A;
+          buf[k * GS + (e - k * A)] = v[u];
+        }
+      }
+    }
+#pragma unroll
+    for (int kq = 0; kq < KQ; ++kq) {
+      const int k = lane + kq * kWave;
+      float t[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) t[c] = xyz_g[(size_t)c * M + min(k, M - 1)];
+      if (k < KP)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) buf[TOFF + c * KP + k] = k < M ? t[c] * inv : 0.f;
+    }
+  }
+  wave_lds_sync();
+  float dyo[KQ][4];
+  {
+    float ta[16][4];  // T[c, a], wave-uniform
+#pragma unroll
+    for (int a4 = 0; a4 < 16; ++a4)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) ta[a4][c] = a4 < A ? buf[TOFF + c * KP + a4] : 0.f;
+#pragma unroll
+    for (int kq = 0; kq < KQ; ++kq) {
+      const int k = lane + kq * kWave;
+      const bool kin = k < M;
+      const int kr = (kin ? k : 0) * GS;
+      float gk[16];
+#pragma unroll
+      for (int a4 = 0; a4 < 16; ++a4) gk[a4] = kin && a4 < A ? buf[kr + a4] : 0.f;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        float d = 0.f;
+#pragma unroll
+        for (int a4 = 0; a4 < 16; ++a4) d += gk[a4] * ta[a4][c];
+        dyo[kq][c] = d;
+      }
+    }
+    // S: A operand lane (i, q) = T[i, 4 s + q] (rows i >= 4 zero), B operand
+    // lane (j, q) = G[4 s + q, j]; lane j < 16 ends with S[0..3, j]
+    const int i16 = lane & 15, q4 = lane >> 4;
+    mfloat4 S = mfloat4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s0 = 0; s0 < 4 * KC; s0 += 4) {
+      float av[4], bv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int m = 4 * (s0 + u) + q4;
+        av[u] = i16 < 4 ? buf[TOFF + i16 * KP + m] : 0.f;
+        bv[u] = m < M && i16 < A ? buf[m * GS + i16] : 0.f;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) S = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv[u], S, 0, 0, 0);
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (lane < A) dyo[0][c] += S[c];
+#pragma unroll
+      for (int kq = 0; kq < KQ; ++kq) dyo[kq][c] *= inv;
+    }
+  }
+  wave_lds_sync();  // dE/dD and T are dead; the buffer holds dy next
+  // power-of-two scale that brings the largest |dy| to [2^13, 2^14)
+  float dmax = 0.f;
+#pragma unroll
+  for (int kq = 0; kq < KQ; ++kq)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) dmax = fmaxf(dmax, fabsf(dyo[kq][c]));
+  for (int off = 32; off > 0; off >>= 1) dmax = fmaxf(dmax, __shfl_xor(dmax, off, kWave));
+  dmax = uni(dmax);
+  int ex = 0;
+  frexpf(dmax, &ex);
+  const float sc = dmax > 0.f ? ldexpf(1.f, 14 - ex) : 1.f, isc = 1.f / sc;
+  // B fragments: lane (n = lane % 16, q = lane / 16) holds columns
+  // k = 16 kc + 4 q + i of dy_hi[n] (n < 4), dy_lo[n - 4] (n < 8), else zero
+#pragma unroll
+  for (int kq = 0; kq < KQ; ++kq) {
+    const int k = lane + kq * kWave;
+    if (k < KP)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) buf[c * KP + k] = dyo[kq][c] * sc;  // zero for k >= M
+  }
+  wave_lds_sync();
+  const int n = lane & 15, q = lane >> 4;
+  mhalf4 bf[KC];
+#pragma unroll
+  for (int kc = 0; kc < KC; ++kc) {
+    const float4 v4 = *reinterpret_cast<const float4*>(buf + (n & 3) * KP + kc * 16 + q * 4);
+    const float v[4] = {v4.x, v4.y, v4.z, v4.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const _Float16 hi = (_Float16)v[i];
+      const _Float16 lo = (_Float16)(v[i] - (float)hi);
+      bf[kc][i] = n < 4 ? hi : (n < 8 ? lo : (_Float16)0.f);
+    }
+  }
+  wave_lds_sync();
+  const Info<float>* infos = (const Info<float>*)a.infos;
+  const float* emg = (const float*)a.em + (size_t)g * nnei * 4;
+  int* srow = s_row[wave];
+  for (int tj = 0; tj < a.ntypes; ++tj) {
+    const int b = a.sec[tj], nb = a.sec[tj + 1] - b;
+    if (nb <= 0) continue;
+    const int net = uni(a.net_of[ti * a.ntypes + tj]);
+    const Info<float> f = infos[net];
+    // wave-uniform table base, so the buffer resource sits in scalar registers
+    const long long toff = a.rtab_off[net];
+    const long long toff_u = ((long long)uni((int)(toff >> 32)) << 32) | (unsigned int)uni((int)(toff & 0xffffffff));
+    const __amdgpu_buffer_rsrc_t rs = pk_rsrc(a.rtab + toff_u);
+    const int lane_b = q * 64 + (lane & 7) * 8;  // byte offset of the lane in a (row, kc) block
+    float psc[6];  // 1 / (sigma_p * dy scale)
+#pragma unroll
+    for (int p = 0; p < 6; ++p) psc[p] = a.rtab_pscale[8 * net + p] * isc;
+    const float* e = emg + (size_t)b * 4;
+    float* nd = ndg + (size_t)b * 4;
+    // first slot of the sorted zero-padding tail (tabulate.cc:188-201): it
+    // carries weight nb - jt, later slots are zero
+    const float last = uni(e[(size_t)(nb - 1) * 4]);
+    const int jt = uni(find_tail(e, nb, last));
+    const int jend = jt < nb ? jt + 1 : nb;
+    for (int c0 = 0; c0 < jend; c0 += kWave) {
+      const int j = c0 + lane;
+      const bool valid = j < jend;
+      const float4 ll = valid ? *reinterpret_cast<const float4*>(e + (size_t)j * 4) : make_float4(0.f, 0.f, 0.f, 0.f);
+      float t = ll.x, delta;
+      int row;
+      locate(f, t, row, delta);
+      const int prow = __shfl_up(row, 1, kWave);
+      const bool start = valid && (lane == 0 || row != prow);
+      const unsigned long long sm = __ballot(start);
+      const int R = __popcll(sm);
+      const int ridx = __popcll(sm & (~0ull >> (63 - lane))) - 1;  // this lane's run
+      if (start) srow[ridx] = row;
+      wave_lds_sync();
+      // H of runs (r0, r0 + 1) and (r0 + 2, r0 + 3): lanes with n < 8 read
+      // the first table row of a pair, n >= 8 the second (clamped runs past
+      // R repeat the last one; their H is not stored)
+      const bool second = lane & 8;
+      for (int r0 = 0; r0 < R; r0 += 4) {
+        // table rows of the two pairs, selected per lane without an indexed
+        // array (which would live in scratch memory); lanes of a run past R,
+        // or whose four columns all lie past M, read zeros through an
+        // out-of-range offset (no memory traffic)
+        const int ra0 = uni(srow[min(r0, R - 1)]), rb0 = uni(srow[min(r0 + 1, R - 1)]);
+        const int ra1 = uni(srow[min(r0 + 2, R - 1)]), rb1 = uni(srow[min(r0 + 3, R - 1)]);
+        const int off0 = (second ? rb0 : ra0) * (KC * 256) + lane_b;
+        const int off1 = (second ? rb1 : ra1) * (KC * 256) + lane_b;
+        const bool live0 = (second ? r0 + 1 : r0) < R, live1 = (second ? r0 + 3 : r0 + 2) < R;
+        mhalf4 av[2][KC];
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc) {
+          const bool col = 16 * kc + 4 * q < M;
+          av[0][kc] = load_h4(rs, live0 && col ? off0 + kc * 256 : kDeadOffset);
+          av[1][kc] = load_h4(rs, live1 && col ? off1 + kc * 256 : kDeadOffset);
+        }
+        mfloat4 h[2] = {mfloat4{0.f, 0.f, 0.f, 0.f}, mfloat4{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+          for (int pp = 0; pp < 2; ++pp) h[pp] = __builtin_amdgcn_mfma_f32_16x16x16f16(av[pp][kc], bf[kc], h[pp], 0, 0, 0);
+        // lane (n, q) holds A-rows 4 q + v of column n; add the dy_lo column
+        // n + 4 to the dy_hi column n
+#pragma unroll
+        for (int pp = 0; pp < 2; ++pp) {
+          float hv[4];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) hv[v] = h[pp][v] + __shfl_xor(h[pp][v], 4, kWave);
+          const int run = r0 + 2 * pp + (q >> 1);
+          if (n < 4 && run < R) {
+            float* hs = buf + run * kRunH;
+            if ((q & 1) == 0) {  // A-rows 0..3: a_0 hi, lo, a_1 hi, lo
+              hs[n] = (hv[0] + hv[1]) * psc[0];
+              hs[4 + n] = (hv[2] + hv[3]) * psc[1];
+            } else {  // A-rows 4..7: a_2 .. a_5
+#pragma unroll
+              for (int v = 0; v < 4; ++v) hs[4 * (2 + v) + n] = hv[v] * psc[2 + v];
+            }
+          }
+        }
+      }
+      wave_lds_sync();
+      if (valid) {
+        const float* hs = buf + ridx * kRunH;
+        const float t2 = t * t, t3 = t2 * t, t4 = t3 * t;
+        const float Td[6] = {0.f, 1.f, 2.f * t, 3.f * t2, 4.f * t3, 5.f * t4};
+        const float Tv[6] = {1.f, t + delta, t2 + delta * Td[2], t3 + delta * Td[3], t4 + delta * Td[4],
+                             t4 * t + delta * Td[5]};
+        float o[4] = {0.f, 0.f, 0.f, 0.f}, sl = 0.f;
+#pragma unroll
+        for (int p = 0; p < 6; ++p) {
+          const float4 H = *reinterpret_cast<const float4*>(hs + 4 * p);
+          o[0] += Tv[p] * H.x;
+          o[1] += Tv[p] * H.y;
+          o[2] += Tv[p] * H.z;
+          o[3] += Tv[p] * H.w;
+          if (p > 0) sl += Td[p] * (ll.x * H.x + ll.y * H.y + ll.z * H.z + ll.w * H.w);
+        }
+        const float scale = j == jt ? (float)(nb - jt) : 1.f;
+        *reinterpret_cast<float4*>(nd + (size_t)j * 4) =
+            make_float4((o[0] + sl) * scale, o[1] * scale, o[2] * scale, o[3] * scale);
+      }
+      wave_lds_sync();  // H and the run rows are rewritten by the next chunk
+    }
+    for (int jj = jend + lane; jj < nb; jj += kWave)
+      *reinterpret_cast<float4*>(nd + (size_t)jj * 4) = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Uncompressed embedding of the FP64 engine (se_a.py:785-839 without
 // compression).  The embedding net runs as row-wise layer products
 // (fit64.hip) over rows laid out net by net within a chunk of descriptor
@@ -1421,6 +1721,77 @@ void launch_embed_bwd_mfma(const EmbedArgs& a, hipStream_t st) {
     case 2: hipLaunchKernelGGL(k_embed_bwd_mfma<2>, grid, block, 0, st, a); break;
     case 3: hipLaunchKernelGGL(k_embed_bwd_mfma<3>, grid, block, 0, st, a); break;
     default: hipLaunchKernelGGL(k_embed_bwd_mfma<4>, grid, block, 0, st, a); break;
+  }
+  DPMI_LAUNCH_CHECK();
+}
+
+void check_table_rows(const double* infos, const int* nrows, int nnets) {
+  for (int n = 0; n < nnets; ++n) {
+    float fi[6];
+    for (int q = 0; q < 6; ++q) fi[q] = (float)infos[n * 6 + q];
+    const Info<double> fd = make_info(infos + n * 6);
+    const Info<float> ff = make_info(fi);
+    if (fd.n0 < 0 || ff.n0 < 0 || fd.ntail >= nrows[n] || ff.ntail >= nrows[n])
+      throw std::invalid_argument("table_info of net " + std::to_string(n) + " addresses rows past its " +
+                                  std::to_string(nrows[n]) + " table rows");
+  }
+}
+
+bool embed_runs_supported(int M, int A) { return M > 0 && M <= 128 && A > 0 && A <= 16; }
+
+void build_run_tables(const double* table, const long long* tab_off, const int* nrows, int nnets, int M,
+                      std::vector<_Float16>& out, std::vector<long long>& off, std::vector<float>& pscale) {
+  const int KC = (M + 15) / 16;
+  off.assign(nnets, 0);
+  long long tot = 0;
+  for (int n = 0; n < nnets; ++n) {
+    off[n] = tot;
+    tot += (long long)nrows[n] * KC * 128;
+  }
+  out.assign((size_t)tot, (_Float16)0.f);  // columns k >= M stay zero
+  pscale.assign(8 * (size_t)nnets, 1.f);
+  for (int n = 0; n < nnets; ++n) {
+    const double* T = table + tab_off[n];
+    // scale per power: 2^e with 2^13 <= sigma_p max|a_p| < 2^14
+    double sig[6];
+    for (int p = 0; p < 6; ++p) {
+      double mx = 0.0;
+      for (size_t q = 0; q < (size_t)nrows[n] * M; ++q) mx = std::max(mx, std::fabs(T[q * 6 + p]));
+      int ex = 0;
+      if (mx > 0.0) std::frexp(mx, &ex);
+      sig[p] = mx > 0.0 ? std::ldexp(1.0, 14 - ex) : 1.0;
+      pscale[8 * n + p] = (float)(1.0 / sig[p]);
+    }
+    for (int r = 0; r < nrows[n]; ++r)
+      for (int k = 0; k < M; ++k) {
+        // element (row r, column k, A-row a) at base + 4 a
+        const size_t base = (size_t)off[n] + (((size_t)r * KC + k / 16) * 4 + (k % 16) / 4) * 32 + k % 4;
+        const double* c = T + ((size_t)r * M + k) * 6;
+        for (int p = 0; p < 2; ++p) {
+          const double v = c[p] * sig[p];
+          const _Float16 hi = (_Float16)v;
+          out[base + 4 * (2 * p)] = hi;
+          out[base + 4 * (2 * p + 1)] = (_Float16)(v - (double)hi);
+        }
+        for (int p = 2; p < 6; ++p) out[base + 4 * (p + 2)] = (_Float16)(c[p] * sig[p]);
+      }
+  }
+}
+
+void launch_embed_bwd_runs(const EmbedArgs& a, hipStream_t st) {
+  if (a.nrows <= 0) return;
+  if (!embed_runs_supported(a.M, a.A) || !a.rtab)
+    throw std::invalid_argument("row-run adjoint needs run tables, M <= 128 and A <= 16");
+  const dim3 grid(ceil_div(a.nrows, kRunWaves)), block(kRunWaves * kWave);
+  switch ((a.M + 15) / 16) {
+    case 1: hipLaunchKernelGGL(k_embed_bwd_runs<1>, grid, block, 0, st, a); break;
+    case 2: hipLaunchKernelGGL(k_embed_bwd_runs<2>, grid, block, 0, st, a); break;
+    case 3: hipLaunchKernelGGL(k_embed_bwd_runs<3>, grid, block, 0, st, a); break;
+    case 4: hipLaunchKernelGGL(k_embed_bwd_runs<4>, grid, block, 0, st, a); break;
+    case 5: hipLaunchKernelGGL(k_embed_bwd_runs<5>, grid, block, 0, st, a); break;
+    case 6: hipLaunchKernelGGL(k_embed_bwd_runs<6>, grid, block, 0, st, a); break;
+    case 7: hipLaunchKernelGGL(k_embed_bwd_runs<7>, grid, block, 0, st, a); break;
+    default: hipLaunchKernelGGL(k_embed_bwd_runs<8>, grid, block, 0, st, a); break;
   }
   DPMI_LAUNCH_CHECK();
 }

@@ -152,6 +152,7 @@ SeaModel::SeaModel(const dpmi_sea_model_desc& d) {
     for (int n = 0; n < nnets; ++n)
       std::memcpy(table_host.data() + tab_off[n], d.tables[n], sizeof(double) * (size_t)d.nspline[n] * 6 * M);
     info_host.assign(d.table_info, d.table_info + nnets * 6);
+    check_table_rows(info_host.data(), d.nspline, nnets);
   } else {
     // uncompressed: the embedding nets themselves (FP64 MFMA operands)
     enl = d.embed_nlayers;
@@ -300,6 +301,16 @@ void SeaModel::build_mixed(const dpmi_sea_model_desc& d) {
     d_tab16_pscale = dupload(psc.data(), psc.size());
     embed_mfma = true;
   }
+  if (embed_runs_supported(M, A)) {
+    std::vector<_Float16> rt;
+    std::vector<long long> roff;
+    std::vector<float> psc;
+    build_run_tables(table_host.data(), tab_off.data(), nspline.data(), nnets, M, rt, roff, psc);
+    d_rtab = dupload(rt.data(), rt.size());
+    d_rtab_off = dupload(roff.data(), roff.size());
+    d_rtab_pscale = dupload(psc.data(), psc.size());
+    embed_runs = true;
+  }
   std::vector<double> shift(ntypes);
   for (int t = 0; t < ntypes; ++t) shift[t] = out_b_host[t] + bias_atom_e[t];
   d_shift = dupload(shift.data(), shift.size());
@@ -401,6 +412,9 @@ SeaModel::~SeaModel() {
   dfree(d_tab16);
   dfree(d_tab16_off);
   dfree(d_tab16_pscale);
+  dfree(d_rtab);
+  dfree(d_rtab_off);
+  dfree(d_rtab_pscale);
   for (auto& p : f16_mem) dfree(p);
   for (auto* v : {&ew, &ewt, &eb, &eidt})
     for (auto& p : *v) dfree(p);
@@ -698,6 +712,9 @@ EmbedArgs SeaEngine::embed_args() const {
   ta.tab16 = m.d_tab16;
   ta.tab16_off = m.d_tab16_off;
   ta.tab16_pscale = m.d_tab16_pscale;
+  ta.rtab = m.d_rtab;
+  ta.rtab_off = m.d_rtab_off;
+  ta.rtab_pscale = m.d_rtab_pscale;
   ta.etiles = d_etiles;
   ta.netiles = netiles;
   ta.flag = d_flag;
@@ -1064,6 +1081,15 @@ void SeaEngine::compute_mixed_descriptor_fitting(hipStream_t st, int& stage) {
   ta.packed = m.d_table_pk;
   ta.packed_off = m.d_tab_off_pk;
   const bool mfma_fwd = m.embed_mfma && (tune.embed_mfma & 1), mfma_bwd = m.embed_mfma && (tune.embed_mfma & 2);
+  const bool runs_bwd = m.embed_runs && (tune.embed_mfma & 4);
+  auto adjoint = [&] {
+    if (runs_bwd)
+      launch_embed_bwd_runs(ta, st);
+    else if (mfma_bwd)
+      launch_embed_bwd_mfma(ta, st);
+    else
+      launch_embed_bwd<float, float>(ta, st);
+  };
   if (mfma_fwd)
     launch_embed_fwd_mfma(ta, st);
   else
@@ -1088,10 +1114,7 @@ void SeaEngine::compute_mixed_descriptor_fitting(hipStream_t st, int& stage) {
     if (tstart[m.ntypes + 1] > tstart[m.ntypes]) launch_fill_double(d_atom_e, 0.0, ntot, st);
     launch_fit_fused(fa, rows == 32 ? ntiles32 : ntiles, rows, st);
     mark(st, stage++);
-    if (mfma_bwd)
-      launch_embed_bwd_mfma(ta, st);
-    else
-      launch_embed_bwd<float, float>(ta, st);
+    adjoint();
     mark(st, stage++);
     return;
   }
@@ -1136,10 +1159,7 @@ void SeaEngine::compute_mixed_descriptor_fitting(hipStream_t st, int& stage) {
   }
   mark(st, stage++);
 
-  if (mfma_bwd)
-    launch_embed_bwd_mfma(ta, st);
-  else
-    launch_embed_bwd<float, float>(ta, st);
+  adjoint();
   mark(st, stage++);
 }
 
@@ -1454,11 +1474,13 @@ void SeaEngine::autotune(const double* coord, const double* box_h, int reps, hip
   const int env_c[3] = {128, 256, 512}, fwd_c[3] = {1, 2, 4}, bwd_c[3] = {1, 2, 4}, frc_c[3] = {4, 8, 16},
             fit_c[3] = {32, 64, 64};
   // candidates 0..2: per-neighbour embedding kernels with the shapes above;
-  // candidate 3 (mixed engine with matrix-core tables): the table-space
-  // embedding kernels, whose launch shape is fixed
-  const bool try_mfma = precision == DPMI_MIXED && model->embed_mfma;
-  const int ncand = try_mfma ? 4 : 3;
-  double t_env[4], t_fwd[4], t_bwd[4], t_frc[4], t_fit[4];
+  // then, in the mixed engine, the table-space embedding kernels (mask 3)
+  // and the row-run adjoint (mask 4), whose launch shapes are fixed
+  int masks[5] = {0, 0, 0, 0, 0};
+  int ncand = 3;
+  if (precision == DPMI_MIXED && model->embed_mfma) masks[ncand++] = 3;
+  if (precision == DPMI_MIXED && model->embed_runs) masks[ncand++] = 4;
+  double t_env[5], t_fwd[5], t_bwd[5], t_frc[5], t_fit[5];
   double *e = nullptr, *f = nullptr, *v = nullptr;
   DPMI_CHECK(hipMalloc(&e, sizeof(double) * nframes));
   DPMI_CHECK(hipMalloc(&f, sizeof(double) * 3 * (size_t)ntot));
@@ -1486,7 +1508,7 @@ void SeaEngine::autotune(const double* coord, const double* box_h, int reps, hip
         tune.bwd_waves = bwd_c[k];
         tune.force_rows = frc_c[k];
         tune.fit_rows = fit_c[k];
-        tune.embed_mfma = c < 3 ? 0 : 3;
+        tune.embed_mfma = masks[c];
         timing = true;
         reset_timing();
         compute(coord, box_h, e, f, v, nullptr, nullptr, st);
@@ -1522,10 +1544,19 @@ void SeaEngine::autotune(const double* coord, const double* box_h, int reps, hip
   tune.force_rows = frc_c[best(t_frc)];
   tune.fit_rows = idx("fitting") >= 0 ? fit_c[best(t_fit)] : 0;
   tune.embed_mfma = 0;
-  if (try_mfma) {
-    if (t_fwd[3] < t_fwd[best(t_fwd)]) tune.embed_mfma |= 1;
-    if (t_bwd[3] < t_bwd[best(t_bwd)]) tune.embed_mfma |= 2;
+  double fbest = t_fwd[best(t_fwd)], bbest = t_bwd[best(t_bwd)];
+  int bmask = 0;
+  for (int c = 3; c < ncand; ++c) {
+    if ((masks[c] & 1) && t_fwd[c] < fbest) {
+      fbest = t_fwd[c];
+      tune.embed_mfma |= 1;
+    }
+    if ((masks[c] & 6) && t_bwd[c] < bbest) {
+      bbest = t_bwd[c];
+      bmask = masks[c] & 6;
+    }
   }
+  tune.embed_mfma |= bmask;
   std::lock_guard<std::mutex> lk(g_tune_mu);
   tune_cache()[tune_key(precision, ntot)] = tune;
 }

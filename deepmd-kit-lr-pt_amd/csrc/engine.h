@@ -56,6 +56,11 @@ struct SeaModel {
   _Float16* d_tab16 = nullptr;
   long long* d_tab16_off = nullptr;
   float* d_tab16_pscale = nullptr;
+  // row-run adjoint on the matrix cores: run tables of every net
+  bool embed_runs = false;
+  _Float16* d_rtab = nullptr;
+  long long* d_rtab_off = nullptr;
+  float* d_rtab_pscale = nullptr;
   std::vector<std::string> type_map;  // names of the types (model files)
   // uncompressed embedding (FP64 engines): per (net, layer) the weights as
   // FP64 MFMA B fragments (W and W^T), bias and timestep

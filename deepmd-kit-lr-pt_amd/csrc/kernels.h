@@ -185,6 +185,10 @@ struct EmbedArgs {
   const int4* etiles = nullptr;          // [netiles] (type or -1, first position, count <= 4, 0)
   int netiles = 0;
   int* flag = nullptr;                   // device status word (kFlagHalfRange)
+  // row-run adjoint of the mixed engine (matrix cores, one wavefront per row)
+  const _Float16* rtab = nullptr;        // run tables of every net (build_run_tables)
+  const long long* rtab_off = nullptr;   // [nnets] element offsets
+  const float* rtab_pscale = nullptr;    // [8 nnets]: 1 / sigma_p
 };
 // Uncompressed embedding (FP64 engine): slots <-> net rows, contraction and
 // adjoint, one wavefront per descriptor position q in [q0, q1) (atom perm[q]);
@@ -241,6 +245,19 @@ void build_split_tables(const double* table, const long long* tab_off, const int
                         std::vector<float>& pscale);
 void launch_embed_fwd_mfma(const EmbedArgs& a, hipStream_t st);
 void launch_embed_bwd_mfma(const EmbedArgs& a, hipStream_t st);
+// Mixed engine, matrix cores: adjoint (net_deriv) from the table rows of each
+// atom's neighbour runs, H[r, p, c] = sum_k a_p[r, k] dy[c, k] once per run.
+// Run tables: per net [row][ceil(M/16)][4][8 A-rows][4 halves] (column
+// 16 kc + 4 kq + i), A-rows sigma_0 a_0 hi, lo, sigma_1 a_1 hi, lo,
+// sigma_p a_p (p = 2..5), sigma_p a power of two per net and p (1 / sigma_p
+// in pscale[8 net + p]).
+bool embed_runs_supported(int M, int A);
+// throws std::invalid_argument when a net's table_info (FP64 or rounded to
+// FP32) locates rows at or past its nrows
+void check_table_rows(const double* infos, const int* nrows, int nnets);
+void build_run_tables(const double* table, const long long* tab_off, const int* nrows, int nnets, int M,
+                      std::vector<_Float16>& out, std::vector<long long>& off, std::vector<float>& pscale);
+void launch_embed_bwd_runs(const EmbedArgs& a, hipStream_t st);
 
 // ------------------------------------------------------------- force.hip --
 template <typename T>
@@ -318,8 +335,10 @@ struct Tune {
   int bwd_waves = 1;
   int force_rows = 16;
   // mixed engine: bit 0 forward, bit 1 adjoint of the table embedding on the
-  // matrix cores (table space); pays off when many neighbours of an atom
-  // share table rows, so it is off until the autotuner measures it faster
+  // matrix cores (table space, four atoms per wavefront); pays off when many
+  // neighbours of an atom share table rows, so it is off until the autotuner
+  // measures it faster; bit 2 the row-run adjoint on the matrix cores (one
+  // atom per wavefront; takes precedence over bit 1)
   int embed_mfma = 0;
   int fit_rows = 0;  // fused fitting rows per workgroup tile (32 or 64; 0: chosen from the size)
 };
