@@ -1142,16 +1142,39 @@ __global__ void __launch_bounds__(256) k_embed_bwd_runs(const EmbedArgs a) {
     const float* G = (const float*)a.gD + (size_t)uni(a.pos[g]) * ldD;
     const float* xyz_g = (const float*)a.xyz + (size_t)g * 4 * M;
     const int MA = M * A;
-    for (int e0 = 0; e0 < MA; e0 += 8 * kWave) {
-      float v[8];
+    if (A == 16 && (ldD & 3) == 0) {
+      // four consecutive values of one row k per lane: LDS index e + e / 16
+      for (int e0 = 0; e0 < MA; e0 += 4 * 4 * kWave) {
+        float4 v[4];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = G[min(e0 + u * kWave + lane, MA - 1)];
+        for (int u = 0; u < 4; ++u) {
+          const int e = e0 + 4 * (u * kWave + lane);
+          v[u] = *reinterpret_cast<const float4*>(G + min(e, MA - 4));
+        }
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int e = e0 + u * kWave + lane;
-        if (e < MA) {
-          const int k = A == 16 ? e >> 4 : e / A;
-          buf[k * GS + (e - k * A)] = v[u];
+        for (int u = 0; u < 4; ++u) {
+          const int e = e0 + 4 * (u * kWave + lane);
+          if (e < MA) {
+            float* d = buf + e + (e >> 4);
+            d[0] = v[u].x;
+            d[1] = v[u].y;
+            d[2] = v[u].z;
+            d[3] = v[u].w;
+          }
+        }
+      }
+    } else {
+      for (int e0 = 0; e0 < MA; e0 += 8 * kWave) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = G[min(e0 + u * kWave + lane, MA - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int e = e0 + u * kWave + lane;
+          if (e < MA) {
+            const int k = e / A;
+            buf[k * GS + (e - k * A)] = v[u];
+          }
         }
       }
     }
@@ -1225,27 +1248,32 @@ __global__ void __launch_bounds__(256) k_embed_bwd_runs(const EmbedArgs a) {
   int ex = 0;
   frexpf(dmax, &ex);
   const float sc = dmax > 0.f ? ldexpf(1.f, 14 - ex) : 1.f, isc = 1.f / sc;
-  // B fragments: lane (n = lane % 16, q = lane / 16) holds columns
-  // k = 16 kc + 4 q + i of dy_hi[n] (n < 4), dy_lo[n - 4] (n < 8), else zero
+  // scaled dy split once per element into FP16 hi and lo parts, staged as
+  // halves [hi / lo][c][KP] (zero for k >= M); B fragments: lane
+  // (n = lane % 16, q = lane / 16) holds columns k = 16 kc + 4 q + i of
+  // dy_hi[n] (n < 4), dy_lo[n - 4] (n < 8), else zero
+  _Float16* hbuf = reinterpret_cast<_Float16*>(buf);
 #pragma unroll
   for (int kq = 0; kq < KQ; ++kq) {
     const int k = lane + kq * kWave;
     if (k < KP)
 #pragma unroll
-      for (int c = 0; c < 4; ++c) buf[c * KP + k] = dyo[kq][c] * sc;  // zero for k >= M
+      for (int c = 0; c < 4; ++c) {
+        const float v = dyo[kq][c] * sc;
+        const _Float16 hi = (_Float16)v;
+        hbuf[c * KP + k] = hi;
+        hbuf[(4 + c) * KP + k] = (_Float16)(v - (float)hi);
+      }
   }
   wave_lds_sync();
   const int n = lane & 15, q = lane >> 4;
   mhalf4 bf[KC];
+  {
+    const _Float16* src = hbuf + (n & 7) * KP + q * 4;
 #pragma unroll
-  for (int kc = 0; kc < KC; ++kc) {
-    const float4 v4 = *reinterpret_cast<const float4*>(buf + (n & 3) * KP + kc * 16 + q * 4);
-    const float v[4] = {v4.x, v4.y, v4.z, v4.w};
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const _Float16 hi = (_Float16)v[i];
-      const _Float16 lo = (_Float16)(v[i] - (float)hi);
-      bf[kc][i] = n < 4 ? hi : (n < 8 ? lo : (_Float16)0.f);
+    for (int kc = 0; kc < KC; ++kc) {
+      const mhalf4 v = *reinterpret_cast<const mhalf4*>(src + kc * 16);
+      bf[kc] = n < 8 ? v : mhalf4{(_Float16)0.f, (_Float16)0.f, (_Float16)0.f, (_Float16)0.f};
     }
   }
   wave_lds_sync();
