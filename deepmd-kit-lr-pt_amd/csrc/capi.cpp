@@ -621,7 +621,7 @@ int dpmi_sea_engine_set_tuning(dpmi_sea_engine* engine, const int* knobs_host) {
       t.force_rows = k[3];
     }
     if (k[4] >= 0) {
-      if (k[4] > 7) throw std::invalid_argument("matrix-core embedding mask: 0 to 7");
+      if (k[4] > 15) throw std::invalid_argument("matrix-core embedding mask: 0 to 15");
       t.embed_mfma = k[4];
     }
     if (k[5] > 0) {

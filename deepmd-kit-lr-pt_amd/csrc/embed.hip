@@ -1389,6 +1389,222 @@ __global__ void __launch_bounds__(256) k_embed_bwd_runs(const EmbedArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Forward embedding of the mixed engine from each atom's table-row runs, on
+// the matrix cores (one wavefront per centre row).
+//
+// With the row's stride s, t' = t / s, delta' = delta / s and a'_p = a_p s^p,
+// the weighted contraction of a block (tabulate.cc:162-232, the sorted
+// padding tail weighted nb - jt) is a sum over the runs of equal rows:
+//   xyz[c, k] = sum_j w_j em[j, c] P_k(x_j) = sum_runs sum_p Q[run, p, c] a'_p[r, k],
+//   Q[run, p, c] = sum_{j in run} w_j em[j, c] T'_j[p],  T'_j[p] = t'^p + p delta' t'^(p-1).
+// Per chunk of 64 neighbours Q of each run is summed in LDS, scaled by a power
+// of two (largest |Q| in [2^13, 2^14)) and split into FP16 hi + lo parts; two
+// runs make one v_mfma_f32_16x16x16_f16 per block of 16 columns: A rows
+// (c, hi) and (c, lo), K = 2 runs x 8 A-rows of the forward run table
+// (sigma a'_0 hi, lo, sigma a'_1 hi, lo, sigma a'_2 .. sigma a'_5 in FP16,
+// sigma a power of two per net), B columns k.  Each chunk's products are
+// unscaled into FP32 sums; T and the descriptor follow as in k_embed_fwd.
+// Forward run tables (build_fwd_run_tables), per net: [row][kb][quad][j][4
+// halves] holds A-rows 4 quad + i of column k = 16 kb + j, 256 bytes per
+// (row, kb).
+template <int KC>
+__global__ void __launch_bounds__(256) k_embed_fwd_runs(const EmbedArgs a) {
+  constexpr int KP = KC * 16;                   // padded columns
+  constexpr int KQ = (KP + kWave - 1) / kWave;  // columns per lane in the output layout
+  // Q of a chunk as [c][8][run] (p -> slot 0, 1, 4, 5, 6, 7): the lanes of
+  // different runs touch consecutive words (no bank conflicts)
+  constexpr int QN = 32 * kWave;
+  __shared__ __attribute__((aligned(16))) float s_q[kRunWaves][QN];
+  __shared__ __attribute__((aligned(16))) float s_x[kRunWaves][4 * KP];  // xyz sums [c][KP]
+  __shared__ int s_row[kRunWaves][kWave];
+  const int lane = lane_id();
+  const int wave = uni((int)(threadIdx.x / kWave));
+  const long long g = (long long)blockIdx.x * kRunWaves + wave;
+  if (g >= a.nrows) return;
+  const int nnei = a.nnei, M = a.M, A = a.A;
+  const int ti = uni(a.row_type[g]);
+  float* sq = s_q[wave];
+  float* sx = s_x[wave];
+  int* srow = s_row[wave];
+  const int j16 = lane & 15, q = lane >> 4;
+  for (int i = lane; i < 4 * KP; i += kWave) sx[i] = 0.f;
+  if (ti >= 0) {
+    const Info<float>* infos = (const Info<float>*)a.infos;
+    const float* emg = (const float*)a.em + (size_t)g * nnei * 4;
+    // A-operand lane roles: rows i = j16 < 8 hold (c = i % 4, hi / lo = i / 4);
+    // K rows 4 q .. 4 q + 3 = run 2 m + q / 2, A-rows 4 (q % 2) .. + 3
+    const int cc = j16 & 3;
+    const bool lo_row = j16 & 4, zero_row = j16 >= 8, odd = q & 1;
+    for (int tj = 0; tj < a.ntypes; ++tj) {
+      const int b = a.sec[tj], nb = a.sec[tj + 1] - b;
+      if (nb <= 0) continue;
+      const int net = uni(a.net_of[ti * a.ntypes + tj]);
+      const Info<float> f = infos[net];
+      const long long toff = a.ftab_off[net];
+      const long long toff_u = ((long long)uni((int)(toff >> 32)) << 32) | (unsigned int)uni((int)(toff & 0xffffffff));
+      const __amdgpu_buffer_rsrc_t rs = pk_rsrc(a.ftab + toff_u);
+      const float inv_sigma = a.ftab_scale[net];
+      const float is0 = 1.f / f.s0, is1 = 1.f / f.s1;
+      const float* e = emg + (size_t)b * 4;
+      const float last = uni(e[(size_t)(nb - 1) * 4]);
+      const int jt = uni(find_tail(e, nb, last));
+      const int jend = jt < nb ? jt + 1 : nb;
+      for (int c0 = 0; c0 < jend; c0 += kWave) {
+        const int j = c0 + lane;
+        const bool valid = j < jend;
+        const float4 ll = valid ? *reinterpret_cast<const float4*>(e + (size_t)j * 4) : make_float4(0.f, 0.f, 0.f, 0.f);
+        float t = ll.x, delta;
+        int row;
+        locate(f, t, row, delta);
+        const int prow = __shfl_up(row, 1, kWave);
+        const bool start = valid && (lane == 0 || row != prow);
+        const unsigned long long sm = __ballot(start);
+        const int R = __popcll(sm);
+        const int ridx = __popcll(sm & (~0ull >> (63 - lane))) - 1;
+        if (start) srow[ridx] = row;
+        // this lane's terms w em[c] T'[p]; a run is a contiguous range of
+        // lanes, summed by a segmented inclusive scan (step d adds the value
+        // d lanes down when that lane is in the same run) whose last lane
+        // holds the run's Q -- deterministic, no atomics
+        float v[4][6];
+        {
+          const float is = row < f.n0 ? is0 : is1;
+          float T[6];
+          powers(t * is, delta * is, T);
+          const float w = valid ? (j == jt ? (float)(nb - jt) : 1.f) : 0.f;
+          const float l4[4] = {ll.x * w, ll.y * w, ll.z * w, ll.w * w};
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int p = 0; p < 6; ++p) v[c][p] = l4[c] * T[p];
+        }
+        const int pos = lane - (63 - __builtin_clzll(sm & (~0ull >> (63 - lane))));  // place in the run
+        int lmax = valid ? pos + 1 : 0;
+        for (int off = 32; off > 0; off >>= 1) lmax = max(lmax, __shfl_xor(lmax, off, kWave));
+        lmax = uni(lmax);
+        for (int d = 1; d < lmax; d <<= 1) {
+          const bool take = pos >= d;
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int p = 0; p < 6; ++p) {
+              const float u = __shfl_up(v[c][p], d, kWave);
+              if (take) v[c][p] += u;
+            }
+        }
+        const bool is_end = valid && (lane == kWave - 1 || j + 1 == jend || ((sm >> (lane + 1)) & 1ull));
+        float qm = 0.f;
+        if (is_end) {
+          float* dst = sq + ridx;
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int p = 0; p < 6; ++p) {
+              dst[(c * 8 + (p < 2 ? p : p + 2)) * kWave] = v[c][p];
+              qm = fmaxf(qm, fabsf(v[c][p]));
+            }
+        }
+        wave_lds_sync();
+        for (int off = 32; off > 0; off >>= 1) qm = fmaxf(qm, __shfl_xor(qm, off, kWave));
+        qm = uni(qm);
+        int ex = 0;
+        frexpf(qm, &ex);
+        const float qsc = qm > 0.f ? ldexpf(1.f, 14 - ex) : 1.f;
+        mfloat4 acc[KC];
+#pragma unroll
+        for (int kb = 0; kb < KC; ++kb) acc[kb] = mfloat4{0.f, 0.f, 0.f, 0.f};
+        for (int m2 = 0; m2 < R; m2 += 2) {
+          const int run = m2 + (q >> 1);
+          const bool rl = run < R;
+          // slots 4 odd + i of Q[c][.][run]: p = 0, 0, 1, 1 (even quad) or 2 .. 5
+          const float* qs = sq + (cc * 8 + (odd ? 4 : 0)) * kWave + min(run, R - 1);
+          const float x0 = qs[0], x1 = qs[kWave], x2 = qs[2 * kWave], x3 = qs[3 * kWave];
+          const float v[4] = {x0, odd ? x1 : x0, odd ? x2 : x1, odd ? x3 : x1};
+          mhalf4 av;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const float s = v[i] * qsc;
+            const _Float16 hi = (_Float16)s;
+            const _Float16 lo = (_Float16)(s - (float)hi);
+            av[i] = zero_row || !rl ? (_Float16)0.f : (lo_row ? lo : hi);
+          }
+          const int r0 = uni(srow[m2]), r1 = uni(srow[min(m2 + 1, R - 1)]);
+          const int base = ((q >> 1) ? r1 : r0) * (KC * 256) + odd * 128 + j16 * 8;
+          mhalf4 bv[KC];
+#pragma unroll
+          for (int kb = 0; kb < KC; ++kb) bv[kb] = load_h4(rs, rl && 16 * kb + j16 < M ? base + kb * 256 : kDeadOffset);
+#pragma unroll
+          for (int kb = 0; kb < KC; ++kb) acc[kb] = __builtin_amdgcn_mfma_f32_16x16x16f16(av, bv[kb], acc[kb], 0, 0, 0);
+        }
+        // rows (c, hi) sit in lanes 0..15, (c, lo) in lanes 16..31; the
+        // chunk's unscaled sums are added to xyz in LDS by lanes 0..15
+        const float us = inv_sigma / qsc;
+#pragma unroll
+        for (int kb = 0; kb < KC; ++kb)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const float sum = (acc[kb][c] + __shfl_xor(acc[kb][c], 16, kWave)) * us;
+            if (lane < 16) sx[c * KP + 16 * kb + lane] += sum;
+          }
+        wave_lds_sync();  // Q and the run rows are rewritten by the next chunk
+      }
+    }
+  }
+  // xyz from LDS into the lanes-over-columns layout
+  wave_lds_sync();
+  float X[KQ][4];
+#pragma unroll
+  for (int kq = 0; kq < KQ; ++kq) {
+    const int k = lane + kq * kWave;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) X[kq][c] = k < M ? sx[c * KP + k] : 0.f;
+  }
+  float* xyz = (float*)a.xyz;
+#pragma unroll
+  for (int kq = 0; kq < KQ; ++kq) {
+    const int k = lane + kq * kWave;
+    if (k < M)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) xyz[((size_t)g * 4 + c) * M + k] = X[kq][c];
+  }
+  // descriptor D[pos[g], k, a] = sum_c T[c, k] T[c, a], T = xyz / nnei (a < A in lanes 0..A-1)
+  const float inv = (float)a.inv_nnei;
+  const int ldD = a.ldD > 0 ? a.ldD : M * A;
+  _Float16* out = (_Float16*)a.D + (size_t)uni(a.pos[g]) * ldD;
+  float ta[16][4];
+#pragma unroll
+  for (int q2 = 0; q2 < 16; ++q2)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) ta[q2][c] = q2 < A ? bcast(X[0][c], q2) * inv : 0.f;
+#pragma unroll
+  for (int kq = 0; kq < KQ; ++kq) {
+    const int k = lane + kq * kWave;
+    if (k >= M) continue;
+    float v[16];
+#pragma unroll
+    for (int q2 = 0; q2 < 16; ++q2) {
+      v[q2] = 0.f;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) v[q2] += (X[kq][c] * inv) * ta[q2][c];
+    }
+    if (A == 16 && (ldD & 7) == 0) {
+      mhalf8 h0, h1;
+#pragma unroll
+      for (int q2 = 0; q2 < 8; ++q2) {
+        h0[q2] = (_Float16)v[q2];
+        h1[q2] = (_Float16)v[q2 + 8];
+      }
+      *reinterpret_cast<mhalf8*>(out + (size_t)k * 16) = h0;
+      *reinterpret_cast<mhalf8*>(out + (size_t)k * 16 + 8) = h1;
+    } else {
+#pragma unroll
+      for (int q2 = 0; q2 < 16; ++q2)
+        if (q2 < A) out[(size_t)k * A + q2] = (_Float16)v[q2];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Uncompressed embedding of the FP64 engine (se_a.py:785-839 without
 // compression).  The embedding net runs as row-wise layer products
 // (fit64.hip) over rows laid out net by net within a chunk of descriptor
@@ -1804,6 +2020,78 @@ void build_run_tables(const double* table, const long long* tab_off, const int* 
         for (int p = 2; p < 6; ++p) out[base + 4 * (p + 2)] = (_Float16)(c[p] * sig[p]);
       }
   }
+}
+
+void build_fwd_run_tables(const double* table, const long long* tab_off, const int* nrows, const double* infos,
+                          int nnets, int M, std::vector<_Float16>& out, std::vector<long long>& off,
+                          std::vector<float>& scale) {
+  const int KC = (M + 15) / 16;
+  off.assign(nnets, 0);
+  long long tot = 0;
+  for (int n = 0; n < nnets; ++n) {
+    off[n] = tot;
+    tot += (long long)nrows[n] * KC * 128;
+  }
+  out.assign((size_t)tot, (_Float16)0.f);  // columns k >= M stay zero
+  scale.assign(nnets, 1.f);
+  for (int n = 0; n < nnets; ++n) {
+    float fi[6];
+    for (int q = 0; q < 6; ++q) fi[q] = (float)infos[n * 6 + q];
+    const Info<float> f = make_info(fi);  // the kernels' segment rule
+    const double* T = table + tab_off[n];
+    auto stride = [&](int r) { return r < f.n0 ? (double)f.s0 : (double)f.s1; };
+    // one power of two per net: 2^13 <= sigma max|a_p s^p| < 2^14
+    double mx = 0.0;
+    for (int r = 0; r < nrows[n]; ++r) {
+      const double s = stride(r);
+      for (int k = 0; k < M; ++k) {
+        double sp = 1.0;
+        for (int p = 0; p < 6; ++p, sp *= s) mx = std::max(mx, std::fabs(T[((size_t)r * M + k) * 6 + p] * sp));
+      }
+    }
+    int ex = 0;
+    if (mx > 0.0) std::frexp(mx, &ex);
+    const double sig = mx > 0.0 ? std::ldexp(1.0, 14 - ex) : 1.0;
+    scale[n] = (float)(1.0 / sig);
+    for (int r = 0; r < nrows[n]; ++r) {
+      const double s = stride(r);
+      for (int k = 0; k < M; ++k) {
+        // A-row u of column k at ((r KC + k / 16) 2 + u / 4) 64 + (k % 16) 4 + u % 4
+        const size_t base = ((size_t)r * KC + k / 16) * 128 + (k % 16) * 4;
+        auto put = [&](int u, _Float16 v) { out[(size_t)off[n] + base + (u / 4) * 64 + u % 4] = v; };
+        const double* c = T + ((size_t)r * M + k) * 6;
+        double sp = sig;
+        for (int p = 0; p < 6; ++p, sp *= s) {
+          const double v = c[p] * sp;
+          const _Float16 hi = (_Float16)v;
+          if (p < 2) {
+            put(2 * p, hi);
+            put(2 * p + 1, (_Float16)(v - (double)hi));
+          } else {
+            put(p + 2, hi);
+          }
+        }
+      }
+    }
+  }
+}
+
+void launch_embed_fwd_runs(const EmbedArgs& a, hipStream_t st) {
+  if (a.nrows <= 0) return;
+  if (!embed_runs_supported(a.M, a.A) || !a.ftab)
+    throw std::invalid_argument("row-run forward needs forward run tables, M <= 128 and A <= 16");
+  const dim3 grid(ceil_div(a.nrows, kRunWaves)), block(kRunWaves * kWave);
+  switch ((a.M + 15) / 16) {
+    case 1: hipLaunchKernelGGL(k_embed_fwd_runs<1>, grid, block, 0, st, a); break;
+    case 2: hipLaunchKernelGGL(k_embed_fwd_runs<2>, grid, block, 0, st, a); break;
+    case 3: hipLaunchKernelGGL(k_embed_fwd_runs<3>, grid, block, 0, st, a); break;
+    case 4: hipLaunchKernelGGL(k_embed_fwd_runs<4>, grid, block, 0, st, a); break;
+    case 5: hipLaunchKernelGGL(k_embed_fwd_runs<5>, grid, block, 0, st, a); break;
+    case 6: hipLaunchKernelGGL(k_embed_fwd_runs<6>, grid, block, 0, st, a); break;
+    case 7: hipLaunchKernelGGL(k_embed_fwd_runs<7>, grid, block, 0, st, a); break;
+    default: hipLaunchKernelGGL(k_embed_fwd_runs<8>, grid, block, 0, st, a); break;
+  }
+  DPMI_LAUNCH_CHECK();
 }
 
 void launch_embed_bwd_runs(const EmbedArgs& a, hipStream_t st) {

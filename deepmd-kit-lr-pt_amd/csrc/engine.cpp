@@ -309,6 +309,13 @@ void SeaModel::build_mixed(const dpmi_sea_model_desc& d) {
     d_rtab = dupload(rt.data(), rt.size());
     d_rtab_off = dupload(roff.data(), roff.size());
     d_rtab_pscale = dupload(psc.data(), psc.size());
+    std::vector<_Float16> ft;
+    std::vector<long long> foff;
+    std::vector<float> fsc;
+    build_fwd_run_tables(table_host.data(), tab_off.data(), nspline.data(), info_host.data(), nnets, M, ft, foff, fsc);
+    d_ftab = dupload(ft.data(), ft.size());
+    d_ftab_off = dupload(foff.data(), foff.size());
+    d_ftab_scale = dupload(fsc.data(), fsc.size());
     embed_runs = true;
   }
   std::vector<double> shift(ntypes);
@@ -415,6 +422,9 @@ SeaModel::~SeaModel() {
   dfree(d_rtab);
   dfree(d_rtab_off);
   dfree(d_rtab_pscale);
+  dfree(d_ftab);
+  dfree(d_ftab_off);
+  dfree(d_ftab_scale);
   for (auto& p : f16_mem) dfree(p);
   for (auto* v : {&ew, &ewt, &eb, &eidt})
     for (auto& p : *v) dfree(p);
@@ -715,6 +725,9 @@ EmbedArgs SeaEngine::embed_args() const {
   ta.rtab = m.d_rtab;
   ta.rtab_off = m.d_rtab_off;
   ta.rtab_pscale = m.d_rtab_pscale;
+  ta.ftab = m.d_ftab;
+  ta.ftab_off = m.d_ftab_off;
+  ta.ftab_scale = m.d_ftab_scale;
   ta.etiles = d_etiles;
   ta.netiles = netiles;
   ta.flag = d_flag;
@@ -1082,6 +1095,7 @@ void SeaEngine::compute_mixed_descriptor_fitting(hipStream_t st, int& stage) {
   ta.packed_off = m.d_tab_off_pk;
   const bool mfma_fwd = m.embed_mfma && (tune.embed_mfma & 1), mfma_bwd = m.embed_mfma && (tune.embed_mfma & 2);
   const bool runs_bwd = m.embed_runs && (tune.embed_mfma & 4);
+  const bool runs_fwd = m.embed_runs && (tune.embed_mfma & 8);
   auto adjoint = [&] {
     if (runs_bwd)
       launch_embed_bwd_runs(ta, st);
@@ -1090,7 +1104,9 @@ void SeaEngine::compute_mixed_descriptor_fitting(hipStream_t st, int& stage) {
     else
       launch_embed_bwd<float, float>(ta, st);
   };
-  if (mfma_fwd)
+  if (runs_fwd)
+    launch_embed_fwd_runs(ta, st);
+  else if (mfma_fwd)
     launch_embed_fwd_mfma(ta, st);
   else
     launch_embed_fwd<float, _Float16>(ta, st);
@@ -1475,11 +1491,12 @@ void SeaEngine::autotune(const double* coord, const double* box_h, int reps, hip
             fit_c[3] = {32, 64, 64};
   // candidates 0..2: per-neighbour embedding kernels with the shapes above;
   // then, in the mixed engine, the table-space embedding kernels (mask 3)
-  // and the row-run adjoint (mask 4), whose launch shapes are fixed
+  // and the row-run forward and adjoint (mask 12), whose launch shapes are
+  // fixed
   int masks[5] = {0, 0, 0, 0, 0};
   int ncand = 3;
   if (precision == DPMI_MIXED && model->embed_mfma) masks[ncand++] = 3;
-  if (precision == DPMI_MIXED && model->embed_runs) masks[ncand++] = 4;
+  if (precision == DPMI_MIXED && model->embed_runs) masks[ncand++] = 12;
   double t_env[5], t_fwd[5], t_bwd[5], t_frc[5], t_fit[5];
   double *e = nullptr, *f = nullptr, *v = nullptr;
   DPMI_CHECK(hipMalloc(&e, sizeof(double) * nframes));
@@ -1546,17 +1563,18 @@ void SeaEngine::autotune(const double* coord, const double* box_h, int reps, hip
   tune.embed_mfma = 0;
   double fbest = t_fwd[best(t_fwd)], bbest = t_bwd[best(t_bwd)];
   int bmask = 0;
+  int fmask = 0;
   for (int c = 3; c < ncand; ++c) {
-    if ((masks[c] & 1) && t_fwd[c] < fbest) {
+    if ((masks[c] & 9) && t_fwd[c] < fbest) {
       fbest = t_fwd[c];
-      tune.embed_mfma |= 1;
+      fmask = masks[c] & 9;
     }
     if ((masks[c] & 6) && t_bwd[c] < bbest) {
       bbest = t_bwd[c];
       bmask = masks[c] & 6;
     }
   }
-  tune.embed_mfma |= bmask;
+  tune.embed_mfma = fmask | bmask;
   std::lock_guard<std::mutex> lk(g_tune_mu);
   tune_cache()[tune_key(precision, ntot)] = tune;
 }

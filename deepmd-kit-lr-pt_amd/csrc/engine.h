@@ -61,6 +61,9 @@ struct SeaModel {
   _Float16* d_rtab = nullptr;
   long long* d_rtab_off = nullptr;
   float* d_rtab_pscale = nullptr;
+  _Float16* d_ftab = nullptr;  // forward run tables
+  long long* d_ftab_off = nullptr;
+  float* d_ftab_scale = nullptr;
   std::vector<std::string> type_map;  // names of the types (model files)
   // uncompressed embedding (FP64 engines): per (net, layer) the weights as
   // FP64 MFMA B fragments (W and W^T), bias and timestep

@@ -189,6 +189,10 @@ struct EmbedArgs {
   const _Float16* rtab = nullptr;        // run tables of every net (build_run_tables)
   const long long* rtab_off = nullptr;   // [nnets] element offsets
   const float* rtab_pscale = nullptr;    // [8 nnets]: 1 / sigma_p
+  // row-run forward of the mixed engine (matrix cores, one wavefront per row)
+  const _Float16* ftab = nullptr;        // forward run tables of every net (build_fwd_run_tables)
+  const long long* ftab_off = nullptr;   // [nnets] element offsets
+  const float* ftab_scale = nullptr;     // [nnets]: 1 / sigma
 };
 // Uncompressed embedding (FP64 engine): slots <-> net rows, contraction and
 // adjoint, one wavefront per descriptor position q in [q0, q1) (atom perm[q]);
@@ -258,6 +262,16 @@ void check_table_rows(const double* infos, const int* nrows, int nnets);
 void build_run_tables(const double* table, const long long* tab_off, const int* nrows, int nnets, int M,
                       std::vector<_Float16>& out, std::vector<long long>& off, std::vector<float>& pscale);
 void launch_embed_bwd_runs(const EmbedArgs& a, hipStream_t st);
+// Forward (xyz, D) from the table rows of each atom's neighbour runs,
+// xyz = sum over runs of Q[run, p, c] a'_p[r, k] (a'_p = a_p s^p, s the
+// stride of row r's segment).  Forward run tables: per net
+// [row][ceil(M/16)][2][16][4 halves] (A-rows 4 quad + i of column
+// 16 kb + j): sigma a'_0 hi, lo, sigma a'_1 hi, lo, sigma a'_p (p = 2..5),
+// sigma one power of two per net (1 / sigma in scale[net]).
+void build_fwd_run_tables(const double* table, const long long* tab_off, const int* nrows, const double* infos,
+                          int nnets, int M, std::vector<_Float16>& out, std::vector<long long>& off,
+                          std::vector<float>& scale);
+void launch_embed_fwd_runs(const EmbedArgs& a, hipStream_t st);
 
 // ------------------------------------------------------------- force.hip --
 template <typename T>
@@ -338,7 +352,8 @@ struct Tune {
   // matrix cores (table space, four atoms per wavefront); pays off when many
   // neighbours of an atom share table rows, so it is off until the autotuner
   // measures it faster; bit 2 the row-run adjoint on the matrix cores (one
-  // atom per wavefront; takes precedence over bit 1)
+  // atom per wavefront; takes precedence over bit 1); bit 3 the row-run
+  // forward on the matrix cores (precedence over bit 0)
   int embed_mfma = 0;
   int fit_rows = 0;  // fused fitting rows per workgroup tile (32 or 64; 0: chosen from the size)
 };

@@ -316,9 +316,9 @@ int dpmi_sea_engine_graph_capture_device_box(dpmi_sea_engine* engine, const doub
  * current knobs (6 ints): {env threads per block, forward embedding waves
  * per workgroup, adjoint waves per workgroup, force rows per workgroup,
  * matrix-core embedding mask (DPMI_MIXED: bit 0 forward, bit 1 adjoint in
- * table space on the matrix cores, bit 2 adjoint from each atom's table-row
- * runs on the matrix cores (precedence over bit 1); else the per-neighbour
- * kernels), fused
+ * table space on the matrix cores, bit 2 adjoint and bit 3 forward from
+ * each atom's table-row runs on the matrix cores (precedence over bits 1
+ * and 0); else the per-neighbour kernels), fused
  * fitting rows per tile (32 or 64; 0 = chosen from the atom count)}.
  * dpmi_sea_engine_set_tuning sets them (a knob <= 0, or < 0 for the mask,
  * keeps its value; out-of-range values give DPMI_ERR_ARGUMENT). */

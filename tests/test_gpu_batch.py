@@ -81,11 +81,25 @@ def test_autotune_keeps_results_and_caches(water, batch, precision):
     assert knobs["env_threads_per_block"] in (128, 256, 512)
     assert knobs["embed_fwd_waves"] in (1, 2, 4) and knobs["embed_bwd_waves"] in (1, 2, 4)
     assert knobs["force_rows_per_workgroup"] in (4, 8, 16)
-    assert knobs["embed_matrix_cores"] in ((0, 1, 2, 3) if precision == "mixed" else (0,))
+    # forward: per-neighbour (0), table space (1) or row runs (8); adjoint:
+    # per-neighbour (0), table space (2) or row runs (4)
+    mixed_masks = {f | b for f in (0, 1, 8) for b in (0, 2, 4)}
+    assert knobs["embed_matrix_cores"] in (mixed_masks if precision == "mixed" else {0})
     assert knobs["fitting_rows_per_tile"] in ((32, 64) if precision == "mixed" else (0,))
     e, f, v = [t.cpu().numpy().copy() for t in eng.compute(x, box)]
-    tol = 1e-12 if precision == "fp64" else 1e-9
-    assert _close(e, e0, tol) and _close(f, f0, tol) and _close(v, v0, tol)
+    if precision == "fp64" or knobs["embed_matrix_cores"] == 0:
+        # launch shapes alone do not change the arithmetic
+        tol = 1e-12 if precision == "fp64" else 1e-9
+        assert _close(e, e0, tol) and _close(f, f0, tol) and _close(v, v0, tol)
+    else:
+        # other embedding kernels round differently: FP32-level agreement
+        # with the default kernels, and bit-equal to an untuned engine given
+        # the same knobs
+        assert _close(e, e0, 1e-6) and _close(f, f0, 1e-4) and _close(v, v0, 1e-4)
+        same = Engine(dm, natoms, atype, precision=precision)
+        same.set_tuning(**knobs)
+        e2, f2, v2 = [t.cpu().numpy().copy() for t in same.compute(x, box)]
+        assert _close(e, e2, 1e-12) and _close(f, f2, 1e-12) and _close(v, v2, 1e-12)
     later = Engine(dm, natoms, atype, precision=precision)
     assert later.tuning() == knobs
 

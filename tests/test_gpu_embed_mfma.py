@@ -80,7 +80,7 @@ def test_matrix_core_embedding_matches_per_neighbour_kernels(case):
     n = len(atype)
     d = _eval(dm, [n], coord, atype, box, precision="fp64")
     ref = _eval(dm, [n], coord, atype, box, mask=0)
-    for mask in (1, 2, 3, 4, 5):
+    for mask in (1, 2, 3, 4, 5, 8, 12):
         got = _eval(dm, [n], coord, atype, box, mask=mask)
         e_ref = {k: _rel(ref[k], d[k]) for k in ("e", "f", "v", "nd", "desc")}
         e_got = {k: _rel(got[k], d[k]) for k in ("e", "f", "v", "nd", "desc")}
@@ -117,4 +117,4 @@ def test_tuning_knob_rejects_bad_mask():
     coord, atype, box = systems.water(8, seed=1)
     eng = Engine(DeviceModel(m), [len(atype)], atype, precision="mixed")
     with pytest.raises(DpmiError, match="ARGUMENT"):
-        eng.set_tuning(embed_matrix_cores=8)
+        eng.set_tuning(embed_matrix_cores=16)
