@@ -35,8 +35,10 @@ def rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
-def _run(m, coord, atype, box, precision="fp64", **kw):
+def _run(m, coord, atype, box, precision="fp64", mask=0, **kw):
     eng = Engine(DeviceModel(m), [len(atype)], atype, precision=precision, **kw)
+    if mask:
+        eng.set_tuning(embed_matrix_cores=mask)
     e, f, v = eng.compute(torch.as_tensor(coord, device="cuda"), box, atom_energy=True)
     eng.status()
     return eng, e.cpu().numpy().copy(), f.cpu().numpy().copy(), v.cpu().numpy().copy()
@@ -67,17 +69,22 @@ def test_sel_truncation():
     assert np.sqrt(np.mean((f_m - f) ** 2)) * 1e3 < 1.0
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_negative_types_are_absent(precision):
-    print(f"case type -1 atoms precision={precision}", flush=True)
+@pytest.mark.parametrize("precision,mask", [("fp64", 0), ("mixed", 0), ("mixed", 12)])
+def test_negative_types_are_absent(precision, mask):
+    """Atoms of type -1 own no neighbours and appear in no list; mask 12
+    runs the mixed engine's row-run embedding kernels (absent rows zeroed
+    there too)."""
+    print(f"case type -1 atoms precision={precision} mask={mask}", flush=True)
     m = dmodel.water_model(seed=1)
     coord, atype, box = systems.water(64, seed=3)
     neg = np.random.default_rng(1).choice(len(atype), 12, replace=False)
     a2 = atype.copy()
     a2[neg] = -1
-    eng, e, f, v = _run(m, coord, a2, box, precision)
+    eng, e, f, v = _run(m, coord, a2, box, precision, mask)
     keep = np.setdiff1d(np.arange(len(atype)), neg)
-    e2, f2, v2 = _run(m, coord[keep], atype[keep], box, precision)[1:]
+    e2, f2, v2 = _run(m, coord[keep], atype[keep], box, precision, mask)[1:]
+    if mask:
+        assert np.all(eng.fetch("net_deriv").reshape(len(atype), -1)[neg] == 0.0)
     assert np.all(f[neg] == 0.0)
     assert np.all(eng.fetch("atom_energy")[neg] == 0.0)
     if precision == "fp64":
