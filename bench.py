@@ -212,7 +212,7 @@ def time_graph(eng, x, box, steps, warmup):
     return dt
 
 
-def roofline(m, nloc, nall, precision, stages):
+def roofline(m, nloc, nall, precision, stages, embed_mask=0):
     model = stage_model(m, nloc, nall, precision)
     dom = max((k for k in stages if k in model), key=lambda k: stages[k])
     kind, amount, peak, nbytes = model[dom]
@@ -229,9 +229,11 @@ def roofline(m, nloc, nall, precision, stages):
         bound = "mfma" if dom.startswith("fitting") else "valu"
         roof = {"kernel": dom, "bound": bound, "achieved": amount / (t_ms * 1e-3) / 1e12, "peak": peak,
                 "unit": "TFLOP/s"}
-        if dom.startswith("embed") and precision == "mixed":
-            roof["note"] = ("achieved = reference per-neighbour FP32 flops / kernel time; the row-run kernels "
-                            "form the column sums per table-row run on v_mfma_f32_16x16x16_f16 split products")
+        runs = (dom == "embed_fwd" and embed_mask & 8) or (dom == "embed_bwd" and embed_mask & 4)
+        if precision == "mixed" and runs:
+            roof["note"] = ("achieved = reference per-neighbour FP32 flops / kernel time; the row-run kernel "
+                            "forms the column sums once per table-row run on v_mfma_f32_16x16x16_f16 split "
+                            "products")
     roof["frac"] = roof["achieved"] / roof["peak"]
     tr = MEASURED_TRAFFIC_MIB.get(precision, {}).get(dom)
     roof["traffic"] = None if tr is None else (tr[0] + tr[1]) * 1048576.0
@@ -446,7 +448,7 @@ def run_copper(args) -> int:
         "config": {"workload": f"fcc Cu {args.ncell}^3 cells ({nloc} atoms, box {box[0]:.2f} A), rcut 8.0, "
                                f"sel [200], embedding [32, 64, 128] compressed, one frame, E+F+V per step",
                    "nloc": nloc, "nall": rm["nall"], "parallelism": "single-gpu"},
-        "roofline": roofline(m, nloc, rm["nall"], "mixed", rm["stages"]),
+        "roofline": roofline(m, nloc, rm["nall"], "mixed", rm["stages"], knobs["mixed"]["embed_matrix_cores"]),
         "stages": per_stage,
         "fp64_value": nloc / rd["dt"],
         "mixed_over_fp64": rd["dt"] / rm["dt"],
@@ -565,7 +567,8 @@ def main() -> int:
         "config": {"workload": f"water {args.nmol} H2O ({nloc} atoms, box {box[0]:.2f} A), se_e2_a compressed, "
                                f"{args.precision} engine, one frame, E+F+V per step", "nloc": nloc,
                    "nall": r["nall"], "precision": args.precision, "parallelism": "single-gpu"},
-        "roofline": roofline(m, nloc, r["nall"], args.precision, r["stages"]),
+        "roofline": roofline(m, nloc, r["nall"], args.precision, r["stages"],
+                             knobs[args.precision]["embed_matrix_cores"]),
         "stages_ms": {k: round(vv, 5) for k, vv in r["stages"].items()},
         "fp64_value": fp64_value,
         "mixed_value": mixed_value,
