@@ -617,7 +617,8 @@ int dpmi_sea_engine_set_tuning(dpmi_sea_engine* engine, const int* knobs_host) {
       t.bwd_waves = k[2];
     }
     if (k[3] > 0) {
-      if (k[3] > 16) throw std::invalid_argument("force rows per workgroup: 1 to 16");
+      if (k[3] > 16 && k[3] != 32 && k[3] != 64 && k[3] != 128 && k[3] != 256)
+        throw std::invalid_argument("force rows per workgroup: 1 to 16, 32, 64, 128 or 256");
       t.force_rows = k[3];
     }
     if (k[4] >= 0) {

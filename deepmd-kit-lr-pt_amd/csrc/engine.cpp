@@ -1487,7 +1487,7 @@ void SeaEngine::load_cached_tune() {
 // them) and keeps, per stage, the value with the fastest single step; the
 // result is cached for the size bucket.
 void SeaEngine::autotune(const double* coord, const double* box_h, int reps, hipStream_t st) {
-  const int env_c[3] = {128, 256, 512}, fwd_c[3] = {1, 2, 4}, bwd_c[3] = {1, 2, 4}, frc_c[3] = {4, 8, 16},
+  const int env_c[3] = {128, 256, 512}, fwd_c[3] = {1, 2, 4}, bwd_c[3] = {1, 2, 4}, frc_c[3] = {8, 32, 64},
             fit_c[3] = {32, 64, 64};
   // candidates 0..2: per-neighbour embedding kernels with the shapes above;
   // then, in the mixed engine, the table-space embedding kernels (mask 3)

@@ -271,6 +271,21 @@ __device__ __forceinline__ int hash_slot(int* keys, int key) {
   }
 }
 
+// Bounded probe for row groups that may hold more distinct keys than the
+// table: -1 after kProbeMax occupied slots (the caller then adds straight
+// into global memory).
+constexpr int kProbeMax = 32;
+template <int HB>
+__device__ __forceinline__ int hash_slot_bounded(int* keys, int key) {
+  unsigned h = ((unsigned)key * 2654435761u) >> (32 - HB);
+  for (int probe = 0; probe < kProbeMax; ++probe) {
+    const int prev = atomicCAS(&keys[h], -1, key);
+    if (prev == -1 || prev == key) return (int)h;
+    h = (h + 1) & ((1u << HB) - 1);
+  }
+  return -1;
+}
+
 // Environment-matrix derivative and displacement of a slot of a centre at
 // ci, rebuilt from the coordinates exactly as the environment kernel stores
 // them (FP64 evaluation; TI float: rounded as the FP32-storing kernel does).
@@ -324,9 +339,15 @@ __global__ void __launch_bounds__(1024) k_force_virial_hash(
     }
   }
   __syncthreads();
-  const int lane = lane_id(), wave = threadIdx.x / kWave;
-  const long long p = (long long)blockIdx.x * R + wave;
-  if (wave < R && p < nrows) {
+  const int lane = lane_id(), wave = threadIdx.x / kWave, W = blockDim.x / kWave;
+  // R rows per workgroup, W waves: wave w takes rows w, w + W, ... of the
+  // group (consecutive in cell order across the waves); with R > the
+  // table's guaranteed capacity the probes are bounded and a slot that finds
+  // no room adds straight into global memory
+  const bool bounded = (long long)R * (nnei + 1) > NH;
+  for (int rr = wave; rr < R; rr += W) {
+  const long long p = (long long)blockIdx.x * R + rr;
+  if (p < nrows) {
     const int f = row_frame[p];
     const long long g = geom[f].loc_off + sorted_loc[p];  // packed row of the p-th atom in cell order
     const int eo = frame_ext_off[f];
@@ -381,11 +402,15 @@ __global__ void __launch_bounds__(1024) k_force_virial_hash(
         TI fs[3], v[9];
         slot_force(n4[u], e12[u], fs);
         slot_virial(n4[u], e12[u], r3[u], v);
-        const int h = hash_slot<HB>(hkey, lo + own[u]);
+        const int key = lo + own[u];
+        const int h = bounded ? hash_slot_bounded<HB>(hkey, key) : hash_slot<HB>(hkey, key);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
           fi[d] -= fs[d];
-          atomicAdd(&hval[h][d], (double)fs[d]);
+          if (h >= 0)
+            atomicAdd(&hval[h][d], (double)fs[d]);
+          else
+            atomicAdd(force + (size_t)key * 3 + d, (double)fs[d]);
         }
 #pragma unroll
         for (int k = 0; k < 9; ++k) vs[k] += v[k];
@@ -400,9 +425,15 @@ __global__ void __launch_bounds__(1024) k_force_virial_hash(
 #pragma unroll
     for (int k = 0; k < 9; ++k) vs[k] = wave_sum(vs[k]);
     if (lane == 0) {
-      const int h = hash_slot<HB>(hkey, lo + (map_ext ? map_ext[eo + i] : eo + i));
+      const int key = lo + (map_ext ? map_ext[eo + i] : eo + i);
+      const int h = bounded ? hash_slot_bounded<HB>(hkey, key) : hash_slot<HB>(hkey, key);
 #pragma unroll
-      for (int d = 0; d < 3; ++d) atomicAdd(&hval[h][d], fi[d]);
+      for (int d = 0; d < 3; ++d) {
+        if (h >= 0)
+          atomicAdd(&hval[h][d], fi[d]);
+        else
+          atomicAdd(force + (size_t)key * 3 + d, fi[d]);
+      }
     }
     if (lane < 9) {
       double out = 0.;
@@ -411,6 +442,7 @@ __global__ void __launch_bounds__(1024) k_force_virial_hash(
         if (lane == k) out = vs[k];
       vpart[(size_t)g * 9 + lane] = out;
     }
+  }
   }
   __syncthreads();
   for (int h = threadIdx.x; h < NH; h += blockDim.x) {
@@ -633,8 +665,14 @@ void launch_force_virial_rows(const ForceRowsArgs& a, hipStream_t st) {
       DPMI_LAUNCH_CHECK();
       return;
     }
-    const int R = std::max(1, std::min(a.rows > 0 ? a.rows : 16, kHashSize / (a.nnei + 1)));
-    const dim3 grid(ceil_div(a.nrows, R)), block(R * kWave);
+    // up to 16 rows: one per wave and a table that holds every key; more
+    // rows (a multiple of 16): several per wave, the 2^12-entry table with
+    // bounded probes (the rows' neighbour sets overlap, so their distinct
+    // owners stay far below R (nnei + 1))
+    const int want = a.rows > 0 ? a.rows : 16;
+    const int R = want > 16 ? (want + 15) / 16 * 16 : std::max(1, std::min(want, kHashSize / (a.nnei + 1)));
+    const int W = std::min(R, 16);
+    const dim3 grid(ceil_div(a.nrows, R)), block(W * kWave);
     const int keys = R * (a.nnei + 1);
     const int hb = keys <= (1 << 10) ? 10 : (keys <= (1 << 11) ? 11 : 12);
 #define DPMI_HASH_LAUNCH(HB)                                                                                          \

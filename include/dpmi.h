@@ -314,7 +314,9 @@ int dpmi_sea_engine_graph_capture_device_box(dpmi_sea_engine* engine, const doub
  * bucket = floor(log2 of the packed atom count)) and applied to engines
  * created later for the same bucket.  dpmi_sea_engine_tuning returns the
  * current knobs (6 ints): {env threads per block, forward embedding waves
- * per workgroup, adjoint waves per workgroup, force rows per workgroup,
+ * per workgroup, adjoint waves per workgroup, force rows per workgroup
+ * (1 to 16: one row per wavefront; 32, 64, 128 or 256: several rows per
+ * wavefront of a 16-wavefront workgroup),
  * matrix-core embedding mask (DPMI_MIXED: bit 0 forward, bit 1 adjoint in
  * table space on the matrix cores, bit 2 adjoint and bit 3 forward from
  * each atom's table-row runs on the matrix cores (precedence over bits 1

@@ -80,7 +80,7 @@ def test_autotune_keeps_results_and_caches(water, batch, precision):
     print(precision, knobs, flush=True)
     assert knobs["env_threads_per_block"] in (128, 256, 512)
     assert knobs["embed_fwd_waves"] in (1, 2, 4) and knobs["embed_bwd_waves"] in (1, 2, 4)
-    assert knobs["force_rows_per_workgroup"] in (4, 8, 16)
+    assert knobs["force_rows_per_workgroup"] in (8, 32, 64)
     # forward: per-neighbour (0), table space (1) or row runs (8); adjoint:
     # per-neighbour (0), table space (2) or row runs (4)
     mixed_masks = {f | b for f in (0, 1, 8) for b in (0, 2, 4)}
