@@ -83,9 +83,9 @@ def stage_model(m, nloc, nall, precision):
 # FETCH_SIZE / WRITE_SIZE on this workload, scaled by a 256 MiB copy; the
 # readings and the commands are in TRAFFIC_PROFILE.
 MEASURED_TRAFFIC_MIB = {
-    "mixed": {"embed_bwd": (123.04, 26.19), "embed_fwd": (31.32, 54.89), "fitting": (61.83, 92.87),
-              "force_virial": (38.36, 28.57), "nlist_env": (13.17, 33.59)},
-    "fp64": {"embed_bwd": (260.66, 50.70), "embed_fwd": (85.34, 200.84), "force_virial": (58.13, 18.89),
+    "mixed": {"embed_bwd": (123.05, 26.19), "embed_fwd": (31.32, 54.89), "fitting": (61.83, 93.03),
+              "force_virial": (40.09, 19.78), "nlist_env": (13.17, 33.59)},
+    "fp64": {"embed_bwd": (261.61, 50.70), "embed_fwd": (85.80, 202.84), "force_virial": (60.19, 11.39),
              "nlist_env": (15.83, 58.91)},
 }
 
