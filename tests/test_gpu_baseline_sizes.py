@@ -52,8 +52,10 @@ def budget(e_m, f_m, v_m, e_d, f_d, v_d, natoms):
     return err
 
 
-def _eval(dm, natoms, coord, atype, box, precision, atom_energy=False):
+def _eval(dm, natoms, coord, atype, box, precision, atom_energy=False, mask=0):
     eng = Engine(dm, natoms, atype, precision=precision)
+    if mask:  # matrix-core embedding kernels (mixed engine)
+        eng.set_tuning(embed_matrix_cores=mask)
     e, f, v = eng.compute(torch.as_tensor(coord, device="cuda"), box, atom_energy=atom_energy)
     eng.status()
     return eng, e.cpu().numpy().copy(), f.cpu().numpy().copy(), v.cpu().numpy().copy()
@@ -88,11 +90,13 @@ def test_water_4096_mixed_within_budget(env_stat):
     m = dmodel.water_model(seed=1, env_stat=env_stat)
     coord, atype, box = systems.water(4096, seed=2024, box=49.68)
     dm = DeviceModel(m)
-    em, e_m, f_m, v_m = _eval(dm, [len(atype)], coord, atype, box, "mixed")
     ed, e_d, f_d, v_d = _eval(dm, [len(atype)], coord, atype, box, "fp64")
-    assert np.array_equal(em.fetch("nlist"), ed.fetch("nlist"))
-    err = budget(e_m[0], f_m, v_m[0], e_d[0], f_d, v_d[0], len(atype))
-    print("errors", err, flush=True)
+    # per-neighbour embedding kernels, then the row-run forward and adjoint
+    for mask in (0, 12):
+        em, e_m, f_m, v_m = _eval(dm, [len(atype)], coord, atype, box, "mixed", mask=mask)
+        assert np.array_equal(em.fetch("nlist"), ed.fetch("nlist"))
+        err = budget(e_m[0], f_m, v_m[0], e_d[0], f_d, v_d[0], len(atype))
+        print("mask", mask, "errors", err, flush=True)
     if env_stat:  # energies of order 10 meV/atom: the comparison is not vacuous
         assert abs(e_d[0]) / len(atype) > 1e-3
 
@@ -108,6 +112,8 @@ def test_batched_256_frames_graph_vs_fp64_and_oracle():
     dm = DeviceModel(m)
     x = torch.as_tensor(coord, device="cuda")
     mixed = Engine(dm, natoms, atype, precision="mixed")
+    # the batched benchmark's choice: the row-run adjoint on the matrix cores
+    mixed.set_tuning(embed_matrix_cores=4, force_rows_per_workgroup=64)
     s = torch.cuda.Stream()
     with torch.cuda.stream(s):
         mixed.capture(x, box)
@@ -170,12 +176,19 @@ def test_copper_1m_mixed_within_budget():
     nl_m = em.fetch("nlist").reshape(n, m.nnei)[rows]
     del em
     torch.cuda.empty_cache()
+    # the row-run forward and adjoint on the matrix cores (copper's long runs)
+    er, e_r, f_r, v_r = _eval(dm, [n], coord, atype, box, "mixed", mask=12)
+    del er
+    torch.cuda.empty_cache()
     ed, e_d, f_d, v_d = _eval(dm, [n], coord, atype, box, "fp64")
     nl_d = ed.fetch("nlist").reshape(n, m.nnei)
     del ed
     assert np.array_equal(nl_m, nl_d[rows])
     err = budget(e_m[0], f_m, v_m[0], e_d[0], f_d, v_d[0], n)
     print("errors", err, flush=True)
+    err = budget(e_r[0], f_r, v_r[0], e_d[0], f_d, v_d[0], n)
+    print("row-run kernels: errors", err, flush=True)
+    assert np.max(np.abs(f_r.sum(0))) < 1e-6 * np.abs(f_r).max() * n
     # translation invariance: the forces sum to zero
     assert np.max(np.abs(f_d.sum(0))) < 1e-9 * np.abs(f_d).max() * n
     assert np.max(np.abs(f_m.sum(0))) < 1e-6 * np.abs(f_m).max() * n
