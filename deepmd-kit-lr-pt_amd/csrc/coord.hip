@@ -19,9 +19,12 @@
 //   4. k_emit_ghosts   one wave per extended cell: sort the source cell's
 //                      atoms by index in registers, write ghost images and
 //                      the extended cells' atom records (coordinates,
-//                      extended index, type) in cell order; the wave of a
-//                      cell's own (non-ghost) image stores the cell's sorted
-//                      atom list and clears its count for the next build
+//                      extended index, type) in cell order, inside a cell
+//                      grouped by sub-cell along the third cell vector (with
+//                      each sub-cell's first record, for the neighbour
+//                      search's windows); the wave of a cell's own
+//                      (non-ghost) image stores the cell's sorted atom list
+//                      and clears its count for the next build
 // Floating-point expressions that decide cell membership and image
 // coordinates are evaluated without contraction, in the reference's order.
 #include "dpmi_common.h"
@@ -216,7 +219,7 @@ __global__ void k_emit_ghosts(const int* __restrict__ xcell_frame, const FrameGe
                               const int* __restrict__ xcnt, const int* __restrict__ xstart,
                               const int* __restrict__ xgstart, double* __restrict__ coord_ext,
                               int* __restrict__ type_ext, int* __restrict__ map_ext,
-                              CellAtom* __restrict__ cell_atoms) {
+                              CellAtom* __restrict__ cell_atoms, int* __restrict__ cell_sub) {
 #pragma clang fp contract(off)
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int lane = threadIdx.x & (kWave - 1);
@@ -228,6 +231,7 @@ __global__ void k_emit_ghosts(const int* __restrict__ xcell_frame, const FrameGe
   bool gh;
   const int s = src_cell(g, e, c, &gh);
   const int cnt = xcnt[wave];
+  const int x0 = xstart[wave];
   const size_t base = (size_t)g.loc_off + cell_start[g.cell_off + s];
   const int* src = cell_list + base;
   double sv[3] = {0., 0., 0.};
@@ -236,15 +240,15 @@ __global__ void k_emit_ghosts(const int* __restrict__ xcell_frame, const FrameGe
     for (int d = 0; d < 3; ++d) sh[d] = (double)image_shift(c[d], g.nc[d]);
     for (int d = 0; d < 3; ++d) sv[d] = sh[0] * g.t[0 * 3 + d] + sh[1] * g.t[1 * 3 + d] + sh[2] * g.t[2 * 3 + d];
   }
-  // atom p is the k-th of its cell in ascending index order
-  auto emit = [&](int p, int k) {
-    const int slot = xstart[wave] + k;
+  // record of atom p, the k-th of its cell in ascending index order (the
+  // reference's ghost order); ghost images are written as a side effect
+  auto record = [&](int p, int k, bool write) {
     const size_t a = (size_t)g.ext_off + p;
     CellAtom rec;
     rec.x = coord_ext[3 * a + 0];
     rec.y = coord_ext[3 * a + 1];
     rec.z = coord_ext[3 * a + 2];
-    rec.type = type_ext[a];
+    rec.type = write ? type_ext[a] : 0;
     rec.j = p;
     if (gh) {
       const int gi = g.nloc + xgstart[wave] + k;
@@ -252,7 +256,7 @@ __global__ void k_emit_ghosts(const int* __restrict__ xcell_frame, const FrameGe
       rec.y = rec.y - sv[1];
       rec.z = rec.z - sv[2];
       rec.j = gi;
-      if (gi < g.ext_cap) {
+      if (write && gi < g.ext_cap) {
         const size_t b = (size_t)g.ext_off + gi;
         coord_ext[3 * b + 0] = rec.x;
         coord_ext[3 * b + 1] = rec.y;
@@ -260,22 +264,76 @@ __global__ void k_emit_ghosts(const int* __restrict__ xcell_frame, const FrameGe
         type_ext[b] = rec.type;
         map_ext[b] = p;
       }
-    } else {
+    } else if (write) {
       sorted_loc[base + k] = p;
     }
-    if (slot < g.ext_cap) cell_atoms[g.ext_off + slot] = rec;
+    return rec;
   };
+  // the records are stored grouped by sub-cell along the third cell vector
+  // (kAxisBins per cell), in index order inside a sub-cell
+  const double q = (double)(e % g.nx[2]);  // the cell spans positions [q, q + 1)
+  auto sub_of = [&](const CellAtom& r) {   // sub-cell of a record (0 below the cell, last above)
+    const double u = geom_axis2_pos(g, r.x, r.y, r.z);
+    int b = 0;
+#pragma unroll
+    for (int k = 1; k < kAxisBins; ++k) b += u >= q + k * (1.0 / kAxisBins) ? 1 : 0;
+    return u == u ? b : kAxisBins - 1;  // NaN: last
+  };
+  int first[kAxisBins];  // first position of each sub-cell's records in the cell
   if (cnt <= kWave) {
     const int v = wave_sort_asc(lane < cnt ? src[lane] : 0x7fffffff, lane);
-    if (lane < cnt) emit(v, lane);
+    CellAtom rec;
+    int sb = kAxisBins;
+    if (lane < cnt) {
+      rec = record(v, lane, true);
+      sb = sub_of(rec);
+    }
+    int pos = 0, run = 0;
+#pragma unroll
+    for (int b = 0; b < kAxisBins; ++b) {
+      const unsigned long long m = __ballot(sb == b);
+      first[b] = run;
+      if (sb == b) pos = run + __popcll(m & ((1ull << lane) - 1ull));
+      run += __popcll(m);
+    }
+    if (lane < cnt && x0 + pos < g.ext_cap) cell_atoms[g.ext_off + x0 + pos] = rec;
   } else {
     // rank by counting (indices in a cell are distinct)
+    int below[kAxisBins];  // this lane's records in each sub-cell
+#pragma unroll
+    for (int b = 0; b < kAxisBins; ++b) below[b] = 0;
     for (int k = lane; k < cnt; k += kWave) {
       const int p = src[k];
       int r = 0;
       for (int m = 0; m < cnt; ++m) r += src[m] < p;
-      emit(p, r);
+      const CellAtom rec = record(p, r, true);
+      const int sb = sub_of(rec);
+      int pos = 0;
+      for (int m = 0; m < cnt; ++m) {
+        const int pm = src[m];
+        const int bm = sub_of(record(pm, 0, false));
+        pos += (bm < sb || (bm == sb && pm < p)) ? 1 : 0;
+      }
+      if (x0 + pos < g.ext_cap) cell_atoms[g.ext_off + x0 + pos] = rec;
+#pragma unroll
+      for (int b = 0; b < kAxisBins; ++b) below[b] += sb == b ? 1 : 0;
     }
+    int run = 0;
+#pragma unroll
+    for (int b = 0; b < kAxisBins; ++b) {
+      int c = below[b];
+#pragma unroll
+      for (int off = 1; off < kWave; off <<= 1) c += __shfl_xor(c, off, kWave);
+      first[b] = run;
+      run += c;
+    }
+  }
+  if (cell_sub) {
+    int mine = 0;
+#pragma unroll
+    for (int b = 0; b < kAxisBins; ++b)
+      if (lane == b) mine = first[b];
+    if (lane < kAxisBins) cell_sub[(size_t)wave * kAxisBins + lane] = x0 + mine;
   }
   if (!gh && lane == 0) cell_cnt[g.cell_off + s] = 0;  // counts start at zero in the next build
 }
@@ -299,7 +357,7 @@ void launch_build_extended(const CellWork& w, hipStream_t st) {
   DPMI_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_emit_ghosts, dim3(ceil_div((long long)w.nx_tot * kWave, tpb)), dim3(tpb), 0, st,
                      w.xcell_frame, w.geom, w.nx_tot, w.cell_start, w.cell_list, w.sorted_loc, w.cell_cnt, w.xcnt,
-                     w.xstart, w.xgstart, w.coord_ext, w.type_ext, w.map_ext, w.cell_atoms);
+                     w.xstart, w.xgstart, w.coord_ext, w.type_ext, w.map_ext, w.cell_atoms, w.cell_sub);
   DPMI_LAUNCH_CHECK();
 }
 

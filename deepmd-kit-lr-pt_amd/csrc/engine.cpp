@@ -578,7 +578,9 @@ SeaEngine::SeaEngine(const SeaModel* m, int nframes_, const int* natoms_h, const
   }
   d_outw.resize(nt, nullptr);
   for (int t = 0; t < nt; ++t) d_outw[t] = dupload(m->out_w_host[t].data(), m->out_w_host[t].size());
-  nlist_cap = nlist_row_capacity(std::max(128, 2 * m->nnei));
+  // keys of the accepted candidates of one centre (rows of nnei + a quarter;
+  // an overflow is flagged and the step retried with twice the capacity)
+  nlist_cap = nlist_row_capacity(std::max(128, m->nnei + m->nnei / 4));
   load_cached_tune();
 }
 
@@ -589,8 +591,8 @@ SeaEngine::~SeaEngine() {
   if (stat_pinned) (void)hipHostFree(stat_pinned);
   for (auto* p : {&d_frame_of, &d_center, &d_type, &d_pos, &d_perm, &d_flag, &d_nall, &d_frame_ext_off, &d_nlist,
                   &d_ncand, &d_cell_of, &d_sorted_loc, &d_cell_frame, &d_xcell_frame, &d_cell_cnt, &d_cell_start,
-                  &d_cell_slot, &d_cell_list, &d_xcnt, &d_xstart, &d_xgcnt, &d_xgstart, &d_type_ext, &d_map_ext,
-                  &d_xframe_of_ext, &d_chunks, &d_chunk_start, &d_ticket})
+                  &d_cell_slot, &d_cell_list, &d_xcnt, &d_xstart, &d_xgcnt, &d_xgstart, &d_cell_sub, &d_type_ext,
+                  &d_map_ext, &d_xframe_of_ext, &d_chunks, &d_chunk_start, &d_ticket})
     dfree(*p);
   dfree(d_cell_atoms);
   for (auto* p : {&d_em, &d_em_deriv, &d_rij, &d_net_deriv, &d_xyz, &d_D, &d_gD, &d_atom_e, &d_vpart,
@@ -652,7 +654,7 @@ void SeaEngine::prepare_geometry(const double* box_h) {
       ext_cap_h[f] = g.ext_cap;
     }
     for (auto* p : {&d_cell_frame, &d_xcell_frame, &d_cell_cnt, &d_cell_start, &d_xcnt, &d_xstart,
-                    &d_xgcnt, &d_xgstart, &d_type_ext, &d_map_ext, &d_xframe_of_ext})
+                    &d_xgcnt, &d_xgstart, &d_cell_sub, &d_type_ext, &d_map_ext, &d_xframe_of_ext})
       dfree(*p);
     dfree(d_cell_atoms);
     dfree(d_coord_ext);
@@ -667,6 +669,7 @@ void SeaEngine::prepare_geometry(const double* box_h) {
     d_xstart = dalloc<int>(nxcells);
     d_xgcnt = dalloc<int>(nxcells);
     d_xgstart = dalloc<int>(nxcells);
+    d_cell_sub = dalloc<int>((size_t)nxcells * kAxisBins);
     d_type_ext = dalloc<int>(next);
     d_map_ext = dalloc<int>(next);
     d_cell_atoms = dalloc<CellAtom>(next);
@@ -1322,6 +1325,7 @@ void SeaEngine::enqueue_step(const double* coord, const double* box_h, double* e
   w.xstart = d_xstart;
   w.xgcnt = d_xgcnt;
   w.xgstart = d_xgstart;
+  w.cell_sub = d_cell_sub;
   w.nall = d_nall;
   w.flag = d_flag;
   w.force_clear = owner_force ? force : nullptr;

@@ -161,6 +161,7 @@ struct SeaEngine {
   int *d_cell_slot = nullptr, *d_cell_list = nullptr;
   bool counts_clean = false;  // d_cell_cnt is zero (left so by the last complete build)
   int *d_xcnt = nullptr, *d_xstart = nullptr, *d_xgcnt = nullptr, *d_xgstart = nullptr;
+  int* d_cell_sub = nullptr;  // [nx_tot, kAxisBins] record table of the cells' sub-cells
   int *d_type_ext = nullptr, *d_map_ext = nullptr, *d_xframe_of_ext = nullptr;
   CellAtom* d_cell_atoms = nullptr;
   double *d_coord_ext = nullptr, *d_force_ext = nullptr;

@@ -80,6 +80,21 @@ __host__ __device__ inline void geom_from_box(FrameGeom& g, const double* boxt, 
   }
 }
 
+// Position of a point along the third cell vector in units of extended
+// cells: the extended cells with third index q span [q, q + 1) (the
+// fractional coordinate of the binning, scaled by nc and shifted by ng).
+// Sub-cell b of cell q holds the records with q + b / kAxisBins <= u <
+// q + (b + 1) / kAxisBins (the first and last also what lies below and
+// above the cell by rounding); cell-ordered records are grouped by it.
+__host__ __device__ inline double geom_axis2_pos(const FrameGeom& g, double x, double y, double z) {
+#pragma clang fp contract(off)
+  const double s = x * g.rec[6] + y * g.rec[7] + z * g.rec[8];
+  return s * g.nc[2] + g.ng[2];
+}
+
+// Sub-cells per cell along the third axis in the records' position table.
+constexpr int kAxisBins = 8;
+
 inline int geom_local_cells(const FrameGeom& g) { return g.nc[0] * g.nc[1] * g.nc[2]; }
 inline int geom_ext_cells(const FrameGeom& g) { return g.nx[0] * g.nx[1] * g.nx[2]; }
 

@@ -31,7 +31,10 @@ struct CellWork {
   double* coord_ext = nullptr;       // [sum ext_cap, 3]
   int* type_ext = nullptr;
   int* map_ext = nullptr;
-  CellAtom* cell_atoms = nullptr;    // [sum ext_cap], extended cells' atoms in cell order
+  CellAtom* cell_atoms = nullptr;    // [sum ext_cap], extended cells' atoms in cell order, each
+                                     // cell's records grouped by sub-cell along the third cell
+                                     // vector (index order inside a sub-cell)
+  int* cell_sub = nullptr;           // [nx_tot, kAxisBins] optional: first record of each sub-cell
   int* cell_of = nullptr;            // [ntot]
   int* cell_slot = nullptr;          // [ntot] slot of the atom in its cell (counting order)
   int* cell_cnt = nullptr;           // [ncells_tot]; left zeroed by the build

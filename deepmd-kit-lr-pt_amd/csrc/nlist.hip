@@ -13,8 +13,13 @@
 // surrounding cells of the extended cell grid, or from a caller-given
 // candidate row) with ballot compaction into an LDS key array
 //   key = type << 56 | float_bits(r^2) << 24 | index      (index < 2^24)
-// which is bitonic-sorted in LDS; per-type first/end positions then map
-// slot s of type t to key[first[t] + s - sec[t]].
+// which is sorted in LDS (by rank up to 128 keys, else a bitonic network);
+// per-type first/end positions then map slot s of type t to
+// key[first[t] + s - sec[t]].  With one ghost layer (cells >= rc) the 27
+// cells are 9 runs of 3 consecutive cells whose records are grouped by
+// sub-cell along the third cell vector, so each run is cut to the sub-cells
+// that can hold pairs within rc (and dropped at a lateral distance >= rc):
+// about 3x the cutoff sphere's volume is tested instead of 7x.
 #include <type_traits>
 
 #include "dpmi_common.h"
@@ -101,15 +106,15 @@ __device__ __forceinline__ void wave_bitonic(unsigned long long (&kr)[KR]) {
   }
 }
 
-struct NoSlotWork {
-  __device__ void operator()(int, int) const {}
-};
-
-// Sorts keys[0, n) (LDS) and writes the formatted row; slot_work(s, j) runs
-// for every slot s as its entry j is written (same lane).
-template <int CAP, typename F = NoSlotWork>
-__device__ void sort_and_slot(unsigned long long* keys, int n, int* tfirst, int* tend, const int* sec, int ntypes,
-                              int nnei, int* __restrict__ row, F slot_work = F()) {
+// Sorts keys[0, n) (LDS) ascending and sets tfirst/tend[t] to the first and
+// end positions of type t's keys.  UNIQUE: no two keys are equal (one record
+// per extended atom); up to 128 such keys are placed by their rank (the
+// number of smaller keys: independent compares against broadcast LDS reads
+// instead of the network's chain of dependent shuffles; at 256 keys the
+// quadratic compare count costs more than the chain), larger or possibly
+// repeated sets go through a bitonic network.
+template <int CAP, bool UNIQUE>
+__device__ void sort_keys(unsigned long long* keys, int n, int* tfirst, int* tend, int ntypes) {
   const int lane = lane_id();
   if (n > CAP) n = CAP;
   int P = 64;
@@ -132,7 +137,32 @@ __device__ void sort_and_slot(unsigned long long* keys, int n, int* tfirst, int*
 #pragma unroll
     for (int r = 0; r < KR; ++r) keys[lane + kWave * r] = kr[r];
   };
-  if (P <= 64)
+  auto rank_in_registers = [&](auto kr_tag) {
+    constexpr int KR = decltype(kr_tag)::value;
+    unsigned long long kr[KR];
+    int rk[KR];
+#pragma unroll
+    for (int r = 0; r < KR; ++r) {
+      const int t = lane + kWave * r;
+      kr[r] = t < n ? keys[t] : ~0ull;
+      rk[r] = 0;
+    }
+#pragma unroll 4
+    for (int m = 0; m < n; ++m) {
+      const unsigned long long km = keys[m];
+#pragma unroll
+      for (int r = 0; r < KR; ++r) rk[r] += km < kr[r] ? 1 : 0;
+    }
+    __syncthreads();  // every read of the unsorted keys is done
+#pragma unroll
+    for (int r = 0; r < KR; ++r)
+      if (lane + kWave * r < n) keys[rk[r]] = kr[r];
+  };
+  if (UNIQUE && P <= 64)
+    rank_in_registers(std::integral_constant<int, 1>{});
+  else if (UNIQUE && P <= 128)
+    rank_in_registers(std::integral_constant<int, 2>{});
+  else if (P <= 64)
     sort_in_registers(std::integral_constant<int, 1>{});
   else if (P <= 128)
     sort_in_registers(std::integral_constant<int, 2>{});
@@ -168,6 +198,19 @@ __device__ void sort_and_slot(unsigned long long* keys, int n, int* tfirst, int*
     if (t == n - 1 || (int)(keys[t + 1] >> 56) != ty) tend[ty] = t + 1;
   }
   __syncthreads();
+}
+
+struct NoSlotWork {
+  __device__ void operator()(int, int) const {}
+};
+
+// Sorts keys[0, n) (LDS) and writes the formatted row; slot_work(s, j) runs
+// for every slot s as its entry j is written (same lane).
+template <int CAP, bool UNIQUE, typename F = NoSlotWork>
+__device__ void sort_and_slot(unsigned long long* keys, int n, int* tfirst, int* tend, const int* sec, int ntypes,
+                              int nnei, int* __restrict__ row, F slot_work = F()) {
+  const int lane = lane_id();
+  sort_keys<CAP, UNIQUE>(keys, n, tfirst, tend, ntypes);
   for (int s = lane; s < nnei; s += kWave) {
     int ty = 0;
     while (ty + 1 < ntypes && s >= sec[ty + 1]) ++ty;
@@ -179,13 +222,88 @@ __device__ void sort_and_slot(unsigned long long* keys, int n, int* tfirst, int*
   }
 }
 
-// Engine path: candidates from the extended cell grid.
+// Records of the run of extended cells (cx + dx, cy + dy, cz - 1 .. cz + 1)
+// (ghost layers of one cell) that can lie within sqrt(rc2) of the centre ci:
+// [*lo, *lo + *cnt) of the frame's cell-ordered records.  Each cell's records
+// are grouped by sub-cell of their position u along the third cell vector
+// (cell q of the extended grid spans [q, q + 1), sub-cell b of it
+// [q + b / kAxisBins, q + (b + 1) / kAxisBins)); sub[q * kAxisBins + b] is
+// the first record of sub-cell b of cell q.  Records before the window's
+// first sub-cell lie below u_i - h, records from the sub-cell after its last
+// one above u_i + h.  The window is widened by margins far above the
+// rounding of u and of the binning, so the records left out are ones that
+// fail the distance test.  Orthorhombic
+// boxes: h follows from the lateral gap to the run's cells (rc^2 - d0^2 -
+// d1^2); other boxes: h = rc and only runs at lateral gap >= rc are dropped.
+__device__ __forceinline__ void run_window(const FrameGeom& g, const double* ci, int cx, int cy, int cz, int dx,
+                                           int dy, double rc2, const int* __restrict__ xs,
+                                           const int* __restrict__ sub, int* lo_out, int* cnt_out) {
+#pragma clang fp contract(off)
+  const double* R = g.rec;
+  const double s0 = ci[0] * R[0] + ci[1] * R[1] + ci[2] * R[2];
+  const double s1 = ci[0] * R[3] + ci[1] * R[4] + ci[2] * R[5];
+  const double ui = geom_axis2_pos(g, ci[0], ci[1], ci[2]);
+  const double n0 = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+  const double n1 = sqrt(R[3] * R[3] + R[4] * R[4] + R[5] * R[5]);
+  const double n2 = sqrt(R[6] * R[6] + R[7] * R[7] + R[8] * R[8]);
+  // perpendicular gap (fractional, less a margin) between ci and the slab of
+  // the neighbouring cell layer along one axis
+  auto gap = [](int d, int c, int n, double s) {
+    const double f = d > 0 ? (double)(c + 1) / n - s : d < 0 ? s - (double)c / n : 0.;
+    return fmax(f - 1e-10, 0.);
+  };
+  const double d0 = gap(dx, cx, g.nc[0], s0) / n0;
+  const double d1 = gap(dy, cy, g.nc[1], s1) / n1;
+  const double* T = g.t;
+  const bool ortho = T[1] == 0. && T[2] == 0. && T[3] == 0. && T[5] == 0. && T[6] == 0. && T[7] == 0.;
+  const double rc2m = rc2 * (1. + 1e-9);
+  double h;
+  bool live;
+  if (ortho) {
+    const double rest = rc2m - d0 * d0 - d1 * d1;
+    live = rest > 0.;
+    h = sqrt(fmax(rest, 0.));
+  } else {
+    const double dm = fmax(d0, d1);
+    live = dm * dm < rc2m;
+    h = sqrt(rc2m);
+  }
+  const double hu = h * n2 * g.nc[2] + 1e-9;
+  const double ulo = ui - hu, uhi = ui + hu;
+  const double flo = floor(ulo), fhi = floor(uhi);
+  const int q0 = cz + g.ng[2] - 1, q2 = q0 + 2;
+  const int eb = ((cx + g.ng[0] + dx) * g.nx[1] + (cy + g.ng[1] + dy)) * g.nx[2];
+  int lo = 0, cnt = 0;
+  if (live && flo <= (double)q2 && fhi >= (double)q0) {
+    if (flo < (double)q0) {
+      lo = xs[eb + q0];
+    } else {
+      const int q = (int)flo;
+      const int b = min(max((int)((ulo - flo) * kAxisBins), 0), kAxisBins - 1);
+      lo = sub[(size_t)(eb + q) * kAxisBins + b];
+    }
+    int hi;
+    if (fhi > (double)q2) {
+      hi = xs[eb + q2 + 1];
+    } else {
+      const int q = (int)fhi;
+      const int b = min(max((int)((uhi - fhi) * kAxisBins), 0), kAxisBins - 1) + 1;
+      hi = b == kAxisBins ? xs[eb + q + 1] : sub[(size_t)(eb + q) * kAxisBins + b];
+    }
+    cnt = max(hi - lo, 0);
+  }
+  *lo_out = lo;
+  *cnt_out = cnt;
+}
+
+// Engine path: candidates from the extended cell grid.  At most 64 VGPRs
+// (eight waves per SIMD): the kernel waits on memory and LDS, not on issue.
 template <int CAP>
-__global__ void __launch_bounds__(64) k_cell_nlist(const int* __restrict__ frame_of, const FrameGeom* __restrict__ geom,
+__global__ void __launch_bounds__(64, 8) k_cell_nlist(const int* __restrict__ frame_of, const FrameGeom* __restrict__ geom,
                                                    int ntot, const double* __restrict__ coord_ext,
                                                    const int* __restrict__ type_ext, const int* __restrict__ cell_of,
                                                    const int* __restrict__ xstart, const CellAtom* __restrict__ cell_atoms,
-                                                   const int* __restrict__ sec_g, int ntypes, int nnei, double rc2,
+                                                   const int* __restrict__ cell_sub, const int* __restrict__ sec_g, int ntypes, int nnei, double rc2,
                                                    float rc2f, int* __restrict__ nlist, int* __restrict__ ncand,
                                                    int* __restrict__ flag, const double* __restrict__ avg,
                                                    const double* __restrict__ stdv, double rmin, double rmax,
@@ -237,7 +355,37 @@ __global__ void __launch_bounds__(64) k_cell_nlist(const int* __restrict__ frame
     n = wave_append<CAP>(keys, n, keep, key, flag);
   };
   const int nruns = (2 * r0 + 1) * w1;  // runs of w2 consecutive cells (z fastest)
-  if (ti >= 0 && nruns <= 32) {
+  if (ti >= 0 && cell_sub && r0 == 1 && r1 == 1 && r2 == 1) {
+    // 9 runs of 3 cells; lane r < 9 bounds run r to the records that can lie
+    // within rc (a window along the third cell vector, the whole run dropped
+    // when its lateral distance reaches rc), then the wave walks the
+    // windows' records as one list of total candidates, 64 at a time
+    int lo = 0, cnt = 0;
+    if (lane < 9) run_window(g, ci, cx, cy, cz, lane / 3 - 1, lane % 3 - 1, rc2, xstart + g.xcell_off,
+                             cell_sub + (size_t)g.xcell_off * kAxisBins, &lo, &cnt);
+    int inc = cnt;
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+      const int v = __shfl_up(inc, off, kWave);
+      if (lane >= off) inc += v;
+    }
+    const int pre = inc - cnt;  // the run's first position in the list
+    const int total = __builtin_amdgcn_readlane(inc, 8);
+    const int dl = lo - pre;    // record of position p of the run: p + dl
+    int rp[9], rd[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+      rp[r] = __builtin_amdgcn_readlane(pre, r);
+      rd[r] = __builtin_amdgcn_readlane(dl, r);
+    }
+    for (int p0 = 0; p0 < total; p0 += kWave) {
+      const int p = p0 + lane;
+      int slot = p + rd[0];
+#pragma unroll
+      for (int r = 1; r < 9; ++r) slot = p >= rp[r] ? p + rd[r] : slot;  // empty runs are overtaken
+      consider(p < total, slot);
+    }
+  } else if (ti >= 0 && nruns <= 32) {
     // each run's records are contiguous: 64 / nruns lanes walk each run
     // (9 runs of 3 cells: 7 lanes per run), no search for a candidate's cell
     const int lpr = kWave / nruns, run = lane / lpr, sub = lane - run * lpr;
@@ -300,17 +448,17 @@ __global__ void __launch_bounds__(64) k_cell_nlist(const int* __restrict__ frame
   int* row = nlist + (size_t)gi * nnei;
   // environment-matrix rows of the slots as the lane formats them
   if (em_f) {
-    sort_and_slot<CAP>(keys, n, tfirst, tend, sec, ntypes, nnei, row, [&](int s, int j) {
+    sort_and_slot<CAP, true>(keys, n, tfirst, tend, sec, ntypes, nnei, row, [&](int s, int j) {
       envc::slot_em<float>(ci, coord_ext + 3 * (eo + (j >= 0 ? j : i)), j, ti, s, nnei, avg, stdv, rmin, rmax,
                            em_f + ((size_t)gi * nnei + s) * 4);
     });
   } else if (em_d) {
-    sort_and_slot<CAP>(keys, n, tfirst, tend, sec, ntypes, nnei, row, [&](int s, int j) {
+    sort_and_slot<CAP, true>(keys, n, tfirst, tend, sec, ntypes, nnei, row, [&](int s, int j) {
       envc::slot_em<double>(ci, coord_ext + 3 * (eo + (j >= 0 ? j : i)), j, ti, s, nnei, avg, stdv, rmin, rmax,
                             em_d + ((size_t)gi * nnei + s) * 4);
     });
   } else {
-    sort_and_slot<CAP>(keys, n, tfirst, tend, sec, ntypes, nnei, row);
+    sort_and_slot<CAP, true>(keys, n, tfirst, tend, sec, ntypes, nnei, row);
   }
 }
 
@@ -374,7 +522,7 @@ __global__ void __launch_bounds__(64) k_row_format(int inum, const int* __restri
     n = wave_append<CAP>(keys, n, keep, key, flag);
   }
   __syncthreads();
-  sort_and_slot<CAP>(keys, n, tfirst, tend, sec, ntypes, nnei, nlist + ((size_t)f * nloc + i) * nnei);
+  sort_and_slot<CAP, false>(keys, n, tfirst, tend, sec, ntypes, nnei, nlist + ((size_t)f * nloc + i) * nnei);
 }
 
 // Brute-force candidate rows in ascending j (the build_nlist contract):
@@ -435,7 +583,7 @@ int nlist_row_capacity(int want) {
 void launch_cell_nlist(const CellWork& w, const NlistArgs& a, hipStream_t st) {
   DPMI_CAP_DISPATCH(a.cap, hipLaunchKernelGGL(k_cell_nlist<CAP>, dim3(w.ntot), dim3(kWave), 0, st, w.frame_of,
                                               w.geom, w.ntot, w.coord_ext, w.type_ext, w.cell_of, w.xstart,
-                                              w.cell_atoms, a.sec, a.ntypes, a.nnei, a.rc2, a.rc2f, a.nlist,
+                                              w.cell_atoms, w.cell_sub, a.sec, a.ntypes, a.nnei, a.rc2, a.rc2f, a.nlist,
                                               a.ncand, w.flag, a.avg, a.stdv, a.rmin, a.rmax, a.em_f, a.em_d));
   DPMI_LAUNCH_CHECK();
 }

@@ -8,6 +8,10 @@
 * types and indices outside the sort-key range: DPMI_ERR_RANGE from the
   kernel-level entry, DPMI_ERR_ARGUMENT from the engine;
 * a triclinic box through the whole path;
+* lattice planes on the cell faces and a neighbour shell at the cutoff: the
+  neighbour walk's per-run windows keep every pair the distance test keeps;
+* more candidates per centre than the initial key capacity: the capacity
+  grows and the step is retried;
 * capacity growth: an engine whose ghost capacity is too small recovers
   and returns the oracle's result; a captured graph is dropped when a later
   step changes the box.
@@ -153,6 +157,40 @@ def test_triclinic_full_path(nmol):
     em, e_m, f_m, v_m = _run(m, coord, atype, box, "mixed")
     assert abs(e_m[0] - e[0]) / len(atype) * 1e3 < 0.1
     assert np.sqrt(np.mean((f_m - f) ** 2)) * 1e3 < 1.0
+
+
+@pytest.mark.parametrize("jitter", [0.0, 1e-12])
+def test_lattice_on_cell_faces_and_cutoff_shell(jitter):
+    """fcc lattice with a = rc / sqrt(5): its tenth neighbour shell lies at the
+    cutoff, and with 10 unit cells per edge the cells are 2.5 lattice
+    constants wide, so lattice planes lie on the cell faces.  Rows equal the
+    oracle's bit for bit (many exactly equal distances and positions)."""
+    print(f"case fcc lattice on cell faces jitter={jitter}", flush=True)
+    m = dmodel.copper_model(seed=2)
+    a = 8.0 / np.sqrt(5.0)
+    coord, atype, box = systems.fcc_copper(10, seed=3, sigma=jitter, a=a)
+    eng, e, f, v = _run(m, coord, atype, box)
+    ref = evaluate(m, coord, atype, box, oracle_lib(), keep=True)
+    assert np.array_equal(eng.fetch("nlist").reshape(ref.nlist.shape), ref.nlist)
+    assert abs(e[0] - ref.energy) / abs(ref.energy) < 1e-10
+    scale = max(float(np.max(np.abs(ref.force))), 1.0)
+    assert float(np.max(np.abs(f - ref.force))) < 1e-10 * scale
+    em, e_m, f_m, v_m = _run(m, coord, atype, box, "mixed")
+    assert np.array_equal(em.fetch("nlist"), eng.fetch("nlist"))
+
+
+def test_candidate_capacity_growth():
+    """sel [10, 20] and rcut 8: about 210 candidates per centre against an
+    initial key capacity of 128.  The step grows the capacity, is retried and
+    returns the oracle's truncated rows and E/F/V."""
+    print("case key capacity growth", flush=True)
+    m = SeAModel.random(2, [10, 20], 8.0, 0.5, [25, 50, 100], 16, [240, 240, 240], seed=3)
+    m.build_tables(0.5)
+    coord, atype, box = systems.water(216, seed=8)
+    eng, e, f, v = _run(m, coord, atype, box)
+    assert int(eng.fetch("row_capacity")[0]) >= 256
+    assert int(eng.fetch("regrows")[0]) >= 1
+    _vs_oracle(m, coord, atype, box, eng, e, f, v)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
