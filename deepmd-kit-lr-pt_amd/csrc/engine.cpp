@@ -526,6 +526,12 @@ SeaEngine::SeaEngine(const SeaModel* m, int nframes_, const int* natoms_h, const
     d_Dh = dalloc<_Float16>((size_t)ntot * k0);
     DPMI_CHECK(hipMemset(d_Dh, 0, sizeof(_Float16) * (size_t)ntot * k0));
     d_gD_f = dalloc<float>((size_t)ntot * k0);
+    if (m->fused && m->nlayers - 1 >= 2) {
+      const size_t nx1 = (size_t)ntot * m->kpad[1];
+      d_fx1 = dalloc<_Float16>(nx1);
+      d_fdv1 = dalloc<_Float16>(nx1);
+      d_fgz0 = dalloc<_Float16>(nx1);
+    }
     d_eperm = dalloc<float>(ntot);
     int kmax = 0;
     for (int l = 0; l + 1 < m->nlayers; ++l) {
@@ -537,14 +543,18 @@ SeaEngine::SeaEngine(const SeaModel* m, int nframes_, const int* natoms_h, const
     d_ga = dalloc<float>((size_t)ntot * kmax);
     d_gb = dalloc<float>((size_t)ntot * kmax);
     const auto chip = chip_layout();
+    nxcd = chip.first;
     const std::vector<int4> tiles = fitting_tile_order(tstart, nt, 64, chip.first, chip.second);
     ntiles = (int)tiles.size();
     d_tiles = dupload(tiles.data(), tiles.size());
     const std::vector<int4> tiles32 = fitting_tile_order(tstart, nt, 32, chip.first, chip.second);
     ntiles32 = (int)tiles32.size();
     d_tiles32 = dupload(tiles32.data(), tiles32.size());
-    // 64-row tiles unless they would leave compute units without a tile
+    // 64-row tiles unless they would leave compute units without a tile;
+    // when even 32-row tiles leave compute units idle, layer 0's products
+    // spread over more workgroups in their own kernels
     fit_rows_default = ntiles < chip.second ? 32 : 64;
+    if (ntiles32 < chip.second && d_fx1) fit_rows_default = 32 | kFitSplit;
     // embedding tiles: groups of kEmbedTileAtoms descriptor positions of one
     // type (atoms of negative type last, as type -1)
     std::vector<int4> et;
@@ -601,6 +611,7 @@ SeaEngine::~SeaEngine() {
   for (auto* p : {&d_em_f, &d_em_deriv_f, &d_rij_f, &d_net_deriv_f, &d_xyz_f, &d_gD_f, &d_eperm, &d_ga, &d_gb})
     dfree(*p);
   dfree(d_Dh);
+  for (auto* p : {&d_fx1, &d_fdv1, &d_fgz0}) dfree(*p);
   dfree(d_tiles);
   dfree(d_tiles32);
   dfree(d_etiles);
@@ -1122,7 +1133,9 @@ void SeaEngine::compute_mixed_descriptor_fitting(hipStream_t st, int& stage) {
     fa.ldD = k0;
     fa.gD = d_gD_f;
     fa.ldG = k0;
-    const int rows = tune.fit_rows ? tune.fit_rows : fit_rows_default;
+    const int knob = tune.fit_rows ? tune.fit_rows : fit_rows_default;
+    const int rows = knob & 255;
+    const bool split = (knob & kFitSplit) && d_fx1;
     fa.tiles = rows == 32 ? d_tiles32 : d_tiles;
     fa.layers = m.d_fused;
     fa.perm = d_perm;
@@ -1131,7 +1144,19 @@ void SeaEngine::compute_mixed_descriptor_fitting(hipStream_t st, int& stage) {
     fa.H = H;
     // atoms of negative type own no tile: their energy is zero
     if (tstart[m.ntypes + 1] > tstart[m.ntypes]) launch_fill_double(d_atom_e, 0.0, ntot, st);
-    launch_fit_fused(fa, rows == 32 ? ntiles32 : ntiles, rows, st);
+    if (split) {
+      fa.x1 = d_fx1;
+      fa.dv1 = d_fdv1;
+      fa.gz0 = d_fgz0;
+      fa.ldX = m.kpad[1];
+      fa.nxcd = nxcd;
+      FusedLayer l0;
+      l0.K = k0;
+      l0.N = m.kpad[1];
+      launch_fit_split(fa, rows == 32 ? ntiles32 : ntiles, rows, l0, st);
+    } else {
+      launch_fit_fused(fa, rows == 32 ? ntiles32 : ntiles, rows, st);
+    }
     mark(st, stage++);
     adjoint();
     mark(st, stage++);
@@ -1492,7 +1517,7 @@ void SeaEngine::load_cached_tune() {
 // result is cached for the size bucket.
 void SeaEngine::autotune(const double* coord, const double* box_h, int reps, hipStream_t st) {
   const int env_c[3] = {128, 256, 512}, fwd_c[3] = {1, 2, 4}, bwd_c[3] = {1, 2, 4}, frc_c[3] = {8, 32, 64},
-            fit_c[3] = {32, 64, 64};
+            fit_c[3] = {32 | kFitSplit, 64, 32};
   // candidates 0..2: per-neighbour embedding kernels with the shapes above;
   // then, in the mixed engine, the table-space embedding kernels (mask 3)
   // and the row-run forward and adjoint (mask 12), whose launch shapes are

@@ -176,6 +176,10 @@ struct SeaEngine {
   float *d_em_f = nullptr, *d_em_deriv_f = nullptr, *d_rij_f = nullptr, *d_net_deriv_f = nullptr;
   float *d_xyz_f = nullptr, *d_gD_f = nullptr, *d_eperm = nullptr, *d_ga = nullptr, *d_gb = nullptr;
   _Float16* d_Dh = nullptr;
+  // layer 0 of the fitting in its own kernels: x_1, idt act'(z_0), dE/dz_0
+  // [ntot, kpad[1]] FP16 (models with at least two hidden layers)
+  _Float16 *d_fx1 = nullptr, *d_fdv1 = nullptr, *d_fgz0 = nullptr;
+  int nxcd = 1;             // XCDs of the device (workgroups dealt round-robin)
   int4* d_tiles = nullptr;  // fused fitting tiles of 64 rows
   int ntiles = 0;
   int4* d_tiles32 = nullptr;  // and of 32 rows

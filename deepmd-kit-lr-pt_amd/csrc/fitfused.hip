@@ -32,6 +32,9 @@
 // as possible.
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
+#include <stdexcept>
+
 #include "dpmi_common.h"
 #include "kernels.h"
 
@@ -149,7 +152,9 @@ __device__ __forceinline__ void zero_acc(floatx4 (&acc)[RT][2]) {
     for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 }
 
-template <int H, int RT>
+// SPLIT: layer 0's products run in k_fit_layer0 / k_fit_dgrad; this kernel
+// reads x_1 and idt_0 act'(z_0) and writes dE/dz_0 instead.
+template <int H, int RT, bool SPLIT = false>
 __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
   constexpr int kRows = 16 * RT;              // rows per workgroup tile
   constexpr int kFr = kKC * RT / kWaves;      // descriptor fragments per wave and chunk
@@ -181,6 +186,34 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
     const int KT = L.K / 32, NT = L.N / 16, ntw = (NT + kWaves - 1) / kWaves;
     const int ct0 = wave * ntw;
     const int src = (l + 1) & 1, dst = l & 1;  // layer l >= 1 reads xs[src]
+    if (SPLIT && l == 0) {
+      // x_1 rows into LDS (rows past the tile: zeros), the activation
+      // derivative at this wave's positions into registers
+      const int N8 = L.N / 8;
+      for (int q = tid; q < kRows * N8; q += kWaves * kWave) {
+        const int r = q / N8, c8 = q - r * N8;
+        const half8 v = r < nrows ? *reinterpret_cast<const half8*>(p.x1 + (size_t)(row0 + r) * p.ldX + c8 * 8)
+                                  : zero8();
+        *reinterpret_cast<half8*>(&xs[dst][r][c8 * 8]) = v;
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int n = min(ct0 + j, NT - 1) * 16 + lr;
+#pragma unroll
+        for (int i = 0; i < RT; ++i) {
+          _Float16 d[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int r = i * 16 + lg * 4 + e;
+            d[e] = r < nrows ? p.dv1[(size_t)(row0 + r) * p.ldX + n] : (_Float16)0.f;
+          }
+          dpk[l][i][j][0] = half2v{d[0], d[1]};
+          dpk[l][i][j][1] = half2v{d[2], d[3]};
+        }
+      }
+      __syncthreads();
+      continue;
+    }
     floatx4 acc[RT][2];
     zero_acc(acc);
     if (l == 0) {
@@ -337,9 +370,16 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const float d = (float)dpk[l][i][j][e >> 1][e & 1];
-          xs[buf][i * 16 + lg * 4 + e][n] = (_Float16)(g[i][j][e] * d);
+          const _Float16 gzv = (_Float16)(g[i][j][e] * d);
+          const int r = i * 16 + lg * 4 + e;
+          if (SPLIT && l == 0) {
+            if (r < nrows) p.gz0[(size_t)(row0 + r) * p.ldX + n] = gzv;
+          } else {
+            xs[buf][r][n] = gzv;
+          }
         }
     }
+    if (SPLIT && l == 0) return;  // the descriptor gradient is k_fit_dgrad's
     __syncthreads();
     const int KT = L.N / 32;  // reduction steps
     const int OT = L.K / 16;  // output column tiles (input width of the layer)
@@ -381,6 +421,165 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
   }
 }
 
+// Workgroup b of a launch over (row tile, column group) pairs: workgroups
+// are dealt round-robin to the nx XCDs, and the groups of one tile go to the
+// XCD that the tile order gives it, one after another (the tile's rows are
+// read from that XCD's L2 after the first group).
+__device__ __forceinline__ void tile_group(int b, int nx, int ngroups, int* tix, int* cg) {
+  const int xi = b % nx, k = b / nx;
+  *cg = k % ngroups;
+  *tix = (k / ngroups) * nx + xi;
+}
+
+// Layer 0 forward as a product tiled over rows and output columns: a
+// workgroup of 4 waves takes one row tile and 4 column tiles (wave w: column
+// tile 4 cg + w, every row of the tile), so it streams a quarter of W_0
+// instead of all of it.  The descriptor rows stage through LDS as in the
+// fused kernel; the epilogue writes x_1 = act(z) idt and idt act'(z) (FP16).
+// Per output the products run in the fused kernel's order (k ascending, hi
+// then lo), so the results are the same bits.
+template <int RT>
+__global__ void __launch_bounds__(256) k_fit_layer0(FusedFitArgs p, int ntiles, int ngroups) {
+  constexpr int kW = 4;
+  constexpr int kFr = kKC * RT / kW;
+  static_assert(kFr * kW == kKC * RT, "a chunk's fragments are shared evenly by the waves");
+  __shared__ half8 afr[2][kKC * RT][kWave];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  int tix, cg;
+  tile_group(blockIdx.x, p.nxcd, ngroups, &tix, &cg);
+  if (tix >= ntiles) return;
+  const int4 ti = p.tiles[tix];
+  const int t = ti.x, row0 = ti.y, nrows = ti.z;
+  if (nrows <= 0) return;
+  const FusedLayer L = p.layers[(size_t)t * p.H];
+  const int KT = L.K / 32, NT = L.N / 16, NC = KT / kKC;
+  const int lr = lane & 15, lg = lane >> 4;
+  const int ct = cg * kW + wave;
+  const bool use = ct < NT;
+  const int c0 = min(ct, NT - 1);
+  typedef __attribute__((address_space(1))) const half8 gh8;
+  const _Float16* dsrc[kFr];
+  bool dval[kFr];
+#pragma unroll
+  for (int m = 0; m < kFr; ++m) {
+    const int f = wave + kW * m, r = (f % RT) * 16 + lr;
+    dval[m] = r < nrows;
+    dsrc[m] = p.D + (size_t)(row0 + (dval[m] ? r : 0)) * p.ldD + (f / RT) * 32 + lg * 8;
+  }
+  floatx4 acc[RT];
+#pragma unroll
+  for (int i = 0; i < RT; ++i) acc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
+  // descriptor chunks in a ring of three register sets: chunk c + 1 is
+  // requested two chunks before it is stored to LDS (at the end of chunk c)
+  half8 ar0[kFr], ar1[kFr], ar2[kFr];
+  half8 bq[kPf], bl[kPf];
+  auto dchunk = [&](half8(&r)[kFr], int c) {
+    const int cc = min(c, NC - 1);
+#pragma unroll
+    for (int m = 0; m < kFr; ++m) r[m] = *(gh8*)(dsrc[m] + cc * kKC * 32);
+  };
+  dchunk(ar0, 0);
+  dchunk(ar1, 1);
+  dchunk(ar2, 2);
+#pragma unroll
+  for (int s = 0; s < kPf; ++s) {
+    bq[s] = load_bfrag(L.wf, s, NT, c0, lane);
+    bl[s] = load_bfrag(L.wf_lo, s, NT, c0, lane);
+  }
+#pragma unroll
+  for (int m = 0; m < kFr; ++m) afr[0][wave + kW * m][lane] = dval[m] ? ar0[m] : zero8();
+  __syncthreads();
+  for (int c = 0; c < NC; ++c) {
+    dchunk(ar0, c + 3);
+#pragma unroll
+    for (int s = 0; s < kKC; ++s) {
+      const int k = c * kKC + s;
+      if (use)
+#pragma unroll
+        for (int i = 0; i < RT; ++i) {
+          const half8 a = afr[c & 1][s * RT + i][lane];
+          acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bq[s], acc[i], 0, 0, 0);
+          acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bl[s], acc[i], 0, 0, 0);
+        }
+      const int kn = min(k + kPf, KT - 1);
+      bq[s] = load_bfrag(L.wf, kn, NT, c0, lane);
+      bl[s] = load_bfrag(L.wf_lo, kn, NT, c0, lane);
+    }
+    if (c + 1 < NC)
+#pragma unroll
+      for (int m = 0; m < kFr; ++m) afr[(c + 1) & 1][wave + kW * m][lane] = dval[m] ? ar1[m] : zero8();
+#pragma unroll
+    for (int m = 0; m < kFr; ++m) {
+      ar1[m] = ar2[m];
+      ar2[m] = ar0[m];
+    }
+    __syncthreads();
+  }
+  if (!use) return;
+  const int n = ct * 16 + lr;
+  const float bn = L.b[n];
+  const float idn = L.idt ? L.idt[n] : 1.f;
+#pragma unroll
+  for (int i = 0; i < RT; ++i)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int r = i * 16 + lg * 4 + e;
+      if (r >= nrows) continue;
+      const float z = acc[i][e] + bn;
+      const float y = L.act ? tanhf(z) : z;
+      const size_t o = (size_t)(row0 + r) * p.ldX + n;
+      p.x1[o] = (_Float16)(y * idn);
+      p.dv1[o] = (_Float16)(L.act ? idn * (1.f - y * y) : idn);
+    }
+}
+
+// Layer 0 backward, dE/dD = dE/dz_0 W_0^T, tiled over rows and output
+// columns: a workgroup of 8 waves takes one row tile and 16 column tiles of
+// the descriptor (wave w: column tiles 16 cg + 2 w, + 1); dE/dz_0 rows stage
+// through LDS.  Same product order as the fused kernel's descriptor gradient.
+template <int RT>
+__global__ void __launch_bounds__(512) k_fit_dgrad(FusedFitArgs p, int ntiles, int ngroups) {
+  constexpr int kRows = 16 * RT;
+  __shared__ _Float16 xs[kRows][kXld];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  int tix, cg;
+  tile_group(blockIdx.x, p.nxcd, ngroups, &tix, &cg);
+  if (tix >= ntiles) return;
+  const int4 ti = p.tiles[tix];
+  const int t = ti.x, row0 = ti.y, nrows = ti.z;
+  if (nrows <= 0) return;
+  const FusedLayer L = p.layers[(size_t)t * p.H];
+  const int N8 = L.N / 8;
+  for (int q = tid; q < kRows * N8; q += kWaves * kWave) {
+    const int r = q / N8, c8 = q - r * N8;
+    const half8 v =
+        r < nrows ? *reinterpret_cast<const half8*>(p.gz0 + (size_t)(row0 + r) * p.ldX + c8 * 8) : zero8();
+    *reinterpret_cast<half8*>(&xs[r][c8 * 8]) = v;
+  }
+  __syncthreads();
+  const int KT = L.N / 32, OT = L.K / 16;
+  const int ot0 = (cg * kWaves + wave) * 2;
+  if (ot0 >= OT) return;
+  const int lr = lane & 15, lg = lane >> 4;
+  floatx4 acc[RT][2];
+  zero_acc(acc);
+  gemm_lds_a<RT, false>(acc, &xs[0][0], kXld, L.wb, L.wb, KT, OT, ot0, 2, lane);
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    if (ot0 + j >= OT) continue;
+    const int col = (ot0 + j) * 16 + lr;
+#pragma unroll
+    for (int i = 0; i < RT; ++i)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int r = i * 16 + lg * 4 + e;
+        if (r < nrows) __builtin_nontemporal_store(acc[i][j][e], p.gD + (size_t)(row0 + r) * p.ldG + col);
+      }
+  }
+}
+
 }  // namespace
 
 template <int RT>
@@ -400,6 +599,33 @@ void launch_fit_fused(const FusedFitArgs& a, int ntiles, int rows_per_tile, hipS
     launch_fit_fused_rt<4>(a, ntiles, st);
   else if (rows_per_tile == 32)
     launch_fit_fused_rt<2>(a, ntiles, st);
+  else
+    throw std::invalid_argument("fused fitting tiles hold 32 or 64 rows");
+  DPMI_LAUNCH_CHECK();
+}
+
+template <int RT>
+void launch_fit_split_rt(const FusedFitArgs& a, int ntiles, const FusedLayer& l0, hipStream_t st) {
+  const int nx = std::max(a.nxcd, 1);
+  const int rounds = (ntiles + nx - 1) / nx;  // tiles per XCD
+  const int g1 = (l0.N / 16 + 3) / 4, g3 = (l0.K / 16 + 2 * kWaves - 1) / (2 * kWaves);
+  hipLaunchKernelGGL((k_fit_layer0<RT>), dim3(rounds * nx * g1), dim3(256), 0, st, a, ntiles, g1);
+  switch (a.H) {
+    case 2: hipLaunchKernelGGL((k_fit_fused<2, RT, true>), dim3(ntiles), dim3(512), 0, st, a); break;
+    case 3: hipLaunchKernelGGL((k_fit_fused<3, RT, true>), dim3(ntiles), dim3(512), 0, st, a); break;
+    case 4: hipLaunchKernelGGL((k_fit_fused<4, RT, true>), dim3(ntiles), dim3(512), 0, st, a); break;
+    default: throw std::invalid_argument("split fitting supports 2 to 4 hidden layers");
+  }
+  hipLaunchKernelGGL((k_fit_dgrad<RT>), dim3(rounds * nx * g3), dim3(512), 0, st, a, ntiles, g3);
+}
+
+void launch_fit_split(const FusedFitArgs& a, int ntiles, int rows_per_tile, const FusedLayer& layer0_host,
+                      hipStream_t st) {
+  if (ntiles <= 0) return;
+  if (rows_per_tile == 64)
+    launch_fit_split_rt<4>(a, ntiles, layer0_host, st);
+  else if (rows_per_tile == 32)
+    launch_fit_split_rt<2>(a, ntiles, layer0_host, st);
   else
     throw std::invalid_argument("fused fitting tiles hold 32 or 64 rows");
   DPMI_LAUNCH_CHECK();

@@ -346,6 +346,7 @@ bool force_recompute_supported(int nnei);
 
 // Launch-shape knobs of the engine stages, picked per size bucket by
 // SeaEngine::autotune.
+constexpr int kFitSplit = 256;
 struct Tune {
   int env_tpb = 256;
   int fwd_waves = 4;
@@ -358,7 +359,9 @@ struct Tune {
   // atom per wavefront; takes precedence over bit 1); bit 3 the row-run
   // forward on the matrix cores (precedence over bit 0)
   int embed_mfma = 0;
-  int fit_rows = 0;  // fused fitting rows per workgroup tile (32 or 64; 0: chosen from the size)
+  // fused fitting rows per workgroup tile (32 or 64; 0: chosen from the
+  // size), plus kFitSplit: layer 0's products in their own kernels
+  int fit_rows = 0;
 };
 void launch_force_virial_rows(const ForceRowsArgs& a, hipStream_t st);
 // force[loc] += force_ext[e] for every extended atom e < nall of each frame
@@ -474,9 +477,24 @@ struct FusedFitArgs {
   double* atom_e = nullptr;     // [atoms] out
   const double* shift = nullptr;  // [ntypes] output bias + bias_atom_e
   int H = 0;                    // hidden layers, 1..4
+  // layer 0 in its own kernels (launch_fit_split): x_1 and idt_0 act'(z_0)
+  // from the forward product, dE/dz_0 into the backward product, [rows, ldX]
+  // FP16 each; nxcd: XCDs the workgroups are dealt to round-robin
+  _Float16* x1 = nullptr;
+  _Float16* dv1 = nullptr;
+  _Float16* gz0 = nullptr;
+  int ldX = 0;
+  int nxcd = 1;
 };
 // tiles of rows_per_tile (32 or 64) descriptor rows
 void launch_fit_fused(const FusedFitArgs& a, int ntiles, int rows_per_tile, hipStream_t st);
+// the same results (bit for bit) from three launches: layer 0 forward as a
+// product tiled over rows and output columns, the other layers fused per row
+// tile, layer 0 backward (dE/dD) tiled over rows and output columns; needs
+// H >= 2 and a.x1/dv1/gz0
+// (layer0_host: the layer-0 widths K, N of the launch grid)
+void launch_fit_split(const FusedFitArgs& a, int ntiles, int rows_per_tile, const FusedLayer& layer0_host,
+                      hipStream_t st);
 // host: Bt [N][K] (row-major) -> MFMA B-fragment order (N % 16 == 0, K % 32 == 0)
 void swizzle_bfrags(const _Float16* bt, int N, int K, _Float16* out);
 

@@ -321,7 +321,9 @@ int dpmi_sea_engine_graph_capture_device_box(dpmi_sea_engine* engine, const doub
  * table space on the matrix cores, bit 2 adjoint and bit 3 forward from
  * each atom's table-row runs on the matrix cores (precedence over bits 1
  * and 0); else the per-neighbour kernels), fused
- * fitting rows per tile (32 or 64; 0 = chosen from the atom count)}.
+ * fitting rows per tile (32 or 64; 0 = chosen from the atom count; plus 256:
+ * layer 0's products in their own kernels tiled over rows and columns, the
+ * same bits, faster when the row tiles leave compute units idle)}.
  * dpmi_sea_engine_set_tuning sets them (a knob <= 0, or < 0 for the mask,
  * keeps its value; out-of-range values give DPMI_ERR_ARGUMENT). */
 int dpmi_sea_engine_autotune(dpmi_sea_engine* engine, const double* coord, const double* box_host, int reps,

@@ -262,18 +262,21 @@ def test_mixed_rejects_width_doubling_residual():
         Engine(dm, [len(atype)], atype, precision="mixed")
 
 
-def test_fitting_tile_rows_give_identical_rows(water_dev):
-    """Fused fitting tiles of 32 and 64 rows compute every row with the same
-    arithmetic, so atomic energies and net_deriv agree bit for bit."""
-    coord, atype, box = systems.water(100, seed=13)
+@pytest.mark.parametrize("nmol", [100, 1500])
+def test_fitting_tile_rows_give_identical_rows(water_dev, nmol):
+    """Fused fitting tiles of 32 and 64 rows, and layer 0 in its own kernels
+    (+256), compute every row with the same arithmetic in the same order, so
+    atomic energies, net_deriv, energy and virial agree bit for bit."""
+    coord, atype, box = systems.water(nmol, seed=13)
     n = len(atype)
     out = {}
-    for rows in (32, 64):
+    for rows in (32, 64, 32 + 256, 64 + 256):
         eng = Engine(water_dev, [n], atype, precision="mixed")
         eng.set_tuning(fitting_rows_per_tile=rows)
         assert eng.tuning()["fitting_rows_per_tile"] == rows
-        eng.compute(torch.as_tensor(coord, device="cuda"), box, atom_energy=True)
+        e, f, v = eng.compute(torch.as_tensor(coord, device="cuda"), box, atom_energy=True)
         eng.status()
-        out[rows] = (eng.fetch("atom_energy"), eng.fetch("net_deriv"))
-    assert np.array_equal(out[32][0], out[64][0])
-    assert np.array_equal(out[32][1], out[64][1])
+        out[rows] = (eng.fetch("atom_energy"), eng.fetch("net_deriv"), e.cpu().numpy(), v.cpu().numpy())
+    for rows in (64, 32 + 256, 64 + 256):
+        for a, b in zip(out[32], out[rows]):
+            assert np.array_equal(a, b), rows
