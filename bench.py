@@ -84,9 +84,9 @@ def stage_model(m, nloc, nall, precision):
 # readings and the commands are in TRAFFIC_PROFILE.
 MEASURED_TRAFFIC_MIB = {
     "mixed": {"embed_bwd": (123.05, 26.19), "embed_fwd": (31.32, 54.89), "fitting": (61.83, 93.03),
-              "force_virial": (40.09, 19.78), "nlist_env": (13.17, 33.59)},
+              "force_virial": (40.09, 19.78), "nlist_env": (13.4, 33.0)},
     "fp64": {"embed_bwd": (261.61, 50.70), "embed_fwd": (85.80, 202.84), "force_virial": (60.19, 11.39),
-             "nlist_env": (15.83, 58.91)},
+             "nlist_env": (14.9, 58.9)},
 }
 
 

@@ -273,11 +273,14 @@ __device__ __forceinline__ int hash_slot(int* keys, int key) {
 
 // Bounded probe for row groups that may hold more distinct keys than the
 // table: -1 after kProbeMax occupied slots (the caller then adds straight
-// into global memory).
+// into global memory).  The loop stays rolled: unrolled, every probe's
+// divergent exit keeps an exec mask in scalar registers, and the kernel
+// spilled them.
 constexpr int kProbeMax = 32;
 template <int HB>
 __device__ __forceinline__ int hash_slot_bounded(int* keys, int key) {
   unsigned h = ((unsigned)key * 2654435761u) >> (32 - HB);
+#pragma unroll 1
   for (int probe = 0; probe < kProbeMax; ++probe) {
     const int prev = atomicCAS(&keys[h], -1, key);
     if (prev == -1 || prev == key) return (int)h;
