@@ -626,9 +626,12 @@ int dpmi_sea_engine_set_tuning(dpmi_sea_engine* engine, const int* knobs_host) {
       t.embed_mfma = k[4];
     }
     if (k[5] > 0) {
-      const int rows = k[5] & ~dpmi::kFitSplit;
-      if (rows != 32 && rows != 64)
-        throw std::invalid_argument("fused fitting rows per tile: 32 or 64, plus 256 for layer 0 in its own kernels");
+      const int rows = k[5] & ~(dpmi::kFitSplit | dpmi::kFitWide);
+      if ((rows != 32 && rows != 64) || ((k[5] & dpmi::kFitSplit) && (k[5] & dpmi::kFitWide)) ||
+          ((k[5] & dpmi::kFitWide) && rows != 64))
+        throw std::invalid_argument(
+            "fused fitting rows per tile: 32 or 64, plus 256 for layer 0 in its own kernels, or 64 + 512 for 16 "
+            "waves per tile");
       t.fit_rows = k[5];
     }
     return kOk;

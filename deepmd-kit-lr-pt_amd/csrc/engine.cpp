@@ -1155,7 +1155,7 @@ void SeaEngine::compute_mixed_descriptor_fitting(hipStream_t st, int& stage) {
       l0.N = m.kpad[1];
       launch_fit_split(fa, rows == 32 ? ntiles32 : ntiles, rows, l0, st);
     } else {
-      launch_fit_fused(fa, rows == 32 ? ntiles32 : ntiles, rows, st);
+      launch_fit_fused(fa, rows == 32 ? ntiles32 : ntiles, rows, st, rows == 64 && (knob & kFitWide));
     }
     mark(st, stage++);
     adjoint();
@@ -1517,7 +1517,7 @@ void SeaEngine::load_cached_tune() {
 // result is cached for the size bucket.
 void SeaEngine::autotune(const double* coord, const double* box_h, int reps, hipStream_t st) {
   const int env_c[3] = {128, 256, 512}, fwd_c[3] = {1, 2, 4}, bwd_c[3] = {1, 2, 4}, frc_c[3] = {8, 32, 64},
-            fit_c[3] = {32 | kFitSplit, 64, 32};
+            fit_c[3] = {32 | kFitSplit, 64 | kFitWide, 32};
   // candidates 0..2: per-neighbour embedding kernels with the shapes above;
   // then, in the mixed engine, the table-space embedding kernels (mask 3)
   // and the row-run forward and adjoint (mask 12), whose launch shapes are

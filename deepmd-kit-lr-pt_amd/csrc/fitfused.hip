@@ -76,8 +76,8 @@ __device__ __forceinline__ half8 load_bfrag(const _Float16* __restrict__ w, int 
 // ct0 + j (used if < NT).  KT is a multiple of kPf.  Loads are
 // unconditional (clamped tile and step indices) so that the prefetch ring is
 // not serialised by the wait counters.
-template <int RT, bool LO, int KTC>
-__device__ __forceinline__ void gemm_lds_a_kt(floatx4 (&acc)[RT][2], const _Float16* a_lds, int ld,
+template <int RT, bool LO, int KTC, int CT = 2>
+__device__ __forceinline__ void gemm_lds_a_kt(floatx4 (&acc)[RT][CT], const _Float16* a_lds, int ld,
                                               const _Float16* __restrict__ w, const _Float16* __restrict__ wl,
                                               int KT_, int NT, int ct0, int ntw, int lane) {
   // KTC > 0: the step count is a compile-time constant, the loop is unrolled
@@ -85,16 +85,18 @@ __device__ __forceinline__ void gemm_lds_a_kt(floatx4 (&acc)[RT][2], const _Floa
   // the last kPf steps issue no refills
   const int KT = KTC > 0 ? KTC : KT_;
   const int lr = lane & 15, lg = lane >> 4;
-  const bool use0 = ntw > 0 && ct0 < NT, use1 = ntw > 1 && ct0 + 1 < NT;
+  // CT column tiles per wave (CT = 1: the second tile's code and loads are
+  // compiled out)
+  const bool use0 = ntw > 0 && ct0 < NT, use1 = CT > 1 && ntw > 1 && ct0 + 1 < NT;
   const int c0 = min(ct0, NT - 1), c1 = min(ct0 + 1, NT - 1);
-  half8 bq[kPf][2], bl[kPf][2];
+  half8 bq[kPf][CT], bl[kPf][CT];
 #pragma unroll
   for (int s = 0; s < kPf; ++s) {
     bq[s][0] = load_bfrag(w, s, NT, c0, lane);
-    bq[s][1] = load_bfrag(w, s, NT, c1, lane);
+    if constexpr (CT > 1) bq[s][CT - 1] = load_bfrag(w, s, NT, c1, lane);
     if constexpr (LO) {
       bl[s][0] = load_bfrag(wl, s, NT, c0, lane);
-      bl[s][1] = load_bfrag(wl, s, NT, c1, lane);
+      if constexpr (CT > 1) bl[s][CT - 1] = load_bfrag(wl, s, NT, c1, lane);
     }
   }
   auto step = [&](int k, int s) {
@@ -107,19 +109,21 @@ __device__ __forceinline__ void gemm_lds_a_kt(floatx4 (&acc)[RT][2], const _Floa
         acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][0], acc[i][0], 0, 0, 0);
         if constexpr (LO) acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bl[s][0], acc[i][0], 0, 0, 0);
       }
-    if (use1)
+    if constexpr (CT > 1)
+      if (use1)
 #pragma unroll
-      for (int i = 0; i < RT; ++i) {
-        acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][1], acc[i][1], 0, 0, 0);
-        if constexpr (LO) acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bl[s][1], acc[i][1], 0, 0, 0);
-      }
+        for (int i = 0; i < RT; ++i) {
+          acc[i][CT - 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][CT - 1], acc[i][CT - 1], 0, 0, 0);
+          if constexpr (LO)
+            acc[i][CT - 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bl[s][CT - 1], acc[i][CT - 1], 0, 0, 0);
+        }
     if (KTC > 0 && k + kPf >= KTC) return;  // folds away after unrolling
     const int kn = min(k + kPf, KT - 1);
     bq[s][0] = load_bfrag(w, kn, NT, c0, lane);
-    bq[s][1] = load_bfrag(w, kn, NT, c1, lane);
+    if constexpr (CT > 1) bq[s][CT - 1] = load_bfrag(w, kn, NT, c1, lane);
     if constexpr (LO) {
       bl[s][0] = load_bfrag(wl, kn, NT, c0, lane);
-      bl[s][1] = load_bfrag(wl, kn, NT, c1, lane);
+      if constexpr (CT > 1) bl[s][CT - 1] = load_bfrag(wl, kn, NT, c1, lane);
     }
   };
   if constexpr (KTC > 0) {
@@ -134,34 +138,39 @@ __device__ __forceinline__ void gemm_lds_a_kt(floatx4 (&acc)[RT][2], const _Floa
   }
 }
 
-template <int RT, bool LO>
-__device__ __forceinline__ void gemm_lds_a(floatx4 (&acc)[RT][2], const _Float16* a_lds, int ld,
+template <int RT, bool LO, int CT = 2>
+__device__ __forceinline__ void gemm_lds_a(floatx4 (&acc)[RT][CT], const _Float16* a_lds, int ld,
                                            const _Float16* __restrict__ w, const _Float16* __restrict__ wl, int KT,
                                            int NT, int ct0, int ntw, int lane) {
   if (KT == 8)  // 256-wide hidden layers
-    gemm_lds_a_kt<RT, LO, 8>(acc, a_lds, ld, w, wl, KT, NT, ct0, ntw, lane);
+    gemm_lds_a_kt<RT, LO, 8, CT>(acc, a_lds, ld, w, wl, KT, NT, ct0, ntw, lane);
   else
-    gemm_lds_a_kt<RT, LO, 0>(acc, a_lds, ld, w, wl, KT, NT, ct0, ntw, lane);
+    gemm_lds_a_kt<RT, LO, 0, CT>(acc, a_lds, ld, w, wl, KT, NT, ct0, ntw, lane);
 }
 
-template <int RT>
-__device__ __forceinline__ void zero_acc(floatx4 (&acc)[RT][2]) {
+template <int RT, int CT = 2>
+__device__ __forceinline__ void zero_acc(floatx4 (&acc)[RT][CT]) {
 #pragma unroll
   for (int i = 0; i < RT; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < CT; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 }
 
 // SPLIT: layer 0's products run in k_fit_layer0 / k_fit_dgrad; this kernel
-// reads x_1 and idt_0 act'(z_0) and writes dE/dz_0 instead.
-template <int H, int RT, bool SPLIT = false>
-__global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
+// reads x_1 and idt_0 act'(z_0) and writes dE/dz_0 instead.  NW waves per
+// workgroup, each with CT = 16 / NW of the (at most 16) column tiles of a
+// layer: 8 waves x 2 tiles, or 16 waves x 1 tile (twice the waves per SIMD
+// to hide the fragment reads, the same fragments fetched once).
+template <int H, int RT, bool SPLIT = false, int NW = 8>
+__global__ void __launch_bounds__(NW * 64) k_fit_fused(FusedFitArgs p) {
+  constexpr int CT = kMaxW / 16 / NW;         // column tiles per wave
+  static_assert(CT * NW * 16 == kMaxW, "the waves share the widest layer's column tiles evenly");
   constexpr int kRows = 16 * RT;              // rows per workgroup tile
-  constexpr int kFr = kKC * RT / kWaves;      // descriptor fragments per wave and chunk
-  static_assert(kFr * kWaves == kKC * RT, "a chunk's fragments are shared evenly by the waves");
+  constexpr int kFr = kKC * RT / NW;          // descriptor fragments per wave and chunk
+  static_assert(kFr * NW == kKC * RT, "a chunk's fragments are shared evenly by the waves");
   __shared__ _Float16 xs[2][kRows][kXld];
   __shared__ half8 afr[2][kKC * RT][kWave];  // descriptor chunks, A-fragment order
-  __shared__ float esum[kWaves][kRows];
+  __shared__ float esum[NW][kRows];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int4 ti = p.tiles[blockIdx.x];  // type, first row, row count
@@ -172,7 +181,7 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
 
   // activation derivative idt * act'(z) of every hidden layer at this wave's
   // output positions (row tile i, column tile j, rows lg*4 + e), two per register
-  half2v dpk[H][RT][2][2];
+  half2v dpk[H][RT][CT][2];
   float epart[RT][4];
 #pragma unroll
   for (int i = 0; i < RT; ++i)
@@ -183,21 +192,21 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
 #pragma unroll
   for (int l = 0; l < H; ++l) {
     const FusedLayer L = Ls[l];
-    const int KT = L.K / 32, NT = L.N / 16, ntw = (NT + kWaves - 1) / kWaves;
+    const int KT = L.K / 32, NT = L.N / 16, ntw = (NT + NW - 1) / NW;
     const int ct0 = wave * ntw;
     const int src = (l + 1) & 1, dst = l & 1;  // layer l >= 1 reads xs[src]
     if (SPLIT && l == 0) {
       // x_1 rows into LDS (rows past the tile: zeros), the activation
       // derivative at this wave's positions into registers
       const int N8 = L.N / 8;
-      for (int q = tid; q < kRows * N8; q += kWaves * kWave) {
+      for (int q = tid; q < kRows * N8; q += NW * kWave) {
         const int r = q / N8, c8 = q - r * N8;
         const half8 v = r < nrows ? *reinterpret_cast<const half8*>(p.x1 + (size_t)(row0 + r) * p.ldX + c8 * 8)
                                   : zero8();
         *reinterpret_cast<half8*>(&xs[dst][r][c8 * 8]) = v;
       }
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
+      for (int j = 0; j < CT; ++j) {
         const int n = min(ct0 + j, NT - 1) * 16 + lr;
 #pragma unroll
         for (int i = 0; i < RT; ++i) {
@@ -214,7 +223,7 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
       __syncthreads();
       continue;
     }
-    floatx4 acc[RT][2];
+    floatx4 acc[RT][CT];
     zero_acc(acc);
     if (l == 0) {
       // descriptor rows through LDS in chunks of kKC k-steps, stored in
@@ -225,29 +234,31 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
       // chunk.  Rows past the tile read the tile's first row and are zeroed.
       typedef __attribute__((address_space(1))) const half8 gh8;
       const int NC = KT / kKC;
-      const bool use0 = ntw > 0 && ct0 < NT, use1 = ntw > 1 && ct0 + 1 < NT;
+      const bool use0 = ntw > 0 && ct0 < NT, use1 = CT > 1 && ntw > 1 && ct0 + 1 < NT;
       const int c0 = min(ct0, NT - 1), c1 = min(ct0 + 1, NT - 1);
       const _Float16* dsrc[kFr];
       bool dval[kFr];
 #pragma unroll
       for (int m = 0; m < kFr; ++m) {
-        const int f = wave + kWaves * m, r = (f % RT) * 16 + lr;
+        const int f = wave + NW * m, r = (f % RT) * 16 + lr;
         dval[m] = r < nrows;
         dsrc[m] = p.D + (size_t)(row0 + (dval[m] ? r : 0)) * p.ldD + (f / RT) * 32 + lg * 8;
       }
       half8 ar[kFr];
-      half8 bq[kPf][2], bl[kPf][2];
+      half8 bq[kPf][CT], bl[kPf][CT];
 #pragma unroll
       for (int m = 0; m < kFr; ++m) ar[m] = __builtin_nontemporal_load((gh8*)dsrc[m]);
 #pragma unroll
       for (int s = 0; s < kPf; ++s) {
         bq[s][0] = load_bfrag(L.wf, s, NT, c0, lane);
-        bq[s][1] = load_bfrag(L.wf, s, NT, c1, lane);
         bl[s][0] = load_bfrag(L.wf_lo, s, NT, c0, lane);
-        bl[s][1] = load_bfrag(L.wf_lo, s, NT, c1, lane);
+        if constexpr (CT > 1) {
+          bq[s][CT - 1] = load_bfrag(L.wf, s, NT, c1, lane);
+          bl[s][CT - 1] = load_bfrag(L.wf_lo, s, NT, c1, lane);
+        }
       }
 #pragma unroll
-      for (int m = 0; m < kFr; ++m) afr[0][wave + kWaves * m][lane] = dval[m] ? ar[m] : zero8();
+      for (int m = 0; m < kFr; ++m) afr[0][wave + NW * m][lane] = dval[m] ? ar[m] : zero8();
       __syncthreads();
       for (int c = 0; c < NC; ++c) {
         if (c + 1 < NC)
@@ -265,31 +276,34 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
               acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][0], acc[i][0], 0, 0, 0);
               acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bl[s][0], acc[i][0], 0, 0, 0);
             }
-          if (use1)
+          if constexpr (CT > 1)
+            if (use1)
 #pragma unroll
-            for (int i = 0; i < RT; ++i) {
-              acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][1], acc[i][1], 0, 0, 0);
-              acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bl[s][1], acc[i][1], 0, 0, 0);
-            }
+              for (int i = 0; i < RT; ++i) {
+                acc[i][CT - 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][CT - 1], acc[i][CT - 1], 0, 0, 0);
+                acc[i][CT - 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bl[s][CT - 1], acc[i][CT - 1], 0, 0, 0);
+              }
           const int kn = min(k + kPf, KT - 1);
           bq[s][0] = load_bfrag(L.wf, kn, NT, c0, lane);
-          bq[s][1] = load_bfrag(L.wf, kn, NT, c1, lane);
           bl[s][0] = load_bfrag(L.wf_lo, kn, NT, c0, lane);
-          bl[s][1] = load_bfrag(L.wf_lo, kn, NT, c1, lane);
+          if constexpr (CT > 1) {
+            bq[s][CT - 1] = load_bfrag(L.wf, kn, NT, c1, lane);
+            bl[s][CT - 1] = load_bfrag(L.wf_lo, kn, NT, c1, lane);
+          }
         }
         // chunk c + 1 goes to the buffer read in chunk c - 1 (every wave has
         // passed the barrier after that chunk)
         if (c + 1 < NC)
 #pragma unroll
-          for (int m = 0; m < kFr; ++m) afr[(c + 1) & 1][wave + kWaves * m][lane] = dval[m] ? ar[m] : zero8();
+          for (int m = 0; m < kFr; ++m) afr[(c + 1) & 1][wave + NW * m][lane] = dval[m] ? ar[m] : zero8();
         __syncthreads();
       }
     } else {
-      gemm_lds_a<RT, true>(acc, &xs[src][0][0], kXld, L.wf, L.wf_lo, KT, NT, ct0, ntw, lane);
+      gemm_lds_a<RT, true, CT>(acc, &xs[src][0][0], kXld, L.wf, L.wf_lo, KT, NT, ct0, ntw, lane);
     }
     // epilogue: lane holds rows lg*4 + e of each 16x16 tile, column lr
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < CT; ++j) {
       if (j >= ntw || ct0 + j >= NT) continue;
       const int n = (ct0 + j) * 16 + lr;
       const float bn = L.b[n];
@@ -334,18 +348,18 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
   if (tid < nrows) {
     float e = 0.f;
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) e += esum[w][tid];
+    for (int w = 0; w < NW; ++w) e += esum[w][tid];
     p.atom_e[p.perm[row0 + tid]] = (double)e + p.shift[t];
   }
 
   // ----------------------------------------------------------- backward --
   // g: dE/d(output of layer l) at this wave's positions; top: w_out
-  float g[RT][2][4];
+  float g[RT][CT][4];
   {
     const FusedLayer L = Ls[H - 1];
-    const int NT = L.N / 16, ntw = (NT + kWaves - 1) / kWaves;
+    const int NT = L.N / 16, ntw = (NT + NW - 1) / NW;
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < CT; ++j) {
       const int ct = wave * ntw + j;
       const float w = (j < ntw && ct < NT) ? L.out_w[ct * 16 + lr] : 0.f;
 #pragma unroll
@@ -357,11 +371,11 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
 #pragma unroll
   for (int l = H - 1; l >= 0; --l) {
     const FusedLayer L = Ls[l];
-    const int NT = L.N / 16, ntw = (NT + kWaves - 1) / kWaves;
+    const int NT = L.N / 16, ntw = (NT + NW - 1) / NW;
     const int buf = l & 1;
     // gz = g * idt * act'(z) -> LDS (the A operand, reduction over N)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < CT; ++j) {
       const int ct = wave * ntw + j;
       if (j >= ntw || ct >= NT) continue;
       const int n = ct * 16 + lr;
@@ -384,28 +398,28 @@ __global__ void __launch_bounds__(512) k_fit_fused(FusedFitArgs p) {
     const int KT = L.N / 32;  // reduction steps
     const int OT = L.K / 16;  // output column tiles (input width of the layer)
     if (l > 0) {
-      const int otw = (OT + kWaves - 1) / kWaves;
-      floatx4 acc[RT][2];
+      const int otw = (OT + NW - 1) / NW;
+      floatx4 acc[RT][CT];
       zero_acc(acc);
-      gemm_lds_a<RT, false>(acc, &xs[buf][0][0], kXld, L.wb, L.wb, KT, OT, wave * otw, otw, lane);
+      gemm_lds_a<RT, false, CT>(acc, &xs[buf][0][0], kXld, L.wb, L.wb, KT, OT, wave * otw, otw, lane);
       // g_l = gz W^T (+ g_{l+1} for residual layers: same positions)
 #pragma unroll
       for (int i = 0; i < RT; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < CT; ++j)
 #pragma unroll
           for (int e = 0; e < 4; ++e)
             g[i][j][e] = (j < otw && wave * otw + j < OT ? acc[i][j][e] : 0.f) + (L.res ? g[i][j][e] : 0.f);
     } else {
-      // descriptor gradient: K0 output columns in chunks of 8 waves x 2 tiles
-      const int nchunk = (OT + 2 * kWaves - 1) / (2 * kWaves);
+      // descriptor gradient: K0 output columns in chunks of NW waves x CT tiles
+      const int nchunk = (OT + CT * NW - 1) / (CT * NW);
       for (int c = 0; c < nchunk; ++c) {
-        const int ot0 = c * 2 * kWaves + wave * 2;
-        floatx4 acc[RT][2];
+        const int ot0 = c * CT * NW + wave * CT;
+        floatx4 acc[RT][CT];
         zero_acc(acc);
-        gemm_lds_a<RT, false>(acc, &xs[buf][0][0], kXld, L.wb, L.wb, KT, OT, ot0, 2, lane);
+        gemm_lds_a<RT, false, CT>(acc, &xs[buf][0][0], kXld, L.wb, L.wb, KT, OT, ot0, CT, lane);
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < CT; ++j) {
           if (ot0 + j >= OT) continue;
           const int col = (ot0 + j) * 16 + lr;
 #pragma unroll
@@ -582,20 +596,23 @@ __global__ void __launch_bounds__(512) k_fit_dgrad(FusedFitArgs p, int ntiles, i
 
 }  // namespace
 
-template <int RT>
+template <int RT, int NW = kWaves>
 void launch_fit_fused_rt(const FusedFitArgs& a, int ntiles, hipStream_t st) {
+  const dim3 block(NW * kWave);
   switch (a.H) {
-    case 1: hipLaunchKernelGGL((k_fit_fused<1, RT>), dim3(ntiles), dim3(512), 0, st, a); break;
-    case 2: hipLaunchKernelGGL((k_fit_fused<2, RT>), dim3(ntiles), dim3(512), 0, st, a); break;
-    case 3: hipLaunchKernelGGL((k_fit_fused<3, RT>), dim3(ntiles), dim3(512), 0, st, a); break;
-    case 4: hipLaunchKernelGGL((k_fit_fused<4, RT>), dim3(ntiles), dim3(512), 0, st, a); break;
+    case 1: hipLaunchKernelGGL((k_fit_fused<1, RT, false, NW>), dim3(ntiles), block, 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_fit_fused<2, RT, false, NW>), dim3(ntiles), block, 0, st, a); break;
+    case 3: hipLaunchKernelGGL((k_fit_fused<3, RT, false, NW>), dim3(ntiles), block, 0, st, a); break;
+    case 4: hipLaunchKernelGGL((k_fit_fused<4, RT, false, NW>), dim3(ntiles), block, 0, st, a); break;
     default: throw std::invalid_argument("fused fitting supports 1 to 4 hidden layers");
   }
 }
 
-void launch_fit_fused(const FusedFitArgs& a, int ntiles, int rows_per_tile, hipStream_t st) {
+void launch_fit_fused(const FusedFitArgs& a, int ntiles, int rows_per_tile, hipStream_t st, bool wide) {
   if (ntiles <= 0) return;
-  if (rows_per_tile == 64)
+  if (rows_per_tile == 64 && wide)
+    launch_fit_fused_rt<4, 16>(a, ntiles, st);
+  else if (rows_per_tile == 64)
     launch_fit_fused_rt<4>(a, ntiles, st);
   else if (rows_per_tile == 32)
     launch_fit_fused_rt<2>(a, ntiles, st);

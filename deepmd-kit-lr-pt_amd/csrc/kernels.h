@@ -347,6 +347,7 @@ bool force_recompute_supported(int nnei);
 // Launch-shape knobs of the engine stages, picked per size bucket by
 // SeaEngine::autotune.
 constexpr int kFitSplit = 256;
+constexpr int kFitWide = 512;
 struct Tune {
   int env_tpb = 256;
   int fwd_waves = 4;
@@ -360,7 +361,8 @@ struct Tune {
   // forward on the matrix cores (precedence over bit 0)
   int embed_mfma = 0;
   // fused fitting rows per workgroup tile (32 or 64; 0: chosen from the
-  // size), plus kFitSplit: layer 0's products in their own kernels
+  // size), plus kFitSplit: layer 0's products in their own kernels, or plus
+  // kFitWide (64-row tiles): 16 waves per workgroup
   int fit_rows = 0;
 };
 void launch_force_virial_rows(const ForceRowsArgs& a, hipStream_t st);
@@ -486,8 +488,10 @@ struct FusedFitArgs {
   int ldX = 0;
   int nxcd = 1;
 };
-// tiles of rows_per_tile (32 or 64) descriptor rows
-void launch_fit_fused(const FusedFitArgs& a, int ntiles, int rows_per_tile, hipStream_t st);
+// tiles of rows_per_tile (32 or 64) descriptor rows; wide (64-row tiles
+// only): 16 waves of one column tile each instead of 8 waves of two (the
+// same products; the energy's FP32 partial sums grouped differently)
+void launch_fit_fused(const FusedFitArgs& a, int ntiles, int rows_per_tile, hipStream_t st, bool wide = false);
 // the same results (bit for bit) from three launches: layer 0 forward as a
 // product tiled over rows and output columns, the other layers fused per row
 // tile, layer 0 backward (dE/dD) tiled over rows and output columns; needs

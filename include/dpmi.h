@@ -323,7 +323,8 @@ int dpmi_sea_engine_graph_capture_device_box(dpmi_sea_engine* engine, const doub
  * and 0); else the per-neighbour kernels), fused
  * fitting rows per tile (32 or 64; 0 = chosen from the atom count; plus 256:
  * layer 0's products in their own kernels tiled over rows and columns, the
- * same bits, faster when the row tiles leave compute units idle)}.
+ * same bits, faster when the row tiles leave compute units idle; or 64 plus
+ * 512: 16 wavefronts of one column tile each per 64-row tile)}.
  * dpmi_sea_engine_set_tuning sets them (a knob <= 0, or < 0 for the mask,
  * keeps its value; out-of-range values give DPMI_ERR_ARGUMENT). */
 int dpmi_sea_engine_autotune(dpmi_sea_engine* engine, const double* coord, const double* box_host, int reps,

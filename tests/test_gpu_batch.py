@@ -85,9 +85,9 @@ def test_autotune_keeps_results_and_caches(water, batch, precision):
     # per-neighbour (0), table space (2) or row runs (4)
     mixed_masks = {f | b for f in (0, 1, 8) for b in (0, 2, 4)}
     assert knobs["embed_matrix_cores"] in (mixed_masks if precision == "mixed" else {0})
-    # fused fitting tiles of 32 or 64 rows, or 32-row tiles with layer 0 in
-    # its own kernels (+256)
-    assert knobs["fitting_rows_per_tile"] in ((32, 64, 32 + 256) if precision == "mixed" else (0,))
+    # fused fitting tiles of 32 rows, 64 rows with 16 waves (+512), or 32-row
+    # tiles with layer 0 in its own kernels (+256)
+    assert knobs["fitting_rows_per_tile"] in ((32, 64 + 512, 32 + 256) if precision == "mixed" else (0,))
     e, f, v = [t.cpu().numpy().copy() for t in eng.compute(x, box)]
     if precision == "fp64" or knobs["embed_matrix_cores"] == 0:
         # launch shapes alone do not change the arithmetic

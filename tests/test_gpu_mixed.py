@@ -266,11 +266,12 @@ def test_mixed_rejects_width_doubling_residual():
 def test_fitting_tile_rows_give_identical_rows(water_dev, nmol):
     """Fused fitting tiles of 32 and 64 rows, and layer 0 in its own kernels
     (+256), compute every row with the same arithmetic in the same order, so
-    atomic energies, net_deriv, energy and virial agree bit for bit."""
+    atomic energies, net_deriv, energy and virial agree bit for bit; 16 waves
+    per 64-row tile (+512) only regroup the energy's FP32 sums."""
     coord, atype, box = systems.water(nmol, seed=13)
     n = len(atype)
     out = {}
-    for rows in (32, 64, 32 + 256, 64 + 256):
+    for rows in (32, 64, 32 + 256, 64 + 256, 64 + 512):
         eng = Engine(water_dev, [n], atype, precision="mixed")
         eng.set_tuning(fitting_rows_per_tile=rows)
         assert eng.tuning()["fitting_rows_per_tile"] == rows
@@ -280,3 +281,11 @@ def test_fitting_tile_rows_give_identical_rows(water_dev, nmol):
     for rows in (64, 32 + 256, 64 + 256):
         for a, b in zip(out[32], out[rows]):
             assert np.array_equal(a, b), rows
+    # 16 waves per 64-row tile: the same products (net_deriv and the virial,
+    # which follows from it, bit for bit); the energy's FP32 partial sums over
+    # the column tiles are grouped by wave, so atom energies agree to FP32
+    # rounding
+    ae, nd, e, v = out[64 + 512]
+    assert np.array_equal(nd, out[32][1])
+    assert np.max(np.abs(ae - out[32][0])) <= 1e-6 * np.max(np.abs(out[32][0]))
+    assert abs(e[0] - out[32][2][0]) <= 1e-6 * abs(out[32][2][0])
