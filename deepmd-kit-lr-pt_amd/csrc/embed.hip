@@ -30,6 +30,8 @@ namespace {
 
 using namespace tab;
 
+typedef _Float16 dhalf8 __attribute__((ext_vector_type(8)));
+
 template <typename TD>
 __device__ __forceinline__ TD to_out(float v);
 template <>
@@ -78,6 +80,61 @@ __global__ void __launch_bounds__(256) k_embed_fwd(T* __restrict__ xyz, TD* __re
   }
   // descriptor row block: columns a < A live in lanes 0..A-1 (kq = 0)
   TD* out = D + (size_t)pos[g] * ldD;
+  if constexpr (sizeof(TD) == 2) {
+    if (A == 16) {
+      // FP16 rows of 16: a lane's values of row k are contiguous, so they
+      // leave as two 16-byte stores instead of sixteen 2-byte ones
+      dhalf8 h[KQ][2];
+#pragma unroll
+      for (int a = 0; a < 16; ++a) {
+        T ta[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) ta[c] = bcast(acc[0][c], a) * inv_nnei;
+#pragma unroll
+        for (int kq = 0; kq < KQ; ++kq) {
+          T v = 0;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) v += (acc[kq][c] * inv_nnei) * ta[c];
+          h[kq][a >> 3][a & 7] = cvt<TD>(v);
+        }
+      }
+#pragma unroll
+      for (int kq = 0; kq < KQ; ++kq) {
+        const int k = lane + kq * kWave;
+        if (k < M) {
+          *reinterpret_cast<dhalf8*>(out + (size_t)k * 16) = h[kq][0];
+          *reinterpret_cast<dhalf8*>(out + (size_t)k * 16 + 8) = h[kq][1];
+        }
+      }
+      return;
+    }
+  } else if constexpr (sizeof(TD) == 8) {
+    if (A == 16) {
+      // FP64 rows of 16 (128 contiguous bytes per lane and row k): pairs
+      // of columns leave as 16-byte stores, eight instead of sixteen 8-byte
+      // ones (one pair at a time keeps the broadcast values few)
+      for (int a2 = 0; a2 < 8; ++a2) {
+        T t0[4], t1[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          t0[c] = bcast(acc[0][c], 2 * a2) * inv_nnei;
+          t1[c] = bcast(acc[0][c], 2 * a2 + 1) * inv_nnei;
+        }
+#pragma unroll
+        for (int kq = 0; kq < KQ; ++kq) {
+          const int k = lane + kq * kWave;
+          T v0 = 0, v1 = 0;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            v0 += (acc[kq][c] * inv_nnei) * t0[c];
+            v1 += (acc[kq][c] * inv_nnei) * t1[c];
+          }
+          if (k < M) *reinterpret_cast<double2*>(out + (size_t)k * 16 + 2 * a2) = make_double2(cvt<TD>(v0), cvt<TD>(v1));
+        }
+      }
+      return;
+    }
+  }
   for (int a = 0; a < A; ++a) {
     T ta[4];
 #pragma unroll
