@@ -157,6 +157,7 @@ int copy_coord_run(FrameGeom g, double* out_c, int* out_t, int* mapping, int* na
     dpmi::CellWork w;
     w.nframes = 1;
     w.ntot = nloc;
+    w.max_nloc = nloc;
     w.ncells_tot = nc;
     w.nx_tot = nx;
     w.do_wrap = 0;

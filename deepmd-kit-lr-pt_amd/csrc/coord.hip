@@ -14,7 +14,8 @@
 //                      kernel when asked)
 //   2. k_scan_cells    one workgroup per frame: exclusive scan of the counts,
 //                      then the prefix over extended cells (all atoms, and
-//                      ghosts only) -> nall
+//                      ghosts only) -> nall; for frames of at most 4096
+//                      atoms the workgroup also does step 3
 //   3. k_scatter_cells one thread per local atom: place into its cell slot
 //   4. k_emit_ghosts   one wave per extended cell: sort the source cell's
 //                      atoms by index in registers, write ghost images and
@@ -171,16 +172,42 @@ __device__ __forceinline__ int src_cell(const FrameGeom& g, int e, int* cx, bool
 
 // One workgroup per frame: exclusive scan of the cell counts (cell_start),
 // then the per-extended-cell counts (all atoms, and ghosts only) and their
-// scans -> nall.
+// scans -> nall.  With cell_list set (frames of at most kScatterInScan
+// atoms) the workgroup also places the frame's atoms at their cell slots,
+// instead of k_scatter_cells: each thread issues the loads of up to four
+// atoms before their stores, so one round of dependent loads covers the
+// frame (for larger frames the chain repeats and the separate kernel wins).
+constexpr int kScatterPerThread = 4;
+constexpr int kScatterInScan = kScatterPerThread * 1024;
 __global__ void __launch_bounds__(1024) k_scan_cells(const FrameGeom* __restrict__ geom, const int* __restrict__ cell_cnt,
                                                      int* __restrict__ cell_start, int* __restrict__ xcnt, int* __restrict__ xstart,
                                                      int* __restrict__ xgcnt, int* __restrict__ xgstart,
-                                                     int* __restrict__ nall_out, int* __restrict__ flag) {
+                                                     int* __restrict__ nall_out, int* __restrict__ flag,
+                                                     const int* __restrict__ cell_of, const int* __restrict__ cell_slot,
+                                                     int* __restrict__ cell_list) {
   __shared__ int sh[2 * kWave];
   const int f = blockIdx.x;
   const FrameGeom& g = geom[f];
   const int nc = g.nc[0] * g.nc[1] * g.nc[2];
-  block_exclusive_scan(cell_cnt + g.cell_off, cell_start + g.cell_off, nc, sh);
+  block_exclusive_scan(cell_cnt + g.cell_off, cell_start + g.cell_off, nc, sh);  // ends with a barrier
+  if (cell_list) {
+    for (int i0 = threadIdx.x; i0 < g.nloc; i0 += kScatterPerThread * blockDim.x) {
+      int c[kScatterPerThread], s[kScatterPerThread];
+#pragma unroll
+      for (int u = 0; u < kScatterPerThread; ++u) {
+        const int i = i0 + u * blockDim.x;
+        c[u] = i < g.nloc ? cell_of[g.loc_off + i] : 0;
+        s[u] = i < g.nloc ? cell_slot[g.loc_off + i] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < kScatterPerThread; ++u) c[u] = cell_start[g.cell_off + c[u]];
+#pragma unroll
+      for (int u = 0; u < kScatterPerThread; ++u) {
+        const int i = i0 + u * blockDim.x;
+        if (i < g.nloc) cell_list[g.loc_off + c[u] + s[u]] = i;
+      }
+    }
+  }
   const int n = g.nx[0] * g.nx[1] * g.nx[2];
   const int off = g.xcell_off;
   for (int e = threadIdx.x; e < n; e += blockDim.x) {
@@ -349,12 +376,16 @@ void launch_build_extended(const CellWork& w, hipStream_t st) {
                      w.geom, w.ntot, w.coord_ext, w.type_ext, w.map_ext, w.cell_of, w.cell_slot, w.cell_cnt,
                      w.do_wrap, w.force_clear);
   DPMI_LAUNCH_CHECK();
+  const bool in_scan = w.max_nloc > 0 && w.max_nloc <= kScatterInScan;
   hipLaunchKernelGGL(k_scan_cells, dim3(w.nframes), dim3(1024), 0, st, w.geom, w.cell_cnt, w.cell_start, w.xcnt,
-                     w.xstart, w.xgcnt, w.xgstart, w.nall, w.flag);
+                     w.xstart, w.xgcnt, w.xgstart, w.nall, w.flag, w.cell_of, w.cell_slot,
+                     in_scan ? w.cell_list : nullptr);
   DPMI_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_scatter_cells, dim3(ceil_div(w.ntot, tpb)), dim3(tpb), 0, st, w.frame_of, w.geom, w.ntot,
-                     w.cell_of, w.cell_slot, w.cell_start, w.cell_list);
-  DPMI_LAUNCH_CHECK();
+  if (!in_scan) {
+    hipLaunchKernelGGL(k_scatter_cells, dim3(ceil_div(w.ntot, tpb)), dim3(tpb), 0, st, w.frame_of, w.geom, w.ntot,
+                       w.cell_of, w.cell_slot, w.cell_start, w.cell_list);
+    DPMI_LAUNCH_CHECK();
+  }
   hipLaunchKernelGGL(k_emit_ghosts, dim3(ceil_div((long long)w.nx_tot * kWave, tpb)), dim3(tpb), 0, st,
                      w.xcell_frame, w.geom, w.nx_tot, w.cell_start, w.cell_list, w.sorted_loc, w.cell_cnt, w.xcnt,
                      w.xstart, w.xgstart, w.coord_ext, w.type_ext, w.map_ext, w.cell_atoms, w.cell_sub);

@@ -1326,6 +1326,7 @@ void SeaEngine::enqueue_step(const double* coord, const double* box_h, double* e
   CellWork w;
   w.nframes = nframes;
   w.ntot = ntot;
+  w.max_nloc = *std::max_element(natoms.begin(), natoms.end());
   w.ncells_tot = ncells_tot;
   w.nx_tot = nx_tot;
   w.do_wrap = 1;

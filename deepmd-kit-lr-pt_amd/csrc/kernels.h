@@ -19,6 +19,7 @@ struct __attribute__((aligned(16))) CellAtom {
 struct CellWork {
   int nframes = 0;
   int ntot = 0;          // packed local atoms of all frames
+  int max_nloc = 0;      // atoms of the largest frame (0: unknown; small frames are placed by the scan)
   int ncells_tot = 0;    // packed local cells of all frames
   int nx_tot = 0;        // packed extended cells (+1 sentinel per frame)
   int do_wrap = 1;
