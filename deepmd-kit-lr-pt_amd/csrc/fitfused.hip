@@ -410,6 +410,57 @@ __global__ void __launch_bounds__(NW * 64) k_fit_fused(FusedFitArgs p) {
 #pragma unroll
           for (int e = 0; e < 4; ++e)
             g[i][j][e] = (j < otw && wave * otw + j < OT ? acc[i][j][e] : 0.f) + (L.res ? g[i][j][e] : 0.f);
+    } else if (KT == 8) {
+      // descriptor gradient, 256-wide layer 0: K0 output columns in chunks of
+      // NW waves x CT tiles, back to back -- the last kPf k-steps of a chunk
+      // fetch the first fragments of the next, so the fragment ring does not
+      // drain between chunks (same products in the same order as below)
+      constexpr int KS = 8;
+      const int nchunk = (OT + CT * NW - 1) / (CT * NW);
+      auto tile = [&](int c, int j) { return min(c * CT * NW + wave * CT + j, OT - 1); };
+      half8 bq[kPf][CT];
+#pragma unroll
+      for (int s = 0; s < kPf; ++s)
+#pragma unroll
+        for (int j = 0; j < CT; ++j) bq[s][j] = load_bfrag(L.wb, s, OT, tile(0, j), lane);
+      for (int c = 0; c < nchunk; ++c) {
+        // the A fragments are the same for every chunk: read them per chunk
+        // (kept across chunks they would take 32 RT registers)
+        asm volatile("" ::: "memory");
+        const int cn = min(c + 1, nchunk - 1);
+        floatx4 acc[RT][CT];
+        zero_acc(acc);
+#pragma unroll
+        for (int k = 0; k < KS; ++k) {
+          const int s = k % kPf;
+          half8 a[RT];
+#pragma unroll
+          for (int i = 0; i < RT; ++i)
+            a[i] = *reinterpret_cast<const half8*>(&xs[buf][i * 16 + lr][k * 32 + lg * 8]);
+#pragma unroll
+          for (int j = 0; j < CT; ++j)
+#pragma unroll
+            for (int i = 0; i < RT; ++i)
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][j], acc[i][j], 0, 0, 0);
+#pragma unroll
+          for (int j = 0; j < CT; ++j)
+            bq[s][j] = k + kPf < KS ? load_bfrag(L.wb, k + kPf, OT, tile(c, j), lane)
+                                    : load_bfrag(L.wb, k + kPf - KS, OT, tile(cn, j), lane);
+        }
+        const int ot0 = c * CT * NW + wave * CT;
+#pragma unroll
+        for (int j = 0; j < CT; ++j) {
+          if (ot0 + j >= OT) continue;
+          const int col = (ot0 + j) * 16 + lr;
+#pragma unroll
+          for (int i = 0; i < RT; ++i)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const int r = i * 16 + lg * 4 + e;
+              if (r < nrows) __builtin_nontemporal_store(acc[i][j][e], p.gD + (size_t)(row0 + r) * p.ldG + col);
+            }
+        }
+      }
     } else {
       // descriptor gradient: K0 output columns in chunks of NW waves x CT tiles
       const int nchunk = (OT + CT * NW - 1) / (CT * NW);
