@@ -156,6 +156,56 @@ __device__ __forceinline__ void zero_acc(floatx4 (&acc)[RT][CT]) {
     for (int j = 0; j < CT; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 }
 
+// One 256-wide product (8 k-steps) of a chain whose fragment ring carries
+// over: the ring holds this product's first kPf steps on entry, and its last
+// kPf steps fetch the first kPf of the next product (hi, and lo when LON).
+// Products and their order are those of gemm_lds_a_kt.
+template <int RT, int CT, bool LO, bool LON>
+__device__ __forceinline__ void gemm8_chain(floatx4 (&acc)[RT][CT], const _Float16* a_lds, half8 (&rq)[kPf][CT],
+                                            half8 (&rl)[kPf][CT], bool use0, bool use1, const _Float16* __restrict__ w,
+                                            const _Float16* __restrict__ wl, int NT, int c0, int c1,
+                                            const _Float16* __restrict__ wn, const _Float16* __restrict__ wln, int NTn,
+                                            int n0, int n1, int lane) {
+  constexpr int KS = 8;
+  const int lr = lane & 15, lg = lane >> 4;
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const int s = k % kPf;
+    half8 a[RT];
+#pragma unroll
+    for (int i = 0; i < RT; ++i) a[i] = *reinterpret_cast<const half8*>(a_lds + (i * 16 + lr) * kXld + k * 32 + lg * 8);
+    if (use0)
+#pragma unroll
+      for (int i = 0; i < RT; ++i) {
+        acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], rq[s][0], acc[i][0], 0, 0, 0);
+        if constexpr (LO) acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], rl[s][0], acc[i][0], 0, 0, 0);
+      }
+    if constexpr (CT > 1)
+      if (use1)
+#pragma unroll
+        for (int i = 0; i < RT; ++i) {
+          acc[i][CT - 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], rq[s][CT - 1], acc[i][CT - 1], 0, 0, 0);
+          if constexpr (LO)
+            acc[i][CT - 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], rl[s][CT - 1], acc[i][CT - 1], 0, 0, 0);
+        }
+    if (k + kPf < KS) {
+      rq[s][0] = load_bfrag(w, k + kPf, NT, c0, lane);
+      if constexpr (CT > 1) rq[s][CT - 1] = load_bfrag(w, k + kPf, NT, c1, lane);
+      if constexpr (LO) {
+        rl[s][0] = load_bfrag(wl, k + kPf, NT, c0, lane);
+        if constexpr (CT > 1) rl[s][CT - 1] = load_bfrag(wl, k + kPf, NT, c1, lane);
+      }
+    } else {
+      rq[s][0] = load_bfrag(wn, k + kPf - KS, NTn, n0, lane);
+      if constexpr (CT > 1) rq[s][CT - 1] = load_bfrag(wn, k + kPf - KS, NTn, n1, lane);
+      if constexpr (LON) {
+        rl[s][0] = load_bfrag(wln, k + kPf - KS, NTn, n0, lane);
+        if constexpr (CT > 1) rl[s][CT - 1] = load_bfrag(wln, k + kPf - KS, NTn, n1, lane);
+      }
+    }
+  }
+}
+
 // SPLIT: layer 0's products run in k_fit_layer0 / k_fit_dgrad; this kernel
 // reads x_1 and idt_0 act'(z_0) and writes dE/dz_0 instead.  NW waves per
 // workgroup, each with CT = 16 / NW of the (at most 16) column tiles of a
@@ -178,6 +228,18 @@ __global__ void __launch_bounds__(NW * 64) k_fit_fused(FusedFitArgs p) {
   if (nrows <= 0) return;  // padding slot of the XCD-aware tile order
   const FusedLayer* Ls = p.layers + (size_t)t * H;
   const int lr = lane & 15, lg = lane >> 4;
+  // destination of this thread's atom energy, read now rather than after the
+  // forward (the backward's first barrier would wait for that load)
+  const int erow = tid < nrows ? p.perm[row0 + tid] : 0;
+  const double eshift = p.shift[t];  // uniform: scalar registers
+  // 16 waves of one column tile: with every hidden width 256, the products
+  // after layer 0's forward run as one chain over a fragment ring that never
+  // drains (gemm8_chain); the ring's 32 registers fit beside the rest
+  constexpr bool CHAIN = NW == 16 && !SPLIT && H >= 2;
+  bool chain = CHAIN;
+#pragma unroll
+  for (int l = 0; l < H; ++l) chain = chain && Ls[l].N == kMaxW && (l == 0 || Ls[l].K == kMaxW);
+  half8 rq[kPf][CT], rl[kPf][CT];  // the chain's fragment ring (hi, lo)
 
   // activation derivative idt * act'(z) of every hidden layer at this wave's
   // output positions (row tile i, column tile j, rows lg*4 + e), two per register
@@ -245,7 +307,12 @@ __global__ void __launch_bounds__(NW * 64) k_fit_fused(FusedFitArgs p) {
         dsrc[m] = p.D + (size_t)(row0 + (dval[m] ? r : 0)) * p.ldD + (f / RT) * 32 + lg * 8;
       }
       half8 ar[kFr];
-      half8 bq[kPf][CT], bl[kPf][CT];
+      half8(&bq)[kPf][CT] = rq;
+      half8(&bl)[kPf][CT] = rl;
+      // layer 1's first fragments (fetched by the last kPf steps when chained)
+      const FusedLayer L1 = Ls[H > 1 ? 1 : 0];
+      const int NT1 = L1.N / 16, ntw1 = (NT1 + NW - 1) / NW;
+      const int n10 = min(wave * ntw1, NT1 - 1), n11 = min(wave * ntw1 + 1, NT1 - 1);
 #pragma unroll
       for (int m = 0; m < kFr; ++m) ar[m] = __builtin_nontemporal_load((gh8*)dsrc[m]);
 #pragma unroll
@@ -283,12 +350,17 @@ __global__ void __launch_bounds__(NW * 64) k_fit_fused(FusedFitArgs p) {
                 acc[i][CT - 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][CT - 1], acc[i][CT - 1], 0, 0, 0);
                 acc[i][CT - 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bl[s][CT - 1], acc[i][CT - 1], 0, 0, 0);
               }
-          const int kn = min(k + kPf, KT - 1);
-          bq[s][0] = load_bfrag(L.wf, kn, NT, c0, lane);
-          bl[s][0] = load_bfrag(L.wf_lo, kn, NT, c0, lane);
+          // unconditional loads (selected sources) keep the wait counts exact
+          const bool nx = chain && k + kPf >= KT;
+          const int kn = nx ? k + kPf - KT : min(k + kPf, KT - 1);
+          const _Float16* wq = nx ? L1.wf : L.wf;
+          const _Float16* wql = nx ? L1.wf_lo : L.wf_lo;
+          const int NTq = nx ? NT1 : NT, q0 = nx ? n10 : c0, q1 = nx ? n11 : c1;
+          bq[s][0] = load_bfrag(wq, kn, NTq, q0, lane);
+          bl[s][0] = load_bfrag(wql, kn, NTq, q0, lane);
           if constexpr (CT > 1) {
-            bq[s][CT - 1] = load_bfrag(L.wf, kn, NT, c1, lane);
-            bl[s][CT - 1] = load_bfrag(L.wf_lo, kn, NT, c1, lane);
+            bq[s][CT - 1] = load_bfrag(wq, kn, NTq, q1, lane);
+            bl[s][CT - 1] = load_bfrag(wql, kn, NTq, q1, lane);
           }
         }
         // chunk c + 1 goes to the buffer read in chunk c - 1 (every wave has
@@ -297,6 +369,19 @@ __global__ void __launch_bounds__(NW * 64) k_fit_fused(FusedFitArgs p) {
 #pragma unroll
           for (int m = 0; m < kFr; ++m) afr[(c + 1) & 1][wave + NW * m][lane] = dval[m] ? ar[m] : zero8();
         __syncthreads();
+      }
+    } else if (CHAIN && chain) {
+      const bool use0 = ntw > 0 && ct0 < NT, use1 = CT > 1 && ntw > 1 && ct0 + 1 < NT;
+      const int c0 = min(ct0, NT - 1), c1 = min(ct0 + 1, NT - 1);
+      if (l + 1 < H) {  // next: layer l + 1 forward
+        const FusedLayer Ln = Ls[l + 1 < H ? l + 1 : l];
+        const int NTn = Ln.N / 16, ntwn = (NTn + NW - 1) / NW;
+        gemm8_chain<RT, CT, true, true>(acc, &xs[src][0][0], rq, rl, use0, use1, L.wf, L.wf_lo, NT, c0, c1, Ln.wf,
+                                        Ln.wf_lo, NTn, min(wave * ntwn, NTn - 1), min(wave * ntwn + 1, NTn - 1), lane);
+      } else {  // next: this layer's backward product
+        const int OT = L.K / 16, otw = (OT + NW - 1) / NW;
+        gemm8_chain<RT, CT, true, false>(acc, &xs[src][0][0], rq, rl, use0, use1, L.wf, L.wf_lo, NT, c0, c1, L.wb,
+                                         L.wb, OT, min(wave * otw, OT - 1), min(wave * otw + 1, OT - 1), lane);
       }
     } else {
       gemm_lds_a<RT, true, CT>(acc, &xs[src][0][0], kXld, L.wf, L.wf_lo, KT, NT, ct0, ntw, lane);
@@ -349,7 +434,7 @@ __global__ void __launch_bounds__(NW * 64) k_fit_fused(FusedFitArgs p) {
     float e = 0.f;
 #pragma unroll
     for (int w = 0; w < NW; ++w) e += esum[w][tid];
-    p.atom_e[p.perm[row0 + tid]] = (double)e + p.shift[t];
+    p.atom_e[erow] = (double)e + eshift;
   }
 
   // ----------------------------------------------------------- backward --
@@ -401,7 +486,25 @@ __global__ void __launch_bounds__(NW * 64) k_fit_fused(FusedFitArgs p) {
       const int otw = (OT + NW - 1) / NW;
       floatx4 acc[RT][CT];
       zero_acc(acc);
-      gemm_lds_a<RT, false, CT>(acc, &xs[buf][0][0], kXld, L.wb, L.wb, KT, OT, wave * otw, otw, lane);
+      if (CHAIN && chain) {
+        const int ct0 = wave * otw;
+        const bool use0 = otw > 0 && ct0 < OT, use1 = CT > 1 && otw > 1 && ct0 + 1 < OT;
+        const FusedLayer Ln = Ls[l - 1];  // next: layer l - 1's backward product
+        const int OTn = Ln.K / 16;
+        int n0, n1;
+        if (l > 1) {
+          const int otwn = (OTn + NW - 1) / NW;
+          n0 = min(wave * otwn, OTn - 1);
+          n1 = min(wave * otwn + 1, OTn - 1);
+        } else {  // layer 0: the first chunk of output tiles
+          n0 = min(wave * CT, OTn - 1);
+          n1 = min(wave * CT + CT - 1, OTn - 1);
+        }
+        gemm8_chain<RT, CT, false, false>(acc, &xs[buf][0][0], rq, rl, use0, use1, L.wb, L.wb, OT, min(ct0, OT - 1),
+                                          min(ct0 + 1, OT - 1), Ln.wb, Ln.wb, OTn, n0, n1, lane);
+      } else {
+        gemm_lds_a<RT, false, CT>(acc, &xs[buf][0][0], kXld, L.wb, L.wb, KT, OT, wave * otw, otw, lane);
+      }
       // g_l = gz W^T (+ g_{l+1} for residual layers: same positions)
 #pragma unroll
       for (int i = 0; i < RT; ++i)
@@ -418,11 +521,12 @@ __global__ void __launch_bounds__(NW * 64) k_fit_fused(FusedFitArgs p) {
       constexpr int KS = 8;
       const int nchunk = (OT + CT * NW - 1) / (CT * NW);
       auto tile = [&](int c, int j) { return min(c * CT * NW + wave * CT + j, OT - 1); };
-      half8 bq[kPf][CT];
+      half8(&bq)[kPf][CT] = rq;  // holds chunk 0's first steps when chained
+      if (!(CHAIN && chain))
 #pragma unroll
-      for (int s = 0; s < kPf; ++s)
+        for (int s = 0; s < kPf; ++s)
 #pragma unroll
-        for (int j = 0; j < CT; ++j) bq[s][j] = load_bfrag(L.wb, s, OT, tile(0, j), lane);
+          for (int j = 0; j < CT; ++j) bq[s][j] = load_bfrag(L.wb, s, OT, tile(0, j), lane);
       for (int c = 0; c < nchunk; ++c) {
         // the A fragments are the same for every chunk: read them per chunk
         // (kept across chunks they would take 32 RT registers)
