@@ -34,6 +34,7 @@
 
 #include <algorithm>
 #include <stdexcept>
+#include <type_traits>
 
 #include "dpmi_common.h"
 #include "kernels.h"
@@ -327,8 +328,11 @@ __global__ void __launch_bounds__(NW * 64) k_fit_fused(FusedFitArgs p) {
 #pragma unroll
       for (int m = 0; m < kFr; ++m) afr[0][wave + NW * m][lane] = dval[m] ? ar[m] : zero8();
       __syncthreads();
-      for (int c = 0; c < NC; ++c) {
-        if (c + 1 < NC)
+      // one chunk; the last one is peeled so that only its refills choose a
+      // source (layer 1's first fragments when chained)
+      auto chunk = [&](int c, auto last_tag) {
+        constexpr bool LAST = decltype(last_tag)::value;
+        if constexpr (!LAST)
 #pragma unroll
           for (int m = 0; m < kFr; ++m) ar[m] = __builtin_nontemporal_load((gh8*)(dsrc[m] + (c + 1) * kKC * 32));
 #pragma unroll
@@ -350,12 +354,20 @@ __global__ void __launch_bounds__(NW * 64) k_fit_fused(FusedFitArgs p) {
                 acc[i][CT - 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bq[s][CT - 1], acc[i][CT - 1], 0, 0, 0);
                 acc[i][CT - 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], bl[s][CT - 1], acc[i][CT - 1], 0, 0, 0);
               }
-          // unconditional loads (selected sources) keep the wait counts exact
-          const bool nx = chain && k + kPf >= KT;
-          const int kn = nx ? k + kPf - KT : min(k + kPf, KT - 1);
-          const _Float16* wq = nx ? L1.wf : L.wf;
-          const _Float16* wql = nx ? L1.wf_lo : L.wf_lo;
-          const int NTq = nx ? NT1 : NT, q0 = nx ? n10 : c0, q1 = nx ? n11 : c1;
+          // unconditional loads keep the wait counts exact; in the last chunk
+          // step k + kPf is step s of the next product (kKC == kPf), or a
+          // repeat of the last step when nothing follows
+          const _Float16* wq = L.wf;
+          const _Float16* wql = L.wf_lo;
+          int kn = k + kPf, NTq = NT, q0 = c0, q1 = c1;
+          if constexpr (LAST) {
+            kn = chain ? s : KT - 1;
+            wq = chain ? L1.wf : L.wf;
+            wql = chain ? L1.wf_lo : L.wf_lo;
+            NTq = chain ? NT1 : NT;
+            q0 = chain ? n10 : c0;
+            q1 = chain ? n11 : c1;
+          }
           bq[s][0] = load_bfrag(wq, kn, NTq, q0, lane);
           bl[s][0] = load_bfrag(wql, kn, NTq, q0, lane);
           if constexpr (CT > 1) {
@@ -365,11 +377,13 @@ __global__ void __launch_bounds__(NW * 64) k_fit_fused(FusedFitArgs p) {
         }
         // chunk c + 1 goes to the buffer read in chunk c - 1 (every wave has
         // passed the barrier after that chunk)
-        if (c + 1 < NC)
+        if constexpr (!LAST)
 #pragma unroll
           for (int m = 0; m < kFr; ++m) afr[(c + 1) & 1][wave + NW * m][lane] = dval[m] ? ar[m] : zero8();
         __syncthreads();
-      }
+      };
+      for (int c = 0; c + 1 < NC; ++c) chunk(c, std::false_type{});
+      chunk(NC - 1, std::true_type{});
     } else if (CHAIN && chain) {
       const bool use0 = ntw > 0 && ct0 < NT, use1 = CT > 1 && ntw > 1 && ct0 + 1 < NT;
       const int c0 = min(ct0, NT - 1), c1 = min(ct0 + 1, NT - 1);
