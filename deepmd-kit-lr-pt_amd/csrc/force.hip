@@ -489,8 +489,19 @@ constexpr int kChunkRows = 1024;
 
 __device__ __forceinline__ void frame_total(const double* __restrict__ part, int c0, int c1, int q,
                                             double* __restrict__ energy, double* __restrict__ virial, int f) {
+  // loads issued eight at a time ahead of the (sequential, chunk-ordered)
+  // compensated additions, so the frame's partials cost one load latency per
+  // eight chunks instead of one per chunk
   KahanD t;
-  for (int c = c0; c < c1; ++c) t.add(part[(size_t)c * 10 + q]);
+  int c = c0;
+  for (; c + 8 <= c1; c += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = part[(size_t)(c + u) * 10 + q];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t.add(v[u]);
+  }
+  for (; c < c1; ++c) t.add(part[(size_t)c * 10 + q]);
   if (q == 0)
     energy[f] = t.value();
   else
@@ -505,18 +516,31 @@ __global__ void __launch_bounds__(256) k_chunk_reduce(const double* __restrict__
   __shared__ double sh[10][256];
   const int4 ch = chunks[blockIdx.x];  // (first row, row count, frame, chunks of the frame)
   KahanD acc[10];
-  for (int r = threadIdx.x; r < ch.y; r += blockDim.x) {
-    const size_t row = (size_t)ch.x + r;
-    acc[0].add(atom_e[row]);
+  // a thread's rows (at most kChunkRows / 256 = 4) loaded before they are added
+  static_assert(kChunkRows <= 4 * 256, "a thread sums at most four rows");
+  double v[4][10];
+  if (ch.y > 0)
 #pragma unroll
-    for (int k = 0; k < 9; ++k) acc[k + 1].add(vpart[row * 9 + k]);
+  for (int u = 0; u < 4; ++u) {
+    const int r = threadIdx.x + u * (int)blockDim.x;
+    const size_t row = (size_t)ch.x + (r < ch.y ? r : 0);
+    v[u][0] = atom_e[row];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) v[u][k + 1] = vpart[row * 9 + k];
   }
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if ((int)threadIdx.x + u * (int)blockDim.x < ch.y)
+#pragma unroll
+      for (int k = 0; k < 10; ++k) acc[k].add(v[u][k]);
 #pragma unroll
   for (int k = 0; k < 10; ++k) sh[k][threadIdx.x] = acc[k].value();
   __syncthreads();
   // pairwise tree over the 256 thread sums (rounding error grows with
   // log2(256), not with the count; the order is fixed, so results repeat)
-  for (int h = 128; h > 0; h >>= 1) {
+  // (levels 128 and 64 through LDS, the six within wave 0 by shuffles: the
+  // same pairs (t, t + h) added in the same order)
+  for (int h = 128; h >= kWave; h >>= 1) {
     if ((int)threadIdx.x < h)
 #pragma unroll
       for (int k = 0; k < 10; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + h];
@@ -524,7 +548,22 @@ __global__ void __launch_bounds__(256) k_chunk_reduce(const double* __restrict__
   }
   if (threadIdx.x >= kWave) return;  // wave 0 writes the partials and takes the ticket
   const int q = threadIdx.x;
-  if (q < 10) part[(size_t)blockIdx.x * 10 + q] = sh[q][0];
+  double tv[10];
+#pragma unroll
+  for (int k = 0; k < 10; ++k) tv[k] = sh[k][q];
+#pragma unroll
+  for (int h = kWave / 2; h > 0; h >>= 1)
+#pragma unroll
+    for (int k = 0; k < 10; ++k) tv[k] += __shfl_down(tv[k], h, kWave);
+  // lane 0 holds the block's sums (every lane takes them: cross-lane reads
+  // outside any condition); lane q < 10 writes value q
+  double mine = 0.;
+#pragma unroll
+  for (int k = 0; k < 10; ++k) {
+    const double s0 = __shfl(tv[k], 0, kWave);
+    mine = q == k ? s0 : mine;
+  }
+  if (q < 10) part[(size_t)blockIdx.x * 10 + q] = mine;
   if (!ticket) return;
   __threadfence();  // this chunk's partials before its ticket
   int last = 0;
