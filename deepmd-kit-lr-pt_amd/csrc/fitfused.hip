@@ -149,6 +149,14 @@ __device__ __forceinline__ void gemm_lds_a(floatx4 (&acc)[RT][CT], const _Float1
     gemm_lds_a_kt<RT, LO, 0, CT>(acc, a_lds, ld, w, wl, KT, NT, ct0, ntw, lane);
 }
 
+// Lane i of each row of 16 takes lane i + N's value (DPP row_shl; lanes past
+// the row end read zero).  Lane 0 of a row then sums like a xor butterfly:
+// s + shl1, + shl2, + shl4, + shl8 adds the same pairs in the same order.
+template <int N>
+__device__ __forceinline__ float dpp_row_shl(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x100 + N, 0xF, 0xF, true));
+}
+
 template <int RT, int CT = 2>
 __device__ __forceinline__ void zero_acc(floatx4 (&acc)[RT][CT]) {
 #pragma unroll
@@ -437,10 +445,10 @@ __global__ void __launch_bounds__(NW * 64) k_fit_fused(FusedFitArgs p) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       float s = epart[i][e];
-      s += __shfl_xor(s, 1, 64);
-      s += __shfl_xor(s, 2, 64);
-      s += __shfl_xor(s, 4, 64);
-      s += __shfl_xor(s, 8, 64);
+      s += dpp_row_shl<1>(s);
+      s += dpp_row_shl<2>(s);
+      s += dpp_row_shl<4>(s);
+      s += dpp_row_shl<8>(s);
       if (lr == 0) esum[wave][i * 16 + lg * 4 + e] = s;
     }
   __syncthreads();
