@@ -1158,6 +1158,12 @@ __device__ __forceinline__ mhalf4 load_h4(__amdgpu_buffer_rsrc_t rs, int off) {
   return __builtin_bit_cast(mhalf4, v);
 }
 
+// Lane i of each row of 16 takes lane i + 4's value (DPP row_shl:4, on the
+// vector ALU instead of an LDS permute); lanes 12..15 of a row read zero.
+__device__ __forceinline__ float dpp_row_shl4(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x104, 0xF, 0xF, true));
+}
+
 // Orders the LDS accesses of one wavefront's lanes (wave-private buffers).
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1398,12 +1404,12 @@ __global__ void __launch_bounds__(256) k_embed_bwd_runs(const EmbedArgs a) {
 #pragma unroll
           for (int pp = 0; pp < 2; ++pp) h[pp] = __builtin_amdgcn_mfma_f32_16x16x16f16(av[pp][kc], bf[kc], h[pp], 0, 0, 0);
         // lane (n, q) holds A-rows 4 q + v of column n; add the dy_lo column
-        // n + 4 to the dy_hi column n
+        // n + 4 to the dy_hi column n (lanes n < 4 use the sum)
 #pragma unroll
         for (int pp = 0; pp < 2; ++pp) {
           float hv[4];
 #pragma unroll
-          for (int v = 0; v < 4; ++v) hv[v] = h[pp][v] + __shfl_xor(h[pp][v], 4, kWave);
+          for (int v = 0; v < 4; ++v) hv[v] = h[pp][v] + dpp_row_shl4(h[pp][v]);
           const int run = r0 + 2 * pp + (q >> 1);
           if (n < 4 && run < R) {
             float* hs = buf + run * kRunH;
