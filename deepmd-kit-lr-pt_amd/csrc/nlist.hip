@@ -39,6 +39,20 @@ __device__ __forceinline__ unsigned long long make_key(int t, float r2, int j) {
 }
 
 // Appends the keys of the lanes with keep set; returns the new count.
+// Lane l ^ j's key: DPP quad_perm moves for j = 1, 2 (vector ALU, no LDS
+// permute), a shuffle otherwise (j is wave-uniform).
+template <int CTRL>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v) {
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)v, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(v >> 32), CTRL, 0xF, 0xF, false);
+  return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+__device__ __forceinline__ unsigned long long xor_lane(unsigned long long v, int j) {
+  if (j == 1) return dpp_u64<0xB1>(v);  // quad_perm [1, 0, 3, 2]
+  if (j == 2) return dpp_u64<0x4E>(v);  // quad_perm [2, 3, 0, 1]
+  return __shfl_xor(v, j, kWave);
+}
+
 template <int CAP>
 __device__ __forceinline__ int wave_append(unsigned long long* keys, int n, bool keep, unsigned long long key,
                                            int* flag) {
@@ -96,7 +110,7 @@ __device__ __forceinline__ void wave_bitonic(unsigned long long (&kr)[KR]) {
         for (int r = 0; r < KR; ++r) {
           const int t = lane + kWave * r;
           const bool up = (t & k) == 0;
-          const unsigned long long o = __shfl_xor(kr[r], j, kWave);
+          const unsigned long long o = xor_lane(kr[r], j);
           // the lower element of an ascending pair keeps the minimum
           const bool keep_min = lower == up;
           kr[r] = keep_min ? (o < kr[r] ? o : kr[r]) : (o > kr[r] ? o : kr[r]);
