@@ -1164,6 +1164,29 @@ __device__ __forceinline__ float dpp_row_shl4(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x104, 0xF, 0xF, true));
 }
 
+// Largest of a non-negative value over the wavefront, in every lane: row
+// maxima by DPP row shifts (lanes past a row's end read zero), then the four
+// rows' lane 0 (max is exact in any order).
+__device__ __forceinline__ float wave_max_nonneg(float v) {
+#pragma unroll
+  for (int n = 1; n < 16; n <<= 1) {
+    const int ctrl = 0x100 + n;
+    float u;
+    switch (ctrl) {
+      case 0x101: u = __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x101, 0xF, 0xF, true)); break;
+      case 0x102: u = __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x102, 0xF, 0xF, true)); break;
+      case 0x104: u = __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x104, 0xF, 0xF, true)); break;
+      default: u = __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x108, 0xF, 0xF, true)); break;
+    }
+    v = fmaxf(v, u);
+  }
+  const float r0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 0));
+  const float r1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 16));
+  const float r2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 32));
+  const float r3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 48));
+  return fmaxf(fmaxf(r0, r1), fmaxf(r2, r3));
+}
+
 // Orders the LDS accesses of one wavefront's lanes (wave-private buffers).
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1306,8 +1329,7 @@ __global__ void __launch_bounds__(256) k_embed_bwd_runs(const EmbedArgs a) {
   for (int kq = 0; kq < KQ; ++kq)
 #pragma unroll
     for (int c = 0; c < 4; ++c) dmax = fmaxf(dmax, fabsf(dyo[kq][c]));
-  for (int off = 32; off > 0; off >>= 1) dmax = fmaxf(dmax, __shfl_xor(dmax, off, kWave));
-  dmax = uni(dmax);
+  dmax = uni(wave_max_nonneg(dmax));
   int ex = 0;
   frexpf(dmax, &ex);
   const float sc = dmax > 0.f ? ldexpf(1.f, 14 - ex) : 1.f, isc = 1.f / sc;
@@ -1370,7 +1392,8 @@ __global__ void __launch_bounds__(256) k_embed_bwd_runs(const EmbedArgs a) {
       float t = ll.x, delta;
       int row;
       locate(f, t, row, delta);
-      const int prow = __shfl_up(row, 1, kWave);
+      // the previous lane's row (DPP wave_shr:1; lane 0 does not use it)
+      const int prow = __builtin_amdgcn_mov_dpp(row, 0x138, 0xF, 0xF, true);
       const bool start = valid && (lane == 0 || row != prow);
       const unsigned long long sm = __ballot(start);
       const int R = __popcll(sm);
